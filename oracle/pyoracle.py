@@ -1,0 +1,149 @@
+"""ctypes wrapper of the CPU oracle (oracle/vsoracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker of the GPU library.  The product
+never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libvsref.so")
+
+
+class Field(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int16), ("y", ctypes.c_int16), ("size", ctypes.c_int16)]
+
+
+class Vec(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int16), ("y", ctypes.c_int16)]
+
+
+class LocalMotion(ctypes.Structure):
+    _fields_ = [("v", Vec), ("f", Field), ("contrast", ctypes.c_double),
+                ("match", ctypes.c_double)]
+
+
+class VSTransform(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("zoom", ctypes.c_double), ("barrel", ctypes.c_double),
+                ("rshutter", ctypes.c_double), ("extra", ctypes.c_int)]
+
+
+class OrFields(ctypes.Structure):
+    _fields_ = [("fieldSize", ctypes.c_int), ("maxShift", ctypes.c_int),
+                ("stepSize", ctypes.c_int), ("fieldNum", ctypes.c_int),
+                ("maxFields", ctypes.c_int), ("fieldRows", ctypes.c_int),
+                ("contrastThreshold", ctypes.c_double), ("fields", ctypes.POINTER(Field))]
+
+
+class OrState(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("shakiness", ctypes.c_int), ("accuracy", ctypes.c_int),
+                ("stepSize", ctypes.c_int), ("virtualTripod", ctypes.c_int),
+                ("contrastThreshold", ctypes.c_double),
+                ("coarse", OrFields), ("fine", OrFields),
+                ("blurSize", ctypes.c_int),
+                ("curr", ctypes.c_void_p), ("prev", ctypes.c_void_p), ("tmp", ctypes.c_void_p),
+                ("hasSeenOneFrame", ctypes.c_int), ("frameNum", ctypes.c_int),
+                ("nCoarse", ctypes.c_int), ("nFineAll", ctypes.c_int), ("nFineKept", ctypes.c_int),
+                ("lastT", VSTransform), ("lastMeanMatch", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.or_new.restype = ctypes.POINTER(OrState)
+        L.or_new.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double, ctypes.c_int]
+        L.or_delete.argtypes = [ctypes.POINTER(OrState)]
+        L.or_detect.restype = ctypes.c_int
+        L.or_detect.argtypes = [ctypes.POINTER(OrState), ctypes.c_void_p, ctypes.c_int,
+                                ctypes.POINTER(LocalMotion), ctypes.c_int]
+        L.or_blurred.restype = ctypes.c_void_p
+        L.or_blurred.argtypes = [ctypes.POINTER(OrState)]
+        L.or_boxblur.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 4
+        L.or_blur_size.restype = ctypes.c_int
+        L.or_blur_size.argtypes = [ctypes.c_int] * 3
+        L.or_sad.restype = ctypes.c_uint32
+        L.or_sad.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6
+        L.or_spiral_rank.restype = ctypes.c_int
+        L.or_spiral_rank.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.or_spiral_walk.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.or_cleanmean.restype = ctypes.c_double
+        L.or_cleanmean.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.or_contrast.restype = ctypes.c_double
+        L.or_contrast.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.or_select.restype = ctypes.c_int
+        L.or_select.argtypes = [ctypes.POINTER(OrFields), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+        L.or_simple_transform.restype = VSTransform
+        L.or_simple_transform.argtypes = [ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(LocalMotion), ctypes.c_int]
+        L.or_transform_point.argtypes = [ctypes.POINTER(VSTransform)] + [ctypes.c_int] * 4 + \
+            [ctypes.POINTER(ctypes.c_int)] * 2
+        _lib = L
+    return _lib
+
+
+def motions_to_list(arr, n):
+    """LocalMotions as comparable tuples (doubles as their IEEE bit patterns)."""
+    out = []
+    for i in range(n):
+        m = arr[i]
+        c = np.array([m.contrast], dtype=np.float64).view(np.uint64)[0]
+        t = np.array([m.match], dtype=np.float64).view(np.uint64)[0]
+        out.append((m.v.x, m.v.y, m.f.x, m.f.y, m.f.size, int(c), int(t)))
+    return out
+
+
+class Detector:
+    """One vsMotionDetection stream on the CPU oracle."""
+
+    def __init__(self, width, height, shakiness=5, accuracy=15, stepsize=6,
+                 mincontrast=0.25, tripod=0):
+        self._L = lib()
+        self.width, self.height = width, height
+        self.s = self._L.or_new(width, height, shakiness, accuracy, stepsize,
+                                float(mincontrast), tripod)
+        if not self.s:
+            raise RuntimeError("or_new failed")
+        st = self.s.contents
+        self.maxout = st.coarse.fieldNum + st.fine.fieldNum + 1
+        self.buf = (LocalMotion * self.maxout)()
+
+    def detect(self, luma):
+        """luma: uint8 array (H, stride) -> list of motion tuples"""
+        luma = np.ascontiguousarray(luma, dtype=np.uint8)
+        n = self._L.or_detect(self.s, luma.ctypes.data, luma.shape[1], self.buf, self.maxout)
+        return motions_to_list(self.buf, n)
+
+    def blurred(self):
+        p = self._L.or_blurred(self.s)
+        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint8)),
+                                     shape=(self.height, self.width)).copy()
+
+    @property
+    def state(self):
+        return self.s.contents
+
+    def close(self):
+        if self.s:
+            self._L.or_delete(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

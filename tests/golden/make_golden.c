@@ -1,0 +1,176 @@
+/*
+ * make_golden.c -- writes the detect-pass golden fixtures from the REFERENCE
+ * implementation (oracle/_ref/libvsref.so, built from /root/reference/src
+ * with the reference's own release flags).
+ *
+ * Run only where /root/reference exists (see make_golden.sh).  For each case
+ * it regenerates the seeded synthetic frames, runs the reference's
+ * vsMotionDetectInit / vsMotionDetection / vsPrepareFile / vsWriteToFile, and
+ * records per frame: a hash of the input luma (pins the generator), a hash of
+ * the reference's blurred plane (md->curr), every LocalMotion with the exact
+ * bits of its doubles, and hashes of the binary and ASCII .trf streams.
+ */
+#define _GNU_SOURCE 1
+#include <inttypes.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "motiondetect.h"
+#include "serialize.h"
+#include "vsynth.h"
+
+static uint64_t fnv(const void* p, size_t n, uint64_t h) {
+  const uint8_t* b = (const uint8_t*)p;
+  for (size_t i = 0; i < n; i++) { h ^= b[i]; h *= 0x100000001B3ull; }
+  return h;
+}
+
+static uint64_t dbits(double d) { uint64_t u; memcpy(&u, &d, 8); return u; }
+
+typedef struct {
+  const char* name;
+  int config, clip;
+  int width, height;     /* 0: from the spec */
+  int nframes;           /* frames to run */
+  int stepsize;          /* 0: from the spec */
+  int accuracy;          /* 0: from the spec */
+  int shakiness;         /* 0: from the spec */
+  int tripod;
+  int stride_pad;
+  double mincontrast;    /* <0: from the spec */
+} Case;
+
+static void run_case(FILE* out, const Case* cs, int first) {
+  VSynSpec sp;
+  vsyn_spec(cs->config, cs->clip, &sp);
+  if (cs->width) { sp.width = cs->width; sp.height = cs->height; }
+  if (cs->stepsize) sp.stepsize = cs->stepsize;
+  if (cs->accuracy) sp.accuracy = cs->accuracy;
+  if (cs->shakiness) sp.shakiness = cs->shakiness;
+  if (cs->mincontrast >= 0) sp.mincontrast = cs->mincontrast;
+  int nfr = cs->nframes < sp.nframes ? cs->nframes : sp.nframes;
+  int total = sp.nframes > nfr ? sp.nframes : nfr;
+  VSynClip clip;
+  VSynMotion* path = (VSynMotion*)calloc((size_t)total, sizeof(VSynMotion));
+  VSynSpec sp2 = sp; sp2.nframes = total;
+  vsyn_open(&sp2, &clip, path);
+
+  VSFrameInfo fi;
+  vsFrameInfoInit(&fi, sp.width, sp.height, PF_YUV420P);
+  VSMotionDetect md;
+  memset(&md, 0, sizeof(md));
+  VSMotionDetectConfig conf = vsMotionDetectGetDefaultConfig("golden");
+  conf.shakiness = sp.shakiness;
+  conf.accuracy = sp.accuracy;
+  conf.stepSize = sp.stepsize;
+  conf.contrastThreshold = sp.mincontrast;
+  conf.virtualTripod = cs->tripod;
+  conf.numThreads = 1;
+  if (vsMotionDetectInit(&md, &conf, &fi) != VS_OK) { fprintf(stderr, "init failed\n"); exit(1); }
+
+  char* bbuf = NULL; size_t blen = 0;
+  char* abuf = NULL; size_t alen = 0;
+  FILE* fb = open_memstream(&bbuf, &blen);
+  FILE* fa = open_memstream(&abuf, &alen);
+  vsPrepareFile(&md, fb);
+  md.serializationMode = ASCII_SERIALIZATION_MODE;
+  vsPrepareFile(&md, fa);
+  md.serializationMode = BINARY_SERIALIZATION_MODE;
+
+  int ls = sp.width + cs->stride_pad;
+  uint8_t* Y = (uint8_t*)malloc((size_t)ls * sp.height);
+  uint8_t* U = (uint8_t*)malloc((size_t)(sp.width / 2) * (sp.height / 2));
+  uint8_t* V = (uint8_t*)malloc((size_t)(sp.width / 2) * (sp.height / 2));
+  memset(Y, 0x5A, (size_t)ls * sp.height);
+  memset(U, 128, (size_t)(sp.width / 2) * (sp.height / 2));
+  memset(V, 128, (size_t)(sp.width / 2) * (sp.height / 2));
+
+  fprintf(out, "%s{\"name\":\"%s\",\"config\":%d,\"clip\":%d,\"width\":%d,\"height\":%d,"
+               "\"stride\":%d,\"shakiness\":%d,\"accuracy\":%d,\"stepsize\":%d,"
+               "\"mincontrast\":\"%016" PRIx64 "\",\"tripod\":%d,\"md_conf\":[%d,%d,%d],"
+               "\"coarse\":[%d,%d,%d,%d],\"fine\":[%d,%d,%d,%d],\"frames\":[",
+          first ? "" : ",\n", cs->name, cs->config, cs->clip, sp.width, sp.height, ls,
+          sp.shakiness, sp.accuracy, sp.stepsize, dbits(sp.mincontrast), cs->tripod,
+          md.conf.shakiness, md.conf.accuracy, md.conf.stepSize,
+          md.fieldscoarse.fieldSize, md.fieldscoarse.maxShift, md.fieldscoarse.fieldNum,
+          md.fieldscoarse.maxFields, md.fieldsfine.fieldSize, md.fieldsfine.maxShift,
+          md.fieldsfine.fieldNum, md.fieldsfine.maxFields);
+  for (int t = 0; t < nfr; t++) {
+    vsyn_render(&clip, path, t, Y, ls);
+    VSFrame fr;
+    memset(&fr, 0, sizeof(fr));
+    fr.data[0] = Y; fr.linesize[0] = ls;
+    fr.data[1] = U; fr.linesize[1] = sp.width / 2;
+    fr.data[2] = V; fr.linesize[2] = sp.width / 2;
+    LocalMotions lms;
+    vsMotionDetection(&md, &lms, &fr);
+    uint64_t hin = vsyn_hash_plane(Y, sp.width, sp.height, ls);
+    uint64_t hbl = vsyn_hash_plane(md.curr.data[0], sp.width, sp.height, md.curr.linesize[0]);
+    vsWriteToFile(&md, fb, &lms);
+    md.serializationMode = ASCII_SERIALIZATION_MODE;
+    vsWriteToFile(&md, fa, &lms);
+    md.serializationMode = BINARY_SERIALIZATION_MODE;
+    int n = vs_vector_size(&lms);
+    fprintf(out, "%s\n {\"in\":\"%016" PRIx64 "\",\"blur\":\"%016" PRIx64 "\",\"lm\":[",
+            t ? "," : "", hin, hbl);
+    for (int i = 0; i < n; i++) {
+      LocalMotion* m = (LocalMotion*)vs_vector_get(&lms, i);
+      fprintf(out, "%s[%d,%d,%d,%d,%d,\"%016" PRIx64 "\",\"%016" PRIx64 "\"]", i ? "," : "",
+              m->v.x, m->v.y, m->f.x, m->f.y, m->f.size, dbits(m->contrast), dbits(m->match));
+    }
+    fprintf(out, "]}");
+    vs_vector_del(&lms);
+  }
+  fflush(fb);
+  fflush(fa);
+  fprintf(out, "],\"trf_bin\":\"%016" PRIx64 "\",\"trf_bin_len\":%zu,"
+               "\"trf_ascii\":\"%016" PRIx64 "\",\"trf_ascii_len\":%zu}",
+          fnv(bbuf, blen, 0xCBF29CE484222325ull), blen,
+          fnv(abuf, alen, 0xCBF29CE484222325ull), alen);
+  fclose(fb);
+  fclose(fa);
+  free(bbuf);
+  free(abuf);
+  vsMotionDetectionCleanup(&md);
+  vsyn_clip_free(&clip);
+  free(path);
+  free(Y); free(U); free(V);
+  fprintf(stderr, "case %s done (%d frames)\n", cs->name, nfr);
+}
+
+int main(int argc, char** argv) {
+  const char* path = argc > 1 ? argv[1] : "detect_golden.json";
+  vs_log_level = 0;
+  static const Case cases[] = {
+    /* name          cfg clip  w     h    nfr step acc shak tripod pad  contrast */
+    { "c1_full",      1, 0,    0,    0,   60,  0,   0,  0,   0,    0,  -1 },
+    { "c1_stride",    1, 1,    0,    0,   8,   0,   0,  0,   0,    40, -1 },
+    { "c1_tripod",    1, 2,    0,    0,   10,  0,   0,  0,   3,    0,  -1 },
+    { "c1_step5",     1, 3,    0,    0,   6,   5,   0,  0,   0,    0,  -1 },
+    { "c1_step3",     1, 4,    0,    0,   6,   3,   0,  0,   0,    0,  -1 },
+    { "c1_step8acc6", 1, 5,    0,    0,   6,   8,   6,  0,   0,    0,  -1 },
+    { "c1_step1",     1, 6,    0,    0,   4,   1,   0,  0,   0,    0,  -1 },
+    { "c1_acc4",      1, 7,    0,    0,   6,   0,   4,  3,   0,    0,  -1 },
+    { "odd_502x298",  1, 8,    502,  298, 6,   0,   0,  0,   0,    3,  -1 },
+    { "c2_1080p",     2, 0,    0,    0,   5,   0,   0,  0,   0,    0,  -1 },
+    { "c3_4k",        3, 0,    0,    0,   3,   0,   0,  0,   0,    0,  -1 },
+    { "c4_clip0",     4, 0,    0,    0,   4,   0,   0,  0,   0,    0,  -1 },
+    { "c4_clip1",     4, 1,    0,    0,   4,   0,   0,  0,   0,    0,  -1 },
+    { "c4_clip2",     4, 2,    0,    0,   4,   0,   0,  0,   0,    0,  -1 },
+    { "c5_4kblur",    5, 0,    0,    0,   2,   0,   0,  0,   0,    0,  -1 },
+  };
+  FILE* out = fopen(path, "w");
+  if (!out) return 1;
+  fprintf(out, "{\"source\":\"reference vsMotionDetection (oracle/_ref/libvsref.so)\",\"cases\":[\n");
+  int only = argc > 2 ? atoi(argv[2]) : -1;
+  int first = 1;
+  for (size_t i = 0; i < sizeof(cases) / sizeof(cases[0]); i++) {
+    if (only >= 0 && (int)i != only) continue;
+    run_case(out, &cases[i], first);
+    first = 0;
+  }
+  fprintf(out, "\n]}\n");
+  fclose(out);
+  return 0;
+}

@@ -1,0 +1,15 @@
+#!/bin/sh
+# Regenerates tests/golden/detect_golden.json from the reference sources.
+# Needs /root/reference (this container only); the GPU box never runs it.
+set -e
+HERE=$(cd "$(dirname "$0")" && pwd)
+REPO=$(cd "$HERE/../.." && pwd)
+REF=${REF:-/root/reference}
+make -C "$REPO/oracle" ref REF="$REF"
+mkdir -p "$REPO/oracle/_ref"
+gcc -O2 -std=gnu11 -ffp-contract=off -fno-fast-math -fopenmp \
+    -I"$REF/src" -I"$REPO/vid.stab_amd/csrc/synth" \
+    -o "$REPO/oracle/_ref/make_golden" "$HERE/make_golden.c" \
+    "$REPO/vid.stab_amd/csrc/synth/vsynth.c" \
+    -L"$REPO/oracle/_ref" -Wl,-rpath,"$REPO/oracle/_ref" -lvsref -lm
+"$REPO/oracle/_ref/make_golden" "$HERE/detect_golden.json" "$@"
