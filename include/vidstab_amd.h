@@ -1,0 +1,37 @@
+/*
+ * vidstab_amd.h -- extensions of the detect API that have no counterpart in
+ * the reference.  They batch what the reference does one call at a time; the
+ * results are exactly those of n consecutive vsMotionDetection calls.
+ *
+ * Environment (read by vsMotionDetectInit):
+ *   VIDSTAB_AMD_DEVICE     HIP device ordinal of new detectors (default 0)
+ *   VIDSTAB_AMD_MAX_BATCH  frames per device batch (default 16)
+ */
+#ifndef VIDSTAB_AMD_H
+#define VIDSTAB_AMD_H
+
+#include <stdint.h>
+#include "motiondetect.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* n consecutive detect steps on host frames; motions[i] receives the
+   LocalMotions of frames[i].  For callers that hold several frames at once
+   (file-based tools); one device batch per VIDSTAB_AMD_MAX_BATCH frames. */
+VS_API int vsMotionDetectionBatch(VSMotionDetect* md, LocalMotions* motions, VSFrame* frames, int n);
+
+/* as vsMotionDetectionBatch, but the luma planes are already in device memory
+   of the detector's GPU (lumaDev[i], linesize[i] bytes per row) */
+VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions,
+                                   const uint8_t* const* lumaDev, const int* linesize, int n);
+
+/* copies the last blurred plane of the detector (the reference's md->curr
+   luma) into dst; for parity checks */
+VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VIDSTAB_AMD_H */

@@ -147,3 +147,58 @@ class Detector:
             self.close()
         except Exception:
             pass
+
+
+class RefDetector:
+    """The reference's own vsMotionDetection (oracle/_ref/libvsref.so: its C
+    sources built with its release flags, SIMD dispatch and OpenMP), driven
+    through the same C API.  Used as the CPU baseline and as a cross-check."""
+
+    def __init__(self, width, height, shakiness=5, accuracy=15, stepsize=6, mincontrast=0.25,
+                 tripod=0, threads=0):
+        from vidstab_amd import api  # struct layouts only (identical ABI)
+        if not os.path.exists(REF_PATH):
+            raise RuntimeError(f"reference build missing: {REF_PATH}")
+        self.api = api
+        L = ctypes.CDLL(REF_PATH)
+        L.vsFrameInfoInit.argtypes = [ctypes.POINTER(api.VSFrameInfo)] + [ctypes.c_int] * 3
+        L.vsMotionDetectGetDefaultConfig.restype = api.VSMotionDetectConfig
+        L.vsMotionDetectGetDefaultConfig.argtypes = [ctypes.c_char_p]
+        L.vsMotionDetectInit.argtypes = [ctypes.POINTER(api.VSMotionDetect),
+                                         ctypes.POINTER(api.VSMotionDetectConfig),
+                                         ctypes.POINTER(api.VSFrameInfo)]
+        L.vsMotionDetection.argtypes = [ctypes.POINTER(api.VSMotionDetect),
+                                        ctypes.POINTER(api.VSVector), ctypes.POINTER(api.VSFrame)]
+        L.vsMotionDetectionCleanup.argtypes = [ctypes.POINTER(api.VSMotionDetect)]
+        L.vs_vector_del.argtypes = [ctypes.POINTER(api.VSVector)]
+        ctypes.c_int.in_dll(L, "vs_log_level").value = 0
+        self.L = L
+        self.fi = api.VSFrameInfo()
+        L.vsFrameInfoInit(ctypes.byref(self.fi), width, height, api.PF_YUV420P)
+        self.md = api.VSMotionDetect()
+        conf = L.vsMotionDetectGetDefaultConfig(b"ref")
+        conf.shakiness, conf.accuracy, conf.stepSize = shakiness, accuracy, stepsize
+        conf.contrastThreshold = float(mincontrast)
+        conf.virtualTripod = tripod
+        conf.numThreads = threads
+        if L.vsMotionDetectInit(ctypes.byref(self.md), ctypes.byref(conf), ctypes.byref(self.fi)) != 0:
+            raise RuntimeError("reference vsMotionDetectInit failed")
+        self.width, self.height = width, height
+        self._chroma = np.full((height // 2) * (width // 2), 128, dtype=np.uint8)
+
+    def detect(self, luma, keep=True):
+        api = self.api
+        fr = api.VSFrame()
+        fr.data[0] = luma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        fr.linesize[0] = luma.shape[1]
+        fr.data[1] = fr.data[2] = self._chroma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        fr.linesize[1] = fr.linesize[2] = self.width // 2
+        vec = api.VSVector()
+        self.L.vsMotionDetection(ctypes.byref(self.md), ctypes.byref(vec), ctypes.byref(fr))
+        out = api.motions_of(vec) if keep else vec.nelems
+        self.L.vs_vector_del(ctypes.byref(vec))
+        return out
+
+    def close(self):
+        if self.md.initialized:
+            self.L.vsMotionDetectionCleanup(ctypes.byref(self.md))

@@ -1,0 +1,69 @@
+/*
+ * vsg_internal.h -- kernel launch interface between vsg_api.hip (device
+ * memory, streams, the C ABI) and vsg_kernels.hip (the gfx950 kernels).
+ */
+#ifndef VSG_INTERNAL_H
+#define VSG_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vsg.h"
+#include "vsg_common.h"
+
+/* one source plane of the blur */
+struct VSGBlurSrc {
+  const uint8_t* p;
+  int pitch;
+  int pad;
+};
+
+/* Two-pass box blur of n planes (src[i] -> dst + i*slotStride, pitch dstPitch).
+   size < 2 copies.  tmp (n*dstPitch*H bytes) is used only for boxes too wide
+   for the one-kernel LDS path. */
+hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t slotStride,
+                           int dstPitch, int W, int H, int size, uint8_t* tmp, hipStream_t st);
+
+/* per-field min/max plus the aligned copy of the field ("patch") */
+struct VSGFieldArgs {
+  const uint8_t* blurred;
+  size_t slotStride;
+  int pitch;
+  const int16_t* fx;
+  const int16_t* fy;
+  const int16_t* fsz;
+  const uint32_t* patchOff;   /* byte offset of each field's patch in a slot */
+  int nfields;
+  uint8_t* patches;
+  size_t patchSlotStride;
+  uint8_t* minmax;            /* [slot][nfields][2] */
+};
+hipError_t vsg_launch_fields(const VSGFieldArgs& a, int nslots, hipStream_t st);
+
+/* one search stage over a job list */
+struct VSGSearchArgs {
+  const VSGJob* jobs;
+  int njobs;
+  const int16_t* fx;
+  const int16_t* fy;
+  const uint8_t* blurred;
+  size_t slotStride;
+  const uint8_t* ref;
+  int pitch;
+  const uint8_t* patches;       /* patch of (slot, field): patches + slot*patchSlotStride
+                                   + stageOff + field*S*S */
+  size_t patchSlotStride;
+  size_t stageOff;
+  VSGGrid g;
+  int tiles;                    /* workgroups per job in the grid kernel */
+  unsigned long long* best;     /* per job: (sad << 32) | spiral rank; ~0 between launches */
+  VSGResult* results;
+};
+
+/* fills J, K, D, CW, m, U, V for a stage; returns the workgroups per job */
+int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift);
+/* grid kernel (argmin into best[]) and refinement kernel (best[] -> results) */
+hipError_t vsg_launch_grid(const VSGSearchArgs& a, hipStream_t st);
+hipError_t vsg_launch_refine(const VSGSearchArgs& a, hipStream_t st);
+
+#endif /* VSG_INTERNAL_H */

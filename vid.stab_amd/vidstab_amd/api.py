@@ -1,0 +1,225 @@
+"""ctypes binding of lib/libvidstab_amd.so: the vid.stab detect C API.
+
+Struct layouts mirror include/*.h (which mirror the reference's ABI), so the
+Python side exercises the same entry points a C caller such as ffmpeg's
+vidstabdetect filter uses:  vsFrameInfoInit -> vsMotionDetectInit ->
+{vsMotionDetection; vsWriteToFile; vs_vector_del} -> vsMotionDetectionCleanup.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from ._paths import lib_path
+
+c_int, c_short, c_double, c_void_p = ctypes.c_int, ctypes.c_short, ctypes.c_double, ctypes.c_void_p
+
+PF_GRAY8, PF_YUV420P, PF_YUV422P, PF_YUV444P = 0, 1, 2, 3
+PF_PACKED, PF_RGB24, PF_BGR24, PF_RGBA = 8, 9, 10, 11
+ASCII_SERIALIZATION_MODE, BINARY_SERIALIZATION_MODE = 1, 2
+
+
+class VSFrameInfo(ctypes.Structure):
+    _fields_ = [("width", c_int), ("height", c_int), ("planes", c_int),
+                ("log2ChromaW", c_int), ("log2ChromaH", c_int), ("pFormat", c_int),
+                ("bytesPerPixel", c_int)]
+
+
+class VSFrame(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(ctypes.c_uint8) * 4), ("linesize", c_int * 4)]
+
+
+class VSMotionDetectConfig(ctypes.Structure):
+    _fields_ = [("shakiness", c_int), ("accuracy", c_int), ("stepSize", c_int), ("algo", c_int),
+                ("virtualTripod", c_int), ("show", c_int), ("contrastThreshold", c_double),
+                ("modName", ctypes.c_char_p), ("numThreads", c_int)]
+
+
+class Field(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int16), ("y", ctypes.c_int16), ("size", ctypes.c_int16)]
+
+
+class Vec(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int16), ("y", ctypes.c_int16)]
+
+
+class LocalMotion(ctypes.Structure):
+    _fields_ = [("v", Vec), ("f", Field), ("contrast", c_double), ("match", c_double)]
+
+
+class VSTransform(ctypes.Structure):
+    _fields_ = [("x", c_double), ("y", c_double), ("alpha", c_double), ("zoom", c_double),
+                ("barrel", c_double), ("rshutter", c_double), ("extra", c_int)]
+
+
+class VSMotionDetectFields(ctypes.Structure):
+    _fields_ = [("maxShift", c_int), ("stepSize", c_int), ("fieldNum", c_int),
+                ("maxFields", c_int), ("contrastThreshold", c_double), ("fieldSize", c_int),
+                ("fieldRows", c_int), ("fields", ctypes.POINTER(Field)), ("useOffset", c_short),
+                ("offset", VSTransform)]
+
+
+class VSMotionDetect(ctypes.Structure):
+    _fields_ = [("fi", VSFrameInfo), ("conf", VSMotionDetectConfig),
+                ("fieldscoarse", VSMotionDetectFields), ("fieldsfine", VSMotionDetectFields),
+                ("curr", VSFrame), ("currorig", VSFrame), ("currtmp", VSFrame), ("prev", VSFrame),
+                ("hasSeenOneFrame", c_short), ("initialized", c_int),
+                ("serializationMode", c_int), ("frameNum", c_int)]
+
+
+class VSVector(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(c_void_p)), ("buffersize", c_int), ("nelems", c_int)]
+
+
+class VSGStageDesc(ctypes.Structure):
+    _fields_ = [("fieldSize", c_int), ("maxShift", c_int), ("stepSize", c_int),
+                ("nfields", c_int), ("fields", ctypes.POINTER(Field))]
+
+
+class VSGJob(ctypes.Structure):
+    _fields_ = [("slot", ctypes.c_int32), ("prev", ctypes.c_int32), ("field", ctypes.c_int32),
+                ("ox", ctypes.c_int16), ("oy", ctypes.c_int16)]
+
+
+class VSGResult(ctypes.Structure):
+    _fields_ = [("tx", ctypes.c_int16), ("ty", ctypes.c_int16), ("minerror", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the native library; raises if it is not built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(lib_path("libvidstab_amd.so"), mode=ctypes.RTLD_GLOBAL)
+        L.vsFrameInfoInit.argtypes = [ctypes.POINTER(VSFrameInfo), c_int, c_int, c_int]
+        L.vsFrameInfoInit.restype = c_int
+        L.vsMotionDetectGetDefaultConfig.argtypes = [ctypes.c_char_p]
+        L.vsMotionDetectGetDefaultConfig.restype = VSMotionDetectConfig
+        L.vsMotionDetectInit.argtypes = [ctypes.POINTER(VSMotionDetect),
+                                         ctypes.POINTER(VSMotionDetectConfig),
+                                         ctypes.POINTER(VSFrameInfo)]
+        L.vsMotionDetectInit.restype = c_int
+        L.vsMotionDetection.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.POINTER(VSVector),
+                                        ctypes.POINTER(VSFrame)]
+        L.vsMotionDetection.restype = c_int
+        L.vsMotionDetectionBatch.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.POINTER(VSVector),
+                                             ctypes.POINTER(VSFrame), c_int]
+        L.vsMotionDetectionBatch.restype = c_int
+        L.vsMotionDetectionDevice.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.POINTER(VSVector),
+                                              ctypes.POINTER(c_void_p), ctypes.POINTER(c_int), c_int]
+        L.vsMotionDetectionDevice.restype = c_int
+        L.vsMotionDetectGetBlurred.argtypes = [ctypes.POINTER(VSMotionDetect), c_void_p, c_int]
+        L.vsMotionDetectGetBlurred.restype = c_int
+        L.vsMotionDetectionCleanup.argtypes = [ctypes.POINTER(VSMotionDetect)]
+        L.vsMotionDetectionCleanup.restype = None
+        L.vs_vector_del.argtypes = [ctypes.POINTER(VSVector)]
+        L.vs_vector_size.argtypes = [ctypes.POINTER(VSVector)]
+        L.vs_vector_get.argtypes = [ctypes.POINTER(VSVector), c_int]
+        L.vs_vector_get.restype = c_void_p
+        L.vsg_spiral_rank.argtypes = [c_int, c_int]
+        L.vsg_spiral_rank.restype = c_int
+        L.vsg_spiral_unrank.argtypes = [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
+        L.vsg_last_error.restype = ctypes.c_char_p
+        L.vsPrepareFile.argtypes = [ctypes.POINTER(VSMotionDetect), c_void_p]
+        L.vsWriteToFile.argtypes = [ctypes.POINTER(VSMotionDetect), c_void_p, ctypes.POINTER(VSVector)]
+        _lib = L
+    return _lib
+
+
+def _bits(x):
+    return int(np.array([x], dtype=np.float64).view(np.uint64)[0])
+
+
+def motions_of(vec):
+    """LocalMotions -> tuples (vx, vy, fx, fy, size, contrast_bits, match_bits)"""
+    L = lib()
+    out = []
+    for i in range(vec.nelems):
+        m = ctypes.cast(vec.data[i], ctypes.POINTER(LocalMotion)).contents
+        out.append((m.v.x, m.v.y, m.f.x, m.f.y, m.f.size, _bits(m.contrast), _bits(m.match)))
+    return out
+
+
+class Detector:
+    """One vsMotionDetect stream on the GPU, driven through the C API."""
+
+    def __init__(self, width, height, shakiness=5, accuracy=15, stepsize=6, mincontrast=0.25,
+                 tripod=0, pformat=PF_YUV420P, mod=b"vidstab_amd"):
+        L = lib()
+        self.L = L
+        self.fi = VSFrameInfo()
+        if not L.vsFrameInfoInit(ctypes.byref(self.fi), width, height, pformat):
+            raise ValueError("vsFrameInfoInit failed")
+        self.md = VSMotionDetect()
+        self._mod = mod
+        conf = L.vsMotionDetectGetDefaultConfig(mod)
+        conf.shakiness, conf.accuracy, conf.stepSize = shakiness, accuracy, stepsize
+        conf.contrastThreshold = float(mincontrast)
+        conf.virtualTripod = tripod
+        if L.vsMotionDetectInit(ctypes.byref(self.md), ctypes.byref(conf), ctypes.byref(self.fi)) != 0:
+            raise RuntimeError("vsMotionDetectInit failed: " + L.vsg_last_error().decode())
+        self.width, self.height = width, height
+        self._chroma = np.full((height // 2) * (width // 2), 128, dtype=np.uint8)
+
+    def _frame(self, luma):
+        fr = VSFrame()
+        fr.data[0] = luma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        fr.linesize[0] = luma.shape[1]
+        fr.data[1] = self._chroma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        fr.data[2] = fr.data[1]
+        fr.linesize[1] = fr.linesize[2] = self.width // 2
+        return fr
+
+    def detect(self, luma):
+        luma = np.ascontiguousarray(luma, dtype=np.uint8)
+        fr = self._frame(luma)
+        vec = VSVector()
+        if self.L.vsMotionDetection(ctypes.byref(self.md), ctypes.byref(vec), ctypes.byref(fr)) != 0:
+            raise RuntimeError("vsMotionDetection failed: " + self.L.vsg_last_error().decode())
+        out = motions_of(vec)
+        self.L.vs_vector_del(ctypes.byref(vec))
+        return out
+
+    def detect_batch(self, lumas):
+        lumas = [np.ascontiguousarray(x, dtype=np.uint8) for x in lumas]
+        frames = (VSFrame * len(lumas))(*[self._frame(x) for x in lumas])
+        vecs = (VSVector * len(lumas))()
+        if self.L.vsMotionDetectionBatch(ctypes.byref(self.md), vecs, frames, len(lumas)) != 0:
+            raise RuntimeError("vsMotionDetectionBatch failed: " + self.L.vsg_last_error().decode())
+        out = []
+        for v in vecs:
+            out.append(motions_of(v))
+            self.L.vs_vector_del(ctypes.byref(v))
+        return out
+
+    def detect_device(self, ptrs, linesizes, keep=False):
+        """frames already in device memory (integer device addresses)"""
+        n = len(ptrs)
+        p = (c_void_p * n)(*ptrs)
+        ls = (c_int * n)(*linesizes)
+        vecs = (VSVector * n)()
+        if self.L.vsMotionDetectionDevice(ctypes.byref(self.md), vecs, p, ls, n) != 0:
+            raise RuntimeError("vsMotionDetectionDevice failed: " + self.L.vsg_last_error().decode())
+        out = []
+        for v in vecs:
+            out.append(motions_of(v) if keep else v.nelems)
+            self.L.vs_vector_del(ctypes.byref(v))
+        return out
+
+    def blurred(self):
+        out = np.zeros((self.height, self.width), dtype=np.uint8)
+        if self.L.vsMotionDetectGetBlurred(ctypes.byref(self.md), out.ctypes.data, self.width) != 0:
+            raise RuntimeError("vsMotionDetectGetBlurred failed")
+        return out
+
+    def close(self):
+        if self.md.initialized:
+            self.L.vsMotionDetectionCleanup(ctypes.byref(self.md))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
