@@ -29,11 +29,12 @@ HBM_PEAK = 8000.0                              # GB/s
 
 
 class VSGProfile(ctypes.Structure):
-    _fields_ = [("launches", ctypes.c_longlong * 5), ("ms", ctypes.c_double * 5),
-                ("work", ctypes.c_double * 5)]
+    _fields_ = [("launches", ctypes.c_longlong * 7), ("ms", ctypes.c_double * 7),
+                ("work", ctypes.c_double * 7)]
 
 
-KNAMES = ["blur", "fields", "grid_coarse", "grid_fine", "refine"]
+KNAMES = ["blur", "fields", "select", "grid_coarse", "refine_coarse", "grid_fine", "refine_fine"]
+K_GRID_COARSE = 3
 
 
 def log(msg):
@@ -127,7 +128,7 @@ def main():
         if prof.launches[k]:
             log(f"kernel {name}: {prof.launches[k]} launches, avg {prof.ms[k] / prof.launches[k]:.4f} ms, "
                 f"total {prof.ms[k] / args.steps:.3f} ms/step")
-    k = 2
+    k = K_GRID_COARSE
     avg_s = prof.ms[k] / prof.launches[k] / 1000.0
     work = prof.work[k] / prof.launches[k]
     achieved = work / avg_s / 1e12

@@ -2,17 +2,23 @@
  * vsg.h -- thin C ABI between the detect library's host C code and its gfx950
  * HIP kernels ("vsg" = vid.stab GPU).  Plain pointers and sizes only.
  *
- * The host side (motiondetect driver) owns every double-precision and
- * ordering decision of the reference (contrast selection, the coarse->fine
- * transform, the match cleanup); this layer owns the device memory and runs
- * the data-parallel integer stages that replace the reference's CPU kernels:
+ * The host side (the motiondetect driver) owns the double-precision steps
+ * that need glibc's libm (the coarse->fine transform) and the match cleanup;
+ * this layer owns the device memory and runs the data-parallel stages that
+ * replace the reference's CPU kernels:
  *
- *   vsg_frames  -- boxblurPlanar (src/boxblur.c:104-332) on a batch of luma
- *                  planes, plus the per-field 8-bit min/max of contrastSubImg
- *                  (src/motiondetect.c:389-407) for every coarse and fine field
- *   vsg_search  -- calcFieldTransPlanar (src/motiondetect.c:439-586) with its
- *                  compareSubImg SAD (src/motiondetect.c:971-996): the spiral
- *                  grid search and the dyadic refinement for a list of fields
+ *   vsg_submit      boxblurPlanar (src/boxblur.c:104-332) of a batch of luma
+ *                   planes; per-field min/max of contrastSubImg
+ *                   (src/motiondetect.c:389-407); selectfields
+ *                   (src/motiondetect.c:676-743) for the coarse and fine grids;
+ *                   the coarse calcFieldTransPlanar (src/motiondetect.c:439-586)
+ *                   with its compareSubImg SAD (src/motiondetect.c:971-996)
+ *   vsg_submit_fine the same search for the fine fields at host-computed offsets
+ *
+ * A context has two sets of maxBatch frame slots (global slot = set*maxBatch +
+ * i), so the host can prepare one batch's fine stage while the device runs the
+ * next batch's coarse stage.  All work of a context runs in order on one HIP
+ * stream; submit calls return at once, wait calls block.
  *
  * Every function returns 0 on success and a negative value on failure;
  * vsg_last_error() then describes the failure (thread-local).
@@ -30,18 +36,21 @@ extern "C" {
 
 typedef struct VSGContext VSGContext;
 
-/* one field grid (coarse or fine): VSMotionDetectFields of the reference */
+/* one field grid with its selection parameters (VSMotionDetectFields) */
 typedef struct {
-  int fieldSize;        /* multiple of 16 */
+  int fieldSize;            /* multiple of 16 */
   int maxShift;
-  int stepSize;         /* >= 1 */
+  int stepSize;             /* >= 1 */
   int nfields;
-  const Field* fields;  /* host array of nfields centres */
+  int fieldRows;
+  int maxFields;
+  double contrastThreshold;
+  const Field* fields;      /* host array of nfields centres */
 } VSGStageDesc;
 
-/* one field search: current frame = batch slot `slot`; previous frame = batch
-   slot `prev`, or the stored reference plane when prev < 0; `field` indexes
-   the stage's field array; (ox, oy) shifts the search window (fine stage) */
+/* one field search: current frame in slot `slot`, previous plane in slot
+   `prev` or the stored reference plane when prev < 0; `field` indexes the
+   stage's fields (< 0: skip); (ox, oy) shifts the search window */
 typedef struct {
   int32_t slot;
   int32_t prev;
@@ -55,39 +64,50 @@ typedef struct {
   uint32_t minerror;
 } VSGResult;
 
-/* Creates the device state for width x height frames on `device`, blur box
-   `blurSize` (< 2: no blur), two field grids, and batches of up to maxBatch
-   frames.  Replaces the buffers vsMotionDetectInit allocates
+/* what vsg_wait_coarse hands back for a batch of n frames (pinned host
+   memory owned by the context, valid until the set is submitted again) */
+typedef struct {
+  const int32_t* count;        /* [n][2]: selected coarse, fine fields */
+  const int16_t* idx;          /* [n][stride]: coarse selection, then fine selection */
+  const double* contrast;      /* [n][stride]: selection contrast */
+  const VSGResult* coarse;     /* [n][ncoarse]: search result of each coarse selection */
+  int stride;                  /* ncoarse + nfine */
+  int ncoarse;
+} VSGBatchView;
+
+/* Creates the device state for width x height frames on `device`: blur box
+   `blurSize` (< 2: plain copy), the coarse and fine grids, two sets of
+   maxBatch slots.  Replaces the buffers of vsMotionDetectInit
    (src/motiondetect.c:104-164). */
 VS_API int vsg_create(VSGContext** ctx, int device, int width, int height, int blurSize,
                       const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch);
 VS_API void vsg_destroy(VSGContext* ctx);
 
-/* Loads n luma planes into batch slots 0..n-1 (host pointers, or device
-   pointers when onDevice != 0; linesize in bytes), blurs them, and returns the
-   8-bit (min, max) of every field: minmaxCoarse[n][ncoarse][2],
-   minmaxFine[n][nfine][2]. */
-VS_API int vsg_frames(VSGContext* ctx, int n, const uint8_t* const* luma, const int* linesize,
-                      int onDevice, uint8_t* minmaxCoarse, uint8_t* minmaxFine);
+/* Enqueues a batch of n <= maxBatch frames into `set`: upload (host luma) or
+   in-place read (onDevice), blur, field min/max, selection of both grids, and
+   the coarse search of every frame with detect[i] != 0 against slot
+   prevSlot[i] (or the stored reference when -1).  If refSlot >= 0, that slot's
+   blurred plane becomes the stored reference right after the blur. */
+VS_API int vsg_submit(VSGContext* ctx, int set, int n, const uint8_t* const* luma,
+                      const int* linesize, int onDevice, const int* prevSlot,
+                      const uint8_t* detect, int refSlot);
+VS_API int vsg_wait_coarse(VSGContext* ctx, int set, VSGBatchView* view);
 
-/* Runs njobs field searches of one stage (0 coarse, 1 fine). */
-VS_API int vsg_search(VSGContext* ctx, int stage, int njobs, const VSGJob* jobs, VSGResult* out);
+/* Enqueues / waits for the fine searches of a set (jobs built by the host). */
+VS_API int vsg_submit_fine(VSGContext* ctx, int set, int njobs, const VSGJob* jobs);
+VS_API int vsg_wait_fine(VSGContext* ctx, int set, const VSGResult** results);
 
-/* Makes batch slot `slot` the stored reference plane (the reference's
-   vsFrameCopy(&md->prev, &md->curr), src/motiondetect.c:299-303). */
-VS_API int vsg_keep_prev(VSGContext* ctx, int slot);
-
-/* Copies a blurred plane to the host (slot < 0: the stored reference). */
+/* Copies a blurred plane (global slot; < 0: the stored reference) to the host. */
 VS_API int vsg_read_blurred(VSGContext* ctx, int slot, uint8_t* dst, int dstStride);
 
 VS_API const char* vsg_last_error(void);
 
-/* Optional per-kernel timing (HIP events on the detector's own stream around
-   each launch; process-wide counters).  work is the kernel's algorithmic unit
-   count: abs-diffs for the searches (candidates x field pixels), bytes for
-   the blur (2 x W x H per frame) and the field pass (field bytes). */
-enum { VSG_K_BLUR = 0, VSG_K_FIELDS = 1, VSG_K_GRID_COARSE = 2, VSG_K_GRID_FINE = 3,
-       VSG_K_REFINE = 4, VSG_K_COUNT = 5 };
+/* Optional per-kernel timing: HIP events on the context's stream around each
+   launch, process-wide counters.  work is the algorithmic unit count:
+   abs-diffs for the searches (candidates x field pixels), bytes for the blur
+   (2 x W x H per frame) and the field pass (2 x field bytes). */
+enum { VSG_K_BLUR = 0, VSG_K_FIELDS = 1, VSG_K_SELECT = 2, VSG_K_GRID_COARSE = 3,
+       VSG_K_REFINE_COARSE = 4, VSG_K_GRID_FINE = 5, VSG_K_REFINE_FINE = 6, VSG_K_COUNT = 7 };
 typedef struct {
   long long launches[VSG_K_COUNT];
   double ms[VSG_K_COUNT];

@@ -2,14 +2,14 @@
  * vsg_api.hip -- the thin C ABI over the detect kernels (declared in vsg.h):
  * device buffers, the HIP stream of one detector, uploads and read-backs.
  *
- * HBM layout of one context (pitch = width rounded up to 64 bytes):
- *   raw[maxBatch]      uploaded luma planes, pitch x H
- *   blur[maxBatch]     blurred planes, pitch x (H + padRows); the pad rows keep
- *                      the grid kernel's idle-candidate reads inside the buffer
- *   ref                the stored previous / tripod reference plane (same shape)
- *   patches[maxBatch]  an aligned copy of every coarse and fine field
- *   minmax[maxBatch]   8-bit (min, max) per field
- *   jobs / best / results   per search job
+ * HBM layout of one context (pitch = width rounded up to 64 bytes; two sets
+ * of B = maxBatch slots, global slot = set*B + i):
+ *   raw[2B]       uploaded luma planes, pitch x H (host-frame batches only)
+ *   blur[2B], ref blurred planes, pitch x (H + padRows); the pad rows keep the
+ *                 grid kernel's idle-candidate reads inside the buffer
+ *   patches[2B]   an aligned, dense copy of every coarse and fine field
+ *   minmax[2B]    8-bit (min, max) per field
+ *   per set       selection lists, coarse/fine job lists, best[] keys, results
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -22,26 +22,71 @@
 
 #include "vsg_internal.h"
 
+namespace {
+
+enum { EV_BLUR0, EV_BLUR1, EV_FIELDS1, EV_SELECT1, EV_GRID1, EV_REFINE1, EV_COUNT };
+enum { EVF_0, EVF_GRID1, EVF_REFINE1, EVF_COUNT };
+
+struct Stage {
+  int S = 0, maxShift = 0, step = 0, nf = 0, rows = 0, maxFields = 0;
+  double ct = 0;
+  int16_t* d_fx = nullptr;
+  int16_t* d_fy = nullptr;
+  VSGGrid g{};
+  int tiles = 0;
+  size_t patchOff = 0;
+  double candsPerJob = 0;     /* (2n+1)^2 */
+  double refinePerJob = 0;    /* refinement candidates the reference visits */
+};
+
+struct Set {
+  /* device */
+  int16_t* d_selIdx = nullptr;
+  double* d_selC = nullptr;
+  int* d_count = nullptr;
+  VSGJob* d_jobsC = nullptr;
+  VSGJob* d_jobsF = nullptr;
+  unsigned long long* d_best = nullptr;
+  VSGResult* d_resC = nullptr;
+  VSGResult* d_resF = nullptr;
+  VSGBlurSrc* d_src = nullptr;
+  int* d_prev = nullptr;
+  uint8_t* d_detect = nullptr;
+  /* pinned host */
+  int16_t* h_selIdx = nullptr;
+  double* h_selC = nullptr;
+  int* h_count = nullptr;
+  VSGResult* h_resC = nullptr;
+  VSGJob* h_jobsF = nullptr;
+  VSGResult* h_resF = nullptr;
+  VSGBlurSrc* h_src = nullptr;
+  int* h_prev = nullptr;
+  uint8_t* h_detect = nullptr;
+  /* state */
+  int n = 0;              /* frames in flight */
+  int njobsF = 0;
+  hipEvent_t done = nullptr, doneF = nullptr;
+  hipEvent_t ev[EV_COUNT] = {};
+  hipEvent_t evf[EVF_COUNT] = {};
+  bool prof = false, profF = false;
+};
+
+}  // namespace
+
 struct VSGContext {
   int device = 0;
   hipStream_t stream = nullptr;
   int W = 0, H = 0, pitch = 0, rowsAlloc = 0;
   size_t slotStride = 0;
   int blurSize = 0;
-  int maxBatch = 0;
-  struct Stage {
-    int S = 0, maxShift = 0, step = 0, nf = 0;
-    int16_t* d_fx = nullptr;
-    int16_t* d_fy = nullptr;
-    VSGGrid g{};
-    int tiles = 0;
-    size_t patchOff = 0;   /* start of this stage's patches inside a slot */
-  } st[2];
+  int B = 0;
+  Stage st[2];
   int nfTotal = 0;
   int16_t* d_allx = nullptr;
   int16_t* d_ally = nullptr;
   int16_t* d_allsz = nullptr;
   uint32_t* d_patchOff = nullptr;
+  double* d_ctab = nullptr;
   uint8_t* d_raw = nullptr;
   uint8_t* d_blur = nullptr;
   uint8_t* d_ref = nullptr;
@@ -49,19 +94,25 @@ struct VSGContext {
   uint8_t* d_patches = nullptr;
   size_t patchSlotStride = 0;
   uint8_t* d_minmax = nullptr;
-  uint8_t* h_minmax = nullptr;
-  int maxJobs = 0;
-  VSGJob* d_jobs = nullptr;
-  VSGJob* h_jobs = nullptr;
-  unsigned long long* d_best = nullptr;
-  VSGResult* d_res = nullptr;
-  VSGResult* h_res = nullptr;
-  VSGBlurSrc* d_src = nullptr;
-  VSGBlurSrc* h_src = nullptr;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  Set set[2];
 };
 
 static thread_local char g_err[512];
+
+static int fail(const char* what, hipError_t e) {
+  snprintf(g_err, sizeof(g_err), "%s: %s", what, e == hipSuccess ? "invalid argument" : hipGetErrorString(e));
+  return -1;
+}
+
+#define VSG_CHECK(call)                                  \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return fail(#call, e_);        \
+  } while (0)
+
+extern "C" const char* vsg_last_error(void) { return g_err; }
+extern "C" int vsg_spiral_rank(int a, int b) { return vsg_rank(a, b); }
+extern "C" void vsg_spiral_unrank(int rank, int* a, int* b) { vsg_unrank(rank, a, b); }
 
 /* ---- optional per-kernel timing ------------------------------------------- */
 
@@ -79,88 +130,106 @@ extern "C" void vsg_profile_get(VSGProfile* out) {
   if (out) *out = g_prof;
 }
 
-/* after the stream has been synchronised: account ev[0]->ev[1] to kind k0 and
-   ev[1]->ev[2] to kind k1 */
-static void prof_account(VSGContext* c, int k0, double w0, int k1, double w1) {
-  float t0 = 0.f, t1 = 0.f;
-  if (hipEventElapsedTime(&t0, c->ev[0], c->ev[1]) != hipSuccess) return;
-  if (hipEventElapsedTime(&t1, c->ev[1], c->ev[2]) != hipSuccess) return;
+static void prof_add(int k, hipEvent_t a, hipEvent_t b, double work) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return;
   std::lock_guard<std::mutex> lk(g_profMu);
-  g_prof.launches[k0] += 1;
-  g_prof.ms[k0] += t0;
-  g_prof.work[k0] += w0;
-  g_prof.launches[k1] += 1;
-  g_prof.ms[k1] += t1;
-  g_prof.work[k1] += w1;
+  g_prof.launches[k] += 1;
+  g_prof.ms[k] += ms;
+  g_prof.work[k] += work;
 }
 
-static int fail(const char* what, hipError_t e) {
-  snprintf(g_err, sizeof(g_err), "%s: %s", what, e == hipSuccess ? "invalid argument" : hipGetErrorString(e));
-  return -1;
+/* ---- lifetime ---------------------------------------------------------------- */
+
+template <class T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
 }
-
-#define VSG_CHECK(call)                                  \
-  do {                                                   \
-    hipError_t e_ = (call);                              \
-    if (e_ != hipSuccess) return fail(#call, e_);        \
-  } while (0)
-
-extern "C" const char* vsg_last_error(void) { return g_err; }
-
-extern "C" int vsg_spiral_rank(int a, int b) { return vsg_rank(a, b); }
-extern "C" void vsg_spiral_unrank(int rank, int* a, int* b) { vsg_unrank(rank, a, b); }
+template <class T>
+static void hfree(T*& p) {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+}
 
 static void free_ctx(VSGContext* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int s = 0; s < 2; ++s) {
-    (void)hipFree(c->st[s].d_fx);
-    (void)hipFree(c->st[s].d_fy);
+  for (Stage& s : c->st) {
+    dfree(s.d_fx);
+    dfree(s.d_fy);
   }
-  (void)hipFree(c->d_allx); (void)hipFree(c->d_ally); (void)hipFree(c->d_allsz); (void)hipFree(c->d_patchOff);
-  (void)hipFree(c->d_raw); (void)hipFree(c->d_blur); (void)hipFree(c->d_ref); (void)hipFree(c->d_tmp);
-  (void)hipFree(c->d_patches); (void)hipFree(c->d_minmax); (void)hipFree(c->d_jobs); (void)hipFree(c->d_best);
-  (void)hipFree(c->d_res); (void)hipFree(c->d_src);
-  (void)hipHostFree(c->h_minmax); (void)hipHostFree(c->h_jobs); (void)hipHostFree(c->h_res); (void)hipHostFree(c->h_src);
-  for (int i = 0; i < 3; ++i)
-    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  dfree(c->d_allx); dfree(c->d_ally); dfree(c->d_allsz); dfree(c->d_patchOff); dfree(c->d_ctab);
+  dfree(c->d_raw); dfree(c->d_blur); dfree(c->d_ref); dfree(c->d_tmp); dfree(c->d_patches);
+  dfree(c->d_minmax);
+  for (Set& s : c->set) {
+    dfree(s.d_selIdx); dfree(s.d_selC); dfree(s.d_count); dfree(s.d_jobsC); dfree(s.d_jobsF);
+    dfree(s.d_best); dfree(s.d_resC); dfree(s.d_resF); dfree(s.d_src); dfree(s.d_prev); dfree(s.d_detect);
+    hfree(s.h_selIdx); hfree(s.h_selC); hfree(s.h_count); hfree(s.h_resC); hfree(s.h_jobsF);
+    hfree(s.h_resF); hfree(s.h_src); hfree(s.h_prev); hfree(s.h_detect);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.doneF) (void)hipEventDestroy(s.doneF);
+    for (hipEvent_t& e : s.ev)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t& e : s.evf)
+      if (e) (void)hipEventDestroy(e);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
 extern "C" void vsg_destroy(VSGContext* ctx) { free_ctx(ctx); }
 
+template <class T>
+static hipError_t dalloc(T** p, size_t n) {
+  return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1));
+}
+template <class T>
+static hipError_t halloc(T** p, size_t n) {
+  return hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1), hipHostMallocDefault);
+}
+
 static int create(VSGContext* c, int device, int width, int height, int blurSize,
-                  const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch) {
+                  const VSGStageDesc* coarse, const VSGStageDesc* fine, int B) {
   c->device = device;
   VSG_CHECK(hipSetDevice(device));
   VSG_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (int i = 0; i < 3; ++i) VSG_CHECK(hipEventCreate(&c->ev[i]));
   c->W = width;
   c->H = height;
   c->pitch = (width + 63) & ~63;
   c->blurSize = blurSize;
-  c->maxBatch = maxBatch;
+  c->B = B;
   const VSGStageDesc* sd[2] = {coarse, fine};
   int maxStep = 1;
   size_t patchBytes = 0;
   std::vector<int16_t> ax, ay, asz;
   std::vector<uint32_t> poff;
-  for (int s = 0; s < 2; ++s) {
-    VSGContext::Stage& S = c->st[s];
-    S.S = sd[s]->fieldSize;
-    S.maxShift = sd[s]->maxShift;
-    S.step = sd[s]->stepSize;
-    S.nf = sd[s]->nfields;
-    if (S.S % 16 != 0 || S.S <= 0 || S.step < 1 || S.nf <= 0 || S.maxShift < 0)
+  for (int k = 0; k < 2; ++k) {
+    Stage& S = c->st[k];
+    S.S = sd[k]->fieldSize;
+    S.maxShift = sd[k]->maxShift;
+    S.step = sd[k]->stepSize;
+    S.nf = sd[k]->nfields;
+    S.rows = sd[k]->fieldRows;
+    S.maxFields = sd[k]->maxFields;
+    S.ct = sd[k]->contrastThreshold;
+    if (S.S % 16 != 0 || S.S <= 0 || S.step < 1 || S.nf <= 0 || S.nf > 4096 || S.maxShift < 0 ||
+        S.rows < 1 || S.rows + 1 > 256)
       return fail("vsg_create: field grid", hipSuccess);
     maxStep = S.step > maxStep ? S.step : maxStep;
     S.tiles = vsg_plan_grid(&S.g, S.S, S.step, S.maxShift);
+    S.candsPerJob = (2.0 * S.g.n + 1.0) * (2.0 * S.g.n + 1.0);
+    int rc = 0;
+    for (int st = S.step; st > 1; st /= 2) {
+      const int ns = st / 2, r = st - ns, ni = (2 * r) / ns + 1;
+      rc += ni * ni - ((r % ns) == 0 ? 1 : 0);
+    }
+    S.refinePerJob = rc;
     S.patchOff = patchBytes;
     std::vector<int16_t> fx(S.nf), fy(S.nf);
     for (int i = 0; i < S.nf; ++i) {
-      const Field& f = sd[s]->fields[i];
+      const Field& f = sd[k]->fields[i];
       if (f.size != S.S) return fail("vsg_create: field size mismatch", hipSuccess);
       fx[i] = f.x;
       fy[i] = f.y;
@@ -170,48 +239,75 @@ static int create(VSGContext* c, int device, int width, int height, int blurSize
       poff.push_back((uint32_t)(patchBytes + (size_t)i * S.S * S.S));
     }
     patchBytes += (size_t)S.nf * S.S * S.S;
-    VSG_CHECK(hipMalloc(&S.d_fx, sizeof(int16_t) * S.nf));
-    VSG_CHECK(hipMalloc(&S.d_fy, sizeof(int16_t) * S.nf));
+    VSG_CHECK(dalloc(&S.d_fx, S.nf));
+    VSG_CHECK(dalloc(&S.d_fy, S.nf));
     VSG_CHECK(hipMemcpy(S.d_fx, fx.data(), sizeof(int16_t) * S.nf, hipMemcpyHostToDevice));
     VSG_CHECK(hipMemcpy(S.d_fy, fy.data(), sizeof(int16_t) * S.nf, hipMemcpyHostToDevice));
   }
   c->nfTotal = (int)ax.size();
-  VSG_CHECK(hipMalloc(&c->d_allx, sizeof(int16_t) * c->nfTotal));
-  VSG_CHECK(hipMalloc(&c->d_ally, sizeof(int16_t) * c->nfTotal));
-  VSG_CHECK(hipMalloc(&c->d_allsz, sizeof(int16_t) * c->nfTotal));
-  VSG_CHECK(hipMalloc(&c->d_patchOff, sizeof(uint32_t) * c->nfTotal));
+  VSG_CHECK(dalloc(&c->d_allx, c->nfTotal));
+  VSG_CHECK(dalloc(&c->d_ally, c->nfTotal));
+  VSG_CHECK(dalloc(&c->d_allsz, c->nfTotal));
+  VSG_CHECK(dalloc(&c->d_patchOff, c->nfTotal));
   VSG_CHECK(hipMemcpy(c->d_allx, ax.data(), sizeof(int16_t) * c->nfTotal, hipMemcpyHostToDevice));
   VSG_CHECK(hipMemcpy(c->d_ally, ay.data(), sizeof(int16_t) * c->nfTotal, hipMemcpyHostToDevice));
   VSG_CHECK(hipMemcpy(c->d_allsz, asz.data(), sizeof(int16_t) * c->nfTotal, hipMemcpyHostToDevice));
   VSG_CHECK(hipMemcpy(c->d_patchOff, poff.data(), sizeof(uint32_t) * c->nfTotal, hipMemcpyHostToDevice));
 
-  /* pad rows: the grid kernel's K candidate rows per lane may reach
-     (K-1)*step rows below a search window, plus one row of column overrun */
+  /* contrast of every (min, max) pair, evaluated on the host exactly as the
+     reference writes it: (max - min) / (max + min + 0.1) */
+  std::vector<double> ctab(65536);
+  for (int mn = 0; mn < 256; ++mn)
+    for (int mx = 0; mx < 256; ++mx) ctab[mn * 256 + mx] = (double)(mx - mn) / ((double)(mx + mn) + 0.1);
+  VSG_CHECK(dalloc(&c->d_ctab, 65536));
+  VSG_CHECK(hipMemcpy(c->d_ctab, ctab.data(), sizeof(double) * 65536, hipMemcpyHostToDevice));
+
+  /* pad rows: a lane's K candidate rows may reach (K-1)*step rows below a
+     search window, plus column overrun into the next row */
   c->rowsAlloc = height + 4 * maxStep + 16;
   c->slotStride = (size_t)c->pitch * c->rowsAlloc;
-  VSG_CHECK(hipMalloc(&c->d_raw, (size_t)maxBatch * c->pitch * height));
-  VSG_CHECK(hipMalloc(&c->d_blur, (size_t)maxBatch * c->slotStride));
-  VSG_CHECK(hipMalloc(&c->d_ref, c->slotStride));
-  VSG_CHECK(hipMemset(c->d_blur, 0, (size_t)maxBatch * c->slotStride));
+  const int slots = 2 * B;
+  VSG_CHECK(dalloc(&c->d_raw, (size_t)slots * c->pitch * height));
+  VSG_CHECK(dalloc(&c->d_blur, (size_t)slots * c->slotStride));
+  VSG_CHECK(dalloc(&c->d_ref, c->slotStride));
+  VSG_CHECK(hipMemset(c->d_blur, 0, (size_t)slots * c->slotStride));
   VSG_CHECK(hipMemset(c->d_ref, 0, c->slotStride));
   if (blurSize >= 2) {
     const int s2 = blurSize / 2;
     const size_t lds = (size_t)(32 + 2 * s2) * (((256 + 2 * s2) + 3) & ~3) + (size_t)(32 + 2 * s2) * 256;
-    if (lds > 64 * 1024) VSG_CHECK(hipMalloc(&c->d_tmp, (size_t)maxBatch * c->pitch * height));
+    if (lds > 64 * 1024) VSG_CHECK(dalloc(&c->d_tmp, (size_t)slots * c->pitch * height));
   }
   c->patchSlotStride = (patchBytes + 255) & ~(size_t)255;
-  VSG_CHECK(hipMalloc(&c->d_patches, (size_t)maxBatch * c->patchSlotStride));
-  VSG_CHECK(hipMalloc(&c->d_minmax, (size_t)maxBatch * c->nfTotal * 2));
-  VSG_CHECK(hipHostMalloc(&c->h_minmax, (size_t)maxBatch * c->nfTotal * 2, hipHostMallocDefault));
-  c->maxJobs = maxBatch * (c->st[0].nf > c->st[1].nf ? c->st[0].nf : c->st[1].nf);
-  VSG_CHECK(hipMalloc(&c->d_jobs, sizeof(VSGJob) * c->maxJobs));
-  VSG_CHECK(hipHostMalloc(&c->h_jobs, sizeof(VSGJob) * c->maxJobs, hipHostMallocDefault));
-  VSG_CHECK(hipMalloc(&c->d_best, sizeof(unsigned long long) * c->maxJobs));
-  VSG_CHECK(hipMemset(c->d_best, 0xFF, sizeof(unsigned long long) * c->maxJobs));
-  VSG_CHECK(hipMalloc(&c->d_res, sizeof(VSGResult) * c->maxJobs));
-  VSG_CHECK(hipHostMalloc(&c->h_res, sizeof(VSGResult) * c->maxJobs, hipHostMallocDefault));
-  VSG_CHECK(hipMalloc(&c->d_src, sizeof(VSGBlurSrc) * maxBatch));
-  VSG_CHECK(hipHostMalloc(&c->h_src, sizeof(VSGBlurSrc) * maxBatch, hipHostMallocDefault));
+  VSG_CHECK(dalloc(&c->d_patches, (size_t)slots * c->patchSlotStride));
+  VSG_CHECK(dalloc(&c->d_minmax, (size_t)slots * c->nfTotal * 2));
+  const int nc = c->st[0].nf, nf = c->st[1].nf, nmax = nc > nf ? nc : nf;
+  for (Set& s : c->set) {
+    VSG_CHECK(dalloc(&s.d_selIdx, (size_t)B * c->nfTotal));
+    VSG_CHECK(dalloc(&s.d_selC, (size_t)B * c->nfTotal));
+    VSG_CHECK(dalloc(&s.d_count, (size_t)B * 2));
+    VSG_CHECK(dalloc(&s.d_jobsC, (size_t)B * nc));
+    VSG_CHECK(dalloc(&s.d_jobsF, (size_t)B * nf));
+    VSG_CHECK(dalloc(&s.d_best, (size_t)B * nmax));
+    VSG_CHECK(hipMemset(s.d_best, 0xFF, sizeof(unsigned long long) * (size_t)B * nmax));
+    VSG_CHECK(dalloc(&s.d_resC, (size_t)B * nc));
+    VSG_CHECK(dalloc(&s.d_resF, (size_t)B * nf));
+    VSG_CHECK(dalloc(&s.d_src, B));
+    VSG_CHECK(dalloc(&s.d_prev, B));
+    VSG_CHECK(dalloc(&s.d_detect, B));
+    VSG_CHECK(halloc(&s.h_selIdx, (size_t)B * c->nfTotal));
+    VSG_CHECK(halloc(&s.h_selC, (size_t)B * c->nfTotal));
+    VSG_CHECK(halloc(&s.h_count, (size_t)B * 2));
+    VSG_CHECK(halloc(&s.h_resC, (size_t)B * nc));
+    VSG_CHECK(halloc(&s.h_jobsF, (size_t)B * nf));
+    VSG_CHECK(halloc(&s.h_resF, (size_t)B * nf));
+    VSG_CHECK(halloc(&s.h_src, B));
+    VSG_CHECK(halloc(&s.h_prev, B));
+    VSG_CHECK(halloc(&s.h_detect, B));
+    VSG_CHECK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    VSG_CHECK(hipEventCreateWithFlags(&s.doneF, hipEventDisableTiming));
+    for (hipEvent_t& e : s.ev) VSG_CHECK(hipEventCreate(&e));
+    for (hipEvent_t& e : s.evf) VSG_CHECK(hipEventCreate(&e));
+  }
   VSG_CHECK(hipDeviceSynchronize());
   return 0;
 }
@@ -234,75 +330,13 @@ extern "C" int vsg_create(VSGContext** out, int device, int width, int height, i
   return 0;
 }
 
-extern "C" int vsg_frames(VSGContext* c, int n, const uint8_t* const* luma, const int* linesize,
-                          int onDevice, uint8_t* mmCoarse, uint8_t* mmFine) {
-  if (!c || n <= 0 || n > c->maxBatch || !luma || !linesize) return fail("vsg_frames", hipSuccess);
-  VSG_CHECK(hipSetDevice(c->device));
-  for (int i = 0; i < n; ++i) {
-    if (linesize[i] < c->W) return fail("vsg_frames: linesize < width", hipSuccess);
-    if (onDevice) {
-      c->h_src[i].p = luma[i];
-      c->h_src[i].pitch = linesize[i];
-    } else {
-      uint8_t* dst = c->d_raw + (size_t)i * c->pitch * c->H;
-      VSG_CHECK(hipMemcpy2DAsync(dst, c->pitch, luma[i], linesize[i], c->W, c->H,
-                                 hipMemcpyHostToDevice, c->stream));
-      c->h_src[i].p = dst;
-      c->h_src[i].pitch = c->pitch;
-    }
-    c->h_src[i].pad = 0;
-  }
-  VSG_CHECK(hipMemcpyAsync(c->d_src, c->h_src, sizeof(VSGBlurSrc) * n, hipMemcpyHostToDevice, c->stream));
-  const int prof = g_profOn.load();
-  if (prof) VSG_CHECK(hipEventRecord(c->ev[0], c->stream));
-  VSG_CHECK(vsg_launch_blur(c->d_src, n, c->d_blur, c->slotStride, c->pitch, c->W, c->H, c->blurSize,
-                            c->d_tmp, c->stream));
-  if (prof) VSG_CHECK(hipEventRecord(c->ev[1], c->stream));
-  VSGFieldArgs fa;
-  fa.blurred = c->d_blur;
-  fa.slotStride = c->slotStride;
-  fa.pitch = c->pitch;
-  fa.fx = c->d_allx;
-  fa.fy = c->d_ally;
-  fa.fsz = c->d_allsz;
-  fa.patchOff = c->d_patchOff;
-  fa.nfields = c->nfTotal;
-  fa.patches = c->d_patches;
-  fa.patchSlotStride = c->patchSlotStride;
-  fa.minmax = c->d_minmax;
-  VSG_CHECK(vsg_launch_fields(fa, n, c->stream));
-  if (prof) VSG_CHECK(hipEventRecord(c->ev[2], c->stream));
-  const size_t mmBytes = (size_t)n * c->nfTotal * 2;
-  VSG_CHECK(hipMemcpyAsync(c->h_minmax, c->d_minmax, mmBytes, hipMemcpyDeviceToHost, c->stream));
-  VSG_CHECK(hipStreamSynchronize(c->stream));
-  if (prof) {
-    double fieldBytes = 0;
-    for (int s = 0; s < 2; ++s) fieldBytes += (double)c->st[s].nf * c->st[s].S * c->st[s].S;
-    prof_account(c, VSG_K_BLUR, 2.0 * c->W * c->H * n, VSG_K_FIELDS, 2.0 * fieldBytes * n);
-  }
-  const int nc = c->st[0].nf, nf = c->st[1].nf;
-  for (int i = 0; i < n; ++i) {
-    const uint8_t* src = c->h_minmax + (size_t)i * c->nfTotal * 2;
-    if (mmCoarse) memcpy(mmCoarse + (size_t)i * nc * 2, src, (size_t)nc * 2);
-    if (mmFine) memcpy(mmFine + (size_t)i * nf * 2, src + (size_t)nc * 2, (size_t)nf * 2);
-  }
-  return 0;
-}
+/* ---- batches ------------------------------------------------------------------ */
 
-extern "C" int vsg_search(VSGContext* c, int stage, int njobs, const VSGJob* jobs, VSGResult* out) {
-  if (!c || stage < 0 || stage > 1 || njobs < 0 || njobs > c->maxJobs) return fail("vsg_search", hipSuccess);
-  if (njobs == 0) return 0;
-  const VSGContext::Stage& S = c->st[stage];
-  for (int i = 0; i < njobs; ++i) {
-    const VSGJob& j = jobs[i];
-    if (j.slot < 0 || j.slot >= c->maxBatch || j.prev >= c->maxBatch || j.field < 0 || j.field >= S.nf)
-      return fail("vsg_search: job out of range", hipSuccess);
-  }
-  VSG_CHECK(hipSetDevice(c->device));
-  memcpy(c->h_jobs, jobs, sizeof(VSGJob) * njobs);
-  VSG_CHECK(hipMemcpyAsync(c->d_jobs, c->h_jobs, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, c->stream));
+static VSGSearchArgs search_args(VSGContext* c, int stage, const VSGJob* jobs, int njobs,
+                                 unsigned long long* best, VSGResult* res) {
+  const Stage& S = c->st[stage];
   VSGSearchArgs a;
-  a.jobs = c->d_jobs;
+  a.jobs = jobs;
   a.njobs = njobs;
   a.fx = S.d_fx;
   a.fy = S.d_fy;
@@ -315,44 +349,179 @@ extern "C" int vsg_search(VSGContext* c, int stage, int njobs, const VSGJob* job
   a.stageOff = S.patchOff;
   a.g = S.g;
   a.tiles = S.tiles;
-  a.best = c->d_best;
-  a.results = c->d_res;
-  const int prof = g_profOn.load();
-  if (prof) VSG_CHECK(hipEventRecord(c->ev[0], c->stream));
-  VSG_CHECK(vsg_launch_grid(a, c->stream));
-  if (prof) VSG_CHECK(hipEventRecord(c->ev[1], c->stream));
-  VSG_CHECK(vsg_launch_refine(a, c->stream));
-  if (prof) VSG_CHECK(hipEventRecord(c->ev[2], c->stream));
-  VSG_CHECK(hipMemcpyAsync(c->h_res, c->d_res, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, c->stream));
-  VSG_CHECK(hipStreamSynchronize(c->stream));
-  if (prof) {
-    /* algorithmic work: every grid candidate over the whole field, and the
-       refinement candidates the reference visits (8 + 24 for step 6 ...) */
-    const double px = (double)S.S * S.S;
-    const double G = 2.0 * S.g.n + 1.0;
-    int refineCands = 0;
-    for (int st = S.step; st > 1; st /= 2) {
-      const int ns = st / 2, r = st - ns, ni = (2 * r) / ns + 1;
-      refineCands += ni * ni - ((r % ns) == 0 ? 1 : 0);
+  a.best = best;
+  a.results = res;
+  return a;
+}
+
+extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* luma, const int* linesize,
+                          int onDevice, const int* prevSlot, const uint8_t* detect, int refSlot) {
+  if (!c || set < 0 || set > 1 || n <= 0 || n > c->B || !luma || !linesize || !prevSlot || !detect)
+    return fail("vsg_submit", hipSuccess);
+  if (refSlot >= 2 * c->B) return fail("vsg_submit: refSlot", hipSuccess);
+  Set& s = c->set[set];
+  const int slot0 = set * c->B;
+  VSG_CHECK(hipSetDevice(c->device));
+  /* this set's pinned buffers are reused below: wait until the previous batch
+     of the set (its coarse read-back and its fine stage) has completed */
+  VSG_CHECK(hipEventSynchronize(s.done));
+  VSG_CHECK(hipEventSynchronize(s.doneF));
+  for (int i = 0; i < n; ++i) {
+    if (linesize[i] < c->W) return fail("vsg_submit: linesize < width", hipSuccess);
+    if (prevSlot[i] >= 2 * c->B) return fail("vsg_submit: prevSlot", hipSuccess);
+    if (onDevice) {
+      s.h_src[i].p = luma[i];
+      s.h_src[i].pitch = linesize[i];
+    } else {
+      uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * c->pitch * c->H;
+      VSG_CHECK(hipMemcpy2DAsync(dst, c->pitch, luma[i], linesize[i], c->W, c->H, hipMemcpyHostToDevice,
+                                 c->stream));
+      s.h_src[i].p = dst;
+      s.h_src[i].pitch = c->pitch;
     }
-    prof_account(c, stage == 0 ? VSG_K_GRID_COARSE : VSG_K_GRID_FINE, njobs * G * G * px, VSG_K_REFINE,
-                 njobs * (double)refineCands * px);
+    s.h_src[i].pad = 0;
+    s.h_prev[i] = prevSlot[i];
+    s.h_detect[i] = detect[i] ? 1 : 0;
   }
-  memcpy(out, c->h_res, sizeof(VSGResult) * njobs);
+  VSG_CHECK(hipMemcpyAsync(s.d_src, s.h_src, sizeof(VSGBlurSrc) * n, hipMemcpyHostToDevice, c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.d_prev, s.h_prev, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.d_detect, s.h_detect, n, hipMemcpyHostToDevice, c->stream));
+  s.prof = g_profOn.load() != 0;
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_BLUR0], c->stream));
+  VSG_CHECK(vsg_launch_blur(s.d_src, n, c->d_blur + (size_t)slot0 * c->slotStride, c->slotStride, c->pitch,
+                            c->W, c->H, c->blurSize, c->d_tmp, c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_BLUR1], c->stream));
+  if (refSlot >= 0)
+    VSG_CHECK(hipMemcpyAsync(c->d_ref, c->d_blur + (size_t)refSlot * c->slotStride, c->slotStride,
+                             hipMemcpyDeviceToDevice, c->stream));
+  VSGFieldArgs fa;
+  fa.blurred = c->d_blur + (size_t)slot0 * c->slotStride;
+  fa.slotStride = c->slotStride;
+  fa.pitch = c->pitch;
+  fa.fx = c->d_allx;
+  fa.fy = c->d_ally;
+  fa.fsz = c->d_allsz;
+  fa.patchOff = c->d_patchOff;
+  fa.nfields = c->nfTotal;
+  fa.patches = c->d_patches + (size_t)slot0 * c->patchSlotStride;
+  fa.patchSlotStride = c->patchSlotStride;
+  fa.minmax = c->d_minmax + (size_t)slot0 * c->nfTotal * 2;
+  VSG_CHECK(vsg_launch_fields(fa, n, c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_FIELDS1], c->stream));
+  VSGSelectArgs sa;
+  sa.minmax = c->d_minmax;
+  sa.nfTotal = c->nfTotal;
+  sa.ctab = c->d_ctab;
+  for (int k = 0; k < 2; ++k) {
+    sa.st[k].nf = c->st[k].nf;
+    sa.st[k].rows = c->st[k].rows;
+    sa.st[k].maxFields = c->st[k].maxFields;
+    sa.st[k].mmOff = k == 0 ? 0 : c->st[0].nf;
+    sa.st[k].ct = c->st[k].ct;
+  }
+  sa.slot0 = slot0;
+  sa.prevSlot = s.d_prev;
+  sa.detect = s.d_detect;
+  sa.selIdx = s.d_selIdx;
+  sa.selC = s.d_selC;
+  sa.count = s.d_count;
+  sa.jobs = s.d_jobsC;
+  VSG_CHECK(vsg_launch_select(sa, n, c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_SELECT1], c->stream));
+  const int nc = c->st[0].nf;
+  VSGSearchArgs ga = search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_resC);
+  VSG_CHECK(vsg_launch_grid(ga, c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_GRID1], c->stream));
+  VSG_CHECK(vsg_launch_refine(ga, c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_REFINE1], c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.h_count, s.d_count, sizeof(int) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.h_selIdx, s.d_selIdx, sizeof(int16_t) * (size_t)n * c->nfTotal,
+                           hipMemcpyDeviceToHost, c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.h_selC, s.d_selC, sizeof(double) * (size_t)n * c->nfTotal,
+                           hipMemcpyDeviceToHost, c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.h_resC, s.d_resC, sizeof(VSGResult) * (size_t)n * nc, hipMemcpyDeviceToHost,
+                           c->stream));
+  VSG_CHECK(hipEventRecord(s.done, c->stream));
+  s.n = n;
   return 0;
 }
 
-extern "C" int vsg_keep_prev(VSGContext* c, int slot) {
-  if (!c || slot < 0 || slot >= c->maxBatch) return fail("vsg_keep_prev", hipSuccess);
-  /* device-to-device copy on the detector's stream, ordered after its searches */
-  uint8_t* p = c->d_blur + (size_t)slot * c->slotStride;
+extern "C" int vsg_wait_coarse(VSGContext* c, int set, VSGBatchView* view) {
+  if (!c || set < 0 || set > 1 || !view) return fail("vsg_wait_coarse", hipSuccess);
+  Set& s = c->set[set];
   VSG_CHECK(hipSetDevice(c->device));
-  VSG_CHECK(hipMemcpyAsync(c->d_ref, p, c->slotStride, hipMemcpyDeviceToDevice, c->stream));
+  VSG_CHECK(hipEventSynchronize(s.done));
+  if (s.prof) {
+    s.prof = false;
+    double fieldBytes = 0;
+    for (const Stage& S : c->st) fieldBytes += (double)S.nf * S.S * S.S;
+    int ncoarse = 0;
+    for (int i = 0; i < s.n; ++i) ncoarse += s.h_count[2 * i];
+    prof_add(VSG_K_BLUR, s.ev[EV_BLUR0], s.ev[EV_BLUR1], 2.0 * c->W * c->H * s.n);
+    prof_add(VSG_K_FIELDS, s.ev[EV_BLUR1], s.ev[EV_FIELDS1], 2.0 * fieldBytes * s.n);
+    prof_add(VSG_K_SELECT, s.ev[EV_FIELDS1], s.ev[EV_SELECT1], (double)c->nfTotal * s.n);
+    const double px = (double)c->st[0].S * c->st[0].S;
+    prof_add(VSG_K_GRID_COARSE, s.ev[EV_SELECT1], s.ev[EV_GRID1], ncoarse * c->st[0].candsPerJob * px);
+    prof_add(VSG_K_REFINE_COARSE, s.ev[EV_GRID1], s.ev[EV_REFINE1], ncoarse * c->st[0].refinePerJob * px);
+  }
+  view->count = s.h_count;
+  view->idx = s.h_selIdx;
+  view->contrast = s.h_selC;
+  view->coarse = s.h_resC;
+  view->stride = c->nfTotal;
+  view->ncoarse = c->st[0].nf;
+  return 0;
+}
+
+extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* jobs) {
+  if (!c || set < 0 || set > 1 || njobs < 0) return fail("vsg_submit_fine", hipSuccess);
+  Set& s = c->set[set];
+  const Stage& S = c->st[1];
+  if (njobs > c->B * S.nf) return fail("vsg_submit_fine: too many jobs", hipSuccess);
+  for (int i = 0; i < njobs; ++i) {
+    const VSGJob& j = jobs[i];
+    if (j.slot < 0 || j.slot >= 2 * c->B || j.prev >= 2 * c->B || j.field >= S.nf)
+      return fail("vsg_submit_fine: job out of range", hipSuccess);
+  }
+  VSG_CHECK(hipSetDevice(c->device));
+  s.njobsF = njobs;
+  s.profF = false;
+  if (njobs == 0) {
+    VSG_CHECK(hipEventRecord(s.doneF, c->stream));
+    return 0;
+  }
+  memcpy(s.h_jobsF, jobs, sizeof(VSGJob) * njobs);
+  VSG_CHECK(hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, c->stream));
+  s.profF = g_profOn.load() != 0;
+  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[EVF_0], c->stream));
+  VSGSearchArgs fa = search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_resF);
+  VSG_CHECK(vsg_launch_grid(fa, c->stream));
+  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[EVF_GRID1], c->stream));
+  VSG_CHECK(vsg_launch_refine(fa, c->stream));
+  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[EVF_REFINE1], c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, c->stream));
+  VSG_CHECK(hipEventRecord(s.doneF, c->stream));
+  return 0;
+}
+
+extern "C" int vsg_wait_fine(VSGContext* c, int set, const VSGResult** results) {
+  if (!c || set < 0 || set > 1 || !results) return fail("vsg_wait_fine", hipSuccess);
+  Set& s = c->set[set];
+  VSG_CHECK(hipSetDevice(c->device));
+  VSG_CHECK(hipEventSynchronize(s.doneF));
+  if (s.profF) {
+    s.profF = false;
+    const Stage& S = c->st[1];
+    const double px = (double)S.S * S.S;
+    prof_add(VSG_K_GRID_FINE, s.evf[EVF_0], s.evf[EVF_GRID1], s.njobsF * S.candsPerJob * px);
+    prof_add(VSG_K_REFINE_FINE, s.evf[EVF_GRID1], s.evf[EVF_REFINE1], s.njobsF * S.refinePerJob * px);
+  }
+  *results = s.h_resF;
   return 0;
 }
 
 extern "C" int vsg_read_blurred(VSGContext* c, int slot, uint8_t* dst, int dstStride) {
-  if (!c || slot >= c->maxBatch || !dst || dstStride < c->W) return fail("vsg_read_blurred", hipSuccess);
+  if (!c || slot >= 2 * c->B || !dst || dstStride < c->W) return fail("vsg_read_blurred", hipSuccess);
   VSG_CHECK(hipSetDevice(c->device));
   const uint8_t* src = slot < 0 ? c->d_ref : c->d_blur + (size_t)slot * c->slotStride;
   VSG_CHECK(hipMemcpy2DAsync(dst, dstStride, src, c->pitch, c->W, c->H, hipMemcpyDeviceToHost, c->stream));

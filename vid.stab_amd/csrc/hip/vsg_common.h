@@ -61,6 +61,7 @@ typedef struct {
   int J, K;     /* candidates per lane along x and y */
   int D;        /* words between a lane's horizontal candidates (m*step/4) */
   int CW;       /* curr words per column chunk */
+  int wg;       /* threads per workgroup of the grid kernel (multiple of 64) */
 } VSGGrid;
 
 #endif /* VSG_COMMON_H */

@@ -60,6 +60,29 @@ struct VSGSearchArgs {
   VSGResult* results;
 };
 
+/* field selection of one stage (selectfields, src/motiondetect.c:676-743) */
+struct VSGSelectStage {
+  int nf;            /* fields of the stage */
+  int rows;          /* fieldRows */
+  int maxFields;
+  int mmOff;         /* first field of the stage inside a slot's min/max row */
+  double ct;         /* contrast threshold */
+};
+struct VSGSelectArgs {
+  const uint8_t* minmax;     /* [slot][nfTotal][2] */
+  int nfTotal;
+  const double* ctab;        /* contrast of (min, max): ctab[min*256 + max] */
+  VSGSelectStage st[2];
+  int slot0;                 /* first slot of the batch */
+  const int* prevSlot;       /* [frame] previous plane (slot or -1) */
+  const uint8_t* detect;     /* [frame] 0: no measurement this frame */
+  int16_t* selIdx;           /* [frame][nfTotal]: coarse picks, then fine picks */
+  double* selC;              /* same layout */
+  int* count;                /* [frame][2] */
+  VSGJob* jobs;              /* [frame][ncoarse]: coarse jobs, field -1 past the count */
+};
+hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st);
+
 /* fills J, K, D, CW, m, U, V for a stage; returns the workgroups per job */
 int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift);
 /* grid kernel (argmin into best[]) and refinement kernel (best[] -> results) */

@@ -186,8 +186,9 @@ template <int J, int K, int D, int CW>
 __global__ __launch_bounds__(256) void k_sad_grid(VSGSearchArgs a) {
   const int job = blockIdx.y;
   const VSGJob jb = a.jobs[job];
+  if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
   const int S = a.g.S, s = a.g.step, n = a.g.n, m = a.g.m, U = a.g.U, V = a.g.V;
-  const int lid = blockIdx.x * 256 + threadIdx.x;
+  const int lid = blockIdx.x * blockDim.x + threadIdx.x;
   const int u = lid % U;
   int v = lid / U;
   const bool laneOk = v < V;
@@ -272,6 +273,7 @@ __global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
   __shared__ uint32_t shMin;
   const int job = blockIdx.x;
   const VSGJob jb = a.jobs[job];
+  if (jb.field < 0) return;
   const int S = a.g.S, words = S >> 2;
   const int fxc = a.fx[jb.field], fyc = a.fy[jb.field];
   const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
@@ -338,11 +340,121 @@ __global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
 
 template <int J, int K, int D, int CW>
 hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL((k_sad_grid<J, K, D, CW>), dim3(a.tiles, a.njobs), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((k_sad_grid<J, K, D, CW>), dim3(a.tiles, a.njobs), dim3(a.g.wg), 0, st, a);
   return hipGetLastError();
 }
 
+/* ---- field selection -----------------------------------------------------------
+   selectfields (src/motiondetect.c:676-743) for one (frame, stage) per workgroup.
+   The reference sorts with glibc's stable qsort, so equal contrasts keep index
+   order; that order is the total order (contrast desc, index asc), and each
+   element's place in it is computed directly as the number of elements that
+   precede it.  Segment picks: the first maxFields/(rows+1) positive entries of
+   each index segment, in segment order.  Leftovers: the stable re-sort of the
+   whole array with the picks zeroed keeps equal contrasts in index order
+   again, so the same rule applies.  Contrast doubles come from a host-built
+   table of (max-min)/(max+min+0.1) over all 65536 (min, max) pairs, so no
+   device double division is involved. */
+__global__ __launch_bounds__(256) void k_select(VSGSelectArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ int segCnt[256];
+  __shared__ int segBase[256];
+  __shared__ int total, nleft;
+  const int f = blockIdx.x, stage = blockIdx.y;
+  const VSGSelectStage st = a.st[stage];
+  const int n = st.nf;
+  const size_t rowOff = (size_t)f * a.nfTotal + (stage ? a.st[0].nf : 0);
+  int16_t* outIdx = a.selIdx + rowOff;
+  double* outC = a.selC + rowOff;
+  const int nc = a.st[0].nf;
+  if (!a.detect[f]) {
+    if (threadIdx.x == 0) a.count[f * 2 + stage] = 0;
+    if (stage == 0)
+      for (int q = threadIdx.x; q < nc; q += 256) a.jobs[(size_t)f * nc + q].field = -1;
+    return;
+  }
+  double* c = reinterpret_cast<double*>(lds);
+  int* pick = reinterpret_cast<int*>(c + n);
+  const uint8_t* mm = a.minmax + ((size_t)(a.slot0 + f) * a.nfTotal + st.mmOff) * 2;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double v = a.ctab[mm[2 * i] * 256 + mm[2 * i + 1]];
+    c[i] = v < st.ct ? 0.0 : v;
+    pick[i] = 0;
+  }
+  const int numsegms = st.rows + 1;
+  const int segmlen = n / numsegms + 1;
+  const int perSeg = st.maxFields / numsegms;
+  for (int s = threadIdx.x; s < numsegms; s += 256) segCnt[s] = 0;
+  if (threadIdx.x == 0) nleft = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double ci = c[i];
+    if (!(ci > 0.0)) continue;
+    const int s = i / segmlen, start = s * segmlen, end = min(start + segmlen, n);
+    int pos = 0;
+    for (int j = start; j < end; ++j) pos += (c[j] > ci || (c[j] == ci && j < i)) ? 1 : 0;
+    if (pos < perSeg) {
+      pick[i] = pos + 1;
+      atomicAdd(&segCnt[s], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int s = 0; s < numsegms; ++s) {
+      segBase[s] = acc;
+      acc += segCnt[s];
+    }
+    total = acc;
+  }
+  __syncthreads();
+  const int picked = total;
+  const int remaining = st.maxFields - picked;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    if (pick[i]) {
+      const int o = segBase[i / segmlen] + pick[i] - 1;
+      outIdx[o] = (int16_t)i;
+      outC[o] = c[i];
+    } else if (remaining > 0 && c[i] > 0.0) {
+      const double ci = c[i];
+      int rank = 0;
+      for (int j = 0; j < n && rank < remaining; ++j) {
+        const double cj = pick[j] ? 0.0 : c[j];
+        rank += (cj > ci || (cj == ci && j < i)) ? 1 : 0;
+      }
+      if (rank < remaining) {
+        outIdx[picked + rank] = (int16_t)i;
+        outC[picked + rank] = ci;
+        atomicAdd(&nleft, 1);
+      }
+    }
+  }
+  __syncthreads();
+  const int cnt = picked + nleft;
+  if (threadIdx.x == 0) a.count[f * 2 + stage] = cnt;
+  if (stage == 0) {
+    for (int q = threadIdx.x; q < nc; q += 256) {
+      VSGJob j;
+      j.slot = a.slot0 + f;
+      j.prev = a.prevSlot[f];
+      j.field = q < cnt ? outIdx[q] : -1;
+      j.ox = 0;
+      j.oy = 0;
+      a.jobs[(size_t)f * nc + q] = j;
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st) {
+  if (nframes <= 0) return hipSuccess;
+  const int nmax = a.st[0].nf > a.st[1].nf ? a.st[0].nf : a.st[1].nf;
+  const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
+  if (lds > 64 * 1024 || a.st[0].rows + 1 > 256 || a.st[1].rows + 1 > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_select, dim3(nframes, 2), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
 
 /* ---- launchers ------------------------------------------------------------- */
 
@@ -403,7 +515,11 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift) {
   const int perClass = (G + g->m - 1) / g->m;
   g->U = g->m * ((perClass + g->J - 1) / g->J);
   g->V = (G + g->K - 1) / g->K;
-  return (g->U * g->V + 255) / 256;
+  /* a job that needs fewer than 256 lanes gets a workgroup of just enough
+     whole waves, so idle lanes stay below one wave */
+  const int lanes = g->U * g->V;
+  g->wg = lanes >= 256 ? 256 : 64 * ((lanes + 63) / 64);
+  return (lanes + g->wg - 1) / g->wg;
 }
 
 hipError_t vsg_launch_grid(const VSGSearchArgs& a, hipStream_t st) {

@@ -1,20 +1,23 @@
 /*
  * vs_motiondetect.c -- the detect-pass driver behind the vid.stab C API
- * (reference: src/motiondetect.c:60-305, 314-358, 665-807).
+ * (reference: src/motiondetect.c:60-305, 314-358, 752-807).
  *
  * Per batch of frames (one frame for vsMotionDetection):
- *   1. GPU: upload, box blur, per-field min/max        (vsg_frames)
- *   2. host: contrast, threshold and stable selection  (selectfields)
- *   3. GPU: coarse grid search + refinement            (vsg_search, stage 0)
- *   4. host: coarse LocalMotions, simple transform, fine offsets
- *   5. GPU: fine search at the offsets                 (vsg_search, stage 1)
- *   6. host: match cleanup ("keep fine with match <= cleanmean(coarse)"),
- *            coarse ++ fine output, previous-frame bookkeeping
- * Frames of one batch are independent given their previous frame, so each
- * GPU stage runs every field of every frame of the batch in one launch.
+ *   GPU  (vsg_submit)      upload, box blur, per-field min/max, field
+ *                          selection of both grids, coarse grid search and
+ *                          refinement for every field of every frame
+ *   host (after the wait)  coarse LocalMotions, vsSimpleMotionsToTransform
+ *                          (glibc libm), fine offsets with the border discard
+ *   GPU  (vsg_submit_fine) fine search of every frame of the batch
+ *   host                   match cleanup, coarse ++ fine output
+ * Batches alternate between the two slot sets of the GPU context, so the
+ * device runs batch k+1's blur/selection/coarse search while the host turns
+ * batch k's fine results into LocalMotions.  Frames of a batch are independent
+ * given their previous plane, so the output is exactly that of one
+ * vsMotionDetection call per frame.
  *
- * VSMotionDetect is caller-owned with a frozen layout, so the GPU context of
- * a detector lives in a side table keyed by the struct's address.
+ * VSMotionDetect is caller-owned with a frozen layout; the GPU context of a
+ * detector lives in a side table keyed by the struct's address.
  */
 #include <pthread.h>
 #include <stdio.h>
@@ -28,32 +31,19 @@
 #include "vidstabdefines.h"
 #include "vsg.h"
 
-/* ---- side table --------------------------------------------------------- */
+#define MD_MAX_BATCH 64
 
-typedef struct {
-  double contrast;
-  int index;
-} SelEntry;
+/* ---- side table --------------------------------------------------------- */
 
 typedef struct MDState {
   const VSMotionDetect* key;
   struct MDState* next;
   VSGContext* gpu;
   int maxBatch;
-  int lastSlot;
-  /* per-batch host scratch */
-  uint8_t* mmC;            /* [maxBatch][ncoarse][2] */
-  uint8_t* mmF;            /* [maxBatch][nfine][2] */
-  SelEntry* selC;          /* [maxBatch][ncoarse] */
-  SelEntry* selF;          /* [maxBatch][nfine] */
-  int* nselC;
-  int* nselF;
-  VSGJob* jobs;            /* [maxBatch * max(nc, nf)] */
-  VSGResult* res;
-  int* jobFrame;           /* frame of each fine job */
-  int* jobSel;             /* selection index of each fine job */
-  SelEntry* sortTmp;
-  double* contrast;
+  int nextSet;     /* slot set of the next batch */
+  int lastSlot;    /* global slot holding the previous frame's blurred plane */
+  VSGJob* jobs;    /* fine jobs of one batch */
+  int* jobSel;     /* fine selection index of each job */
 } MDState;
 
 static MDState* g_states = NULL;
@@ -70,9 +60,8 @@ static MDState* state_find(const VSMotionDetect* md) {
 static void state_free(MDState* s) {
   if (!s) return;
   if (s->gpu) vsg_destroy(s->gpu);
-  free(s->mmC); free(s->mmF); free(s->selC); free(s->selF); free(s->nselC); free(s->nselF);
-  free(s->jobs); free(s->res); free(s->jobFrame); free(s->jobSel); free(s->sortTmp);
-  free(s->contrast);
+  free(s->jobs);
+  free(s->jobSel);
   free(s);
 }
 
@@ -173,7 +162,18 @@ static int blur_box_size(const VSMotionDetect* md) {
   return VS_CLAMP(size, 3, VS_MIN(md->fi.height / 2, md->fi.width / 2));
 }
 
-static void* xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
+static VSGStageDesc stage_desc(const VSMotionDetectFields* fs) {
+  VSGStageDesc d;
+  d.fieldSize = fs->fieldSize;
+  d.maxShift = fs->maxShift;
+  d.stepSize = fs->stepSize;
+  d.nfields = fs->fieldNum;
+  d.fieldRows = fs->fieldRows;
+  d.maxFields = fs->maxFields;
+  d.contrastThreshold = fs->contrastThreshold;
+  d.fields = fs->fields;
+  return d;
+}
 
 VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* conf, const VSFrameInfo* fi) {
   if (!md || !conf || !fi) return VS_ERROR;
@@ -233,38 +233,25 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
   MDState* s = (MDState*)calloc(1, sizeof(MDState));
   if (!s) return VS_ERROR;
   s->key = md;
-  s->maxBatch = VS_MAX(1, env_int("VIDSTAB_AMD_MAX_BATCH", 16));
+  s->maxBatch = VS_MIN(MD_MAX_BATCH, VS_MAX(1, env_int("VIDSTAB_AMD_MAX_BATCH", 16)));
+  s->lastSlot = -1;
   const int device = env_int("VIDSTAB_AMD_DEVICE", 0);
-  const int nc = md->fieldscoarse.fieldNum, nf = md->fieldsfine.fieldNum;
-  const int B = s->maxBatch, nmax = VS_MAX(nc, nf);
-  s->mmC = (uint8_t*)xcalloc((size_t)B * nc * 2, 1);
-  s->mmF = (uint8_t*)xcalloc((size_t)B * nf * 2, 1);
-  s->selC = (SelEntry*)xcalloc((size_t)B * nc, sizeof(SelEntry));
-  s->selF = (SelEntry*)xcalloc((size_t)B * nf, sizeof(SelEntry));
-  s->nselC = (int*)xcalloc((size_t)B, sizeof(int));
-  s->nselF = (int*)xcalloc((size_t)B, sizeof(int));
-  s->jobs = (VSGJob*)xcalloc((size_t)B * nmax, sizeof(VSGJob));
-  s->res = (VSGResult*)xcalloc((size_t)B * nmax, sizeof(VSGResult));
-  s->jobFrame = (int*)xcalloc((size_t)B * nmax, sizeof(int));
-  s->jobSel = (int*)xcalloc((size_t)B * nmax, sizeof(int));
-  s->sortTmp = (SelEntry*)xcalloc((size_t)nmax, sizeof(SelEntry));
-  s->contrast = (double*)xcalloc((size_t)nmax, sizeof(double));
-  if (!s->mmC || !s->mmF || !s->selC || !s->selF || !s->nselC || !s->nselF || !s->jobs || !s->res ||
-      !s->jobFrame || !s->jobSel || !s->sortTmp || !s->contrast) {
+  const size_t njmax = (size_t)s->maxBatch * md->fieldsfine.fieldNum;
+  s->jobs = (VSGJob*)calloc(njmax, sizeof(VSGJob));
+  s->jobSel = (int*)calloc(njmax, sizeof(int));
+  if (!s->jobs || !s->jobSel) {
     state_free(s);
     vs_log_error(md->conf.modName, "malloc failed");
     return VS_ERROR;
   }
-  VSGStageDesc dc = {md->fieldscoarse.fieldSize, md->fieldscoarse.maxShift, md->fieldscoarse.stepSize,
-                     nc, md->fieldscoarse.fields};
-  VSGStageDesc df = {md->fieldsfine.fieldSize, md->fieldsfine.maxShift, md->fieldsfine.stepSize, nf,
-                     md->fieldsfine.fields};
-  if (vsg_create(&s->gpu, device, md->fi.width, md->fi.height, blur_box_size(md), &dc, &df, B) != 0) {
+  VSGStageDesc dc = stage_desc(&md->fieldscoarse), df = stage_desc(&md->fieldsfine);
+  if (vsg_create(&s->gpu, device, md->fi.width, md->fi.height, blur_box_size(md), &dc, &df,
+                 s->maxBatch) != 0) {
     vs_log_error(md->conf.modName, "GPU initialisation failed: %s\n", vsg_last_error());
     state_free(s);
     return VS_ERROR;
   }
-  vs_log_info(md->conf.modName, "GPU: HIP device %i, batch %i\n", device, B);
+  vs_log_info(md->conf.modName, "GPU: HIP device %i, batch %i\n", device, s->maxBatch);
   state_insert(s);
   md->initialized = 2;
   return VS_OK;
@@ -284,63 +271,6 @@ VS_API void vsMotionDetectionCleanup(VSMotionDetect* md) {
   vsFrameFree(&md->curr);
   vsFrameFree(&md->currtmp);
   md->initialized = 0;
-}
-
-/* ---- field selection: src/motiondetect.c:665-743 ---------------------------
-   The reference sorts with glibc qsort, a stable merge sort, so equal
-   contrasts keep ascending index order; the merge sort below is stable too. */
-
-static void sort_desc_stable(SelEntry* a, int n, SelEntry* tmp) {
-  for (int width = 1; width < n; width *= 2) {
-    for (int lo = 0; lo < n; lo += 2 * width) {
-      const int mid = VS_MIN(lo + width, n), hi = VS_MIN(lo + 2 * width, n);
-      int p = lo, q = mid, o = lo;
-      while (p < mid && q < hi) tmp[o++] = (a[q].contrast > a[p].contrast) ? a[q++] : a[p++];
-      while (p < mid) tmp[o++] = a[p++];
-      while (q < hi) tmp[o++] = a[q++];
-    }
-    memcpy(a, tmp, sizeof(SelEntry) * (size_t)n);
-  }
-}
-
-/* mm: (min,max) per field -> ordered selection in out; returns its length */
-static int select_fields(const VSMotionDetectFields* fs, const uint8_t* mm, SelEntry* out,
-                         MDState* s) {
-  const int n = fs->fieldNum;
-  double* c = s->contrast;
-  SelEntry* seg = (SelEntry*)malloc(sizeof(SelEntry) * (size_t)n);
-  if (!seg) return 0;
-  for (int i = 0; i < n; i++) {
-    const int mn = mm[2 * i], mx = mm[2 * i + 1];
-    double v = (double)(mx - mn) / ((double)(mx + mn) + 0.1);
-    c[i] = v < fs->contrastThreshold ? 0.0 : v;
-    seg[i].contrast = c[i];
-    seg[i].index = i;
-  }
-  int cnt = 0;
-  const int numsegms = fs->fieldRows + 1;
-  const int segmlen = n / (fs->fieldRows + 1) + 1;
-  for (int g = 0; g < numsegms; g++) {
-    const int start = segmlen * g, end = VS_MIN(segmlen * (g + 1), n);
-    if (end > start) sort_desc_stable(seg + start, end - start, s->sortTmp);
-    for (int j = 0; j < fs->maxFields / numsegms; j++) {
-      if (start + j >= end) continue;
-      if (seg[start + j].contrast > 0) {
-        out[cnt].index = seg[start + j].index;
-        out[cnt].contrast = c[seg[start + j].index];
-        cnt++;
-        seg[start + j].contrast = 0;
-      }
-    }
-  }
-  const int remaining = fs->maxFields - cnt;
-  if (remaining > 0) {
-    sort_desc_stable(seg, n, s->sortTmp);
-    for (int j = 0; j < remaining && j < n; j++)
-      if (seg[j].contrast > 0) out[cnt++] = seg[j];
-  }
-  free(seg);
-  return cnt;
 }
 
 /* ---- per-field result -> LocalMotion: src/motiondetect.c:573-585 ---------- */
@@ -364,144 +294,154 @@ static short lm_match_better(void* thresh, void* lm) {
   return ((LocalMotion*)lm)->match <= *((double*)thresh) ? 1 : 0;
 }
 
-/* ---- the batch core --------------------------------------------------------- */
+/* ---- batches ------------------------------------------------------------------ */
 
-static int detect_chunk(VSMotionDetect* md, MDState* s, LocalMotions* motions,
-                        const uint8_t* const* luma, const int* linesize, int onDevice, int nb) {
+typedef struct {
+  int set, n, slot0;
+  int fnum0;                   /* frame number of the batch's first frame */
+  int prevSlot[MD_MAX_BATCH];
+  uint8_t detect[MD_MAX_BATCH];
+  LocalMotions* out;
+  LocalMotions coarse[MD_MAX_BATCH];
+  int njobs;
+} Batch;
+
+/* decides, for the next n frames, which are measured and against which plane
+   (src/motiondetect.c:211-218, 294-303), and submits them */
+static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t* const* luma,
+                        const int* linesize, int onDevice, int n, LocalMotions* out) {
+  const int tripod = md->conf.virtualTripod;
+  b->set = s->nextSet;
+  s->nextSet ^= 1;
+  b->n = n;
+  b->slot0 = b->set * s->maxBatch;
+  b->fnum0 = md->frameNum;
+  b->out = out;
+  b->njobs = 0;
+  int refSlot = -1;
+  int seen = md->hasSeenOneFrame;
+  for (int i = 0; i < n; i++) {
+    const int fnum = md->frameNum + i;
+    const int leadIn = tripod >= 1 && fnum < tripod;
+    b->detect[i] = (uint8_t)(seen && !leadIn);
+    if (!b->detect[i]) seen = 1;
+    /* tripod: every measured frame compares against frame tripod-1, kept as
+       the stored reference plane; otherwise against the frame before */
+    b->prevSlot[i] = tripod >= 1 ? -1 : (i == 0 ? s->lastSlot : b->slot0 + i - 1);
+    if (tripod >= 1 && fnum == tripod - 1) refSlot = b->slot0 + i;
+  }
+  if (vsg_submit(s->gpu, b->set, n, luma, linesize, onDevice, b->prevSlot, b->detect, refSlot) != 0) {
+    vs_log_error(md->conf.modName, "GPU detect failed: %s\n", vsg_last_error());
+    return VS_ERROR;
+  }
+  md->hasSeenOneFrame = (short)seen;
+  md->frameNum += n;
+  s->lastSlot = b->slot0 + n - 1;
+  return VS_OK;
+}
+
+/* coarse results -> coarse LocalMotions, transform, fine jobs; submits them */
+static int batch_coarse(VSMotionDetect* md, MDState* s, Batch* b) {
   const char* mod = md->conf.modName;
   VSMotionDetectFields* fc = &md->fieldscoarse;
   VSMotionDetectFields* ff = &md->fieldsfine;
-  const int nc = fc->fieldNum, nf = ff->fieldNum;
-  const int tripod = md->conf.virtualTripod;
-
-  if (vsg_frames(s->gpu, nb, luma, linesize, onDevice, s->mmC, s->mmF) != 0) {
-    vs_log_error(mod, "GPU blur/contrast failed: %s\n", vsg_last_error());
+  VSGBatchView v;
+  if (vsg_wait_coarse(s->gpu, b->set, &v) != 0) {
+    vs_log_error(mod, "GPU detect failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
-
-  /* which frames are measured, and against which plane */
-  int detect[64], prevSlot[64];
-  int lastKept = -1;   /* last slot of this batch that became the stored previous plane */
-  int seen = md->hasSeenOneFrame;
-  for (int i = 0; i < nb; i++) {
-    const int fnum = md->frameNum + i;
-    const int leadIn = tripod >= 1 && fnum < tripod;
-    detect[i] = seen && !leadIn;
-    prevSlot[i] = lastKept;
-    if (!detect[i]) seen = 1;
-    if (tripod < 1 || fnum < tripod) lastKept = i;
-  }
-
-  /* coarse selection and jobs */
-  int njobs = 0;
-  for (int i = 0; i < nb; i++) {
-    s->nselC[i] = 0;
-    if (!detect[i]) continue;
-    SelEntry* sel = s->selC + (size_t)i * nc;
-    s->nselC[i] = select_fields(fc, s->mmC + (size_t)i * nc * 2, sel, s);
-    for (int q = 0; q < s->nselC[i]; q++) {
-      VSGJob* j = &s->jobs[njobs++];
-      j->slot = i;
-      j->prev = prevSlot[i];
-      j->field = sel[q].index;
-      j->ox = 0;
-      j->oy = 0;
-    }
-  }
-  if (vsg_search(s->gpu, 0, njobs, s->jobs, s->res) != 0) {
-    vs_log_error(mod, "GPU coarse search failed: %s\n", vsg_last_error());
-    return VS_ERROR;
-  }
-
-  /* coarse LocalMotions, transform, fine jobs */
-  LocalMotions coarse[64];
-  int jc = 0, njf = 0;
-  for (int i = 0; i < nb; i++) {
-    vs_vector_init(&coarse[i], 0);
-    if (!detect[i]) continue;
-    vs_vector_init(&coarse[i], fc->maxFields);
-    const SelEntry* sel = s->selC + (size_t)i * nc;
-    for (int q = 0; q < s->nselC[i]; q++, jc++) {
+  int nj = 0;
+  for (int i = 0; i < b->n; i++) {
+    vs_vector_init(&b->coarse[i], 0);
+    if (!b->detect[i]) continue;
+    vs_vector_init(&b->coarse[i], fc->maxFields);
+    const int16_t* idx = v.idx + (size_t)i * v.stride;
+    const double* con = v.contrast + (size_t)i * v.stride;
+    const VSGResult* res = v.coarse + (size_t)i * v.ncoarse;
+    for (int q = 0; q < v.count[2 * i]; q++) {
       LocalMotion lm;
-      const Field* f = &fc->fields[sel[q].index];
-      if (make_motion(fc, f, &s->res[jc], 0, 0, sel[q].contrast, &lm))
-        vs_vector_append_dup(&coarse[i], &lm, sizeof(LocalMotion));
+      if (make_motion(fc, &fc->fields[idx[q]], &res[q], 0, 0, con[q], &lm))
+        vs_vector_append_dup(&b->coarse[i], &lm, sizeof(LocalMotion));
     }
-    if (vs_vector_size(&coarse[i]) < 1) {
-      vs_log_warn(mod, "too low contrast. (no translations are detected in frame %i)\n", md->frameNum + i);
+    if (vs_vector_size(&b->coarse[i]) < 1) {
+      vs_log_warn(mod, "too low contrast. (no translations are detected in frame %i)\n", b->fnum0 + i);
       continue;
     }
-    VSTransform t = vsSimpleMotionsToTransform(md->fi, mod, &coarse[i]);
-    ff->offset = t;
+    /* src/motiondetect.c:238-240, 418-434 */
+    ff->offset = vsSimpleMotionsToTransform(md->fi, mod, &b->coarse[i]);
     ff->useOffset = 1;
     PreparedTransform pt = prepare_transform(&ff->offset, &md->fi);
-    SelEntry* sel2 = s->selF + (size_t)i * nf;
-    s->nselF[i] = select_fields(ff, s->mmF + (size_t)i * nf * 2, sel2, s);
-    for (int q = 0; q < s->nselF[i]; q++) {
-      const Field* f = &ff->fields[sel2[q].index];
+    const int16_t* fidx = idx + v.ncoarse;
+    for (int q = 0; q < v.count[2 * i + 1]; q++) {
+      const Field* f = &ff->fields[fidx[q]];
       Vec pos = {f->x, f->y};
       Vec off = sub_vec(transform_vec(&pt, &pos), pos);
-      /* src/motiondetect.c:418-434: discard fields whose window leaves the frame */
       const int border = f->size / 2 + ff->maxShift + ff->stepSize;
       const int x = pos.x + off.x, y = pos.y + off.y;
       if (x - border < 0 || x + border >= md->fi.width || y - border < 0 || y + border >= md->fi.height)
         continue;
-      VSGJob* j = &s->jobs[njf];
-      j->slot = i;
-      j->prev = prevSlot[i];
-      j->field = sel2[q].index;
+      VSGJob* j = &s->jobs[nj];
+      j->slot = b->slot0 + i;
+      j->prev = b->prevSlot[i];
+      j->field = fidx[q];
       j->ox = off.x;
       j->oy = off.y;
-      s->jobFrame[njf] = i;
-      s->jobSel[njf] = q;
-      njf++;
+      s->jobSel[nj] = q;
+      nj++;
     }
   }
-  if (vsg_search(s->gpu, 1, njf, s->jobs, s->res) != 0) {
+  b->njobs = nj;
+  if (vsg_submit_fine(s->gpu, b->set, nj, s->jobs) != 0) {
     vs_log_error(mod, "GPU fine search failed: %s\n", vsg_last_error());
-    for (int i = 0; i < nb; i++) vs_vector_del(&coarse[i]);
     return VS_ERROR;
   }
+  return VS_OK;
+}
 
-  /* fine LocalMotions, match cleanup, output */
+/* fine results -> cleanup and output (src/motiondetect.c:249-291) */
+static int batch_fine(VSMotionDetect* md, MDState* s, Batch* b) {
+  VSMotionDetectFields* ff = &md->fieldsfine;
+  const VSGResult* res = NULL;
+  VSGBatchView v;
+  if (vsg_wait_fine(s->gpu, b->set, &res) != 0 || vsg_wait_coarse(s->gpu, b->set, &v) != 0) {
+    vs_log_error(md->conf.modName, "GPU fine search failed: %s\n", vsg_last_error());
+    return VS_ERROR;
+  }
   int jf = 0;
-  for (int i = 0; i < nb; i++) {
-    if (!detect[i]) {
-      vs_vector_fini(&coarse[i]);
-      vs_vector_init(&motions[i], 1);   /* the reference's empty "dummy" vector */
+  for (int i = 0; i < b->n; i++) {
+    if (!b->detect[i]) {
+      vs_vector_fini(&b->coarse[i]);
+      vs_vector_init(&b->out[i], 1);   /* the reference's empty "dummy" vector */
       continue;
     }
     LocalMotions fine;
     vs_vector_init(&fine, 0);
-    if (vs_vector_size(&coarse[i]) >= 1) {
-      VSArray mq = localmotionsGetMatch(&coarse[i]);
+    if (vs_vector_size(&b->coarse[i]) >= 1) {
+      VSArray mq = localmotionsGetMatch(&b->coarse[i]);
       double meanMatch = cleanmean(mq.dat, mq.len, NULL, NULL);
       vs_array_free(mq);
-      const SelEntry* sel2 = s->selF + (size_t)i * nf;
-      for (; jf < njf && s->jobFrame[jf] == i; jf++) {
+      const double* fcon = v.contrast + (size_t)i * v.stride + v.ncoarse;
+      for (; jf < b->njobs && s->jobs[jf].slot == b->slot0 + i; jf++) {
         const VSGJob* j = &s->jobs[jf];
         LocalMotion lm;
-        const int q = s->jobSel[jf];
-        if (!make_motion(ff, &ff->fields[j->field], &s->res[jf], j->ox, j->oy, sel2[q].contrast, &lm))
+        if (!make_motion(ff, &ff->fields[j->field], &res[jf], j->ox, j->oy, fcon[s->jobSel[jf]], &lm))
           continue;
         if (lm_match_better(&meanMatch, &lm)) vs_vector_append_dup(&fine, &lm, sizeof(LocalMotion));
       }
     }
-    motions[i] = vs_vector_concat(&coarse[i], &fine);
-    vs_vector_fini(&coarse[i]);
+    b->out[i] = vs_vector_concat(&b->coarse[i], &fine);
+    vs_vector_fini(&b->coarse[i]);
     vs_vector_fini(&fine);
   }
-
-  if (lastKept >= 0 && vsg_keep_prev(s->gpu, lastKept) != 0) {
-    vs_log_error(mod, "GPU: %s\n", vsg_last_error());
-    return VS_ERROR;
-  }
-  s->lastSlot = nb - 1;
-  md->hasSeenOneFrame = (short)seen;
-  md->frameNum += nb;
   return VS_OK;
 }
 
+static void batch_discard(Batch* b) {
+  for (int i = 0; i < b->n; i++) vs_vector_del(&b->coarse[i]);
+}
+
+/* Runs n frames through batches of maxBatch; batch k+1 is submitted before
+   batch k's fine results are waited for. */
 static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_t* const* luma,
                          const int* linesize, int onDevice, int n) {
   if (!md || md->initialized != 2 || !motions || n < 0) return VS_ERROR;
@@ -510,14 +450,43 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
     vs_log_error(md->conf.modName, "detector has no GPU state (not initialised?)\n");
     return VS_ERROR;
   }
-  const int B = VS_MIN(s->maxBatch, 64);
-  for (int done = 0; done < n;) {
-    const int nb = VS_MIN(B, n - done);
-    if (detect_chunk(md, s, motions + done, luma + done, linesize + done, onDevice, nb) != VS_OK) {
-      for (int i = 0; i < done; i++) vs_vector_del(&motions[i]);
-      return VS_ERROR;
+  if (n == 0) return VS_OK;
+  static __thread Batch bufs[2];
+  Batch* cur = &bufs[0];
+  Batch* nxt = &bufs[1];
+  const int B = s->maxBatch;
+  int done = 0, next = VS_MIN(B, n);
+  if (batch_submit(md, s, cur, luma, linesize, onDevice, next, motions) != VS_OK) return VS_ERROR;
+  int ok = 1;
+  while (ok) {
+    if (batch_coarse(md, s, cur) != VS_OK) {
+      ok = 0;
+      break;
     }
-    done += nb;
+    int haveNext = 0;
+    if (next < n) {
+      const int nb = VS_MIN(B, n - next);
+      if (batch_submit(md, s, nxt, luma + next, linesize + next, onDevice, nb, motions + next) != VS_OK) {
+        batch_discard(cur);
+        ok = 0;
+        break;
+      }
+      haveNext = 1;
+      next += nb;
+    }
+    if (batch_fine(md, s, cur) != VS_OK) {
+      ok = 0;
+      break;
+    }
+    done += cur->n;
+    if (!haveNext) break;
+    Batch* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (!ok) {
+    for (int i = 0; i < done; i++) vs_vector_del(&motions[i]);
+    return VS_ERROR;
   }
   return VS_OK;
 }
@@ -559,6 +528,6 @@ VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions, co
 
 VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride) {
   MDState* s = state_find(md);
-  if (!s) return VS_ERROR;
+  if (!s || s->lastSlot < 0) return VS_ERROR;
   return vsg_read_blurred(s->gpu, s->lastSlot, dst, dstStride) == 0 ? VS_OK : VS_ERROR;
 }
