@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="clip length (0: the workload's)")
     ap.add_argument("--cpu-frames", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed region, one 256 MiB device copy (known bytes) to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the detect pass shards by frame range or clip with no "
@@ -122,6 +125,12 @@ def main():
     prof = VSGProfile()
     L.vsg_profile_get(ctypes.byref(prof))
     det.close()
+    if args.calib_copy:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0").fill_(7)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 268435456 bytes read, 268435456 bytes written")
     ms_step = (t1 - t0) * 1000.0 / args.steps
     value = nfr * args.steps / (t1 - t0)
     for k, name in enumerate(KNAMES):

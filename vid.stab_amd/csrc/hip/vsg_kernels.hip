@@ -263,22 +263,31 @@ __global__ __launch_bounds__(256) void k_sad_grid(VSGSearchArgs a) {
 }
 
 /* ---- refinement --------------------------------------------------------------
-   One workgroup per job.  Each level's candidates are evaluated in parallel
-   (rows spread over the workgroup, LDS atomic adds), then one lane scans them
-   in the reference's order (i outer, j inner, centre skipped) with the strict
-   "<" against the running minimum, which is exactly the sequential loop. */
+   One workgroup per job.  A level's candidates form an ni x ni grid (spacing
+   ns around the current best); an item is one (candidate row, field row)
+   pair, whose prev row serves all ni candidates of that row: 16-byte loads of
+   the prev row at each candidate's byte offset (gfx950 global loads accept any
+   byte alignment) against 16-byte loads of the aligned field patch, four
+   v_sad_u8 per load.  Row sums meet in LDS; then one lane scans the
+   candidates in the reference's order (i outer, j inner, centre skipped) with
+   the strict "<" against the running minimum, which is exactly the
+   sequential loop. */
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef u32x4 __attribute__((aligned(1))) u32x4_any;
+
 __global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
-  __shared__ uint32_t sads[32];
+  constexpr int kMaxNi = 5;   /* ni = 2r/ns + 1 <= 5 for every step */
+  __shared__ uint32_t sads[kMaxNi * kMaxNi];
   __shared__ int shTx, shTy;
   __shared__ uint32_t shMin;
   const int job = blockIdx.x;
   const VSGJob jb = a.jobs[job];
   if (jb.field < 0) return;
-  const int S = a.g.S, words = S >> 2;
+  const int S = a.g.S, w16 = S >> 4;
   const int fxc = a.fx[jb.field], fyc = a.fy[jb.field];
   const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
-  const uint32_t* patch = reinterpret_cast<const uint32_t*>(a.patches + (size_t)jb.slot * a.patchSlotStride +
-                                                            a.stageOff + (size_t)jb.field * S * S);
+  const u32x4* patch = reinterpret_cast<const u32x4*>(a.patches + (size_t)jb.slot * a.patchSlotStride +
+                                                      a.stageOff + (size_t)jb.field * S * S);
   if (threadIdx.x == 0) {
     unsigned long long key = a.best[job];
     int ga, gb;
@@ -294,29 +303,43 @@ __global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
   while (st > 1) {
     const int ns = st / 2, r = st - ns;
     const int ni = (2 * r) / ns + 1;
-    const int nc = ni * ni;
-    if (threadIdx.x < 32) sads[threadIdx.x] = 0;
+    if (threadIdx.x < kMaxNi * kMaxNi) sads[threadIdx.x] = 0;
     __syncthreads();
-    for (int item = threadIdx.x; item < nc * S; item += 256) {
-      const int c = item / S, y = item - c * S;
-      const int ti = c / ni, tj = c - ti * ni;
-      const int i = tx - r + ns * ti, j = ty - r + ns * tj;
-      if (i == tx && j == ty) continue;
-      const uint8_t* prow = prev + (size_t)(fyc - S / 2 + y + j + jb.oy) * a.pitch + (fxc - S / 2 + i + jb.ox);
-      const uint32_t* crow = patch + y * words;
-      uint32_t sum = 0;
-      for (int w = 0; w < words; ++w) sum = __builtin_amdgcn_sad_u8(load_u32_unaligned(prow + 4 * w), crow[w], sum);
-      atomicAdd(&sads[c], sum);
+    const int col0 = fxc - S / 2 + tx - r + jb.ox;
+    for (int item = threadIdx.x; item < ni * S; item += 256) {
+      const int tj = item / S, y = item - tj * S;
+      const int j = ty - r + ns * tj;
+      const uint8_t* prow = prev + (size_t)(fyc - S / 2 + y + j + jb.oy) * a.pitch + col0;
+      const u32x4* crow = patch + y * w16;
+      uint32_t acc[kMaxNi] = {0, 0, 0, 0, 0};
+      for (int w = 0; w < w16; ++w) {
+        const u32x4 c = crow[w];
+#pragma unroll
+        for (int t = 0; t < kMaxNi; ++t) {
+          if (t < ni) {
+            const u32x4 p = *reinterpret_cast<const u32x4_any*>(prow + ns * t + 16 * w);
+            uint32_t s = acc[t];
+            s = __builtin_amdgcn_sad_u8(p.x, c.x, s);
+            s = __builtin_amdgcn_sad_u8(p.y, c.y, s);
+            s = __builtin_amdgcn_sad_u8(p.z, c.z, s);
+            acc[t] = __builtin_amdgcn_sad_u8(p.w, c.w, s);
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kMaxNi; ++t)
+        if (t < ni) atomicAdd(&sads[t * ni + tj], acc[t]);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
       int btx = tx, bty = ty;
-      for (int c = 0; c < nc; ++c) {
-        const int ti = c / ni, tj = c - ti * ni;
-        const int i = tx - r + ns * ti, j = ty - r + ns * tj;
-        if (i == tx && j == ty) continue;
-        if (sads[c] < minerr) { minerr = sads[c]; btx = i; bty = j; }
-      }
+      for (int ti = 0; ti < ni; ++ti)
+        for (int tj = 0; tj < ni; ++tj) {
+          const int i = tx - r + ns * ti, j = ty - r + ns * tj;
+          if (i == tx && j == ty) continue;
+          const uint32_t e = sads[ti * ni + tj];
+          if (e < minerr) { minerr = e; btx = i; bty = j; }
+        }
       shTx = btx;
       shTy = bty;
       shMin = minerr;
@@ -502,7 +525,15 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift) {
   const int lcm4 = 4 * step / gcd_int(4, step);
   const int D = lcm4 / 4;
   g->K = 4;
-  g->CW = ((S / 4) % 8 == 0) ? 8 : 4;
+  /* the widest column chunk (<= 28 words) that divides a field row: one prev
+     row segment load then feeds J*K*CW v_sad_u8 */
+  static const int kCW[] = {28, 20, 16, 12, 8, 4};
+  g->CW = 4;
+  for (int cw : kCW)
+    if ((S / 4) % cw == 0) {
+      g->CW = cw;
+      break;
+    }
   if (D == 1 || D == 3) {
     g->J = 4;
     g->D = D;
@@ -522,14 +553,24 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift) {
   return (lanes + g->wg - 1) / g->wg;
 }
 
+template <int J, int D>
+static hipError_t launch_grid_cw(const VSGSearchArgs& a, hipStream_t st) {
+  switch (a.g.CW) {
+    case 28: return launch_grid<J, 4, D, 28>(a, st);
+    case 20: return launch_grid<J, 4, D, 20>(a, st);
+    case 16: return launch_grid<J, 4, D, 16>(a, st);
+    case 12: return launch_grid<J, 4, D, 12>(a, st);
+    case 8: return launch_grid<J, 4, D, 8>(a, st);
+    default: return launch_grid<J, 4, D, 4>(a, st);
+  }
+}
+
 hipError_t vsg_launch_grid(const VSGSearchArgs& a, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
   const VSGGrid& g = a.g;
-  if (g.J == 4 && g.D == 1)
-    return g.CW == 8 ? launch_grid<4, 4, 1, 8>(a, st) : launch_grid<4, 4, 1, 4>(a, st);
-  if (g.J == 4 && g.D == 3)
-    return g.CW == 8 ? launch_grid<4, 4, 3, 8>(a, st) : launch_grid<4, 4, 3, 4>(a, st);
-  return g.CW == 8 ? launch_grid<1, 4, 1, 8>(a, st) : launch_grid<1, 4, 1, 4>(a, st);
+  if (g.J == 4 && g.D == 1) return launch_grid_cw<4, 1>(a, st);
+  if (g.J == 4 && g.D == 3) return launch_grid_cw<4, 3>(a, st);
+  return launch_grid_cw<1, 1>(a, st);
 }
 
 hipError_t vsg_launch_refine(const VSGSearchArgs& a, hipStream_t st) {
