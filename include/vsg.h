@@ -15,9 +15,10 @@
  *                   with its compareSubImg SAD (src/motiondetect.c:971-996)
  *   vsg_submit_fine the same search for the fine fields at host-computed offsets
  *
- * A context has two sets of maxBatch frame slots (global slot = set*maxBatch +
- * i), so the host can prepare one batch's fine stage while the device runs the
- * next batch's coarse stage.  All work of a context runs in order on one HIP
+ * A context has VSG_NSETS sets of maxBatch frame slots (global slot =
+ * set*maxBatch + i), so the host can prepare one batch's fine stage while the
+ * device runs the next batch's coarse stage, and a batch's planes survive
+ * until the batch after next.  All work of a context runs in order on one HIP
  * stream; submit calls return at once, wait calls block.
  *
  * Every function returns 0 on success and a negative value on failure;
@@ -35,6 +36,9 @@ extern "C" {
 #endif
 
 typedef struct VSGContext VSGContext;
+
+/* slot sets of a context (batches k, k+1, k+2 use different sets) */
+#define VSG_NSETS 3
 
 /* one field grid with its selection parameters (VSMotionDetectFields) */
 typedef struct {

@@ -20,86 +20,11 @@
 #include <new>
 #include <vector>
 
-#include "vsg_internal.h"
-
-namespace {
-
-enum { EV_BLUR0, EV_BLUR1, EV_FIELDS1, EV_SELECT1, EV_GRID1, EV_REFINE1, EV_COUNT };
-enum { EVF_0, EVF_GRID1, EVF_REFINE1, EVF_COUNT };
-
-struct Stage {
-  int S = 0, maxShift = 0, step = 0, nf = 0, rows = 0, maxFields = 0;
-  double ct = 0;
-  int16_t* d_fx = nullptr;
-  int16_t* d_fy = nullptr;
-  VSGGrid g{};
-  int tiles = 0;
-  size_t patchOff = 0;
-  double candsPerJob = 0;     /* (2n+1)^2 */
-  double refinePerJob = 0;    /* refinement candidates the reference visits */
-};
-
-struct Set {
-  /* device */
-  int16_t* d_selIdx = nullptr;
-  double* d_selC = nullptr;
-  int* d_count = nullptr;
-  VSGJob* d_jobsC = nullptr;
-  VSGJob* d_jobsF = nullptr;
-  unsigned long long* d_best = nullptr;
-  VSGResult* d_resC = nullptr;
-  VSGResult* d_resF = nullptr;
-  VSGBlurSrc* d_src = nullptr;
-  int* d_prev = nullptr;
-  uint8_t* d_detect = nullptr;
-  /* pinned host */
-  int16_t* h_selIdx = nullptr;
-  double* h_selC = nullptr;
-  int* h_count = nullptr;
-  VSGResult* h_resC = nullptr;
-  VSGJob* h_jobsF = nullptr;
-  VSGResult* h_resF = nullptr;
-  VSGBlurSrc* h_src = nullptr;
-  int* h_prev = nullptr;
-  uint8_t* h_detect = nullptr;
-  /* state */
-  int n = 0;              /* frames in flight */
-  int njobsF = 0;
-  hipEvent_t done = nullptr, doneF = nullptr;
-  hipEvent_t ev[EV_COUNT] = {};
-  hipEvent_t evf[EVF_COUNT] = {};
-  bool prof = false, profF = false;
-};
-
-}  // namespace
-
-struct VSGContext {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int W = 0, H = 0, pitch = 0, rowsAlloc = 0;
-  size_t slotStride = 0;
-  int blurSize = 0;
-  int B = 0;
-  Stage st[2];
-  int nfTotal = 0;
-  int16_t* d_allx = nullptr;
-  int16_t* d_ally = nullptr;
-  int16_t* d_allsz = nullptr;
-  uint32_t* d_patchOff = nullptr;
-  double* d_ctab = nullptr;
-  uint8_t* d_raw = nullptr;
-  uint8_t* d_blur = nullptr;
-  uint8_t* d_ref = nullptr;
-  uint8_t* d_tmp = nullptr;
-  uint8_t* d_patches = nullptr;
-  size_t patchSlotStride = 0;
-  uint8_t* d_minmax = nullptr;
-  Set set[2];
-};
+#include "vsg_context.h"
 
 static thread_local char g_err[512];
 
-static int fail(const char* what, hipError_t e) {
+int vsg_fail(const char* what, hipError_t e) {
   snprintf(g_err, sizeof(g_err), "%s: %s", what, e == hipSuccess ? "invalid argument" : hipGetErrorString(e));
   return -1;
 }
@@ -107,7 +32,7 @@ static int fail(const char* what, hipError_t e) {
 #define VSG_CHECK(call)                                  \
   do {                                                   \
     hipError_t e_ = (call);                              \
-    if (e_ != hipSuccess) return fail(#call, e_);        \
+    if (e_ != hipSuccess) return vsg_fail(#call, e_);    \
   } while (0)
 
 extern "C" const char* vsg_last_error(void) { return g_err; }
@@ -152,18 +77,18 @@ static void hfree(T*& p) {
   p = nullptr;
 }
 
-static void free_ctx(VSGContext* c) {
+void vsg_ctx_free(VSGContext* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (Stage& s : c->st) {
+  for (VSGStage& s : c->st) {
     dfree(s.d_fx);
     dfree(s.d_fy);
   }
   dfree(c->d_allx); dfree(c->d_ally); dfree(c->d_allsz); dfree(c->d_patchOff); dfree(c->d_ctab);
   dfree(c->d_raw); dfree(c->d_blur); dfree(c->d_ref); dfree(c->d_tmp); dfree(c->d_patches);
   dfree(c->d_minmax);
-  for (Set& s : c->set) {
+  for (VSGSet& s : c->set) {
     dfree(s.d_selIdx); dfree(s.d_selC); dfree(s.d_count); dfree(s.d_jobsC); dfree(s.d_jobsF);
     dfree(s.d_best); dfree(s.d_resC); dfree(s.d_resF); dfree(s.d_src); dfree(s.d_prev); dfree(s.d_detect);
     hfree(s.h_selIdx); hfree(s.h_selC); hfree(s.h_count); hfree(s.h_resC); hfree(s.h_jobsF);
@@ -174,8 +99,15 @@ static void free_ctx(VSGContext* c) {
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t& e : s.evf)
       if (e) (void)hipEventDestroy(e);
+    s = VSGSet();
   }
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->ownStream) (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+static void free_ctx(VSGContext* c) {
+  if (!c) return;
+  vsg_ctx_free(c);
   delete c;
 }
 
@@ -190,11 +122,17 @@ static hipError_t halloc(T** p, size_t n) {
   return hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1), hipHostMallocDefault);
 }
 
-static int create(VSGContext* c, int device, int width, int height, int blurSize,
-                  const VSGStageDesc* coarse, const VSGStageDesc* fine, int B) {
+int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize, const VSGStageDesc* coarse,
+                 const VSGStageDesc* fine, int B, hipStream_t shared) {
   c->device = device;
   VSG_CHECK(hipSetDevice(device));
-  VSG_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (shared) {
+    c->stream = shared;
+    c->ownStream = false;
+  } else {
+    VSG_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->ownStream = true;
+  }
   c->W = width;
   c->H = height;
   c->pitch = (width + 63) & ~63;
@@ -206,7 +144,7 @@ static int create(VSGContext* c, int device, int width, int height, int blurSize
   std::vector<int16_t> ax, ay, asz;
   std::vector<uint32_t> poff;
   for (int k = 0; k < 2; ++k) {
-    Stage& S = c->st[k];
+    VSGStage& S = c->st[k];
     S.S = sd[k]->fieldSize;
     S.maxShift = sd[k]->maxShift;
     S.step = sd[k]->stepSize;
@@ -214,9 +152,9 @@ static int create(VSGContext* c, int device, int width, int height, int blurSize
     S.rows = sd[k]->fieldRows;
     S.maxFields = sd[k]->maxFields;
     S.ct = sd[k]->contrastThreshold;
-    if (S.S % 16 != 0 || S.S <= 0 || S.step < 1 || S.nf <= 0 || S.nf > 4096 || S.maxShift < 0 ||
+    if (S.S % 16 != 0 || S.S <= 0 || S.S > 256 || S.step < 1 || S.nf <= 0 || S.nf > 4096 || S.maxShift < 0 ||
         S.rows < 1 || S.rows + 1 > 256)
-      return fail("vsg_create: field grid", hipSuccess);
+      return vsg_fail("vsg_create: field grid", hipSuccess);
     maxStep = S.step > maxStep ? S.step : maxStep;
     S.tiles = vsg_plan_grid(&S.g, S.S, S.step, S.maxShift);
     S.candsPerJob = (2.0 * S.g.n + 1.0) * (2.0 * S.g.n + 1.0);
@@ -230,7 +168,7 @@ static int create(VSGContext* c, int device, int width, int height, int blurSize
     std::vector<int16_t> fx(S.nf), fy(S.nf);
     for (int i = 0; i < S.nf; ++i) {
       const Field& f = sd[k]->fields[i];
-      if (f.size != S.S) return fail("vsg_create: field size mismatch", hipSuccess);
+      if (f.size != S.S) return vsg_fail("vsg_create: field size mismatch", hipSuccess);
       fx[i] = f.x;
       fy[i] = f.y;
       ax.push_back(f.x);
@@ -266,7 +204,7 @@ static int create(VSGContext* c, int device, int width, int height, int blurSize
      search window, plus column overrun into the next row */
   c->rowsAlloc = height + 4 * maxStep + 16;
   c->slotStride = (size_t)c->pitch * c->rowsAlloc;
-  const int slots = 2 * B;
+  const int slots = VSG_NSETS * B;
   VSG_CHECK(dalloc(&c->d_raw, (size_t)slots * c->pitch * height));
   VSG_CHECK(dalloc(&c->d_blur, (size_t)slots * c->slotStride));
   VSG_CHECK(dalloc(&c->d_ref, c->slotStride));
@@ -281,7 +219,7 @@ static int create(VSGContext* c, int device, int width, int height, int blurSize
   VSG_CHECK(dalloc(&c->d_patches, (size_t)slots * c->patchSlotStride));
   VSG_CHECK(dalloc(&c->d_minmax, (size_t)slots * c->nfTotal * 2));
   const int nc = c->st[0].nf, nf = c->st[1].nf, nmax = nc > nf ? nc : nf;
-  for (Set& s : c->set) {
+  for (VSGSet& s : c->set) {
     VSG_CHECK(dalloc(&s.d_selIdx, (size_t)B * c->nfTotal));
     VSG_CHECK(dalloc(&s.d_selC, (size_t)B * c->nfTotal));
     VSG_CHECK(dalloc(&s.d_count, (size_t)B * 2));
@@ -315,11 +253,11 @@ static int create(VSGContext* c, int device, int width, int height, int blurSize
 extern "C" int vsg_create(VSGContext** out, int device, int width, int height, int blurSize,
                           const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch) {
   if (!out || !coarse || !fine || width <= 0 || height <= 0 || maxBatch <= 0)
-    return fail("vsg_create", hipSuccess);
+    return vsg_fail("vsg_create", hipSuccess);
   *out = nullptr;
   VSGContext* c = new (std::nothrow) VSGContext();
-  if (!c) return fail("vsg_create: out of host memory", hipSuccess);
-  if (create(c, device, width, height, blurSize, coarse, fine, maxBatch) != 0) {
+  if (!c) return vsg_fail("vsg_create: out of host memory", hipSuccess);
+  if (vsg_ctx_init(c, device, width, height, blurSize, coarse, fine, maxBatch, nullptr) != 0) {
     char keep[sizeof(g_err)];
     memcpy(keep, g_err, sizeof(keep));
     free_ctx(c);
@@ -332,9 +270,9 @@ extern "C" int vsg_create(VSGContext** out, int device, int width, int height, i
 
 /* ---- batches ------------------------------------------------------------------ */
 
-static VSGSearchArgs search_args(VSGContext* c, int stage, const VSGJob* jobs, int njobs,
-                                 unsigned long long* best, VSGResult* res) {
-  const Stage& S = c->st[stage];
+VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, int njobs,
+                                  unsigned long long* best, VSGResult* res) {
+  const VSGStage& S = c->st[stage];
   VSGSearchArgs a;
   a.jobs = jobs;
   a.njobs = njobs;
@@ -349,51 +287,26 @@ static VSGSearchArgs search_args(VSGContext* c, int stage, const VSGJob* jobs, i
   a.stageOff = S.patchOff;
   a.g = S.g;
   a.tiles = S.tiles;
+  /* one frame's worth of jobs per XCD chunk: its planes stay in one L2 */
+  a.chunk = S.nf * S.tiles;
   a.best = best;
   a.results = res;
   return a;
 }
 
-extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* luma, const int* linesize,
-                          int onDevice, const int* prevSlot, const uint8_t* detect, int refSlot) {
-  if (!c || set < 0 || set > 1 || n <= 0 || n > c->B || !luma || !linesize || !prevSlot || !detect)
-    return fail("vsg_submit", hipSuccess);
-  if (refSlot >= 2 * c->B) return fail("vsg_submit: refSlot", hipSuccess);
-  Set& s = c->set[set];
+/* the pinned per-frame descriptors of set `set` (h_src / h_prev / h_detect)
+   must be filled for n frames; enqueues their upload, the blur, the field pass
+   and the selection */
+int vsg_ctx_enqueue_front(VSGContext* c, int set, int n) {
+  VSGSet& s = c->set[set];
   const int slot0 = set * c->B;
-  VSG_CHECK(hipSetDevice(c->device));
-  /* this set's pinned buffers are reused below: wait until the previous batch
-     of the set (its coarse read-back and its fine stage) has completed */
-  VSG_CHECK(hipEventSynchronize(s.done));
-  VSG_CHECK(hipEventSynchronize(s.doneF));
-  for (int i = 0; i < n; ++i) {
-    if (linesize[i] < c->W) return fail("vsg_submit: linesize < width", hipSuccess);
-    if (prevSlot[i] >= 2 * c->B) return fail("vsg_submit: prevSlot", hipSuccess);
-    if (onDevice) {
-      s.h_src[i].p = luma[i];
-      s.h_src[i].pitch = linesize[i];
-    } else {
-      uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * c->pitch * c->H;
-      VSG_CHECK(hipMemcpy2DAsync(dst, c->pitch, luma[i], linesize[i], c->W, c->H, hipMemcpyHostToDevice,
-                                 c->stream));
-      s.h_src[i].p = dst;
-      s.h_src[i].pitch = c->pitch;
-    }
-    s.h_src[i].pad = 0;
-    s.h_prev[i] = prevSlot[i];
-    s.h_detect[i] = detect[i] ? 1 : 0;
-  }
   VSG_CHECK(hipMemcpyAsync(s.d_src, s.h_src, sizeof(VSGBlurSrc) * n, hipMemcpyHostToDevice, c->stream));
   VSG_CHECK(hipMemcpyAsync(s.d_prev, s.h_prev, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
   VSG_CHECK(hipMemcpyAsync(s.d_detect, s.h_detect, n, hipMemcpyHostToDevice, c->stream));
-  s.prof = g_profOn.load() != 0;
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_BLUR0], c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR0], c->stream));
   VSG_CHECK(vsg_launch_blur(s.d_src, n, c->d_blur + (size_t)slot0 * c->slotStride, c->slotStride, c->pitch,
                             c->W, c->H, c->blurSize, c->d_tmp, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_BLUR1], c->stream));
-  if (refSlot >= 0)
-    VSG_CHECK(hipMemcpyAsync(c->d_ref, c->d_blur + (size_t)refSlot * c->slotStride, c->slotStride,
-                             hipMemcpyDeviceToDevice, c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR1], c->stream));
   VSGFieldArgs fa;
   fa.blurred = c->d_blur + (size_t)slot0 * c->slotStride;
   fa.slotStride = c->slotStride;
@@ -407,7 +320,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   fa.patchSlotStride = c->patchSlotStride;
   fa.minmax = c->d_minmax + (size_t)slot0 * c->nfTotal * 2;
   VSG_CHECK(vsg_launch_fields(fa, n, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_FIELDS1], c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_FIELDS1], c->stream));
   VSGSelectArgs sa;
   sa.minmax = c->d_minmax;
   sa.nfTotal = c->nfTotal;
@@ -427,13 +340,13 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   sa.count = s.d_count;
   sa.jobs = s.d_jobsC;
   VSG_CHECK(vsg_launch_select(sa, n, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_SELECT1], c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_SELECT1], c->stream));
+  return 0;
+}
+
+int vsg_ctx_enqueue_coarse_readback(VSGContext* c, int set, int n) {
+  VSGSet& s = c->set[set];
   const int nc = c->st[0].nf;
-  VSGSearchArgs ga = search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_resC);
-  VSG_CHECK(vsg_launch_grid(ga, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_GRID1], c->stream));
-  VSG_CHECK(vsg_launch_refine(ga, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[EV_REFINE1], c->stream));
   VSG_CHECK(hipMemcpyAsync(s.h_count, s.d_count, sizeof(int) * 2 * n, hipMemcpyDeviceToHost, c->stream));
   VSG_CHECK(hipMemcpyAsync(s.h_selIdx, s.d_selIdx, sizeof(int16_t) * (size_t)n * c->nfTotal,
                            hipMemcpyDeviceToHost, c->stream));
@@ -441,28 +354,74 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
                            hipMemcpyDeviceToHost, c->stream));
   VSG_CHECK(hipMemcpyAsync(s.h_resC, s.d_resC, sizeof(VSGResult) * (size_t)n * nc, hipMemcpyDeviceToHost,
                            c->stream));
+  return 0;
+}
+
+extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* luma, const int* linesize,
+                          int onDevice, const int* prevSlot, const uint8_t* detect, int refSlot) {
+  if (!c || set < 0 || set >= VSG_NSETS || n <= 0 || n > c->B || !luma || !linesize || !prevSlot || !detect)
+    return vsg_fail("vsg_submit", hipSuccess);
+  if (refSlot >= VSG_NSETS * c->B) return vsg_fail("vsg_submit: refSlot", hipSuccess);
+  VSGSet& s = c->set[set];
+  const int slot0 = set * c->B;
+  VSG_CHECK(hipSetDevice(c->device));
+  /* this set's pinned buffers are reused below: wait until the previous batch
+     of the set (its coarse read-back and its fine stage) has completed */
+  VSG_CHECK(hipEventSynchronize(s.done));
+  VSG_CHECK(hipEventSynchronize(s.doneF));
+  for (int i = 0; i < n; ++i) {
+    if (linesize[i] < c->W) return vsg_fail("vsg_submit: linesize < width", hipSuccess);
+    if (prevSlot[i] >= VSG_NSETS * c->B) return vsg_fail("vsg_submit: prevSlot", hipSuccess);
+    if (onDevice) {
+      s.h_src[i].p = luma[i];
+      s.h_src[i].pitch = linesize[i];
+    } else {
+      uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * c->pitch * c->H;
+      VSG_CHECK(hipMemcpy2DAsync(dst, c->pitch, luma[i], linesize[i], c->W, c->H, hipMemcpyHostToDevice,
+                                 c->stream));
+      s.h_src[i].p = dst;
+      s.h_src[i].pitch = c->pitch;
+    }
+    s.h_src[i].pad = 0;
+    s.h_prev[i] = prevSlot[i];
+    s.h_detect[i] = detect[i] ? 1 : 0;
+  }
+  s.prof = g_profOn.load() != 0;
+  if (vsg_ctx_enqueue_front(c, set, n) != 0) return -1;
+  if (refSlot >= 0) {
+    /* the frozen tripod plane: after the blur, before any search reads it */
+    VSG_CHECK(hipMemcpyAsync(c->d_ref, c->d_blur + (size_t)refSlot * c->slotStride, c->slotStride,
+                             hipMemcpyDeviceToDevice, c->stream));
+  }
+  const int nc = c->st[0].nf;
+  VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_resC);
+  VSG_CHECK(vsg_launch_grid(ga, c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID1], c->stream));
+  VSG_CHECK(vsg_launch_refine(ga, c->stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_REFINE1], c->stream));
+  if (vsg_ctx_enqueue_coarse_readback(c, set, n) != 0) return -1;
   VSG_CHECK(hipEventRecord(s.done, c->stream));
   s.n = n;
   return 0;
 }
 
 extern "C" int vsg_wait_coarse(VSGContext* c, int set, VSGBatchView* view) {
-  if (!c || set < 0 || set > 1 || !view) return fail("vsg_wait_coarse", hipSuccess);
-  Set& s = c->set[set];
+  if (!c || set < 0 || set >= VSG_NSETS || !view) return vsg_fail("vsg_wait_coarse", hipSuccess);
+  VSGSet& s = c->set[set];
   VSG_CHECK(hipSetDevice(c->device));
   VSG_CHECK(hipEventSynchronize(s.done));
   if (s.prof) {
     s.prof = false;
     double fieldBytes = 0;
-    for (const Stage& S : c->st) fieldBytes += (double)S.nf * S.S * S.S;
+    for (const VSGStage& S : c->st) fieldBytes += (double)S.nf * S.S * S.S;
     int ncoarse = 0;
     for (int i = 0; i < s.n; ++i) ncoarse += s.h_count[2 * i];
-    prof_add(VSG_K_BLUR, s.ev[EV_BLUR0], s.ev[EV_BLUR1], 2.0 * c->W * c->H * s.n);
-    prof_add(VSG_K_FIELDS, s.ev[EV_BLUR1], s.ev[EV_FIELDS1], 2.0 * fieldBytes * s.n);
-    prof_add(VSG_K_SELECT, s.ev[EV_FIELDS1], s.ev[EV_SELECT1], (double)c->nfTotal * s.n);
+    prof_add(VSG_K_BLUR, s.ev[VSG_EV_BLUR0], s.ev[VSG_EV_BLUR1], 2.0 * c->W * c->H * s.n);
+    prof_add(VSG_K_FIELDS, s.ev[VSG_EV_BLUR1], s.ev[VSG_EV_FIELDS1], 2.0 * fieldBytes * s.n);
+    prof_add(VSG_K_SELECT, s.ev[VSG_EV_FIELDS1], s.ev[VSG_EV_SELECT1], (double)c->nfTotal * s.n);
     const double px = (double)c->st[0].S * c->st[0].S;
-    prof_add(VSG_K_GRID_COARSE, s.ev[EV_SELECT1], s.ev[EV_GRID1], ncoarse * c->st[0].candsPerJob * px);
-    prof_add(VSG_K_REFINE_COARSE, s.ev[EV_GRID1], s.ev[EV_REFINE1], ncoarse * c->st[0].refinePerJob * px);
+    prof_add(VSG_K_GRID_COARSE, s.ev[VSG_EV_SELECT1], s.ev[VSG_EV_GRID1], ncoarse * c->st[0].candsPerJob * px);
+    prof_add(VSG_K_REFINE_COARSE, s.ev[VSG_EV_GRID1], s.ev[VSG_EV_REFINE1], ncoarse * c->st[0].refinePerJob * px);
   }
   view->count = s.h_count;
   view->idx = s.h_selIdx;
@@ -474,14 +433,14 @@ extern "C" int vsg_wait_coarse(VSGContext* c, int set, VSGBatchView* view) {
 }
 
 extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* jobs) {
-  if (!c || set < 0 || set > 1 || njobs < 0) return fail("vsg_submit_fine", hipSuccess);
-  Set& s = c->set[set];
-  const Stage& S = c->st[1];
-  if (njobs > c->B * S.nf) return fail("vsg_submit_fine: too many jobs", hipSuccess);
+  if (!c || set < 0 || set >= VSG_NSETS || njobs < 0) return vsg_fail("vsg_submit_fine", hipSuccess);
+  VSGSet& s = c->set[set];
+  const VSGStage& S = c->st[1];
+  if (njobs > c->B * S.nf) return vsg_fail("vsg_submit_fine: too many jobs", hipSuccess);
   for (int i = 0; i < njobs; ++i) {
     const VSGJob& j = jobs[i];
-    if (j.slot < 0 || j.slot >= 2 * c->B || j.prev >= 2 * c->B || j.field >= S.nf)
-      return fail("vsg_submit_fine: job out of range", hipSuccess);
+    if (j.slot < 0 || j.slot >= VSG_NSETS * c->B || j.prev >= VSG_NSETS * c->B || j.field >= S.nf)
+      return vsg_fail("vsg_submit_fine: job out of range", hipSuccess);
   }
   VSG_CHECK(hipSetDevice(c->device));
   s.njobsF = njobs;
@@ -490,38 +449,38 @@ extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* 
     VSG_CHECK(hipEventRecord(s.doneF, c->stream));
     return 0;
   }
-  memcpy(s.h_jobsF, jobs, sizeof(VSGJob) * njobs);
+  if (jobs != s.h_jobsF) memcpy(s.h_jobsF, jobs, sizeof(VSGJob) * njobs);
   VSG_CHECK(hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, c->stream));
   s.profF = g_profOn.load() != 0;
-  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[EVF_0], c->stream));
-  VSGSearchArgs fa = search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_resF);
+  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_0], c->stream));
+  VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_resF);
   VSG_CHECK(vsg_launch_grid(fa, c->stream));
-  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[EVF_GRID1], c->stream));
+  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_GRID1], c->stream));
   VSG_CHECK(vsg_launch_refine(fa, c->stream));
-  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[EVF_REFINE1], c->stream));
+  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_REFINE1], c->stream));
   VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, c->stream));
   VSG_CHECK(hipEventRecord(s.doneF, c->stream));
   return 0;
 }
 
 extern "C" int vsg_wait_fine(VSGContext* c, int set, const VSGResult** results) {
-  if (!c || set < 0 || set > 1 || !results) return fail("vsg_wait_fine", hipSuccess);
-  Set& s = c->set[set];
+  if (!c || set < 0 || set >= VSG_NSETS || !results) return vsg_fail("vsg_wait_fine", hipSuccess);
+  VSGSet& s = c->set[set];
   VSG_CHECK(hipSetDevice(c->device));
   VSG_CHECK(hipEventSynchronize(s.doneF));
   if (s.profF) {
     s.profF = false;
-    const Stage& S = c->st[1];
+    const VSGStage& S = c->st[1];
     const double px = (double)S.S * S.S;
-    prof_add(VSG_K_GRID_FINE, s.evf[EVF_0], s.evf[EVF_GRID1], s.njobsF * S.candsPerJob * px);
-    prof_add(VSG_K_REFINE_FINE, s.evf[EVF_GRID1], s.evf[EVF_REFINE1], s.njobsF * S.refinePerJob * px);
+    prof_add(VSG_K_GRID_FINE, s.evf[VSG_EVF_0], s.evf[VSG_EVF_GRID1], s.njobsF * S.candsPerJob * px);
+    prof_add(VSG_K_REFINE_FINE, s.evf[VSG_EVF_GRID1], s.evf[VSG_EVF_REFINE1], s.njobsF * S.refinePerJob * px);
   }
   *results = s.h_resF;
   return 0;
 }
 
 extern "C" int vsg_read_blurred(VSGContext* c, int slot, uint8_t* dst, int dstStride) {
-  if (!c || slot >= 2 * c->B || !dst || dstStride < c->W) return fail("vsg_read_blurred", hipSuccess);
+  if (!c || slot >= VSG_NSETS * c->B || !dst || dstStride < c->W) return vsg_fail("vsg_read_blurred", hipSuccess);
   VSG_CHECK(hipSetDevice(c->device));
   const uint8_t* src = slot < 0 ? c->d_ref : c->d_blur + (size_t)slot * c->slotStride;
   VSG_CHECK(hipMemcpy2DAsync(dst, dstStride, src, c->pitch, c->W, c->H, hipMemcpyDeviceToHost, c->stream));

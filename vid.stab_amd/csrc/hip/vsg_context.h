@@ -1,0 +1,95 @@
+/*
+ * vsg_context.h -- the device state of one detector (one clip): buffers,
+ * the two slot sets and their pinned host twins.  Shared by the single-clip
+ * ABI (vsg_api.hip) and the multi-clip batched form (vsg_multi.hip).
+ */
+#ifndef VSG_CONTEXT_H
+#define VSG_CONTEXT_H
+
+#include "vsg_internal.h"
+
+enum { VSG_EV_BLUR0, VSG_EV_BLUR1, VSG_EV_FIELDS1, VSG_EV_SELECT1, VSG_EV_GRID1, VSG_EV_REFINE1, VSG_EV_COUNT };
+enum { VSG_EVF_0, VSG_EVF_GRID1, VSG_EVF_REFINE1, VSG_EVF_COUNT };
+
+struct VSGStage {
+  int S = 0, maxShift = 0, step = 0, nf = 0, rows = 0, maxFields = 0;
+  double ct = 0;
+  int16_t* d_fx = nullptr;
+  int16_t* d_fy = nullptr;
+  VSGGrid g{};
+  int tiles = 0;
+  size_t patchOff = 0;
+  double candsPerJob = 0;     /* (2n+1)^2 */
+  double refinePerJob = 0;    /* refinement candidates the reference visits */
+};
+
+struct VSGSet {
+  /* device */
+  int16_t* d_selIdx = nullptr;
+  double* d_selC = nullptr;
+  int* d_count = nullptr;
+  VSGJob* d_jobsC = nullptr;
+  VSGJob* d_jobsF = nullptr;
+  unsigned long long* d_best = nullptr;
+  VSGResult* d_resC = nullptr;
+  VSGResult* d_resF = nullptr;
+  VSGBlurSrc* d_src = nullptr;
+  int* d_prev = nullptr;
+  uint8_t* d_detect = nullptr;
+  /* pinned host */
+  int16_t* h_selIdx = nullptr;
+  double* h_selC = nullptr;
+  int* h_count = nullptr;
+  VSGResult* h_resC = nullptr;
+  VSGJob* h_jobsF = nullptr;
+  VSGResult* h_resF = nullptr;
+  VSGBlurSrc* h_src = nullptr;
+  int* h_prev = nullptr;
+  uint8_t* h_detect = nullptr;
+  /* state */
+  int n = 0;              /* frames in flight */
+  int njobsF = 0;
+  hipEvent_t done = nullptr, doneF = nullptr;
+  hipEvent_t ev[VSG_EV_COUNT] = {};
+  hipEvent_t evf[VSG_EVF_COUNT] = {};
+  bool prof = false, profF = false;
+};
+
+struct VSGContext {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool ownStream = true;
+  int W = 0, H = 0, pitch = 0, rowsAlloc = 0;
+  size_t slotStride = 0;
+  int blurSize = 0;
+  int B = 0;
+  VSGStage st[2];
+  int nfTotal = 0;
+  int16_t* d_allx = nullptr;
+  int16_t* d_ally = nullptr;
+  int16_t* d_allsz = nullptr;
+  uint32_t* d_patchOff = nullptr;
+  double* d_ctab = nullptr;
+  uint8_t* d_raw = nullptr;
+  uint8_t* d_blur = nullptr;
+  uint8_t* d_ref = nullptr;
+  uint8_t* d_tmp = nullptr;
+  uint8_t* d_patches = nullptr;
+  size_t patchSlotStride = 0;
+  uint8_t* d_minmax = nullptr;
+  VSGSet set[VSG_NSETS];
+};
+
+/* creation / destruction shared by both ABIs (vsg_api.hip); a null stream
+   argument creates a private stream */
+int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize, const VSGStageDesc* coarse,
+                 const VSGStageDesc* fine, int maxBatch, hipStream_t shared);
+void vsg_ctx_free(VSGContext* c);
+/* enqueue helpers (no host synchronisation) */
+int vsg_ctx_enqueue_front(VSGContext* c, int set, int n);      /* H2D descr., blur, fields, select */
+int vsg_ctx_enqueue_coarse_readback(VSGContext* c, int set, int n);
+VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, int njobs,
+                                  unsigned long long* best, VSGResult* res);
+int vsg_fail(const char* what, hipError_t e);
+
+#endif /* VSG_CONTEXT_H */

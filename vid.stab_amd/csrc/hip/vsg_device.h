@@ -1,0 +1,113 @@
+/*
+ * vsg_device.h -- device-side building blocks shared by the per-job kernels
+ * (vsg_kernels.hip) and the persistent multi-clip kernels (vsg_multi.hip).
+ */
+#ifndef VSG_DEVICE_H
+#define VSG_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vsg_common.h"
+
+typedef const __attribute__((address_space(4))) uint32_t* vsg_cword_ptr;   /* scalar loads */
+typedef const __attribute__((address_space(1))) uint32_t* vsg_gword_ptr;   /* global loads */
+
+/* one field's search: geometry of its candidate grid and where its planes are */
+struct VSGTile {
+  const uint8_t* prev;     /* previous blurred plane */
+  int pitch;               /* bytes per row of prev (multiple of 64) */
+  vsg_cword_ptr patch;     /* the current field, S x S, 4-byte aligned */
+  int S, s, n, m, U, V;    /* field size, grid step, half width, lane layout */
+  int fxc, fyc;            /* field centre */
+  int ox, oy;              /* search offset */
+};
+
+/* ---- SAD grid search, one lane ------------------------------------------------
+   Lane lid = (u, v) of a job owns the candidates a = a0 + m*j (j < J) and
+   b = b0 + k (k < K) of the (2n+1)^2 grid, a0 = -n + (u % m) + m*J*(u / m),
+   b0 = -n + K*v.  m*step is a multiple of 4, so the J candidates of a lane
+   sit D = m*step/4 words apart in the same prev row: one load of a prev row
+   segment feeds J*K candidates, each prev word is shifted into the field's
+   alignment once (v_alignbyte), and the current field's words come in as
+   wave-uniform scalar loads (SGPR operand of v_sad_u8).  Iterating over prev
+   rows p, candidate k pairs prev row p with field row p - k*step.
+   Returns the lane's best (sad << 32 | spiral rank), or ~0 for idle lanes. */
+template <int J, int K, int D, int CW>
+__device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int lid) {
+  const int S = t.S, s = t.s, n = t.n, m = t.m;
+  const int u = lid % t.U;
+  int v = lid / t.U;
+  const bool laneOk = v < t.V;
+  if (!laneOk) v = t.V - 1;
+  const int a0 = -n + (u % m) + m * J * (u / m);
+  const int b0 = -n + K * v;
+  const int prow0 = t.fyc - S / 2 + b0 * s + t.oy;
+  const int pcol0 = t.fxc - S / 2 + a0 * s + t.ox;
+  const int wordsPerRow = S >> 2;
+  const int pitchWords = t.pitch >> 2;
+
+  uint32_t acc[K][J];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int j = 0; j < J; ++j) acc[k][j] = 0;
+
+  constexpr int NP = CW + D * (J - 1);   /* shifted prev words per row segment */
+  const int nchunks = wordsPerRow / CW;
+  const int prows = S + (K - 1) * s;
+  for (int c = 0; c < nchunks; ++c) {
+    const uintptr_t start = reinterpret_cast<uintptr_t>(t.prev + (size_t)prow0 * t.pitch + pcol0 + 4 * CW * c);
+    vsg_gword_ptr wp = (vsg_gword_ptr)(start & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(start & 3);
+    vsg_cword_ptr cbase = t.patch + CW * c;
+    for (int p = 0; p < prows; ++p) {
+      uint32_t raw[NP + 1];
+#pragma unroll
+      for (int q = 0; q <= NP; ++q) raw[q] = wp[q];
+      uint32_t P[NP];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) P[q] = __builtin_amdgcn_alignbyte(raw[q + 1], raw[q], sh);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int y = p - k * s;
+        if (y >= 0 && y < S) {
+          vsg_cword_ptr crow = cbase + y * wordsPerRow;
+#pragma unroll
+          for (int w = 0; w < CW; ++w) {
+            const uint32_t cw = crow[w];
+#pragma unroll
+            for (int j = 0; j < J; ++j) acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw, acc[k][j]);
+          }
+        }
+      }
+      wp += pitchWords;
+    }
+  }
+
+  unsigned long long best = ~0ull;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int ga = a0 + m * j, gb = b0 + k;
+      if (laneOk && ga <= n && gb <= n) {
+        const unsigned long long key = ((unsigned long long)acc[k][j] << 32) | (uint32_t)vsg_rank(ga, gb);
+        best = key < best ? key : best;
+      }
+    }
+  return best;
+}
+
+/* minimum over the 64 lanes of a wave */
+__device__ __forceinline__ unsigned long long vsg_wave_min_u64(unsigned long long best) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)best, off);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(best >> 32), off);
+    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+    best = o < best ? o : best;
+  }
+  return best;
+}
+
+#endif /* VSG_DEVICE_H */

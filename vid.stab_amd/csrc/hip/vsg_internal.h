@@ -56,6 +56,7 @@ struct VSGSearchArgs {
   size_t stageOff;
   VSGGrid g;
   int tiles;                    /* workgroups per job in the grid kernel */
+  int chunk;                    /* (job, tile) units kept together on one XCD */
   unsigned long long* best;     /* per job: (sad << 32) | spiral rank; ~0 between launches */
   VSGResult* results;
 };

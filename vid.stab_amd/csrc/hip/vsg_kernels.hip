@@ -1,20 +1,17 @@
 /*
- * vsg_kernels.hip -- gfx950 kernels of the detect pass.
+ * vsg_kernels.hip -- gfx950 kernels of the detect pass that precede the
+ * field searches (the searches are in vsg_search.hip).
  *
  *   k_blur        separable integer box blur, row pass and column pass through
  *                 LDS in one kernel (reference: src/boxblur.c:190-332)
  *   k_fields      per-field 8-bit min/max (src/motiondetect.c:389-407) and an
  *                 aligned copy of the field for the SAD kernels
- *   k_sad_grid    exhaustive grid search of the reference's spiral
- *                 (src/motiondetect.c:466-518, SAD :971-996): candidate shifts
- *                 spread over lanes, packed-u8 SAD (v_sad_u8) with the current
- *                 field as a wave-uniform operand, wave argmin on
- *                 (sad, spiral rank), one 64-bit atomic min per wave
- *   k_sad_refine  the dyadic refinement around the grid winner, replayed in
- *                 the reference's visit order (src/motiondetect.c:545-567)
+ *   k_select      selectfields (src/motiondetect.c:676-743) for every frame
+ *                 and both field grids, writing the coarse job lists
  *
- * No MFMA: SAD is not a dense contraction.  All arithmetic is integer.
+ * No MFMA: none of this is a dense contraction.  All arithmetic is integer.
  */
+#include "vsg_device.h"
 #include "vsg_internal.h"
 
 namespace {
@@ -40,12 +37,25 @@ __global__ __launch_bounds__(256) void k_blur(const VSGBlurSrc* __restrict__ src
   uint8_t* sIn = lds;
   uint8_t* sHor = lds + rowsIn * inPitch;
 
-  /* stage the input tile with edge replication */
-  for (int i = threadIdx.x; i < rowsIn * colsIn; i += 256) {
-    int r = i / colsIn, c = i - r * colsIn;
-    int gy = min(max(y0 - s2 + r, 0), H - 1);
-    int gx = min(max(x0 - s2 + c, 0), W - 1);
-    sIn[r * inPitch + c] = src.p[(size_t)gy * src.pitch + gx];
+  /* stage the input tile: interior tiles with dword loads (any source
+     alignment: gfx950 global loads take byte-aligned addresses), border tiles
+     byte by byte with edge replication */
+  const int nw = inPitch >> 2;
+  if (x0 - s2 >= 0 && x0 - s2 + inPitch <= W && y0 - s2 >= 0 && y0 + kBlurTH + s2 <= H) {
+    typedef uint32_t __attribute__((aligned(1))) u32_any;
+    const uint8_t* base = src.p + (size_t)(y0 - s2) * src.pitch + (x0 - s2);
+    for (int i = threadIdx.x; i < rowsIn * nw; i += 256) {
+      const int r = i / nw, q = i - r * nw;
+      *reinterpret_cast<uint32_t*>(sIn + r * inPitch + 4 * q) =
+          *reinterpret_cast<const u32_any*>(base + (size_t)r * src.pitch + 4 * q);
+    }
+  } else {
+    for (int i = threadIdx.x; i < rowsIn * colsIn; i += 256) {
+      int r = i / colsIn, c = i - r * colsIn;
+      int gy = min(max(y0 - s2 + r, 0), H - 1);
+      int gx = min(max(x0 - s2 + c, 0), W - 1);
+      sIn[r * inPitch + c] = src.p[(size_t)gy * src.pitch + gx];
+    }
   }
   __syncthreads();
 
@@ -169,204 +179,6 @@ __global__ __launch_bounds__(256) void k_fields(VSGFieldArgs a) {
   }
 }
 
-/* ---- SAD grid search ---------------------------------------------------------
-   Lane (u, v) of a job owns the candidates a = a0 + m*j (j < J) and
-   b = b0 + k (k < K) of the (2n+1)^2 grid, a0 = -n + (u % m) + m*J*(u / m),
-   b0 = -n + K*v.  m*step is a multiple of 4, so the J candidates of a lane
-   sit D = m*step/4 words apart in the same prev row: one load of a prev row
-   segment feeds J*K candidates, each prev word is shifted into the field's
-   alignment once (v_alignbyte), and the current field's words come in as
-   wave-uniform scalar loads (SGPR operand of v_sad_u8).  Iterating over prev
-   rows p, candidate k pairs prev row p with field row p - k*step. */
-
-typedef const __attribute__((address_space(4))) uint32_t* cword_ptr;   /* scalar loads */
-typedef const __attribute__((address_space(1))) uint32_t* gword_ptr;   /* global loads */
-
-template <int J, int K, int D, int CW>
-__global__ __launch_bounds__(256) void k_sad_grid(VSGSearchArgs a) {
-  const int job = blockIdx.y;
-  const VSGJob jb = a.jobs[job];
-  if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
-  const int S = a.g.S, s = a.g.step, n = a.g.n, m = a.g.m, U = a.g.U, V = a.g.V;
-  const int lid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int u = lid % U;
-  int v = lid / U;
-  const bool laneOk = v < V;
-  if (!laneOk) v = V - 1;
-  const int a0 = -n + (u % m) + m * J * (u / m);
-  const int b0 = -n + K * v;
-  const int fxc = a.fx[jb.field], fyc = a.fy[jb.field];
-  const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
-  cword_ptr patch = (cword_ptr)(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
-                                (size_t)jb.field * S * S);
-  const int prow0 = fyc - S / 2 + b0 * s + jb.oy;
-  const int pcol0 = fxc - S / 2 + a0 * s + jb.ox;
-  const int wordsPerRow = S >> 2;
-  const int pitchWords = a.pitch >> 2;
-
-  uint32_t acc[K][J];
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-#pragma unroll
-    for (int j = 0; j < J; ++j) acc[k][j] = 0;
-
-  constexpr int NP = CW + D * (J - 1);   /* shifted prev words per row segment */
-  const int nchunks = wordsPerRow / CW;
-  const int prows = S + (K - 1) * s;
-  for (int c = 0; c < nchunks; ++c) {
-    const uintptr_t start = reinterpret_cast<uintptr_t>(prev + (size_t)prow0 * a.pitch + pcol0 + 4 * CW * c);
-    gword_ptr wp = (gword_ptr)(start & ~(uintptr_t)3);
-    const uint32_t sh = (uint32_t)(start & 3);
-    cword_ptr cbase = patch + CW * c;
-    for (int p = 0; p < prows; ++p) {
-      uint32_t raw[NP + 1];
-#pragma unroll
-      for (int q = 0; q <= NP; ++q) raw[q] = wp[q];
-      uint32_t P[NP];
-#pragma unroll
-      for (int q = 0; q < NP; ++q) P[q] = __builtin_amdgcn_alignbyte(raw[q + 1], raw[q], sh);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int y = p - k * s;
-        if (y >= 0 && y < S) {
-          cword_ptr crow = cbase + y * wordsPerRow;
-#pragma unroll
-          for (int w = 0; w < CW; ++w) {
-            const uint32_t cw = crow[w];
-#pragma unroll
-            for (int j = 0; j < J; ++j) acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw, acc[k][j]);
-          }
-        }
-      }
-      wp += pitchWords;
-    }
-  }
-
-  unsigned long long best = ~0ull;
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int ga = a0 + m * j, gb = b0 + k;
-      if (laneOk && ga <= n && gb <= n) {
-        unsigned long long key = ((unsigned long long)acc[k][j] << 32) | (uint32_t)vsg_rank(ga, gb);
-        best = key < best ? key : best;
-      }
-    }
-  for (int off = 32; off > 0; off >>= 1) {
-    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)best, off);
-    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(best >> 32), off);
-    unsigned long long o = ((unsigned long long)hi << 32) | lo;
-    best = o < best ? o : best;
-  }
-  if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(&a.best[job], best);
-}
-
-/* ---- refinement --------------------------------------------------------------
-   One workgroup per job.  A level's candidates form an ni x ni grid (spacing
-   ns around the current best); an item is one (candidate row, field row)
-   pair, whose prev row serves all ni candidates of that row: 16-byte loads of
-   the prev row at each candidate's byte offset (gfx950 global loads accept any
-   byte alignment) against 16-byte loads of the aligned field patch, four
-   v_sad_u8 per load.  Row sums meet in LDS; then one lane scans the
-   candidates in the reference's order (i outer, j inner, centre skipped) with
-   the strict "<" against the running minimum, which is exactly the
-   sequential loop. */
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef u32x4 __attribute__((aligned(1))) u32x4_any;
-
-__global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
-  constexpr int kMaxNi = 5;   /* ni = 2r/ns + 1 <= 5 for every step */
-  __shared__ uint32_t sads[kMaxNi * kMaxNi];
-  __shared__ int shTx, shTy;
-  __shared__ uint32_t shMin;
-  const int job = blockIdx.x;
-  const VSGJob jb = a.jobs[job];
-  if (jb.field < 0) return;
-  const int S = a.g.S, w16 = S >> 4;
-  const int fxc = a.fx[jb.field], fyc = a.fy[jb.field];
-  const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
-  const u32x4* patch = reinterpret_cast<const u32x4*>(a.patches + (size_t)jb.slot * a.patchSlotStride +
-                                                      a.stageOff + (size_t)jb.field * S * S);
-  if (threadIdx.x == 0) {
-    unsigned long long key = a.best[job];
-    int ga, gb;
-    vsg_unrank((int)(uint32_t)key, &ga, &gb);
-    shTx = ga * a.g.step;
-    shTy = gb * a.g.step;
-    shMin = (uint32_t)(key >> 32);
-  }
-  __syncthreads();
-  int tx = shTx, ty = shTy;
-  uint32_t minerr = shMin;
-  int st = a.g.step;
-  while (st > 1) {
-    const int ns = st / 2, r = st - ns;
-    const int ni = (2 * r) / ns + 1;
-    if (threadIdx.x < kMaxNi * kMaxNi) sads[threadIdx.x] = 0;
-    __syncthreads();
-    const int col0 = fxc - S / 2 + tx - r + jb.ox;
-    for (int item = threadIdx.x; item < ni * S; item += 256) {
-      const int tj = item / S, y = item - tj * S;
-      const int j = ty - r + ns * tj;
-      const uint8_t* prow = prev + (size_t)(fyc - S / 2 + y + j + jb.oy) * a.pitch + col0;
-      const u32x4* crow = patch + y * w16;
-      uint32_t acc[kMaxNi] = {0, 0, 0, 0, 0};
-      for (int w = 0; w < w16; ++w) {
-        const u32x4 c = crow[w];
-#pragma unroll
-        for (int t = 0; t < kMaxNi; ++t) {
-          if (t < ni) {
-            const u32x4 p = *reinterpret_cast<const u32x4_any*>(prow + ns * t + 16 * w);
-            uint32_t s = acc[t];
-            s = __builtin_amdgcn_sad_u8(p.x, c.x, s);
-            s = __builtin_amdgcn_sad_u8(p.y, c.y, s);
-            s = __builtin_amdgcn_sad_u8(p.z, c.z, s);
-            acc[t] = __builtin_amdgcn_sad_u8(p.w, c.w, s);
-          }
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < kMaxNi; ++t)
-        if (t < ni) atomicAdd(&sads[t * ni + tj], acc[t]);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int btx = tx, bty = ty;
-      for (int ti = 0; ti < ni; ++ti)
-        for (int tj = 0; tj < ni; ++tj) {
-          const int i = tx - r + ns * ti, j = ty - r + ns * tj;
-          if (i == tx && j == ty) continue;
-          const uint32_t e = sads[ti * ni + tj];
-          if (e < minerr) { minerr = e; btx = i; bty = j; }
-        }
-      shTx = btx;
-      shTy = bty;
-      shMin = minerr;
-    }
-    __syncthreads();
-    tx = shTx;
-    ty = shTy;
-    minerr = shMin;
-    st /= 2;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    VSGResult res;
-    res.tx = (int16_t)tx;
-    res.ty = (int16_t)ty;
-    res.minerror = minerr;
-    a.results[job] = res;
-    a.best[job] = ~0ull;   /* ready for the next launch */
-  }
-}
-
-template <int J, int K, int D, int CW>
-hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL((k_sad_grid<J, K, D, CW>), dim3(a.tiles, a.njobs), dim3(a.g.wg), 0, st, a);
-  return hipGetLastError();
-}
-
 /* ---- field selection -----------------------------------------------------------
    selectfields (src/motiondetect.c:676-743) for one (frame, stage) per workgroup.
    The reference sorts with glibc's stable qsort, so equal contrasts keep index
@@ -378,11 +190,13 @@ hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
    again, so the same rule applies.  Contrast doubles come from a host-built
    table of (max-min)/(max+min+0.1) over all 65536 (min, max) pairs, so no
    device double division is involved. */
-__global__ __launch_bounds__(256) void k_select(VSGSelectArgs a) {
+constexpr int kSelectThreads = 512;
+
+__global__ __launch_bounds__(kSelectThreads) void k_select(VSGSelectArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ int segCnt[256];
   __shared__ int segBase[256];
-  __shared__ int total, nleft;
+  __shared__ int total, nleft, nunp;
   const int f = blockIdx.x, stage = blockIdx.y;
   const VSGSelectStage st = a.st[stage];
   const int n = st.nf;
@@ -393,13 +207,13 @@ __global__ __launch_bounds__(256) void k_select(VSGSelectArgs a) {
   if (!a.detect[f]) {
     if (threadIdx.x == 0) a.count[f * 2 + stage] = 0;
     if (stage == 0)
-      for (int q = threadIdx.x; q < nc; q += 256) a.jobs[(size_t)f * nc + q].field = -1;
+      for (int q = threadIdx.x; q < nc; q += kSelectThreads) a.jobs[(size_t)f * nc + q].field = -1;
     return;
   }
   double* c = reinterpret_cast<double*>(lds);
   int* pick = reinterpret_cast<int*>(c + n);
   const uint8_t* mm = a.minmax + ((size_t)(a.slot0 + f) * a.nfTotal + st.mmOff) * 2;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += kSelectThreads) {
     const double v = a.ctab[mm[2 * i] * 256 + mm[2 * i + 1]];
     c[i] = v < st.ct ? 0.0 : v;
     pick[i] = 0;
@@ -407,10 +221,13 @@ __global__ __launch_bounds__(256) void k_select(VSGSelectArgs a) {
   const int numsegms = st.rows + 1;
   const int segmlen = n / numsegms + 1;
   const int perSeg = st.maxFields / numsegms;
-  for (int s = threadIdx.x; s < numsegms; s += 256) segCnt[s] = 0;
-  if (threadIdx.x == 0) nleft = 0;
+  for (int s = threadIdx.x; s < numsegms; s += kSelectThreads) segCnt[s] = 0;
+  if (threadIdx.x == 0) {
+    nleft = 0;
+    nunp = 0;
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += kSelectThreads) {
     const double ci = c[i];
     if (!(ci > 0.0)) continue;
     const int s = i / segmlen, start = s * segmlen, end = min(start + segmlen, n);
@@ -433,30 +250,40 @@ __global__ __launch_bounds__(256) void k_select(VSGSelectArgs a) {
   __syncthreads();
   const int picked = total;
   const int remaining = st.maxFields - picked;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  /* picks go to their slots; unpicked positive fields are gathered into a
+     short list (the zeroed picks can never precede a positive contrast in the
+     leftover order, so only this list needs ranking) */
+  int* unp = pick + n;
+  for (int i = threadIdx.x; i < n; i += kSelectThreads) {
     if (pick[i]) {
       const int o = segBase[i / segmlen] + pick[i] - 1;
       outIdx[o] = (int16_t)i;
       outC[o] = c[i];
     } else if (remaining > 0 && c[i] > 0.0) {
-      const double ci = c[i];
-      int rank = 0;
-      for (int j = 0; j < n && rank < remaining; ++j) {
-        const double cj = pick[j] ? 0.0 : c[j];
-        rank += (cj > ci || (cj == ci && j < i)) ? 1 : 0;
-      }
-      if (rank < remaining) {
-        outIdx[picked + rank] = (int16_t)i;
-        outC[picked + rank] = ci;
-        atomicAdd(&nleft, 1);
-      }
+      unp[atomicAdd(&nunp, 1)] = i;
+    }
+  }
+  __syncthreads();
+  const int L = nunp;
+  for (int k = threadIdx.x; k < L; k += kSelectThreads) {
+    const int i = unp[k];
+    const double ci = c[i];
+    int rank = 0;
+    for (int m = 0; m < L; ++m) {
+      const int j = unp[m];
+      rank += (c[j] > ci || (c[j] == ci && j < i)) ? 1 : 0;
+    }
+    if (rank < remaining) {
+      outIdx[picked + rank] = (int16_t)i;
+      outC[picked + rank] = ci;
+      atomicAdd(&nleft, 1);
     }
   }
   __syncthreads();
   const int cnt = picked + nleft;
   if (threadIdx.x == 0) a.count[f * 2 + stage] = cnt;
   if (stage == 0) {
-    for (int q = threadIdx.x; q < nc; q += 256) {
+    for (int q = threadIdx.x; q < nc; q += kSelectThreads) {
       VSGJob j;
       j.slot = a.slot0 + f;
       j.prev = a.prevSlot[f];
@@ -473,9 +300,9 @@ __global__ __launch_bounds__(256) void k_select(VSGSelectArgs a) {
 hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st) {
   if (nframes <= 0) return hipSuccess;
   const int nmax = a.st[0].nf > a.st[1].nf ? a.st[0].nf : a.st[1].nf;
-  const size_t lds = (size_t)nmax * (sizeof(double) + sizeof(int));
+  const size_t lds = (size_t)nmax * (sizeof(double) + 2 * sizeof(int));
   if (lds > 64 * 1024 || a.st[0].rows + 1 > 256 || a.st[1].rows + 1 > 256) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_select, dim3(nframes, 2), dim3(256), lds, st, a);
+  hipLaunchKernelGGL(k_select, dim3(nframes, 2), dim3(kSelectThreads), lds, st, a);
   return hipGetLastError();
 }
 
@@ -512,69 +339,3 @@ hipError_t vsg_launch_fields(const VSGFieldArgs& a, int nslots, hipStream_t st) 
   return hipGetLastError();
 }
 
-static int gcd_int(int x, int y) {
-  while (y) { int t = x % y; x = y; y = t; }
-  return x;
-}
-
-int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift) {
-  g->S = S;
-  g->step = step;
-  g->n = maxShift / step;
-  const int G = 2 * g->n + 1;
-  const int lcm4 = 4 * step / gcd_int(4, step);
-  const int D = lcm4 / 4;
-  g->K = 4;
-  /* the widest column chunk (<= 28 words) that divides a field row: one prev
-     row segment load then feeds J*K*CW v_sad_u8 */
-  static const int kCW[] = {28, 20, 16, 12, 8, 4};
-  g->CW = 4;
-  for (int cw : kCW)
-    if ((S / 4) % cw == 0) {
-      g->CW = cw;
-      break;
-    }
-  if (D == 1 || D == 3) {
-    g->J = 4;
-    g->D = D;
-    g->m = lcm4 / step;
-  } else {
-    g->J = 1;   /* any step: one column per lane, no horizontal sharing */
-    g->D = 1;
-    g->m = 1;
-  }
-  const int perClass = (G + g->m - 1) / g->m;
-  g->U = g->m * ((perClass + g->J - 1) / g->J);
-  g->V = (G + g->K - 1) / g->K;
-  /* a job that needs fewer than 256 lanes gets a workgroup of just enough
-     whole waves, so idle lanes stay below one wave */
-  const int lanes = g->U * g->V;
-  g->wg = lanes >= 256 ? 256 : 64 * ((lanes + 63) / 64);
-  return (lanes + g->wg - 1) / g->wg;
-}
-
-template <int J, int D>
-static hipError_t launch_grid_cw(const VSGSearchArgs& a, hipStream_t st) {
-  switch (a.g.CW) {
-    case 28: return launch_grid<J, 4, D, 28>(a, st);
-    case 20: return launch_grid<J, 4, D, 20>(a, st);
-    case 16: return launch_grid<J, 4, D, 16>(a, st);
-    case 12: return launch_grid<J, 4, D, 12>(a, st);
-    case 8: return launch_grid<J, 4, D, 8>(a, st);
-    default: return launch_grid<J, 4, D, 4>(a, st);
-  }
-}
-
-hipError_t vsg_launch_grid(const VSGSearchArgs& a, hipStream_t st) {
-  if (a.njobs <= 0) return hipSuccess;
-  const VSGGrid& g = a.g;
-  if (g.J == 4 && g.D == 1) return launch_grid_cw<4, 1>(a, st);
-  if (g.J == 4 && g.D == 3) return launch_grid_cw<4, 3>(a, st);
-  return launch_grid_cw<1, 1>(a, st);
-}
-
-hipError_t vsg_launch_refine(const VSGSearchArgs& a, hipStream_t st) {
-  if (a.njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sad_refine, dim3(a.njobs), dim3(256), 0, st, a);
-  return hipGetLastError();
-}

@@ -1,0 +1,248 @@
+/*
+ * vsg_search.hip -- the field search kernels of the detect pass on gfx950.
+ *
+ *   k_sad_grid    exhaustive grid search of the reference's spiral
+ *                 (src/motiondetect.c:466-518, SAD :971-996); the lane body is
+ *                 vsg_sad_lane (vsg_device.h): candidates across lanes,
+ *                 v_sad_u8 with the field as a wave-uniform SGPR operand,
+ *                 wave argmin on (sad, spiral rank), one u64 atomic min per wave
+ *   k_sad_refine  the dyadic refinement around the grid winner, replayed in
+ *                 the reference's visit order (src/motiondetect.c:545-567)
+ *
+ * Both launch 1-D grids whose block -> work mapping is XCD-aware: the
+ * hardware hands workgroups to the 8 XCDs round-robin (block b runs on XCD
+ * b % 8), so units of work are grouped in chunks (normally the fields of one
+ * frame) and every chunk is placed on a single XCD.  A frame's blurred planes
+ * then stay in that XCD's L2 while all of its fields are searched.
+ */
+#include "vsg_device.h"
+#include "vsg_internal.h"
+
+namespace {
+
+constexpr int kXcds = 8;
+
+/* unit handled by block b, or -1: XCD x = b % 8 takes chunks x, x+8, ... */
+__device__ __forceinline__ int xcd_unit(int b, int chunk, int nunits) {
+  const int x = b % kXcds, i = b / kXcds;
+  const int k = x + kXcds * (i / chunk);
+  const int u = k * chunk + (i % chunk);
+  return u < nunits ? u : -1;
+}
+
+static int xcd_blocks(int nunits, int chunk) {
+  const int nchunks = (nunits + chunk - 1) / chunk;
+  return kXcds * ((nchunks + kXcds - 1) / kXcds) * chunk;
+}
+
+template <int J, int K, int D, int CW>
+__global__ __launch_bounds__(256) void k_sad_grid(VSGSearchArgs a) {
+  const int unit = xcd_unit(blockIdx.x, a.chunk, a.njobs * a.tiles);
+  if (unit < 0) return;
+  const int job = unit / a.tiles, tile = unit - job * a.tiles;
+  const VSGJob jb = a.jobs[job];
+  if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
+  VSGTile t;
+  t.S = a.g.S;
+  t.s = a.g.step;
+  t.n = a.g.n;
+  t.m = a.g.m;
+  t.U = a.g.U;
+  t.V = a.g.V;
+  t.fxc = a.fx[jb.field];
+  t.fyc = a.fy[jb.field];
+  t.ox = jb.ox;
+  t.oy = jb.oy;
+  t.pitch = a.pitch;
+  t.prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
+  t.patch = (vsg_cword_ptr)(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
+                            (size_t)jb.field * t.S * t.S);
+  unsigned long long best = vsg_sad_lane<J, K, D, CW>(t, tile * blockDim.x + threadIdx.x);
+  best = vsg_wave_min_u64(best);
+  if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(&a.best[job], best);
+}
+
+/* ---- refinement --------------------------------------------------------------
+   One workgroup (4 waves) per job.  Thread t handles word w = t % (S/4) of
+   field rows t / (S/4), + rows-per-pass, ...: its loads of the field patch and
+   of the prev rows are consecutive dwords across the wave (coalesced; prev
+   words at any byte offset, which gfx950 global loads accept).  A level's
+   ni x ni candidates (spacing ns around the current best) accumulate per
+   thread, are summed across each wave and then across the 4 waves in LDS,
+   and every thread scans them in the reference's order (i outer, j inner,
+   centre skipped) with the strict "<" against the running minimum --
+   exactly the sequential loop. */
+typedef uint32_t __attribute__((aligned(1))) u32_any;
+constexpr int kMaxNi = 5;   /* ni = 2r/ns + 1 <= 5 for every step */
+constexpr int kNc = kMaxNi * kMaxNi;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
+  __shared__ uint32_t part[4][kNc];
+  __shared__ uint32_t sums[kNc];
+  const int job = xcd_unit(blockIdx.x, a.chunk, a.njobs);
+  if (job < 0) return;
+  const VSGJob jb = a.jobs[job];
+  if (jb.field < 0) return;
+  const int S = a.g.S, W4 = S >> 2;
+  const int rowsPerPass = 256 / W4;           /* S <= 256 */
+  const int w = threadIdx.x % W4, r0 = threadIdx.x / W4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int fxc = a.fx[jb.field], fyc = a.fy[jb.field];
+  const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
+  const uint32_t* patch = reinterpret_cast<const uint32_t*>(a.patches + (size_t)jb.slot * a.patchSlotStride +
+                                                            a.stageOff + (size_t)jb.field * S * S);
+  const unsigned long long key = a.best[job];
+  int ga, gb;
+  vsg_unrank((int)(uint32_t)key, &ga, &gb);
+  int tx = ga * a.g.step, ty = gb * a.g.step;
+  uint32_t minerr = (uint32_t)(key >> 32);
+  for (int st = a.g.step; st > 1; st /= 2) {
+    const int ns = st / 2, r = st - ns;
+    const int ni = (2 * r) / ns + 1;
+    uint32_t acc[kNc];
+#pragma unroll
+    for (int c = 0; c < kNc; ++c) acc[c] = 0;
+    if (r0 < rowsPerPass) {
+      const uint8_t* colBase = prev + (fxc - S / 2 + tx - r + jb.ox) + 4 * w;
+      const int rowBase = fyc - S / 2 + ty - r + jb.oy;
+#pragma unroll 2
+      for (int y = r0; y < S; y += rowsPerPass) {
+        const uint32_t c = patch[y * W4 + w];
+#pragma unroll
+        for (int tj = 0; tj < kMaxNi; ++tj) {
+          if (tj < ni) {
+            const uint8_t* p = colBase + (size_t)(rowBase + y + ns * tj) * a.pitch;
+#pragma unroll
+            for (int ti = 0; ti < kMaxNi; ++ti)
+              if (ti < ni)
+                acc[ti * kMaxNi + tj] =
+                    __builtin_amdgcn_sad_u8(*reinterpret_cast<const u32_any*>(p + ns * ti), c, acc[ti * kMaxNi + tj]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kNc; ++c) {
+      const uint32_t v = wave_sum(acc[c]);
+      if (lane == 0) part[wave][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNc)
+      sums[threadIdx.x] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+    __syncthreads();
+    int btx = tx, bty = ty;
+    for (int ti = 0; ti < ni; ++ti)
+      for (int tj = 0; tj < ni; ++tj) {
+        const int i = tx - r + ns * ti, j = ty - r + ns * tj;
+        const uint32_t e = sums[ti * kMaxNi + tj];
+        if (!(i == tx && j == ty) && e < minerr) {
+          minerr = e;
+          btx = i;
+          bty = j;
+        }
+      }
+    tx = btx;
+    ty = bty;
+    __syncthreads();   /* part[] / sums[] are rewritten by the next level */
+  }
+  if (threadIdx.x == 0) {
+    VSGResult res;
+    res.tx = (int16_t)tx;
+    res.ty = (int16_t)ty;
+    res.minerror = minerr;
+    a.results[job] = res;
+    a.best[job] = ~0ull;   /* ready for the next launch */
+  }
+}
+
+template <int J, int K, int D, int CW>
+hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
+  const int nb = xcd_blocks(a.njobs * a.tiles, a.chunk);
+  hipLaunchKernelGGL((k_sad_grid<J, K, D, CW>), dim3(nb), dim3(a.g.wg), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int J, int D>
+hipError_t launch_grid_cw(const VSGSearchArgs& a, hipStream_t st) {
+  switch (a.g.CW) {
+    case 28: return launch_grid<J, 4, D, 28>(a, st);
+    case 20: return launch_grid<J, 4, D, 20>(a, st);
+    case 16: return launch_grid<J, 4, D, 16>(a, st);
+    case 12: return launch_grid<J, 4, D, 12>(a, st);
+    case 8: return launch_grid<J, 4, D, 8>(a, st);
+    default: return launch_grid<J, 4, D, 4>(a, st);
+  }
+}
+
+int gcd_int(int x, int y) {
+  while (y) {
+    int t = x % y;
+    x = y;
+    y = t;
+  }
+  return x;
+}
+
+}  // namespace
+
+int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift) {
+  g->S = S;
+  g->step = step;
+  g->n = maxShift / step;
+  const int G = 2 * g->n + 1;
+  const int lcm4 = 4 * step / gcd_int(4, step);
+  const int D = lcm4 / 4;
+  g->K = 4;
+  /* the widest column chunk (<= 28 words) that divides a field row: one prev
+     row segment load then feeds J*K*CW v_sad_u8 */
+  static const int kCW[] = {28, 20, 16, 12, 8, 4};
+  g->CW = 4;
+  for (int cw : kCW)
+    if ((S / 4) % cw == 0) {
+      g->CW = cw;
+      break;
+    }
+  if (D == 1 || D == 3) {
+    g->J = 4;
+    g->D = D;
+    g->m = lcm4 / step;
+  } else {
+    g->J = 1;   /* any step: one column per lane, no horizontal sharing */
+    g->D = 1;
+    g->m = 1;
+  }
+  const int perClass = (G + g->m - 1) / g->m;
+  g->U = g->m * ((perClass + g->J - 1) / g->J);
+  g->V = (G + g->K - 1) / g->K;
+  /* a job that needs fewer than 256 lanes gets a workgroup of just enough
+     whole waves, so idle lanes stay below one wave */
+  const int lanes = g->U * g->V;
+  g->wg = lanes >= 256 ? 256 : 64 * ((lanes + 63) / 64);
+  return (lanes + g->wg - 1) / g->wg;
+}
+
+hipError_t vsg_launch_grid(const VSGSearchArgs& a, hipStream_t st) {
+  if (a.njobs <= 0) return hipSuccess;
+  if (a.chunk <= 0) return hipErrorInvalidValue;
+  const VSGGrid& g = a.g;
+  if (g.J == 4 && g.D == 1) return launch_grid_cw<4, 1>(a, st);
+  if (g.J == 4 && g.D == 3) return launch_grid_cw<4, 3>(a, st);
+  return launch_grid_cw<1, 1>(a, st);
+}
+
+hipError_t vsg_launch_refine(const VSGSearchArgs& a, hipStream_t st) {
+  if (a.njobs <= 0) return hipSuccess;
+  if (a.g.S > 256 || a.chunk <= 0) return hipErrorInvalidValue;
+  /* one workgroup per job; the grid kernel's chunk counted (job, tile) units */
+  VSGSearchArgs r = a;
+  r.chunk = a.chunk / a.tiles;
+  if (r.chunk < 1) r.chunk = 1;
+  const int nb = xcd_blocks(a.njobs, r.chunk);
+  hipLaunchKernelGGL(k_sad_refine, dim3(nb), dim3(256), 0, st, r);
+  return hipGetLastError();
+}

@@ -1,0 +1,34 @@
+/*
+ * vs_detect_internal.h -- host steps of the detect pass shared by the
+ * single-clip driver (vs_motiondetect.c) and the multi-clip batched form
+ * (vs_multidetect.c).  Not exported.
+ */
+#ifndef VS_DETECT_INTERNAL_H
+#define VS_DETECT_INTERNAL_H
+
+#include "motiondetect.h"
+#include "vsg.h"
+
+/* vsMotionDetectInit without the GPU context: validation, config clamps, the
+   coarse and fine field grids (src/motiondetect.c:83-165, 314-358) */
+int vsd_configure(VSMotionDetect* md, const VSMotionDetectConfig* conf, const VSFrameInfo* fi);
+
+/* box size of boxblurPlanar for this detector (src/boxblur.c:109-122) */
+int vsd_blur_size(const VSMotionDetect* md);
+
+/* GPU description of one field grid */
+VSGStageDesc vsd_stage_desc(const VSMotionDetectFields* fs);
+
+/* Frame i of a batch view: builds its coarse LocalMotions (in selection
+   order, rejected fields dropped), computes vsSimpleMotionsToTransform and
+   appends the fine search jobs (slot / prev given) to jobs[]; jobSel[] gets
+   each job's index in the fine selection.  Returns the number of jobs. */
+int vsd_frame_coarse(VSMotionDetect* md, const VSGBatchView* v, int i, int fnum, int slot, int prevSlot,
+                     LocalMotions* coarse, VSGJob* jobs, int* jobSel);
+
+/* Frame i: fine results of its njobs jobs -> the cleaned fine motions, then
+   out = coarse ++ fine (the elements of coarse move into out). */
+void vsd_frame_fine(VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
+                    const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, LocalMotions* out);
+
+#endif /* VS_DETECT_INTERNAL_H */

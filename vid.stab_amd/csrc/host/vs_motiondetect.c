@@ -10,11 +10,12 @@
  *                          (glibc libm), fine offsets with the border discard
  *   GPU  (vsg_submit_fine) fine search of every frame of the batch
  *   host                   match cleanup, coarse ++ fine output
- * Batches alternate between the two slot sets of the GPU context, so the
- * device runs batch k+1's blur/selection/coarse search while the host turns
- * batch k's fine results into LocalMotions.  Frames of a batch are independent
- * given their previous plane, so the output is exactly that of one
- * vsMotionDetection call per frame.
+ * Batches rotate through the GPU context's VSG_NSETS slot sets: batch k+1 is
+ * already queued when the host works on batch k's coarse results, and batch
+ * k+2 is queued as soon as batch k is finished, so the device is kept busy
+ * while the host does its double-precision steps.  Frames of a batch are
+ * independent given their previous plane, so the output is exactly that of
+ * one vsMotionDetection call per frame.
  *
  * VSMotionDetect is caller-owned with a frozen layout; the GPU context of a
  * detector lives in a side table keyed by the struct's address.
@@ -29,6 +30,7 @@
 #include "transformtype_operations.h"
 #include "vidstab_amd.h"
 #include "vidstabdefines.h"
+#include "vs_detect_internal.h"
 #include "vsg.h"
 
 #define MD_MAX_BATCH 64
@@ -42,8 +44,6 @@ typedef struct MDState {
   int maxBatch;
   int nextSet;     /* slot set of the next batch */
   int lastSlot;    /* global slot holding the previous frame's blurred plane */
-  VSGJob* jobs;    /* fine jobs of one batch */
-  int* jobSel;     /* fine selection index of each job */
 } MDState;
 
 static MDState* g_states = NULL;
@@ -60,8 +60,6 @@ static MDState* state_find(const VSMotionDetect* md) {
 static void state_free(MDState* s) {
   if (!s) return;
   if (s->gpu) vsg_destroy(s->gpu);
-  free(s->jobs);
-  free(s->jobSel);
   free(s);
 }
 
@@ -154,15 +152,14 @@ static int init_fields(VSMotionDetect* md, VSMotionDetectFields* fs, int size, i
   return 1;
 }
 
-/* src/boxblur.c:109-122 */
-static int blur_box_size(const VSMotionDetect* md) {
+int vsd_blur_size(const VSMotionDetect* md) {
   int size = md->conf.stepSize;
   if (size < 2) return 1;
   size = (size / 2) * 2 + 1;
   return VS_CLAMP(size, 3, VS_MIN(md->fi.height / 2, md->fi.width / 2));
 }
 
-static VSGStageDesc stage_desc(const VSMotionDetectFields* fs) {
+VSGStageDesc vsd_stage_desc(const VSMotionDetectFields* fs) {
   VSGStageDesc d;
   d.fieldSize = fs->fieldSize;
   d.maxShift = fs->maxShift;
@@ -175,10 +172,12 @@ static VSGStageDesc stage_desc(const VSMotionDetectFields* fs) {
   return d;
 }
 
-VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* conf, const VSFrameInfo* fi) {
+int vsd_configure(VSMotionDetect* md, const VSMotionDetectConfig* conf, const VSFrameInfo* fi) {
   if (!md || !conf || !fi) return VS_ERROR;
   md->conf = *conf;
   md->fi = *fi;
+  md->fieldscoarse.fields = NULL;
+  md->fieldsfine.fields = NULL;
   if (fi->pFormat <= PF_NONE || fi->pFormat == PF_PACKED || fi->pFormat >= PF_NUMBER) {
     vs_log_warn(md->conf.modName, "unsupported Pixel Format (%i)\n", md->fi.pFormat);
     return VS_ERROR;
@@ -192,8 +191,6 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
     vs_log_error(md->conf.modName, "stepsize must be at least 1\n");
     return VS_ERROR;
   }
-  state_remove(md);   /* a re-initialised detector drops its old device state */
-
   vsFrameNull(&md->prev);
   vsFrameNull(&md->curr);
   vsFrameNull(&md->currorig);
@@ -221,34 +218,48 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
   /* always rounded up to the next multiple of 16, as the reference does */
   fieldSize = (fieldSize / 16 + 1) * 16;
   fieldSizeFine = (fieldSizeFine / 16 + 1) * 16;
-  md->fieldscoarse.fields = NULL;
-  md->fieldsfine.fields = NULL;
   if (!init_fields(md, &md->fieldscoarse, fieldSize, maxShift, md->conf.stepSize, 1, 0,
                    md->conf.contrastThreshold))
     return VS_ERROR;
   if (!init_fields(md, &md->fieldsfine, fieldSizeFine, fieldSizeFine, 2, 1, fieldSizeFine,
                    md->conf.contrastThreshold / 2))
     return VS_ERROR;
+  return VS_OK;
+}
 
-  MDState* s = (MDState*)calloc(1, sizeof(MDState));
-  if (!s) return VS_ERROR;
-  s->key = md;
-  s->maxBatch = VS_MIN(MD_MAX_BATCH, VS_MAX(1, env_int("VIDSTAB_AMD_MAX_BATCH", 16)));
-  s->lastSlot = -1;
-  const int device = env_int("VIDSTAB_AMD_DEVICE", 0);
-  const size_t njmax = (size_t)s->maxBatch * md->fieldsfine.fieldNum;
-  s->jobs = (VSGJob*)calloc(njmax, sizeof(VSGJob));
-  s->jobSel = (int*)calloc(njmax, sizeof(int));
-  if (!s->jobs || !s->jobSel) {
-    state_free(s);
-    vs_log_error(md->conf.modName, "malloc failed");
+static void free_fields(VSMotionDetect* md) {
+  if (md->fieldscoarse.fields) {
+    vs_free(md->fieldscoarse.fields);
+    md->fieldscoarse.fields = 0;
+  }
+  if (md->fieldsfine.fields) {
+    vs_free(md->fieldsfine.fields);
+    md->fieldsfine.fields = 0;
+  }
+}
+
+VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* conf, const VSFrameInfo* fi) {
+  if (!md || !conf || !fi) return VS_ERROR;
+  state_remove(md);   /* a re-initialised detector drops its old device state */
+  if (vsd_configure(md, conf, fi) != VS_OK) {
+    free_fields(md);
     return VS_ERROR;
   }
-  VSGStageDesc dc = stage_desc(&md->fieldscoarse), df = stage_desc(&md->fieldsfine);
-  if (vsg_create(&s->gpu, device, md->fi.width, md->fi.height, blur_box_size(md), &dc, &df,
+  MDState* s = (MDState*)calloc(1, sizeof(MDState));
+  if (!s) {
+    free_fields(md);
+    return VS_ERROR;
+  }
+  s->key = md;
+  s->maxBatch = VS_MIN(MD_MAX_BATCH, VS_MAX(1, env_int("VIDSTAB_AMD_MAX_BATCH", 32)));
+  s->lastSlot = -1;
+  const int device = env_int("VIDSTAB_AMD_DEVICE", 0);
+  VSGStageDesc dc = vsd_stage_desc(&md->fieldscoarse), df = vsd_stage_desc(&md->fieldsfine);
+  if (vsg_create(&s->gpu, device, md->fi.width, md->fi.height, vsd_blur_size(md), &dc, &df,
                  s->maxBatch) != 0) {
     vs_log_error(md->conf.modName, "GPU initialisation failed: %s\n", vsg_last_error());
     state_free(s);
+    free_fields(md);
     return VS_ERROR;
   }
   vs_log_info(md->conf.modName, "GPU: HIP device %i, batch %i\n", device, s->maxBatch);
@@ -259,22 +270,16 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
 
 VS_API void vsMotionDetectionCleanup(VSMotionDetect* md) {
   state_remove(md);
-  if (md->fieldscoarse.fields) {
-    vs_free(md->fieldscoarse.fields);
-    md->fieldscoarse.fields = 0;
-  }
-  if (md->fieldsfine.fields) {
-    vs_free(md->fieldsfine.fields);
-    md->fieldsfine.fields = 0;
-  }
+  free_fields(md);
   vsFrameFree(&md->prev);
   vsFrameFree(&md->curr);
   vsFrameFree(&md->currtmp);
   md->initialized = 0;
 }
 
-/* ---- per-field result -> LocalMotion: src/motiondetect.c:573-585 ---------- */
+/* ---- per-frame host steps ------------------------------------------------------ */
 
+/* src/motiondetect.c:573-585 */
 static int make_motion(const VSMotionDetectFields* fs, const Field* f, const VSGResult* r, int ox,
                        int oy, double contrast, LocalMotion* lm) {
   int st = fs->stepSize;
@@ -294,6 +299,74 @@ static short lm_match_better(void* thresh, void* lm) {
   return ((LocalMotion*)lm)->match <= *((double*)thresh) ? 1 : 0;
 }
 
+int vsd_frame_coarse(VSMotionDetect* md, const VSGBatchView* v, int i, int fnum, int slot, int prevSlot,
+                     LocalMotions* coarse, VSGJob* jobs, int* jobSel) {
+  const char* mod = md->conf.modName;
+  VSMotionDetectFields* fc = &md->fieldscoarse;
+  VSMotionDetectFields* ff = &md->fieldsfine;
+  vs_vector_init(coarse, fc->maxFields);
+  const int16_t* idx = v->idx + (size_t)i * v->stride;
+  const double* con = v->contrast + (size_t)i * v->stride;
+  const VSGResult* res = v->coarse + (size_t)i * v->ncoarse;
+  for (int q = 0; q < v->count[2 * i]; q++) {
+    LocalMotion lm;
+    if (make_motion(fc, &fc->fields[idx[q]], &res[q], 0, 0, con[q], &lm))
+      vs_vector_append_dup(coarse, &lm, sizeof(LocalMotion));
+  }
+  if (vs_vector_size(coarse) < 1) {
+    vs_log_warn(mod, "too low contrast. (no translations are detected in frame %i)\n", fnum);
+    return 0;
+  }
+  /* src/motiondetect.c:238-240, 418-434 */
+  ff->offset = vsSimpleMotionsToTransform(md->fi, mod, coarse);
+  ff->useOffset = 1;
+  PreparedTransform pt = prepare_transform(&ff->offset, &md->fi);
+  const int16_t* fidx = idx + v->ncoarse;
+  int nj = 0;
+  for (int q = 0; q < v->count[2 * i + 1]; q++) {
+    const Field* f = &ff->fields[fidx[q]];
+    Vec pos = {f->x, f->y};
+    Vec off = sub_vec(transform_vec(&pt, &pos), pos);
+    const int border = f->size / 2 + ff->maxShift + ff->stepSize;
+    const int x = pos.x + off.x, y = pos.y + off.y;
+    if (x - border < 0 || x + border >= md->fi.width || y - border < 0 || y + border >= md->fi.height)
+      continue;
+    VSGJob* j = &jobs[nj];
+    j->slot = slot;
+    j->prev = prevSlot;
+    j->field = fidx[q];
+    j->ox = off.x;
+    j->oy = off.y;
+    jobSel[nj] = q;
+    nj++;
+  }
+  return nj;
+}
+
+void vsd_frame_fine(VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
+                    const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, LocalMotions* out) {
+  VSMotionDetectFields* ff = &md->fieldsfine;
+  LocalMotions fine;
+  vs_vector_init(&fine, 0);
+  if (vs_vector_size(coarse) >= 1) {
+    /* "cleanmotions": keep fine motions no worse than the trimmed mean of the
+       coarse matches (src/motiondetect.c:249-258) */
+    VSArray mq = localmotionsGetMatch(coarse);
+    double meanMatch = cleanmean(mq.dat, mq.len, NULL, NULL);
+    vs_array_free(mq);
+    const double* fcon = v->contrast + (size_t)i * v->stride + v->ncoarse;
+    for (int k = 0; k < njobs; k++) {
+      LocalMotion lm;
+      if (!make_motion(ff, &ff->fields[jobs[k].field], &res[k], jobs[k].ox, jobs[k].oy, fcon[jobSel[k]], &lm))
+        continue;
+      if (lm_match_better(&meanMatch, &lm)) vs_vector_append_dup(&fine, &lm, sizeof(LocalMotion));
+    }
+  }
+  *out = vs_vector_concat(coarse, &fine);
+  vs_vector_fini(coarse);
+  vs_vector_fini(&fine);
+}
+
 /* ---- batches ------------------------------------------------------------------ */
 
 typedef struct {
@@ -303,7 +376,11 @@ typedef struct {
   uint8_t detect[MD_MAX_BATCH];
   LocalMotions* out;
   LocalMotions coarse[MD_MAX_BATCH];
+  int jobStart[MD_MAX_BATCH + 1];
+  VSGJob* jobs;                /* fine jobs of the batch (frame order) */
+  int* jobSel;
   int njobs;
+  int active;
 } Batch;
 
 /* decides, for the next n frames, which are measured and against which plane
@@ -312,7 +389,7 @@ static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t*
                         const int* linesize, int onDevice, int n, LocalMotions* out) {
   const int tripod = md->conf.virtualTripod;
   b->set = s->nextSet;
-  s->nextSet ^= 1;
+  s->nextSet = (s->nextSet + 1) % VSG_NSETS;
   b->n = n;
   b->slot0 = b->set * s->maxBatch;
   b->fnum0 = md->frameNum;
@@ -337,62 +414,29 @@ static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t*
   md->hasSeenOneFrame = (short)seen;
   md->frameNum += n;
   s->lastSlot = b->slot0 + n - 1;
+  b->active = 1;
   return VS_OK;
 }
 
 /* coarse results -> coarse LocalMotions, transform, fine jobs; submits them */
 static int batch_coarse(VSMotionDetect* md, MDState* s, Batch* b) {
-  const char* mod = md->conf.modName;
-  VSMotionDetectFields* fc = &md->fieldscoarse;
-  VSMotionDetectFields* ff = &md->fieldsfine;
   VSGBatchView v;
   if (vsg_wait_coarse(s->gpu, b->set, &v) != 0) {
-    vs_log_error(mod, "GPU detect failed: %s\n", vsg_last_error());
+    vs_log_error(md->conf.modName, "GPU detect failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
   int nj = 0;
   for (int i = 0; i < b->n; i++) {
+    b->jobStart[i] = nj;
     vs_vector_init(&b->coarse[i], 0);
     if (!b->detect[i]) continue;
-    vs_vector_init(&b->coarse[i], fc->maxFields);
-    const int16_t* idx = v.idx + (size_t)i * v.stride;
-    const double* con = v.contrast + (size_t)i * v.stride;
-    const VSGResult* res = v.coarse + (size_t)i * v.ncoarse;
-    for (int q = 0; q < v.count[2 * i]; q++) {
-      LocalMotion lm;
-      if (make_motion(fc, &fc->fields[idx[q]], &res[q], 0, 0, con[q], &lm))
-        vs_vector_append_dup(&b->coarse[i], &lm, sizeof(LocalMotion));
-    }
-    if (vs_vector_size(&b->coarse[i]) < 1) {
-      vs_log_warn(mod, "too low contrast. (no translations are detected in frame %i)\n", b->fnum0 + i);
-      continue;
-    }
-    /* src/motiondetect.c:238-240, 418-434 */
-    ff->offset = vsSimpleMotionsToTransform(md->fi, mod, &b->coarse[i]);
-    ff->useOffset = 1;
-    PreparedTransform pt = prepare_transform(&ff->offset, &md->fi);
-    const int16_t* fidx = idx + v.ncoarse;
-    for (int q = 0; q < v.count[2 * i + 1]; q++) {
-      const Field* f = &ff->fields[fidx[q]];
-      Vec pos = {f->x, f->y};
-      Vec off = sub_vec(transform_vec(&pt, &pos), pos);
-      const int border = f->size / 2 + ff->maxShift + ff->stepSize;
-      const int x = pos.x + off.x, y = pos.y + off.y;
-      if (x - border < 0 || x + border >= md->fi.width || y - border < 0 || y + border >= md->fi.height)
-        continue;
-      VSGJob* j = &s->jobs[nj];
-      j->slot = b->slot0 + i;
-      j->prev = b->prevSlot[i];
-      j->field = fidx[q];
-      j->ox = off.x;
-      j->oy = off.y;
-      s->jobSel[nj] = q;
-      nj++;
-    }
+    nj += vsd_frame_coarse(md, &v, i, b->fnum0 + i, b->slot0 + i, b->prevSlot[i], &b->coarse[i],
+                           b->jobs + nj, b->jobSel + nj);
   }
+  b->jobStart[b->n] = nj;
   b->njobs = nj;
-  if (vsg_submit_fine(s->gpu, b->set, nj, s->jobs) != 0) {
-    vs_log_error(mod, "GPU fine search failed: %s\n", vsg_last_error());
+  if (vsg_submit_fine(s->gpu, b->set, nj, b->jobs) != 0) {
+    vs_log_error(md->conf.modName, "GPU fine search failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
   return VS_OK;
@@ -400,48 +444,29 @@ static int batch_coarse(VSMotionDetect* md, MDState* s, Batch* b) {
 
 /* fine results -> cleanup and output (src/motiondetect.c:249-291) */
 static int batch_fine(VSMotionDetect* md, MDState* s, Batch* b) {
-  VSMotionDetectFields* ff = &md->fieldsfine;
   const VSGResult* res = NULL;
   VSGBatchView v;
   if (vsg_wait_fine(s->gpu, b->set, &res) != 0 || vsg_wait_coarse(s->gpu, b->set, &v) != 0) {
     vs_log_error(md->conf.modName, "GPU fine search failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
-  int jf = 0;
   for (int i = 0; i < b->n; i++) {
     if (!b->detect[i]) {
       vs_vector_fini(&b->coarse[i]);
       vs_vector_init(&b->out[i], 1);   /* the reference's empty "dummy" vector */
       continue;
     }
-    LocalMotions fine;
-    vs_vector_init(&fine, 0);
-    if (vs_vector_size(&b->coarse[i]) >= 1) {
-      VSArray mq = localmotionsGetMatch(&b->coarse[i]);
-      double meanMatch = cleanmean(mq.dat, mq.len, NULL, NULL);
-      vs_array_free(mq);
-      const double* fcon = v.contrast + (size_t)i * v.stride + v.ncoarse;
-      for (; jf < b->njobs && s->jobs[jf].slot == b->slot0 + i; jf++) {
-        const VSGJob* j = &s->jobs[jf];
-        LocalMotion lm;
-        if (!make_motion(ff, &ff->fields[j->field], &res[jf], j->ox, j->oy, fcon[s->jobSel[jf]], &lm))
-          continue;
-        if (lm_match_better(&meanMatch, &lm)) vs_vector_append_dup(&fine, &lm, sizeof(LocalMotion));
-      }
-    }
-    b->out[i] = vs_vector_concat(&b->coarse[i], &fine);
-    vs_vector_fini(&b->coarse[i]);
-    vs_vector_fini(&fine);
+    const int j0 = b->jobStart[i];
+    vsd_frame_fine(md, &v, i, &b->coarse[i], b->jobs + j0, b->jobSel + j0, res + j0,
+                   b->jobStart[i + 1] - j0, &b->out[i]);
   }
+  b->active = 0;
   return VS_OK;
 }
 
-static void batch_discard(Batch* b) {
-  for (int i = 0; i < b->n; i++) vs_vector_del(&b->coarse[i]);
-}
-
-/* Runs n frames through batches of maxBatch; batch k+1 is submitted before
-   batch k's fine results are waited for. */
+/* Runs n frames through batches of maxBatch.  Batch k+1 is queued before the
+   host turns batch k's coarse results into fine jobs, batch k+2 right after
+   batch k completes. */
 static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_t* const* luma,
                          const int* linesize, int onDevice, int n) {
   if (!md || md->initialized != 2 || !motions || n < 0) return VS_ERROR;
@@ -451,44 +476,50 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
     return VS_ERROR;
   }
   if (n == 0) return VS_OK;
-  static __thread Batch bufs[2];
-  Batch* cur = &bufs[0];
-  Batch* nxt = &bufs[1];
   const int B = s->maxBatch;
-  int done = 0, next = VS_MIN(B, n);
-  if (batch_submit(md, s, cur, luma, linesize, onDevice, next, motions) != VS_OK) return VS_ERROR;
-  int ok = 1;
-  while (ok) {
-    if (batch_coarse(md, s, cur) != VS_OK) {
-      ok = 0;
-      break;
-    }
-    int haveNext = 0;
-    if (next < n) {
-      const int nb = VS_MIN(B, n - next);
-      if (batch_submit(md, s, nxt, luma + next, linesize + next, onDevice, nb, motions + next) != VS_OK) {
-        batch_discard(cur);
-        ok = 0;
-        break;
-      }
-      haveNext = 1;
-      next += nb;
-    }
-    if (batch_fine(md, s, cur) != VS_OK) {
-      ok = 0;
-      break;
-    }
-    done += cur->n;
-    if (!haveNext) break;
-    Batch* t = cur;
-    cur = nxt;
-    nxt = t;
-  }
-  if (!ok) {
-    for (int i = 0; i < done; i++) vs_vector_del(&motions[i]);
+  const int nb = (n + B - 1) / B;
+  const size_t nfj = (size_t)B * md->fieldsfine.fieldNum;
+  Batch* bs = (Batch*)calloc(2, sizeof(Batch));
+  VSGJob* jobs = (VSGJob*)malloc(sizeof(VSGJob) * nfj * 2);
+  int* sel = (int*)malloc(sizeof(int) * nfj * 2);
+  if (!bs || !jobs || !sel) {
+    free(bs);
+    free(jobs);
+    free(sel);
     return VS_ERROR;
   }
-  return VS_OK;
+  for (int k = 0; k < 2; k++) {
+    bs[k].jobs = jobs + nfj * k;
+    bs[k].jobSel = sel + nfj * k;
+  }
+  int ok = 1, done = 0;
+  /* batches k and k+1 are in flight; bs[k % 2] */
+  for (int k = 0; k < nb && k < 2 && ok; k++) {
+    const int f0 = k * B, cnt = VS_MIN(B, n - f0);
+    ok = batch_submit(md, s, &bs[k], luma + f0, linesize + f0, onDevice, cnt, motions + f0) == VS_OK;
+  }
+  for (int k = 0; k < nb && ok; k++) {
+    Batch* b = &bs[k % 2];
+    if (batch_coarse(md, s, b) != VS_OK || batch_fine(md, s, b) != VS_OK) {
+      ok = 0;
+      break;
+    }
+    done += b->n;
+    if (k + 2 < nb) {
+      const int f0 = (k + 2) * B, cnt = VS_MIN(B, n - f0);
+      ok = batch_submit(md, s, b, luma + f0, linesize + f0, onDevice, cnt, motions + f0) == VS_OK;
+    }
+  }
+  if (!ok) {
+    for (int k = 0; k < 2; k++)
+      if (bs[k].active)
+        for (int i = 0; i < bs[k].n; i++) vs_vector_del(&bs[k].coarse[i]);
+    for (int i = 0; i < done; i++) vs_vector_del(&motions[i]);
+  }
+  free(bs);
+  free(jobs);
+  free(sel);
+  return ok ? VS_OK : VS_ERROR;
 }
 
 VS_API int vsMotionDetection(VSMotionDetect* md, LocalMotions* motions, VSFrame* frame) {
