@@ -90,7 +90,7 @@ void vsg_ctx_free(VSGContext* c) {
   dfree(c->d_minmax);
   for (VSGSet& s : c->set) {
     dfree(s.d_selIdx); dfree(s.d_selC); dfree(s.d_count); dfree(s.d_jobsC); dfree(s.d_jobsF);
-    dfree(s.d_best); dfree(s.d_resC); dfree(s.d_resF); dfree(s.d_src); dfree(s.d_prev); dfree(s.d_detect);
+    dfree(s.d_best); dfree(s.d_done); dfree(s.d_resC); dfree(s.d_resF); dfree(s.d_src); dfree(s.d_prev); dfree(s.d_detect);
     hfree(s.h_selIdx); hfree(s.h_selC); hfree(s.h_count); hfree(s.h_resC); hfree(s.h_jobsF);
     hfree(s.h_resF); hfree(s.h_src); hfree(s.h_prev); hfree(s.h_detect);
     if (s.done) (void)hipEventDestroy(s.done);
@@ -227,6 +227,8 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
     VSG_CHECK(dalloc(&s.d_jobsF, (size_t)B * nf));
     VSG_CHECK(dalloc(&s.d_best, (size_t)B * nmax));
     VSG_CHECK(hipMemset(s.d_best, 0xFF, sizeof(unsigned long long) * (size_t)B * nmax));
+    VSG_CHECK(dalloc(&s.d_done, (size_t)B * nmax));
+    VSG_CHECK(hipMemset(s.d_done, 0, sizeof(unsigned) * (size_t)B * nmax));
     VSG_CHECK(dalloc(&s.d_resC, (size_t)B * nc));
     VSG_CHECK(dalloc(&s.d_resF, (size_t)B * nf));
     VSG_CHECK(dalloc(&s.d_src, B));
@@ -271,7 +273,7 @@ extern "C" int vsg_create(VSGContext** out, int device, int width, int height, i
 /* ---- batches ------------------------------------------------------------------ */
 
 VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, int njobs,
-                                  unsigned long long* best, VSGResult* res) {
+                                  unsigned long long* best, unsigned* done, VSGResult* res) {
   const VSGStage& S = c->st[stage];
   VSGSearchArgs a;
   a.jobs = jobs;
@@ -290,6 +292,7 @@ VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, 
   /* one frame's worth of jobs per XCD chunk: its planes stay in one L2 */
   a.chunk = S.nf * S.tiles;
   a.best = best;
+  a.done = done;
   a.results = res;
   return a;
 }
@@ -394,11 +397,9 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
                              hipMemcpyDeviceToDevice, c->stream));
   }
   const int nc = c->st[0].nf;
-  VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_resC);
-  VSG_CHECK(vsg_launch_grid(ga, c->stream));
+  VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
+  VSG_CHECK(vsg_launch_search(ga, c->stream));
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID1], c->stream));
-  VSG_CHECK(vsg_launch_refine(ga, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_REFINE1], c->stream));
   if (vsg_ctx_enqueue_coarse_readback(c, set, n) != 0) return -1;
   VSG_CHECK(hipEventRecord(s.done, c->stream));
   s.n = n;
@@ -420,8 +421,8 @@ extern "C" int vsg_wait_coarse(VSGContext* c, int set, VSGBatchView* view) {
     prof_add(VSG_K_FIELDS, s.ev[VSG_EV_BLUR1], s.ev[VSG_EV_FIELDS1], 2.0 * fieldBytes * s.n);
     prof_add(VSG_K_SELECT, s.ev[VSG_EV_FIELDS1], s.ev[VSG_EV_SELECT1], (double)c->nfTotal * s.n);
     const double px = (double)c->st[0].S * c->st[0].S;
-    prof_add(VSG_K_GRID_COARSE, s.ev[VSG_EV_SELECT1], s.ev[VSG_EV_GRID1], ncoarse * c->st[0].candsPerJob * px);
-    prof_add(VSG_K_REFINE_COARSE, s.ev[VSG_EV_GRID1], s.ev[VSG_EV_REFINE1], ncoarse * c->st[0].refinePerJob * px);
+    prof_add(VSG_K_GRID_COARSE, s.ev[VSG_EV_SELECT1], s.ev[VSG_EV_GRID1],
+             ncoarse * (c->st[0].candsPerJob + c->st[0].refinePerJob) * px);
   }
   view->count = s.h_count;
   view->idx = s.h_selIdx;
@@ -453,11 +454,9 @@ extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* 
   VSG_CHECK(hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, c->stream));
   s.profF = g_profOn.load() != 0;
   if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_0], c->stream));
-  VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_resF);
-  VSG_CHECK(vsg_launch_grid(fa, c->stream));
+  VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
+  VSG_CHECK(vsg_launch_search(fa, c->stream));
   if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_GRID1], c->stream));
-  VSG_CHECK(vsg_launch_refine(fa, c->stream));
-  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_REFINE1], c->stream));
   VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, c->stream));
   VSG_CHECK(hipEventRecord(s.doneF, c->stream));
   return 0;
@@ -472,8 +471,8 @@ extern "C" int vsg_wait_fine(VSGContext* c, int set, const VSGResult** results) 
     s.profF = false;
     const VSGStage& S = c->st[1];
     const double px = (double)S.S * S.S;
-    prof_add(VSG_K_GRID_FINE, s.evf[VSG_EVF_0], s.evf[VSG_EVF_GRID1], s.njobsF * S.candsPerJob * px);
-    prof_add(VSG_K_REFINE_FINE, s.evf[VSG_EVF_GRID1], s.evf[VSG_EVF_REFINE1], s.njobsF * S.refinePerJob * px);
+    prof_add(VSG_K_GRID_FINE, s.evf[VSG_EVF_0], s.evf[VSG_EVF_GRID1],
+             s.njobsF * (S.candsPerJob + S.refinePerJob) * px);
   }
   *results = s.h_resF;
   return 0;

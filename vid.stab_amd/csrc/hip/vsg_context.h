@@ -8,8 +8,8 @@
 
 #include "vsg_internal.h"
 
-enum { VSG_EV_BLUR0, VSG_EV_BLUR1, VSG_EV_FIELDS1, VSG_EV_SELECT1, VSG_EV_GRID1, VSG_EV_REFINE1, VSG_EV_COUNT };
-enum { VSG_EVF_0, VSG_EVF_GRID1, VSG_EVF_REFINE1, VSG_EVF_COUNT };
+enum { VSG_EV_BLUR0, VSG_EV_BLUR1, VSG_EV_FIELDS1, VSG_EV_SELECT1, VSG_EV_GRID1, VSG_EV_COUNT };
+enum { VSG_EVF_0, VSG_EVF_GRID1, VSG_EVF_COUNT };
 
 struct VSGStage {
   int S = 0, maxShift = 0, step = 0, nf = 0, rows = 0, maxFields = 0;
@@ -31,6 +31,7 @@ struct VSGSet {
   VSGJob* d_jobsC = nullptr;
   VSGJob* d_jobsF = nullptr;
   unsigned long long* d_best = nullptr;
+  unsigned* d_done = nullptr;
   VSGResult* d_resC = nullptr;
   VSGResult* d_resF = nullptr;
   VSGBlurSrc* d_src = nullptr;
@@ -89,7 +90,7 @@ void vsg_ctx_free(VSGContext* c);
 int vsg_ctx_enqueue_front(VSGContext* c, int set, int n);      /* H2D descr., blur, fields, select */
 int vsg_ctx_enqueue_coarse_readback(VSGContext* c, int set, int n);
 VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, int njobs,
-                                  unsigned long long* best, VSGResult* res);
+                                  unsigned long long* best, unsigned* done, VSGResult* res);
 int vsg_fail(const char* what, hipError_t e);
 
 #endif /* VSG_CONTEXT_H */

@@ -58,6 +58,7 @@ struct VSGSearchArgs {
   int tiles;                    /* workgroups per job in the grid kernel */
   int chunk;                    /* (job, tile) units kept together on one XCD */
   unsigned long long* best;     /* per job: (sad << 32) | spiral rank; ~0 between launches */
+  unsigned* done;               /* per job: grid workgroups finished; 0 between launches */
   VSGResult* results;
 };
 
@@ -86,8 +87,8 @@ hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st
 
 /* fills J, K, D, CW, m, U, V for a stage; returns the workgroups per job */
 int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift);
-/* grid kernel (argmin into best[]) and refinement kernel (best[] -> results) */
-hipError_t vsg_launch_grid(const VSGSearchArgs& a, hipStream_t st);
-hipError_t vsg_launch_refine(const VSGSearchArgs& a, hipStream_t st);
+/* grid search (argmin into best[]) fused with the refinement, which the
+   last workgroup of each job runs (best[] -> results) */
+hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st);
 
 #endif /* VSG_INTERNAL_H */

@@ -1,15 +1,16 @@
 /*
- * vsg_search.hip -- the field search kernels of the detect pass on gfx950.
+ * vsg_search.hip -- the field search kernel of the detect pass on gfx950.
  *
- *   k_sad_grid    exhaustive grid search of the reference's spiral
+ *   k_sad_search  exhaustive grid search of the reference's spiral
  *                 (src/motiondetect.c:466-518, SAD :971-996); the lane body is
  *                 vsg_sad_lane (vsg_device.h): candidates across lanes,
  *                 v_sad_u8 with the field as a wave-uniform SGPR operand,
- *                 wave argmin on (sad, spiral rank), one u64 atomic min per wave
- *   k_sad_refine  the dyadic refinement around the grid winner, replayed in
- *                 the reference's visit order (src/motiondetect.c:545-567)
+ *                 argmin on (sad, spiral rank); then the dyadic refinement
+ *                 around the grid winner, replayed in the reference's visit
+ *                 order (src/motiondetect.c:545-567), by the job's last
+ *                 workgroup
  *
- * Both launch 1-D grids whose block -> work mapping is XCD-aware: the
+ * The launch is a 1-D grid whose block -> work mapping is XCD-aware: the
  * hardware hands workgroups to the 8 XCDs round-robin (block b runs on XCD
  * b % 8), so units of work are grouped in chunks (normally the fields of one
  * frame) and every chunk is placed on a single XCD.  A frame's blurred planes
@@ -35,68 +36,44 @@ static int xcd_blocks(int nunits, int chunk) {
   return kXcds * ((nchunks + kXcds - 1) / kXcds) * chunk;
 }
 
-template <int J, int K, int D, int CW>
-__global__ __launch_bounds__(256) void k_sad_grid(VSGSearchArgs a) {
-  const int unit = xcd_unit(blockIdx.x, a.chunk, a.njobs * a.tiles);
-  if (unit < 0) return;
-  const int job = unit / a.tiles, tile = unit - job * a.tiles;
-  const VSGJob jb = a.jobs[job];
-  if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
-  VSGTile t;
-  t.S = a.g.S;
-  t.s = a.g.step;
-  t.n = a.g.n;
-  t.m = a.g.m;
-  t.U = a.g.U;
-  t.V = a.g.V;
-  t.fxc = a.fx[jb.field];
-  t.fyc = a.fy[jb.field];
-  t.ox = jb.ox;
-  t.oy = jb.oy;
-  t.pitch = a.pitch;
-  t.prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
-  t.patch = (vsg_cword_ptr)(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
-                            (size_t)jb.field * t.S * t.S);
-  unsigned long long best = vsg_sad_lane<J, K, D, CW>(t, tile * blockDim.x + threadIdx.x);
-  best = vsg_wave_min_u64(best);
-  if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(&a.best[job], best);
-}
-
 /* ---- refinement --------------------------------------------------------------
-   One workgroup (4 waves) per job.  Thread t handles word w = t % (S/4) of
-   field rows t / (S/4), + rows-per-pass, ...: its loads of the field patch and
-   of the prev rows are consecutive dwords across the wave (coalesced; prev
-   words at any byte offset, which gfx950 global loads accept).  A level's
-   ni x ni candidates (spacing ns around the current best) accumulate per
-   thread, are summed across each wave and then across the 4 waves in LDS,
-   and every thread scans them in the reference's order (i outer, j inner,
-   centre skipped) with the strict "<" against the running minimum --
-   exactly the sequential loop. */
+   Run by one whole workgroup (1-4 waves) per job.  Thread t handles word
+   w = t % (S/4) of field rows t / (S/4), + rows-per-pass, ...: its loads of
+   the field patch and of the prev rows are consecutive dwords across the
+   wave (coalesced; prev words at any byte offset, which gfx950 global loads
+   accept).  A level's ni x ni candidates (spacing ns around the current best)
+   accumulate per thread, are summed across each wave and then across the
+   waves in LDS, and every thread scans them in the reference's order
+   (i outer, j inner, centre skipped) with the strict "<" against the running
+   minimum -- exactly the sequential loop. */
 typedef uint32_t __attribute__((aligned(1))) u32_any;
 constexpr int kMaxNi = 5;   /* ni = 2r/ns + 1 <= 5 for every step */
 constexpr int kNc = kMaxNi * kMaxNi;
+
+struct RefineLds {
+  uint32_t part[4][kNc];
+  uint32_t sums[kNc];
+  unsigned long long wmin[4];
+  unsigned long long key;
+  int last;
+};
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
-  __shared__ uint32_t part[4][kNc];
-  __shared__ uint32_t sums[kNc];
-  const int job = xcd_unit(blockIdx.x, a.chunk, a.njobs);
-  if (job < 0) return;
-  const VSGJob jb = a.jobs[job];
-  if (jb.field < 0) return;
+__device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned long long key, int job,
+                           RefineLds& L) {
+  const int nthr = blockDim.x, nwaves = nthr >> 6;
   const int S = a.g.S, W4 = S >> 2;
-  const int rowsPerPass = 256 / W4;           /* S <= 256 */
+  const int rowsPerPass = nthr / W4;          /* S <= 256, nthr >= 64 */
   const int w = threadIdx.x % W4, r0 = threadIdx.x / W4;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int fxc = a.fx[jb.field], fyc = a.fy[jb.field];
   const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
   const uint32_t* patch = reinterpret_cast<const uint32_t*>(a.patches + (size_t)jb.slot * a.patchSlotStride +
                                                             a.stageOff + (size_t)jb.field * S * S);
-  const unsigned long long key = a.best[job];
   int ga, gb;
   vsg_unrank((int)(uint32_t)key, &ga, &gb);
   int tx = ga * a.g.step, ty = gb * a.g.step;
@@ -129,17 +106,20 @@ __global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
 #pragma unroll
     for (int c = 0; c < kNc; ++c) {
       const uint32_t v = wave_sum(acc[c]);
-      if (lane == 0) part[wave][c] = v;
+      if (lane == 0) L.part[wave][c] = v;
     }
     __syncthreads();
-    if (threadIdx.x < kNc)
-      sums[threadIdx.x] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+    if (threadIdx.x < kNc) {
+      uint32_t t = 0;
+      for (int q = 0; q < nwaves; ++q) t += L.part[q][threadIdx.x];
+      L.sums[threadIdx.x] = t;
+    }
     __syncthreads();
     int btx = tx, bty = ty;
     for (int ti = 0; ti < ni; ++ti)
       for (int tj = 0; tj < ni; ++tj) {
         const int i = tx - r + ns * ti, j = ty - r + ns * tj;
-        const uint32_t e = sums[ti * kMaxNi + tj];
+        const uint32_t e = L.sums[ti * kMaxNi + tj];
         if (!(i == tx && j == ty) && e < minerr) {
           minerr = e;
           btx = i;
@@ -156,14 +136,77 @@ __global__ __launch_bounds__(256) void k_sad_refine(VSGSearchArgs a) {
     res.ty = (int16_t)ty;
     res.minerror = minerr;
     a.results[job] = res;
-    a.best[job] = ~0ull;   /* ready for the next launch */
   }
+}
+
+/* ---- grid search + refinement ---------------------------------------------------
+   Workgroup (job, tile) searches its share of the job's candidate grid.  With
+   one tile per job the workgroup's argmin is reduced in LDS; otherwise every
+   wave folds its argmin into best[job] with a u64 atomic min, and the
+   workgroup that finishes the job last (counted in done[job]) reads the
+   winner and runs the refinement, then resets best / done for the next
+   launch.  The refinement of early jobs overlaps the grid of later ones. */
+template <int J, int K, int D, int CW>
+__global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
+  __shared__ RefineLds L;
+  const int unit = xcd_unit(blockIdx.x, a.chunk, a.njobs * a.tiles);
+  if (unit < 0) return;
+  const int job = unit / a.tiles, tile = unit - job * a.tiles;
+  const VSGJob jb = a.jobs[job];
+  if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
+  VSGTile t;
+  t.S = a.g.S;
+  t.s = a.g.step;
+  t.n = a.g.n;
+  t.m = a.g.m;
+  t.U = a.g.U;
+  t.V = a.g.V;
+  t.fxc = a.fx[jb.field];
+  t.fyc = a.fy[jb.field];
+  t.ox = jb.ox;
+  t.oy = jb.oy;
+  t.pitch = a.pitch;
+  t.prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
+  t.patch = (vsg_cword_ptr)(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
+                            (size_t)jb.field * t.S * t.S);
+  unsigned long long best = vsg_sad_lane<J, K, D, CW>(t, tile * blockDim.x + threadIdx.x);
+  best = vsg_wave_min_u64(best);
+  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  if (a.tiles == 1) {
+    if ((threadIdx.x & 63) == 0) L.wmin[wave] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long k = L.wmin[0];
+      for (int q = 1; q < nwaves; ++q) k = L.wmin[q] < k ? L.wmin[q] : k;
+      L.key = k;
+    }
+    __syncthreads();
+  } else {
+    if ((threadIdx.x & 63) == 0) {
+      if (best != ~0ull) atomicMin(&a.best[job], best);
+      __threadfence();   /* the min is visible before this workgroup counts as done */
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned before = atomicAdd(&a.done[job], 1u);
+      L.last = before == (unsigned)a.tiles - 1;
+      if (L.last) {
+        __threadfence();
+        L.key = __hip_atomic_load(&a.best[job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.best[job] = ~0ull;   /* ready for the next launch */
+        a.done[job] = 0;
+      }
+    }
+    __syncthreads();
+    if (!L.last) return;
+  }
+  refine_job(a, jb, L.key, job, L);
 }
 
 template <int J, int K, int D, int CW>
 hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
   const int nb = xcd_blocks(a.njobs * a.tiles, a.chunk);
-  hipLaunchKernelGGL((k_sad_grid<J, K, D, CW>), dim3(nb), dim3(a.g.wg), 0, st, a);
+  hipLaunchKernelGGL((k_sad_search<J, K, D, CW>), dim3(nb), dim3(a.g.wg), 0, st, a);
   return hipGetLastError();
 }
 
@@ -226,23 +269,11 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift) {
   return (lanes + g->wg - 1) / g->wg;
 }
 
-hipError_t vsg_launch_grid(const VSGSearchArgs& a, hipStream_t st) {
+hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
-  if (a.chunk <= 0) return hipErrorInvalidValue;
+  if (a.chunk <= 0 || a.g.S > 256 || a.g.wg < 64 || a.g.wg > 256) return hipErrorInvalidValue;
   const VSGGrid& g = a.g;
   if (g.J == 4 && g.D == 1) return launch_grid_cw<4, 1>(a, st);
   if (g.J == 4 && g.D == 3) return launch_grid_cw<4, 3>(a, st);
   return launch_grid_cw<1, 1>(a, st);
-}
-
-hipError_t vsg_launch_refine(const VSGSearchArgs& a, hipStream_t st) {
-  if (a.njobs <= 0) return hipSuccess;
-  if (a.g.S > 256 || a.chunk <= 0) return hipErrorInvalidValue;
-  /* one workgroup per job; the grid kernel's chunk counted (job, tile) units */
-  VSGSearchArgs r = a;
-  r.chunk = a.chunk / a.tiles;
-  if (r.chunk < 1) r.chunk = 1;
-  const int nb = xcd_blocks(a.njobs, r.chunk);
-  hipLaunchKernelGGL(k_sad_refine, dim3(nb), dim3(256), 0, st, r);
-  return hipGetLastError();
 }

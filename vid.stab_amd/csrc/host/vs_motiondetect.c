@@ -464,9 +464,11 @@ static int batch_fine(VSMotionDetect* md, MDState* s, Batch* b) {
   return VS_OK;
 }
 
-/* Runs n frames through batches of maxBatch.  Batch k+1 is queued before the
-   host turns batch k's coarse results into fine jobs, batch k+2 right after
-   batch k completes. */
+/* Runs n frames through batches of maxBatch, three in flight: while the host
+   waits for batch k's fine search and cleans it up, the device already runs
+   batch k+2's front stages and coarse search (queued right after batch k's
+   fine search), and batch k+1 waits with its coarse results. */
+#define MD_INFLIGHT 3
 static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_t* const* luma,
                          const int* linesize, int onDevice, int n) {
   if (!md || md->initialized != 2 || !motions || n < 0) return VS_ERROR;
@@ -479,39 +481,46 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
   const int B = s->maxBatch;
   const int nb = (n + B - 1) / B;
   const size_t nfj = (size_t)B * md->fieldsfine.fieldNum;
-  Batch* bs = (Batch*)calloc(2, sizeof(Batch));
-  VSGJob* jobs = (VSGJob*)malloc(sizeof(VSGJob) * nfj * 2);
-  int* sel = (int*)malloc(sizeof(int) * nfj * 2);
+  Batch* bs = (Batch*)calloc(MD_INFLIGHT, sizeof(Batch));
+  VSGJob* jobs = (VSGJob*)malloc(sizeof(VSGJob) * nfj * MD_INFLIGHT);
+  int* sel = (int*)malloc(sizeof(int) * nfj * MD_INFLIGHT);
   if (!bs || !jobs || !sel) {
     free(bs);
     free(jobs);
     free(sel);
     return VS_ERROR;
   }
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < MD_INFLIGHT; k++) {
     bs[k].jobs = jobs + nfj * k;
     bs[k].jobSel = sel + nfj * k;
   }
   int ok = 1, done = 0;
-  /* batches k and k+1 are in flight; bs[k % 2] */
   for (int k = 0; k < nb && k < 2 && ok; k++) {
     const int f0 = k * B, cnt = VS_MIN(B, n - f0);
     ok = batch_submit(md, s, &bs[k], luma + f0, linesize + f0, onDevice, cnt, motions + f0) == VS_OK;
   }
   for (int k = 0; k < nb && ok; k++) {
-    Batch* b = &bs[k % 2];
-    if (batch_coarse(md, s, b) != VS_OK || batch_fine(md, s, b) != VS_OK) {
+    Batch* b = &bs[k % MD_INFLIGHT];
+    if (batch_coarse(md, s, b) != VS_OK) {
+      ok = 0;
+      break;
+    }
+    if (k + 2 < nb) {
+      const int f0 = (k + 2) * B, cnt = VS_MIN(B, n - f0);
+      if (batch_submit(md, s, &bs[(k + 2) % MD_INFLIGHT], luma + f0, linesize + f0, onDevice, cnt,
+                       motions + f0) != VS_OK) {
+        ok = 0;
+        break;
+      }
+    }
+    if (batch_fine(md, s, b) != VS_OK) {
       ok = 0;
       break;
     }
     done += b->n;
-    if (k + 2 < nb) {
-      const int f0 = (k + 2) * B, cnt = VS_MIN(B, n - f0);
-      ok = batch_submit(md, s, b, luma + f0, linesize + f0, onDevice, cnt, motions + f0) == VS_OK;
-    }
   }
   if (!ok) {
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < MD_INFLIGHT; k++)
       if (bs[k].active)
         for (int i = 0; i < bs[k].n; i++) vs_vector_del(&bs[k].coarse[i]);
     for (int i = 0; i < done; i++) vs_vector_del(&motions[i]);
