@@ -137,6 +137,11 @@ def main():
         if prof.launches[k]:
             log(f"kernel {name}: {prof.launches[k]} launches, avg {prof.ms[k] / prof.launches[k]:.4f} ms, "
                 f"total {prof.ms[k] / args.steps:.3f} ms/step")
+    L.vsg_sad_peak.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    peak_meas = ctypes.c_double(0.0)
+    if L.vsg_sad_peak(0, ctypes.byref(peak_meas), None) != 0:
+        peak_meas.value = 0.0
+    log(f"measured v_sad_u8 ceiling: {peak_meas.value / 1e12:.1f} T abs-diff/s")
     k = K_GRID_COARSE
     avg_s = prof.ms[k] / prof.launches[k] / 1000.0
     work = prof.work[k] / prof.launches[k]
@@ -144,6 +149,8 @@ def main():
     roofline = {"bound": "valu", "kernel": "k_sad_grid (coarse)", "achieved": round(achieved, 3),
                 "peak": round(SAD_PEAK_T, 1), "unit": "T abs-diff/s", "frac": round(achieved / SAD_PEAK_T, 4),
                 "traffic": None, "avg_launch_ms": round(avg_s * 1000.0, 4),
+                "peak_measured": round(peak_meas.value / 1e12, 1),
+                "frac_of_measured": round(achieved * 1e12 / peak_meas.value, 4) if peak_meas.value else None,
                 "work_per_launch": work}
     bk = 0
     blur_gbs = prof.work[bk] / (prof.ms[bk] / 1000.0) / 1e9 if prof.ms[bk] else 0.0

@@ -5,7 +5,9 @@
  *
  * Environment (read by vsMotionDetectInit):
  *   VIDSTAB_AMD_DEVICE     HIP device ordinal of new detectors (default 0)
- *   VIDSTAB_AMD_MAX_BATCH  frames per device batch (default 16)
+ *   VIDSTAB_AMD_MAX_BATCH  frames per device batch (default 32, at most 64)
+ *   VIDSTAB_AMD_HOST_THREADS  threads for the per-frame host steps of a batch
+ *                          (default: numThreads, else 0.8 x cores; at most 16)
  */
 #ifndef VIDSTAB_AMD_H
 #define VIDSTAB_AMD_H
@@ -30,6 +32,9 @@ VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions,
 /* copies the last blurred plane of the detector (the reference's md->curr
    luma) into dst; for parity checks */
 VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride);
+
+/* vs_vector_del on motions[0..n-1]: the elements go back through vs_free */
+VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n);
 
 #ifdef __cplusplus
 }

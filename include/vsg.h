@@ -17,9 +17,13 @@
  *
  * A context has VSG_NSETS sets of maxBatch frame slots (global slot =
  * set*maxBatch + i), so the host can prepare one batch's fine stage while the
- * device runs the next batch's coarse stage, and a batch's planes survive
- * until the batch after next.  All work of a context runs in order on one HIP
- * stream; submit calls return at once, wait calls block.
+ * device runs the next batches, and a batch's planes survive until its set is
+ * submitted again.  Each set runs on its own HIP stream; the context orders
+ * what crosses sets (a batch's previous-frame planes, the tripod plane) with
+ * events.  Contract: a batch's prevSlot entries name slots of its own set or
+ * of the batch submitted just before it, and a set is submitted again only
+ * after the fine stage of the batch that followed it has been submitted (or
+ * will never be).  Submit calls return at once, wait calls block.
  *
  * Every function returns 0 on success and a negative value on failure;
  * vsg_last_error() then describes the failure (thread-local).
@@ -120,6 +124,16 @@ typedef struct {
 VS_API void vsg_profile_enable(int on);
 VS_API void vsg_profile_get(VSGProfile* out);
 VS_API void vsg_profile_reset(void);
+
+/* The search plan of a stage (0 coarse, 1 fine) chosen by the autotuner:
+   out[6] = J, K, D, CW, workgroup size, workgroups per job; tuneMs[3] (may
+   be NULL) = the timed batch search with K = 4, 6, 8 (0 when not timed). */
+VS_API int vsg_get_plan(VSGContext* ctx, int stage, int* out, float* tuneMs);
+
+/* Measured packed-u8 SAD ceiling of `device`: a kernel of independent
+   v_sad_u8 chains with a wave-uniform operand and no memory traffic;
+   *absdiffPerSec = 4 abs-diffs per lane per instruction.  Host-synchronous. */
+VS_API int vsg_sad_peak(int device, double* absdiffPerSec, double* ms);
 
 /* host versions of the device spiral helpers, for tests */
 VS_API int vsg_spiral_rank(int a, int b);

@@ -1,15 +1,23 @@
 /*
  * vsg_api.hip -- the thin C ABI over the detect kernels (declared in vsg.h):
- * device buffers, the HIP stream of one detector, uploads and read-backs.
+ * device buffers, the per-set HIP streams of one detector, uploads and read-backs.
  *
- * HBM layout of one context (pitch = width rounded up to 64 bytes; two sets
- * of B = maxBatch slots, global slot = set*B + i):
- *   raw[2B]       uploaded luma planes, pitch x H (host-frame batches only)
- *   blur[2B], ref blurred planes, pitch x (H + padRows); the pad rows keep the
+ * HBM layout of one context (pitch = width rounded up to 64 bytes; VSG_NSETS
+ * sets of B = maxBatch slots, global slot = set*B + i):
+ *   raw[3B]       uploaded luma planes, pitch x H (host-frame batches only)
+ *   blur[3B], ref blurred planes, pitch x (H + padRows); the pad rows keep the
  *                 grid kernel's idle-candidate reads inside the buffer
- *   patches[2B]   an aligned, dense copy of every coarse and fine field
- *   minmax[2B]    8-bit (min, max) per field
+ *   patches[3B]   an aligned, dense copy of every coarse and fine field
+ *   minmax[3B]    8-bit (min, max) per field
  *   per set       selection lists, coarse/fine job lists, best[] keys, results
+ *
+ * Each set has its own stream, so one batch's search can fill the tail of the
+ * previous batch's kernels.  Cross-set hazards are ordered with events: a
+ * batch's coarse search waits for the previous batch's planesReady (which in
+ * turn waited for the one before, so the tripod plane is covered), and a set
+ * is overwritten only after the batch that followed its previous use has
+ * finished its latest search (it read the set's last plane as its previous
+ * frame).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -80,7 +88,8 @@ static void hfree(T*& p) {
 void vsg_ctx_free(VSGContext* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (VSGSet& s : c->set)
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
   for (VSGStage& s : c->st) {
     dfree(s.d_fx);
     dfree(s.d_fy);
@@ -95,14 +104,21 @@ void vsg_ctx_free(VSGContext* c) {
     hfree(s.h_resF); hfree(s.h_src); hfree(s.h_prev); hfree(s.h_detect);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.doneF) (void)hipEventDestroy(s.doneF);
+    if (s.planesReady) (void)hipEventDestroy(s.planesReady);
+    if (s.searchDone) (void)hipEventDestroy(s.searchDone);
+    if (s.stream && c->ownStreams) (void)hipStreamDestroy(s.stream);
     for (hipEvent_t& e : s.ev)
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t& e : s.evf)
       if (e) (void)hipEventDestroy(e);
     s = VSGSet();
   }
-  if (c->stream && c->ownStream) (void)hipStreamDestroy(c->stream);
-  c->stream = nullptr;
+}
+
+int vsg_ctx_sync(VSGContext* c) {
+  VSG_CHECK(hipSetDevice(c->device));
+  for (VSGSet& s : c->set) VSG_CHECK(hipStreamSynchronize(s.stream));
+  return 0;
 }
 
 static void free_ctx(VSGContext* c) {
@@ -123,15 +139,15 @@ static hipError_t halloc(T** p, size_t n) {
 }
 
 int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize, const VSGStageDesc* coarse,
-                 const VSGStageDesc* fine, int B, hipStream_t shared) {
+                 const VSGStageDesc* fine, int B, const hipStream_t* shared) {
   c->device = device;
   VSG_CHECK(hipSetDevice(device));
-  if (shared) {
-    c->stream = shared;
-    c->ownStream = false;
-  } else {
-    VSG_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->ownStream = true;
+  c->ownStreams = shared == nullptr;
+  for (int k = 0; k < VSG_NSETS; ++k) {
+    if (shared)
+      c->set[k].stream = shared[k];
+    else
+      VSG_CHECK(hipStreamCreateWithFlags(&c->set[k].stream, hipStreamNonBlocking));
   }
   c->W = width;
   c->H = height;
@@ -156,7 +172,7 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
         S.rows < 1 || S.rows + 1 > 256)
       return vsg_fail("vsg_create: field grid", hipSuccess);
     maxStep = S.step > maxStep ? S.step : maxStep;
-    S.tiles = vsg_plan_grid(&S.g, S.S, S.step, S.maxShift);
+    S.tiles = vsg_plan_model(&S.g, S.S, S.step, S.maxShift);
     S.candsPerJob = (2.0 * S.g.n + 1.0) * (2.0 * S.g.n + 1.0);
     int rc = 0;
     for (int st = S.step; st > 1; st /= 2) {
@@ -200,9 +216,9 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
   VSG_CHECK(dalloc(&c->d_ctab, 65536));
   VSG_CHECK(hipMemcpy(c->d_ctab, ctab.data(), sizeof(double) * 65536, hipMemcpyHostToDevice));
 
-  /* pad rows: a lane's K candidate rows may reach (K-1)*step rows below a
-     search window, plus column overrun into the next row */
-  c->rowsAlloc = height + 4 * maxStep + 16;
+  /* pad rows: a lane's K <= 8 candidate rows may reach (K-1)*step rows below
+     a search window, plus column overrun into the next row */
+  c->rowsAlloc = height + 8 * maxStep + 16;
   c->slotStride = (size_t)c->pitch * c->rowsAlloc;
   const int slots = VSG_NSETS * B;
   VSG_CHECK(dalloc(&c->d_raw, (size_t)slots * c->pitch * height));
@@ -229,6 +245,7 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
     VSG_CHECK(hipMemset(s.d_best, 0xFF, sizeof(unsigned long long) * (size_t)B * nmax));
     VSG_CHECK(dalloc(&s.d_done, (size_t)B * nmax));
     VSG_CHECK(hipMemset(s.d_done, 0, sizeof(unsigned) * (size_t)B * nmax));
+
     VSG_CHECK(dalloc(&s.d_resC, (size_t)B * nc));
     VSG_CHECK(dalloc(&s.d_resF, (size_t)B * nf));
     VSG_CHECK(dalloc(&s.d_src, B));
@@ -245,11 +262,14 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
     VSG_CHECK(halloc(&s.h_detect, B));
     VSG_CHECK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     VSG_CHECK(hipEventCreateWithFlags(&s.doneF, hipEventDisableTiming));
+    VSG_CHECK(hipEventCreateWithFlags(&s.planesReady, hipEventDisableTiming));
+    VSG_CHECK(hipEventCreateWithFlags(&s.searchDone, hipEventDisableTiming));
     for (hipEvent_t& e : s.ev) VSG_CHECK(hipEventCreate(&e));
     for (hipEvent_t& e : s.evf) VSG_CHECK(hipEventCreate(&e));
   }
+  VSG_CHECK(hipMemset(c->d_patches, 0, (size_t)slots * c->patchSlotStride));
   VSG_CHECK(hipDeviceSynchronize());
-  return 0;
+  return vsg_ctx_tune(c);
 }
 
 extern "C" int vsg_create(VSGContext** out, int device, int width, int height, int blurSize,
@@ -302,14 +322,15 @@ VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, 
    and the selection */
 int vsg_ctx_enqueue_front(VSGContext* c, int set, int n) {
   VSGSet& s = c->set[set];
+  hipStream_t stream = s.stream;
   const int slot0 = set * c->B;
-  VSG_CHECK(hipMemcpyAsync(s.d_src, s.h_src, sizeof(VSGBlurSrc) * n, hipMemcpyHostToDevice, c->stream));
-  VSG_CHECK(hipMemcpyAsync(s.d_prev, s.h_prev, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
-  VSG_CHECK(hipMemcpyAsync(s.d_detect, s.h_detect, n, hipMemcpyHostToDevice, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR0], c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.d_src, s.h_src, sizeof(VSGBlurSrc) * n, hipMemcpyHostToDevice, stream));
+  VSG_CHECK(hipMemcpyAsync(s.d_prev, s.h_prev, sizeof(int) * n, hipMemcpyHostToDevice, stream));
+  VSG_CHECK(hipMemcpyAsync(s.d_detect, s.h_detect, n, hipMemcpyHostToDevice, stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR0], stream));
   VSG_CHECK(vsg_launch_blur(s.d_src, n, c->d_blur + (size_t)slot0 * c->slotStride, c->slotStride, c->pitch,
-                            c->W, c->H, c->blurSize, c->d_tmp, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR1], c->stream));
+                            c->W, c->H, c->blurSize, c->d_tmp, stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR1], stream));
   VSGFieldArgs fa;
   fa.blurred = c->d_blur + (size_t)slot0 * c->slotStride;
   fa.slotStride = c->slotStride;
@@ -322,8 +343,8 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n) {
   fa.patches = c->d_patches + (size_t)slot0 * c->patchSlotStride;
   fa.patchSlotStride = c->patchSlotStride;
   fa.minmax = c->d_minmax + (size_t)slot0 * c->nfTotal * 2;
-  VSG_CHECK(vsg_launch_fields(fa, n, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_FIELDS1], c->stream));
+  VSG_CHECK(vsg_launch_fields(fa, n, stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_FIELDS1], stream));
   VSGSelectArgs sa;
   sa.minmax = c->d_minmax;
   sa.nfTotal = c->nfTotal;
@@ -342,21 +363,22 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n) {
   sa.selC = s.d_selC;
   sa.count = s.d_count;
   sa.jobs = s.d_jobsC;
-  VSG_CHECK(vsg_launch_select(sa, n, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_SELECT1], c->stream));
+  VSG_CHECK(vsg_launch_select(sa, n, stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_SELECT1], stream));
   return 0;
 }
 
 int vsg_ctx_enqueue_coarse_readback(VSGContext* c, int set, int n) {
   VSGSet& s = c->set[set];
+  hipStream_t stream = s.stream;
   const int nc = c->st[0].nf;
-  VSG_CHECK(hipMemcpyAsync(s.h_count, s.d_count, sizeof(int) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.h_count, s.d_count, sizeof(int) * 2 * n, hipMemcpyDeviceToHost, stream));
   VSG_CHECK(hipMemcpyAsync(s.h_selIdx, s.d_selIdx, sizeof(int16_t) * (size_t)n * c->nfTotal,
-                           hipMemcpyDeviceToHost, c->stream));
+                           hipMemcpyDeviceToHost, stream));
   VSG_CHECK(hipMemcpyAsync(s.h_selC, s.d_selC, sizeof(double) * (size_t)n * c->nfTotal,
-                           hipMemcpyDeviceToHost, c->stream));
+                           hipMemcpyDeviceToHost, stream));
   VSG_CHECK(hipMemcpyAsync(s.h_resC, s.d_resC, sizeof(VSGResult) * (size_t)n * nc, hipMemcpyDeviceToHost,
-                           c->stream));
+                           stream));
   return 0;
 }
 
@@ -364,9 +386,11 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
                           int onDevice, const int* prevSlot, const uint8_t* detect, int refSlot) {
   if (!c || set < 0 || set >= VSG_NSETS || n <= 0 || n > c->B || !luma || !linesize || !prevSlot || !detect)
     return vsg_fail("vsg_submit", hipSuccess);
-  if (refSlot >= VSG_NSETS * c->B) return vsg_fail("vsg_submit: refSlot", hipSuccess);
-  VSGSet& s = c->set[set];
   const int slot0 = set * c->B;
+  if (refSlot >= 0 && (refSlot < slot0 || refSlot >= slot0 + n))
+    return vsg_fail("vsg_submit: refSlot outside the batch", hipSuccess);
+  VSGSet& s = c->set[set];
+  hipStream_t stream = s.stream;
   VSG_CHECK(hipSetDevice(c->device));
   /* this set's pinned buffers are reused below: wait until the previous batch
      of the set (its coarse read-back and its fine stage) has completed */
@@ -375,13 +399,19 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   for (int i = 0; i < n; ++i) {
     if (linesize[i] < c->W) return vsg_fail("vsg_submit: linesize < width", hipSuccess);
     if (prevSlot[i] >= VSG_NSETS * c->B) return vsg_fail("vsg_submit: prevSlot", hipSuccess);
+  }
+  /* the set's planes are overwritten below; the batch that followed the set's
+     previous batch reads its last plane as the previous frame, so its latest
+     search must be over first */
+  if (s.follower >= 0) VSG_CHECK(hipStreamWaitEvent(stream, c->set[s.follower].searchDone, 0));
+  for (int i = 0; i < n; ++i) {
     if (onDevice) {
       s.h_src[i].p = luma[i];
       s.h_src[i].pitch = linesize[i];
     } else {
       uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * c->pitch * c->H;
       VSG_CHECK(hipMemcpy2DAsync(dst, c->pitch, luma[i], linesize[i], c->W, c->H, hipMemcpyHostToDevice,
-                                 c->stream));
+                                 stream));
       s.h_src[i].p = dst;
       s.h_src[i].pitch = c->pitch;
     }
@@ -394,14 +424,25 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   if (refSlot >= 0) {
     /* the frozen tripod plane: after the blur, before any search reads it */
     VSG_CHECK(hipMemcpyAsync(c->d_ref, c->d_blur + (size_t)refSlot * c->slotStride, c->slotStride,
-                             hipMemcpyDeviceToDevice, c->stream));
+                             hipMemcpyDeviceToDevice, stream));
   }
+  /* planes of earlier batches (previous frames, the tripod plane) come from
+     other sets' streams: planesReady chains them, so one wait covers all */
+  if (c->lastSet >= 0 && c->lastSet != set) {
+    VSG_CHECK(hipStreamWaitEvent(stream, c->set[c->lastSet].planesReady, 0));
+    c->set[c->lastSet].follower = set;
+  }
+  VSG_CHECK(hipEventRecord(s.planesReady, stream));
+  c->lastSet = set;
+  s.follower = -1;
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID0], stream));
   const int nc = c->st[0].nf;
   VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
-  VSG_CHECK(vsg_launch_search(ga, c->stream));
-  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID1], c->stream));
+  VSG_CHECK(vsg_launch_search(ga, stream));
+  if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID1], stream));
+  VSG_CHECK(hipEventRecord(s.searchDone, stream));
   if (vsg_ctx_enqueue_coarse_readback(c, set, n) != 0) return -1;
-  VSG_CHECK(hipEventRecord(s.done, c->stream));
+  VSG_CHECK(hipEventRecord(s.done, stream));
   s.n = n;
   return 0;
 }
@@ -421,7 +462,7 @@ extern "C" int vsg_wait_coarse(VSGContext* c, int set, VSGBatchView* view) {
     prof_add(VSG_K_FIELDS, s.ev[VSG_EV_BLUR1], s.ev[VSG_EV_FIELDS1], 2.0 * fieldBytes * s.n);
     prof_add(VSG_K_SELECT, s.ev[VSG_EV_FIELDS1], s.ev[VSG_EV_SELECT1], (double)c->nfTotal * s.n);
     const double px = (double)c->st[0].S * c->st[0].S;
-    prof_add(VSG_K_GRID_COARSE, s.ev[VSG_EV_SELECT1], s.ev[VSG_EV_GRID1],
+    prof_add(VSG_K_GRID_COARSE, s.ev[VSG_EV_GRID0], s.ev[VSG_EV_GRID1],
              ncoarse * (c->st[0].candsPerJob + c->st[0].refinePerJob) * px);
   }
   view->count = s.h_count;
@@ -447,18 +488,19 @@ extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* 
   s.njobsF = njobs;
   s.profF = false;
   if (njobs == 0) {
-    VSG_CHECK(hipEventRecord(s.doneF, c->stream));
+    VSG_CHECK(hipEventRecord(s.doneF, s.stream));
     return 0;
   }
   if (jobs != s.h_jobsF) memcpy(s.h_jobsF, jobs, sizeof(VSGJob) * njobs);
-  VSG_CHECK(hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, c->stream));
+  VSG_CHECK(hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, s.stream));
   s.profF = g_profOn.load() != 0;
-  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_0], c->stream));
+  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_0], s.stream));
   VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
-  VSG_CHECK(vsg_launch_search(fa, c->stream));
-  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_GRID1], c->stream));
-  VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, c->stream));
-  VSG_CHECK(hipEventRecord(s.doneF, c->stream));
+  VSG_CHECK(vsg_launch_search(fa, s.stream));
+  if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_GRID1], s.stream));
+  VSG_CHECK(hipEventRecord(s.searchDone, s.stream));
+  VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, s.stream));
+  VSG_CHECK(hipEventRecord(s.doneF, s.stream));
   return 0;
 }
 
@@ -482,7 +524,7 @@ extern "C" int vsg_read_blurred(VSGContext* c, int slot, uint8_t* dst, int dstSt
   if (!c || slot >= VSG_NSETS * c->B || !dst || dstStride < c->W) return vsg_fail("vsg_read_blurred", hipSuccess);
   VSG_CHECK(hipSetDevice(c->device));
   const uint8_t* src = slot < 0 ? c->d_ref : c->d_blur + (size_t)slot * c->slotStride;
-  VSG_CHECK(hipMemcpy2DAsync(dst, dstStride, src, c->pitch, c->W, c->H, hipMemcpyDeviceToHost, c->stream));
-  VSG_CHECK(hipStreamSynchronize(c->stream));
+  if (vsg_ctx_sync(c) != 0) return -1;
+  VSG_CHECK(hipMemcpy2D(dst, dstStride, src, c->pitch, c->W, c->H, hipMemcpyDeviceToHost));
   return 0;
 }

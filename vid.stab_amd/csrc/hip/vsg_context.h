@@ -1,6 +1,6 @@
 /*
  * vsg_context.h -- the device state of one detector (one clip): buffers,
- * the two slot sets and their pinned host twins.  Shared by the single-clip
+ * the slot sets, their streams and their pinned host twins.  Shared by the single-clip
  * ABI (vsg_api.hip) and the multi-clip batched form (vsg_multi.hip).
  */
 #ifndef VSG_CONTEXT_H
@@ -8,7 +8,7 @@
 
 #include "vsg_internal.h"
 
-enum { VSG_EV_BLUR0, VSG_EV_BLUR1, VSG_EV_FIELDS1, VSG_EV_SELECT1, VSG_EV_GRID1, VSG_EV_COUNT };
+enum { VSG_EV_BLUR0, VSG_EV_BLUR1, VSG_EV_FIELDS1, VSG_EV_SELECT1, VSG_EV_GRID0, VSG_EV_GRID1, VSG_EV_COUNT };
 enum { VSG_EVF_0, VSG_EVF_GRID1, VSG_EVF_COUNT };
 
 struct VSGStage {
@@ -19,6 +19,7 @@ struct VSGStage {
   VSGGrid g{};
   int tiles = 0;
   size_t patchOff = 0;
+  float tuneMs[3] = {0, 0, 0};   /* timed plans K = 4, 6, 8 (0: not timed) */
   double candsPerJob = 0;     /* (2n+1)^2 */
   double refinePerJob = 0;    /* refinement candidates the reference visits */
 };
@@ -48,9 +49,13 @@ struct VSGSet {
   int* h_prev = nullptr;
   uint8_t* h_detect = nullptr;
   /* state */
+  hipStream_t stream = nullptr;   /* all work of the set runs in order here */
   int n = 0;              /* frames in flight */
   int njobsF = 0;
-  hipEvent_t done = nullptr, doneF = nullptr;
+  int follower = -1;      /* set submitted right after this set's latest batch */
+  hipEvent_t done = nullptr, doneF = nullptr;   /* coarse / fine read-back complete */
+  hipEvent_t planesReady = nullptr;   /* this and every earlier batch's blurred planes written */
+  hipEvent_t searchDone = nullptr;    /* latest search of the set (coarse, then fine) finished */
   hipEvent_t ev[VSG_EV_COUNT] = {};
   hipEvent_t evf[VSG_EVF_COUNT] = {};
   bool prof = false, profF = false;
@@ -58,8 +63,8 @@ struct VSGSet {
 
 struct VSGContext {
   int device = 0;
-  hipStream_t stream = nullptr;
-  bool ownStream = true;
+  bool ownStreams = true;
+  int lastSet = -1;       /* set of the latest batch */
   int W = 0, H = 0, pitch = 0, rowsAlloc = 0;
   size_t slotStride = 0;
   int blurSize = 0;
@@ -81,10 +86,16 @@ struct VSGContext {
   VSGSet set[VSG_NSETS];
 };
 
-/* creation / destruction shared by both ABIs (vsg_api.hip); a null stream
-   argument creates a private stream */
+/* creation / destruction shared by both ABIs (vsg_api.hip); a null `shared`
+   creates one private stream per set, else set k runs on shared[k] */
 int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize, const VSGStageDesc* coarse,
-                 const VSGStageDesc* fine, int maxBatch, hipStream_t shared);
+                 const VSGStageDesc* fine, int maxBatch, const hipStream_t* shared);
+/* all streams of the context idle */
+int vsg_ctx_sync(VSGContext* c);
+/* plan with the lowest modelled cost; returns workgroups per job */
+int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift);
+/* picks each stage's plan (vsg_tune.hip) */
+int vsg_ctx_tune(VSGContext* c);
 void vsg_ctx_free(VSGContext* c);
 /* enqueue helpers (no host synchronisation) */
 int vsg_ctx_enqueue_front(VSGContext* c, int set, int n);      /* H2D descr., blur, fields, select */

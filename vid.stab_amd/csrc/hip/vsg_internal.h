@@ -85,8 +85,11 @@ struct VSGSelectArgs {
 };
 hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st);
 
-/* fills J, K, D, CW, m, U, V for a stage; returns the workgroups per job */
-int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift);
+/* fills J, D, CW, m, U, V, wg for a stage with K candidate rows per lane
+   (4, 6 or 8); returns the workgroups per job */
+int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K);
+/* modelled instruction count of one job under a plan (for ranking plans) */
+double vsg_plan_cost(const VSGGrid& g, int tiles);
 /* grid search (argmin into best[]) fused with the refinement, which the
    last workgroup of each job runs (best[] -> results) */
 hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st);

@@ -210,15 +210,24 @@ hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int J, int D>
+template <int J, int K, int D>
 hipError_t launch_grid_cw(const VSGSearchArgs& a, hipStream_t st) {
   switch (a.g.CW) {
-    case 28: return launch_grid<J, 4, D, 28>(a, st);
-    case 20: return launch_grid<J, 4, D, 20>(a, st);
-    case 16: return launch_grid<J, 4, D, 16>(a, st);
-    case 12: return launch_grid<J, 4, D, 12>(a, st);
-    case 8: return launch_grid<J, 4, D, 8>(a, st);
-    default: return launch_grid<J, 4, D, 4>(a, st);
+    case 28: return launch_grid<J, K, D, 28>(a, st);
+    case 20: return launch_grid<J, K, D, 20>(a, st);
+    case 16: return launch_grid<J, K, D, 16>(a, st);
+    case 12: return launch_grid<J, K, D, 12>(a, st);
+    case 8: return launch_grid<J, K, D, 8>(a, st);
+    default: return launch_grid<J, K, D, 4>(a, st);
+  }
+}
+
+template <int J, int D>
+hipError_t launch_grid_k(const VSGSearchArgs& a, hipStream_t st) {
+  switch (a.g.K) {
+    case 8: return launch_grid_cw<J, 8, D>(a, st);
+    case 6: return launch_grid_cw<J, 6, D>(a, st);
+    default: return launch_grid_cw<J, 4, D>(a, st);
   }
 }
 
@@ -233,14 +242,14 @@ int gcd_int(int x, int y) {
 
 }  // namespace
 
-int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift) {
+int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K) {
   g->S = S;
   g->step = step;
   g->n = maxShift / step;
   const int G = 2 * g->n + 1;
   const int lcm4 = 4 * step / gcd_int(4, step);
   const int D = lcm4 / 4;
-  g->K = 4;
+  g->K = (K == 6 || K == 8) ? K : 4;
   /* the widest column chunk (<= 28 words) that divides a field row: one prev
      row segment load then feeds J*K*CW v_sad_u8 */
   static const int kCW[] = {28, 20, 16, 12, 8, 4};
@@ -269,11 +278,22 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift) {
   return (lanes + g->wg - 1) / g->wg;
 }
 
+double vsg_plan_cost(const VSGGrid& g, int tiles) {
+  /* instructions one job issues: per lane and prev row, NP+1 loads, NP
+     alignbytes and a few address / scalar-load slots; per lane the J*K*S*CW
+     v_sad_u8 of each column chunk; times the lanes the job occupies */
+  const int NP = g.CW + g.D * (g.J - 1);
+  const int chunks = (g.S / 4) / g.CW;
+  const double rows = g.S + (g.K - 1) * g.step;
+  const double perLane = chunks * (rows * (2.0 * NP + 5.0) + (double)g.K * g.J * g.S * g.CW);
+  return perLane * (double)tiles * g.wg;
+}
+
 hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
   if (a.chunk <= 0 || a.g.S > 256 || a.g.wg < 64 || a.g.wg > 256) return hipErrorInvalidValue;
   const VSGGrid& g = a.g;
-  if (g.J == 4 && g.D == 1) return launch_grid_cw<4, 1>(a, st);
-  if (g.J == 4 && g.D == 3) return launch_grid_cw<4, 3>(a, st);
-  return launch_grid_cw<1, 1>(a, st);
+  if (g.J == 4 && g.D == 1) return launch_grid_k<4, 1>(a, st);
+  if (g.J == 4 && g.D == 3) return launch_grid_k<4, 3>(a, st);
+  return launch_grid_k<1, 1>(a, st);
 }

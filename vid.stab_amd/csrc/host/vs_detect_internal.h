@@ -20,15 +20,16 @@ int vsd_blur_size(const VSMotionDetect* md);
 VSGStageDesc vsd_stage_desc(const VSMotionDetectFields* fs);
 
 /* Frame i of a batch view: builds its coarse LocalMotions (in selection
-   order, rejected fields dropped), computes vsSimpleMotionsToTransform and
-   appends the fine search jobs (slot / prev given) to jobs[]; jobSel[] gets
-   each job's index in the fine selection.  Returns the number of jobs. */
-int vsd_frame_coarse(VSMotionDetect* md, const VSGBatchView* v, int i, int fnum, int slot, int prevSlot,
-                     LocalMotions* coarse, VSGJob* jobs, int* jobSel);
+   order, rejected fields dropped), computes vsSimpleMotionsToTransform into
+   *offset and writes the fine search jobs (slot / prev given) to jobs[];
+   jobSel[] gets each job's index in the fine selection.  Returns the number
+   of jobs.  Reads md only, so frames may run concurrently. */
+int vsd_frame_coarse(const VSMotionDetect* md, const VSGBatchView* v, int i, int fnum, int slot, int prevSlot,
+                     LocalMotions* coarse, VSTransform* offset, VSGJob* jobs, int* jobSel);
 
 /* Frame i: fine results of its njobs jobs -> the cleaned fine motions, then
-   out = coarse ++ fine (the elements of coarse move into out). */
-void vsd_frame_fine(VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
+   out = coarse ++ fine (the elements of coarse move into out).  Reads md only. */
+void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
                     const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, LocalMotions* out);
 
 #endif /* VS_DETECT_INTERNAL_H */

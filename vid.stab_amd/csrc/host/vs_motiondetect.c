@@ -24,6 +24,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "localmotion2transform.h"
 #include "motiondetect.h"
@@ -31,6 +32,7 @@
 #include "vidstab_amd.h"
 #include "vidstabdefines.h"
 #include "vs_detect_internal.h"
+#include "vs_pool.h"
 #include "vsg.h"
 
 #define MD_MAX_BATCH 64
@@ -41,6 +43,7 @@ typedef struct MDState {
   const VSMotionDetect* key;
   struct MDState* next;
   VSGContext* gpu;
+  VSPool* pool;    /* per-frame host steps of a batch */
   int maxBatch;
   int nextSet;     /* slot set of the next batch */
   int lastSlot;    /* global slot holding the previous frame's blurred plane */
@@ -60,6 +63,7 @@ static MDState* state_find(const VSMotionDetect* md) {
 static void state_free(MDState* s) {
   if (!s) return;
   if (s->gpu) vsg_destroy(s->gpu);
+  vs_pool_destroy(s->pool);
   free(s);
 }
 
@@ -262,7 +266,25 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
     free_fields(md);
     return VS_ERROR;
   }
-  vs_log_info(md->conf.modName, "GPU: HIP device %i, batch %i\n", device, s->maxBatch);
+  /* host threads: numThreads if set, else 0.8 x the online cores as the
+     reference does (src/motiondetect.c:98-100), at most 16 */
+  int nth = md->conf.numThreads;
+  if (nth <= 0) {
+    const long cores = sysconf(_SC_NPROCESSORS_ONLN);
+    nth = (int)(cores * 0.8);
+  }
+  nth = VS_MIN(16, VS_MAX(1, env_int("VIDSTAB_AMD_HOST_THREADS", nth)));
+  s->pool = vs_pool_create(nth);
+  for (int k = 0; k < 2; k++) {
+    int pl[6];
+    float tm[3];
+    if (vsg_get_plan(s->gpu, k, pl, tm) == 0)
+      vs_log_info(md->conf.modName,
+                  "%s search plan: J=%i K=%i D=%i CW=%i workgroup %i x %i per field "
+                  "(batch timed at K=4/6/8: %.3f/%.3f/%.3f ms)\n",
+                  k == 0 ? "coarse" : "fine", pl[0], pl[1], pl[2], pl[3], pl[4], pl[5], tm[0], tm[1], tm[2]);
+  }
+  vs_log_info(md->conf.modName, "GPU: HIP device %i, batch %i, %i host threads\n", device, s->maxBatch, nth);
   state_insert(s);
   md->initialized = 2;
   return VS_OK;
@@ -299,11 +321,11 @@ static short lm_match_better(void* thresh, void* lm) {
   return ((LocalMotion*)lm)->match <= *((double*)thresh) ? 1 : 0;
 }
 
-int vsd_frame_coarse(VSMotionDetect* md, const VSGBatchView* v, int i, int fnum, int slot, int prevSlot,
-                     LocalMotions* coarse, VSGJob* jobs, int* jobSel) {
+int vsd_frame_coarse(const VSMotionDetect* md, const VSGBatchView* v, int i, int fnum, int slot, int prevSlot,
+                     LocalMotions* coarse, VSTransform* offset, VSGJob* jobs, int* jobSel) {
   const char* mod = md->conf.modName;
-  VSMotionDetectFields* fc = &md->fieldscoarse;
-  VSMotionDetectFields* ff = &md->fieldsfine;
+  const VSMotionDetectFields* fc = &md->fieldscoarse;
+  const VSMotionDetectFields* ff = &md->fieldsfine;
   vs_vector_init(coarse, fc->maxFields);
   const int16_t* idx = v->idx + (size_t)i * v->stride;
   const double* con = v->contrast + (size_t)i * v->stride;
@@ -318,9 +340,8 @@ int vsd_frame_coarse(VSMotionDetect* md, const VSGBatchView* v, int i, int fnum,
     return 0;
   }
   /* src/motiondetect.c:238-240, 418-434 */
-  ff->offset = vsSimpleMotionsToTransform(md->fi, mod, coarse);
-  ff->useOffset = 1;
-  PreparedTransform pt = prepare_transform(&ff->offset, &md->fi);
+  *offset = vsSimpleMotionsToTransform(md->fi, mod, coarse);
+  PreparedTransform pt = prepare_transform(offset, &md->fi);
   const int16_t* fidx = idx + v->ncoarse;
   int nj = 0;
   for (int q = 0; q < v->count[2 * i + 1]; q++) {
@@ -343,9 +364,9 @@ int vsd_frame_coarse(VSMotionDetect* md, const VSGBatchView* v, int i, int fnum,
   return nj;
 }
 
-void vsd_frame_fine(VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
+void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
                     const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, LocalMotions* out) {
-  VSMotionDetectFields* ff = &md->fieldsfine;
+  const VSMotionDetectFields* ff = &md->fieldsfine;
   LocalMotions fine;
   vs_vector_init(&fine, 0);
   if (vs_vector_size(coarse) >= 1) {
@@ -377,6 +398,9 @@ typedef struct {
   LocalMotions* out;
   LocalMotions coarse[MD_MAX_BATCH];
   int jobStart[MD_MAX_BATCH + 1];
+  int jobCount[MD_MAX_BATCH];
+  VSTransform offset[MD_MAX_BATCH];
+  int jobCap;                  /* fine jobs per frame (fine field count) */
   VSGJob* jobs;                /* fine jobs of the batch (frame order) */
   int* jobSel;
   int njobs;
@@ -418,6 +442,39 @@ static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t*
   return VS_OK;
 }
 
+typedef struct {
+  VSMotionDetect* md;
+  Batch* b;
+  const VSGBatchView* v;
+  const VSGResult* res;
+} BatchWork;
+
+/* frame i: coarse LocalMotions, transform and fine jobs, into the frame's own
+   region of the job list */
+static void frame_coarse_fn(void* ctx, int i) {
+  BatchWork* w = (BatchWork*)ctx;
+  Batch* b = w->b;
+  vs_vector_init(&b->coarse[i], 0);
+  b->jobCount[i] = 0;
+  if (!b->detect[i]) return;
+  const size_t at = (size_t)i * b->jobCap;
+  b->jobCount[i] = vsd_frame_coarse(w->md, w->v, i, b->fnum0 + i, b->slot0 + i, b->prevSlot[i], &b->coarse[i],
+                                    &b->offset[i], b->jobs + at, b->jobSel + at);
+}
+
+static void frame_fine_fn(void* ctx, int i) {
+  BatchWork* w = (BatchWork*)ctx;
+  Batch* b = w->b;
+  if (!b->detect[i]) {
+    vs_vector_fini(&b->coarse[i]);
+    vs_vector_init(&b->out[i], 1);   /* the reference's empty "dummy" vector */
+    return;
+  }
+  const int j0 = b->jobStart[i];
+  vsd_frame_fine(w->md, w->v, i, &b->coarse[i], b->jobs + j0, b->jobSel + j0, w->res + j0,
+                 b->jobStart[i + 1] - j0, &b->out[i]);
+}
+
 /* coarse results -> coarse LocalMotions, transform, fine jobs; submits them */
 static int batch_coarse(VSMotionDetect* md, MDState* s, Batch* b) {
   VSGBatchView v;
@@ -425,16 +482,28 @@ static int batch_coarse(VSMotionDetect* md, MDState* s, Batch* b) {
     vs_log_error(md->conf.modName, "GPU detect failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
-  int nj = 0;
+  BatchWork w = {md, b, &v, NULL};
+  vs_pool_run(s->pool, b->n, frame_coarse_fn, &w);
+  /* pack the per-frame job regions into one list (destinations never pass
+     their sources, so front-to-back moves are safe) */
+  int nj = 0, last = -1;
   for (int i = 0; i < b->n; i++) {
+    const size_t at = (size_t)i * b->jobCap;
     b->jobStart[i] = nj;
-    vs_vector_init(&b->coarse[i], 0);
-    if (!b->detect[i]) continue;
-    nj += vsd_frame_coarse(md, &v, i, b->fnum0 + i, b->slot0 + i, b->prevSlot[i], &b->coarse[i],
-                           b->jobs + nj, b->jobSel + nj);
+    if (b->jobCount[i] > 0 && (size_t)nj != at) {
+      memmove(b->jobs + nj, b->jobs + at, sizeof(VSGJob) * (size_t)b->jobCount[i]);
+      memmove(b->jobSel + nj, b->jobSel + at, sizeof(int) * (size_t)b->jobCount[i]);
+    }
+    nj += b->jobCount[i];
+    if (vs_vector_size(&b->coarse[i]) >= 1) last = i;
   }
   b->jobStart[b->n] = nj;
   b->njobs = nj;
+  if (last >= 0) {
+    /* as after the reference's frame: the last transform stays in fieldsfine */
+    md->fieldsfine.offset = b->offset[last];
+    md->fieldsfine.useOffset = 1;
+  }
   if (vsg_submit_fine(s->gpu, b->set, nj, b->jobs) != 0) {
     vs_log_error(md->conf.modName, "GPU fine search failed: %s\n", vsg_last_error());
     return VS_ERROR;
@@ -450,16 +519,8 @@ static int batch_fine(VSMotionDetect* md, MDState* s, Batch* b) {
     vs_log_error(md->conf.modName, "GPU fine search failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
-  for (int i = 0; i < b->n; i++) {
-    if (!b->detect[i]) {
-      vs_vector_fini(&b->coarse[i]);
-      vs_vector_init(&b->out[i], 1);   /* the reference's empty "dummy" vector */
-      continue;
-    }
-    const int j0 = b->jobStart[i];
-    vsd_frame_fine(md, &v, i, &b->coarse[i], b->jobs + j0, b->jobSel + j0, res + j0,
-                   b->jobStart[i + 1] - j0, &b->out[i]);
-  }
+  BatchWork w = {md, b, &v, res};
+  vs_pool_run(s->pool, b->n, frame_fine_fn, &w);
   b->active = 0;
   return VS_OK;
 }
@@ -493,6 +554,7 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
   for (int k = 0; k < MD_INFLIGHT; k++) {
     bs[k].jobs = jobs + nfj * k;
     bs[k].jobSel = sel + nfj * k;
+    bs[k].jobCap = md->fieldsfine.fieldNum;
   }
   int ok = 1, done = 0;
   for (int k = 0; k < nb && k < 2 && ok; k++) {
@@ -564,6 +626,30 @@ VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions, co
                                    const int* linesize, int n) {
   if (n == 0) return VS_OK;
   return detect_frames(md, motions, lumaDev, linesize, 1, n);
+}
+
+/* bulk release on a library-wide pool: the vectors are independent */
+static VSPool* g_delPool = NULL;
+static pthread_once_t g_delOnce = PTHREAD_ONCE_INIT;
+static void del_pool_init(void) {
+  const long cores = sysconf(_SC_NPROCESSORS_ONLN);
+  g_delPool = vs_pool_create(VS_MIN(8, VS_MAX(1, (int)(cores * 0.8))));
+}
+static void del_fn(void* ctx, int i) {
+  LocalMotions* m = (LocalMotions*)ctx;
+  for (int k = 0; k < 16; k++) vs_vector_del(&m[16 * i + k]);
+}
+
+VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n) {
+  if (!motions || n <= 0) return;
+  const int full = n / 16;
+  if (full >= 2) {
+    pthread_once(&g_delOnce, del_pool_init);
+    vs_pool_run(g_delPool, full, del_fn, motions);
+  } else {
+    for (int i = 0; i < 16 * full; i++) vs_vector_del(&motions[i]);
+  }
+  for (int i = 16 * full; i < n; i++) vs_vector_del(&motions[i]);
 }
 
 VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride) {

@@ -115,6 +115,8 @@ def lib():
         L.vsMotionDetectionCleanup.argtypes = [ctypes.POINTER(VSMotionDetect)]
         L.vsMotionDetectionCleanup.restype = None
         L.vs_vector_del.argtypes = [ctypes.POINTER(VSVector)]
+        L.vsLocalMotionsArrayDel.argtypes = [ctypes.POINTER(VSVector), c_int]
+        L.vsLocalMotionsArrayDel.restype = None
         L.vs_vector_size.argtypes = [ctypes.POINTER(VSVector)]
         L.vs_vector_get.argtypes = [ctypes.POINTER(VSVector), c_int]
         L.vs_vector_get.restype = c_void_p
@@ -202,10 +204,8 @@ class Detector:
         vecs = (VSVector * n)()
         if self.L.vsMotionDetectionDevice(ctypes.byref(self.md), vecs, p, ls, n) != 0:
             raise RuntimeError("vsMotionDetectionDevice failed: " + self.L.vsg_last_error().decode())
-        out = []
-        for v in vecs:
-            out.append(motions_of(v) if keep else v.nelems)
-            self.L.vs_vector_del(ctypes.byref(v))
+        out = [motions_of(v) for v in vecs] if keep else [v.nelems for v in vecs]
+        self.L.vsLocalMotionsArrayDel(vecs, n)
         return out
 
     def blurred(self):
