@@ -33,6 +33,17 @@ VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions,
    luma) into dst; for parity checks */
 VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride);
 
+/* The batched multi-clip form: the detect pass of nclips independent clips
+   at once, frames already in device memory.  Clip c (config confs[c], frame
+   info fis[c]; virtualTripod must be 0) has nframes[c] luma planes
+   lumaDev[c][0..nframes[c]-1] with linesize[c] bytes per row; motions[c][t]
+   receives exactly what the t-th of nframes[c] vsMotionDetection calls on a
+   fresh detector for that clip would return.  Up to VIDSTAB_AMD_MULTI_BATCH
+   (default 8) frames of every clip go through each device batch. */
+VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameInfo* fis, int nclips,
+                               const int* nframes, const uint8_t* const* const* lumaDev, const int* linesize,
+                               LocalMotions* const* motions);
+
 /* vs_vector_del on motions[0..n-1]: the elements go back through vs_free */
 VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n);
 

@@ -125,6 +125,29 @@ VS_API void vsg_profile_enable(int on);
 VS_API void vsg_profile_get(VSGProfile* out);
 VS_API void vsg_profile_reset(void);
 
+/* ---- the batched multi-clip form ---------------------------------------------
+   One context per clip, all on one stream; the searches of all clips that
+   share a plan run as one launch over their flattened job lists, and each
+   slot set's stages are captured once as HIP graphs (fixed shapes: B slots
+   per clip, idle slots and padded fine lists skipped on the device).  Frames
+   must be in device memory.  Per-clip results are read with vsg_wait_coarse
+   / vsg_wait_fine on vsg_multi_clip(m, c) after the multi waits. */
+typedef struct VSGMulti VSGMulti;
+VS_API int vsg_multi_create(VSGMulti** m, int device, int nclips, const int* width, const int* height,
+                            const int* blurSize, const VSGStageDesc* coarse, const VSGStageDesc* fine,
+                            int maxBatch);
+VS_API void vsg_multi_destroy(VSGMulti* m);
+VS_API VSGContext* vsg_multi_clip(VSGMulti* m, int clip);
+/* launches per batch of a stage (clips with one plan share a launch) */
+VS_API int vsg_multi_groups(VSGMulti* m, int stage);
+/* n[c] <= maxBatch frames of clip c (0: idle); lumaDev[c][i], linesize[c];
+   prevSlot[c][i] a slot of the clip (>= 0), detect[c][i] as for vsg_submit */
+VS_API int vsg_multi_submit(VSGMulti* m, int set, const int* n, const uint8_t* const* const* lumaDev,
+                            const int* linesize, const int* const* prevSlot, const uint8_t* const* detect);
+VS_API int vsg_multi_wait_coarse(VSGMulti* m, int set);
+VS_API int vsg_multi_submit_fine(VSGMulti* m, int set, const int* njobs, const VSGJob* const* jobs);
+VS_API int vsg_multi_wait_fine(VSGMulti* m, int set);
+
 /* The search plan of a stage (0 coarse, 1 fine) chosen by the autotuner:
    out[6] = J, K, D, CW, workgroup size, workgroups per job; tuneMs[3] (may
    be NULL) = the timed batch search with K = 4, 6, 8 (0 when not timed). */

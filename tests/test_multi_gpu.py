@@ -1,0 +1,83 @@
+"""The batched multi-clip form (vsMotionDetectMulti) against per-clip plain
+runs of the same frames: every clip's LocalMotions must be bit-identical to
+what one vsMotionDetection stream per clip returns (which the fixtures of
+test_gpu_parity.py pin to the reference).  Clips of uneven size and length
+go through one call; small batch sizes leave idle slots and shorter clips
+finishing early, which the fixed-shape graphs must skip."""
+import sys
+
+import numpy as np
+import pytest
+
+from vidstab_amd import api
+from vidstab_amd.synth import Clip
+
+pytestmark = pytest.mark.gpu
+
+# (config, clip, width, height, frames, stepsize); a mix of geometries, two
+# clips sharing a geometry (one search launch), lengths 1 .. 11
+CLIPS = [(4, 40, 640, 360, 11, 6), (4, 41, 1280, 720, 7, 6), (4, 42, 640, 360, 5, 6),
+         (1, 43, 502, 298, 9, 6), (4, 44, 1920, 1080, 4, 6), (1, 45, 640, 360, 1, 6),
+         (1, 46, 640, 360, 6, 4)]
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _device_clips(clips):
+    import torch
+    out, keep = [], []
+    for config, cid, w, h, n, step in clips:
+        clip = Clip(config, cid, width=w, height=h, nframes=n)
+        s = clip.spec
+        host = np.stack([clip.render(t) for t in range(n)])
+        dev = torch.from_numpy(host).to("cuda:0")
+        keep.append(dev)
+        fb = w * h
+        out.append(dict(width=w, height=h, shakiness=s.shakiness, accuracy=s.accuracy, stepsize=step,
+                        mincontrast=s.mincontrast, linesize=w,
+                        ptrs=[dev.data_ptr() + t * fb for t in range(n)]))
+    torch.cuda.synchronize()
+    return out, keep
+
+
+@pytest.mark.parametrize("batch", [8, 3, 1])
+def test_multi_clip_equals_per_clip(monkeypatch, batch):
+    monkeypatch.setenv("VIDSTAB_AMD_MULTI_BATCH", str(batch))
+    _say(f"[gpu] multi-clip, {len(CLIPS)} clips, batch {batch}")
+    clips, keep = _device_clips(CLIPS)
+    got = api.detect_multi(clips)
+    assert len(got) == len(clips)
+    for c, cl in enumerate(clips):
+        det = api.Detector(cl["width"], cl["height"], shakiness=cl["shakiness"], accuracy=cl["accuracy"],
+                           stepsize=cl["stepsize"], mincontrast=cl["mincontrast"])
+        want = det.detect_device(cl["ptrs"], [cl["linesize"]] * len(cl["ptrs"]), keep=True)
+        det.close()
+        assert len(got[c]) == len(want)
+        for t in range(len(want)):
+            assert got[c][t] == want[t], f"clip {c} frame {t}"
+        assert got[c][0] == []
+        if len(want) > 1:
+            assert len(got[c][1]) > 0
+    del keep
+
+
+def test_multi_clip_rejects_tripod():
+    """virtualTripod has no multi-clip form: an error, while the same call
+    without it (and no frames) succeeds"""
+    import ctypes
+    L = api.lib()
+    fi = (api.VSFrameInfo * 1)()
+    assert L.vsFrameInfoInit(fi, 640, 360, api.PF_YUV420P)
+    nf = (ctypes.c_int * 1)(0)
+    ls = (ctypes.c_int * 1)(640)
+    rows = (ctypes.c_void_p * 1)()
+    vecs = (api.VSVector * 1)()
+    outs = (ctypes.c_void_p * 1)(ctypes.cast(vecs, ctypes.c_void_p))
+    for tripod, want_ok in ((2, False), (0, True)):
+        conf = (api.VSMotionDetectConfig * 1)()
+        conf[0] = L.vsMotionDetectGetDefaultConfig(b"t")
+        conf[0].virtualTripod = tripod
+        rc = L.vsMotionDetectMulti(conf, fi, 1, nf, rows, ls, outs)
+        assert (rc == 0) == want_ok, f"tripod {tripod}: rc {rc}"

@@ -343,6 +343,7 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n) {
   fa.patches = c->d_patches + (size_t)slot0 * c->patchSlotStride;
   fa.patchSlotStride = c->patchSlotStride;
   fa.minmax = c->d_minmax + (size_t)slot0 * c->nfTotal * 2;
+  fa.src = s.d_src;
   VSG_CHECK(vsg_launch_fields(fa, n, stream));
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_FIELDS1], stream));
   VSGSelectArgs sa;

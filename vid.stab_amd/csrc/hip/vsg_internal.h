@@ -37,6 +37,7 @@ struct VSGFieldArgs {
   uint8_t* patches;
   size_t patchSlotStride;
   uint8_t* minmax;            /* [slot][nfields][2] */
+  const VSGBlurSrc* src;      /* per slot of the launch; p == null: idle slot (skipped) */
 };
 hipError_t vsg_launch_fields(const VSGFieldArgs& a, int nslots, hipStream_t st);
 
@@ -93,5 +94,15 @@ double vsg_plan_cost(const VSGGrid& g, int tiles);
 /* grid search (argmin into best[]) fused with the refinement, which the
    last workgroup of each job runs (best[] -> results) */
 hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st);
+
+/* one clip's stage inside a multi-clip search launch */
+struct VSGMultiSeg {
+  VSGSearchArgs a;
+  int unitBase;     /* first unit (workgroup) of the segment */
+  int pad;
+};
+/* all segments share g's plan (J, K, D, CW, wg); dSegs is a device table
+   sorted by unitBase */
+hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, const VSGGrid& g, hipStream_t st);
 
 #endif /* VSG_INTERNAL_H */

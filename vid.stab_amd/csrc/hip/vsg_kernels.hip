@@ -30,6 +30,7 @@ __global__ __launch_bounds__(256) void k_blur(const VSGBlurSrc* __restrict__ src
   const int s2 = size >> 1;
   const int x0 = blockIdx.x * kBlurTW, y0 = blockIdx.y * kBlurTH;
   const VSGBlurSrc src = srcs[blockIdx.z];
+  if (!src.p) return;   /* idle frame slot of a fixed-shape (graph) launch */
   uint8_t* out = dst + (size_t)blockIdx.z * slotStride;
   const int rowsIn = kBlurTH + 2 * s2;
   const int colsIn = kBlurTW + 2 * s2;
@@ -111,18 +112,19 @@ __global__ __launch_bounds__(256) void k_blur_rows(const VSGBlurSrc* __restrict_
                                                    int W, int H, int size, uint32_t mul) {
   const VSGBlurSrc src = srcs[blockIdx.z];
   int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
-  if (x >= W) return;
+  if (x >= W || !src.p) return;
   int s2 = size >> 1;
   uint32_t acc = 0;
   for (int k = -s2; k <= s2; ++k) acc += src.p[(size_t)y * src.pitch + min(max(x + k, 0), W - 1)];
   tmp[(size_t)blockIdx.z * tmpPitch * H + (size_t)y * tmpPitch + x] = (uint8_t)__umulhi(acc, mul);
 }
 
-__global__ __launch_bounds__(256) void k_blur_cols(const uint8_t* __restrict__ tmp, int tmpPitch,
+__global__ __launch_bounds__(256) void k_blur_cols(const VSGBlurSrc* __restrict__ srcs,
+                                                   const uint8_t* __restrict__ tmp, int tmpPitch,
                                                    uint8_t* __restrict__ dst, size_t slotStride,
                                                    int dstPitch, int W, int H, int size, uint32_t mul) {
   int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
-  if (x >= W) return;
+  if (x >= W || !srcs[blockIdx.z].p) return;
   int s2 = size >> 1;
   const uint8_t* t = tmp + (size_t)blockIdx.z * tmpPitch * H;
   uint32_t acc = 0;
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(256) void k_copy_plane(const VSGBlurSrc* __restrict
                                                     int dstPitch, int W, int H) {
   const VSGBlurSrc src = srcs[blockIdx.z];
   int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
-  if (x >= W) return;
+  if (x >= W || !src.p) return;
   dst[(size_t)blockIdx.z * slotStride + (size_t)y * dstPitch + x] = src.p[(size_t)y * src.pitch + x];
 }
 
@@ -154,6 +156,7 @@ __global__ __launch_bounds__(256) void k_fields(VSGFieldArgs a) {
   const int f = blockIdx.x * 4 + wave;
   if (f >= a.nfields) return;
   const int slot = blockIdx.y;
+  if (a.src && !a.src[slot].p) return;   /* idle frame slot */
   const int S = a.fsz[f];
   const int words = S >> 2;
   const int left = a.fx[f] - S / 2, top = a.fy[f] - S / 2;
@@ -328,7 +331,7 @@ hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t 
   if (!tmp) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_blur_rows, dim3((W + 255) / 256, H, n), dim3(256), 0, st, d_src, tmp, dstPitch,
                      W, H, size, mul);
-  hipLaunchKernelGGL(k_blur_cols, dim3((W + 255) / 256, H, n), dim3(256), 0, st, tmp, dstPitch, dst,
+  hipLaunchKernelGGL(k_blur_cols, dim3((W + 255) / 256, H, n), dim3(256), 0, st, d_src, tmp, dstPitch, dst,
                      slotStride, dstPitch, W, H, size, mul);
   return hipGetLastError();
 }

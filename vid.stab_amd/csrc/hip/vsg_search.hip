@@ -10,6 +10,9 @@
  *                 order (src/motiondetect.c:545-567), by the job's last
  *                 workgroup
  *
+ * k_sad_multi runs the same body over the flattened job lists of several
+ * clips at once (the batched multi-clip form).
+ *
  * The launch is a 1-D grid whose block -> work mapping is XCD-aware: the
  * hardware hands workgroups to the 8 XCDs round-robin (block b runs on XCD
  * b % 8), so units of work are grouped in chunks (normally the fields of one
@@ -147,11 +150,7 @@ __device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned lo
    winner and runs the refinement, then resets best / done for the next
    launch.  The refinement of early jobs overlaps the grid of later ones. */
 template <int J, int K, int D, int CW>
-__global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
-  __shared__ RefineLds L;
-  const int unit = xcd_unit(blockIdx.x, a.chunk, a.njobs * a.tiles);
-  if (unit < 0) return;
-  const int job = unit / a.tiles, tile = unit - job * a.tiles;
+__device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int tile, RefineLds& L) {
   const VSGJob jb = a.jobs[job];
   if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
   VSGTile t;
@@ -204,6 +203,40 @@ __global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
 }
 
 template <int J, int K, int D, int CW>
+__global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
+  __shared__ RefineLds L;
+  const int unit = xcd_unit(blockIdx.x, a.chunk, a.njobs * a.tiles);
+  if (unit < 0) return;
+  const int job = unit / a.tiles;
+  search_wg<J, K, D, CW>(a, job, unit - job * a.tiles, L);
+}
+
+/* The multi-clip form: one launch over the flattened units of several
+   segments (the same stage of clips that share a plan, so one workgroup
+   shape fits all).  Segment s owns units [unitBase, unitBase + njobs*tiles);
+   a workgroup finds its segment by binary search over the (wave-uniform)
+   table. */
+template <int J, int K, int D, int CW>
+__global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict__ segs, int nseg, int totalUnits,
+                                                   int chunk) {
+  __shared__ RefineLds L;
+  const int unit = xcd_unit(blockIdx.x, chunk, totalUnits);
+  if (unit < 0) return;
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].unitBase <= unit)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const VSGMultiSeg* sg = segs + lo;
+  const int local = unit - sg->unitBase;
+  const int job = local / sg->a.tiles;
+  search_wg<J, K, D, CW>(sg->a, job, local - job * sg->a.tiles, L);
+}
+
+template <int J, int K, int D, int CW>
 hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
   const int nb = xcd_blocks(a.njobs * a.tiles, a.chunk);
   hipLaunchKernelGGL((k_sad_search<J, K, D, CW>), dim3(nb), dim3(a.g.wg), 0, st, a);
@@ -228,6 +261,35 @@ hipError_t launch_grid_k(const VSGSearchArgs& a, hipStream_t st) {
     case 8: return launch_grid_cw<J, 8, D>(a, st);
     case 6: return launch_grid_cw<J, 6, D>(a, st);
     default: return launch_grid_cw<J, 4, D>(a, st);
+  }
+}
+
+template <int J, int K, int D, int CW>
+hipError_t launch_multi(const VSGMultiSeg* segs, int nseg, int totalUnits, int wg, hipStream_t st) {
+  const int chunk = 32;
+  const int nb = xcd_blocks(totalUnits, chunk);
+  hipLaunchKernelGGL((k_sad_multi<J, K, D, CW>), dim3(nb), dim3(wg), 0, st, segs, nseg, totalUnits, chunk);
+  return hipGetLastError();
+}
+
+template <int J, int K, int D>
+hipError_t launch_multi_cw(const VSGMultiSeg* segs, int nseg, int units, const VSGGrid& g, hipStream_t st) {
+  switch (g.CW) {
+    case 28: return launch_multi<J, K, D, 28>(segs, nseg, units, g.wg, st);
+    case 20: return launch_multi<J, K, D, 20>(segs, nseg, units, g.wg, st);
+    case 16: return launch_multi<J, K, D, 16>(segs, nseg, units, g.wg, st);
+    case 12: return launch_multi<J, K, D, 12>(segs, nseg, units, g.wg, st);
+    case 8: return launch_multi<J, K, D, 8>(segs, nseg, units, g.wg, st);
+    default: return launch_multi<J, K, D, 4>(segs, nseg, units, g.wg, st);
+  }
+}
+
+template <int J, int D>
+hipError_t launch_multi_k(const VSGMultiSeg* segs, int nseg, int units, const VSGGrid& g, hipStream_t st) {
+  switch (g.K) {
+    case 8: return launch_multi_cw<J, 8, D>(segs, nseg, units, g, st);
+    case 6: return launch_multi_cw<J, 6, D>(segs, nseg, units, g, st);
+    default: return launch_multi_cw<J, 4, D>(segs, nseg, units, g, st);
   }
 }
 
@@ -296,4 +358,12 @@ hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st) {
   if (g.J == 4 && g.D == 1) return launch_grid_k<4, 1>(a, st);
   if (g.J == 4 && g.D == 3) return launch_grid_k<4, 3>(a, st);
   return launch_grid_k<1, 1>(a, st);
+}
+
+hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, const VSGGrid& g, hipStream_t st) {
+  if (totalUnits <= 0 || nseg <= 0) return hipSuccess;
+  if (!dSegs || g.S > 256 || g.wg < 64 || g.wg > 256) return hipErrorInvalidValue;
+  if (g.J == 4 && g.D == 1) return launch_multi_k<4, 1>(dSegs, nseg, totalUnits, g, st);
+  if (g.J == 4 && g.D == 3) return launch_multi_k<4, 3>(dSegs, nseg, totalUnits, g, st);
+  return launch_multi_k<1, 1>(dSegs, nseg, totalUnits, g, st);
 }

@@ -1,0 +1,341 @@
+/*
+ * vs_multidetect.c -- host driver of the batched multi-clip form
+ * (vsMotionDetectMulti, declared in vidstab_amd.h).
+ *
+ * Each clip is an independent detect stream with its own VSMotionDetect
+ * state (configured exactly as vsMotionDetectInit does, src/motiondetect.c:
+ * 83-165) and its own previous-frame chain.  A batch takes up to B frames of
+ * every clip; batches rotate through the VSG_NSETS slot sets in the same
+ * three-in-flight order as the single-clip driver (vs_motiondetect.c).  The
+ * per-frame host steps (coarse LocalMotions, vsSimpleMotionsToTransform,
+ * fine job lists, match cleanup) are the single-clip ones and run over all
+ * (clip, frame) pairs of a batch on a thread pool.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include "motiondetect.h"
+#include "vidstab_amd.h"
+#include "vidstabdefines.h"
+#include "vs_detect_internal.h"
+#include "vs_pool.h"
+#include "vsg.h"
+
+#define MULTI_MAX_B 32
+#define MULTI_INFLIGHT 3
+
+typedef struct {
+  VSMotionDetect md;
+  int lastSlot;          /* slot of the previous frame, -1 before the first */
+  int next;              /* next frame to submit */
+  int jobCap;            /* fine fields */
+} MClip;
+
+typedef struct {
+  int n, fnum0, slot0;
+  int prevSlot[MULTI_MAX_B];
+  uint8_t detect[MULTI_MAX_B];
+  const uint8_t* luma[MULTI_MAX_B];
+  LocalMotions coarse[MULTI_MAX_B];
+  VSTransform offset[MULTI_MAX_B];
+  int jobStart[MULTI_MAX_B + 1];
+  int jobCount[MULTI_MAX_B];
+  VSGJob* jobs;
+  int* jobSel;
+  VSGBatchView view;
+  const VSGResult* res;
+} MClipBatch;
+
+typedef struct {
+  int set;
+  int active;
+  MClipBatch* cb;        /* [nclips] */
+  int* itemClip;         /* (clip, frame) items of the batch */
+  int* itemFrame;
+  int nitems;
+} MBatch;
+
+typedef struct {
+  int nclips, B;
+  MClip* clips;
+  VSGMulti* gpu;
+  VSPool* pool;
+  LocalMotions* const* motions;
+  const uint8_t* const* const* luma;
+  const int* nframes;
+  const int* linesize;
+  /* per clip scratch for the multi ABI */
+  int* n;
+  const uint8_t* const** rows;
+  const int** prevs;
+  const uint8_t** dets;
+  const VSGJob** jobs;
+} Multi;
+
+typedef struct {
+  Multi* M;
+  MBatch* b;
+} Work;
+
+static int env_int(const char* name, int def) {
+  const char* v = getenv(name);
+  if (!v || !*v) return def;
+  char* end = NULL;
+  long x = strtol(v, &end, 10);
+  return (end && *end == 0) ? (int)x : def;
+}
+
+static void coarse_fn(void* ctx, int k) {
+  Work* w = (Work*)ctx;
+  const int c = w->b->itemClip[k], i = w->b->itemFrame[k];
+  MClip* cl = &w->M->clips[c];
+  MClipBatch* cb = &w->b->cb[c];
+  vs_vector_init(&cb->coarse[i], 0);
+  cb->jobCount[i] = 0;
+  if (!cb->detect[i]) return;
+  const size_t at = (size_t)i * cl->jobCap;
+  cb->jobCount[i] = vsd_frame_coarse(&cl->md, &cb->view, i, cb->fnum0 + i, cb->slot0 + i, cb->prevSlot[i],
+                                     &cb->coarse[i], &cb->offset[i], cb->jobs + at, cb->jobSel + at);
+}
+
+static void fine_fn(void* ctx, int k) {
+  Work* w = (Work*)ctx;
+  const int c = w->b->itemClip[k], i = w->b->itemFrame[k];
+  MClip* cl = &w->M->clips[c];
+  MClipBatch* cb = &w->b->cb[c];
+  LocalMotions* out = &w->M->motions[c][cb->fnum0 + i];
+  if (!cb->detect[i]) {
+    vs_vector_fini(&cb->coarse[i]);
+    vs_vector_init(out, 1);   /* the reference's empty "dummy" vector */
+    return;
+  }
+  const int j0 = cb->jobStart[i];
+  vsd_frame_fine(&cl->md, &cb->view, i, &cb->coarse[i], cb->jobs + j0, cb->jobSel + j0, cb->res + j0,
+                 cb->jobStart[i + 1] - j0, out);
+}
+
+/* next frames of every clip into set `set` */
+static int batch_submit(Multi* M, MBatch* b, int set) {
+  b->set = set;
+  b->nitems = 0;
+  for (int c = 0; c < M->nclips; c++) {
+    MClip* cl = &M->clips[c];
+    MClipBatch* cb = &b->cb[c];
+    const int n = VS_MIN(M->B, M->nframes[c] - cl->next);
+    cb->n = n;
+    cb->fnum0 = cl->next;
+    cb->slot0 = set * M->B;
+    for (int i = 0; i < n; i++) {
+      const int fnum = cl->next + i;
+      /* every frame after a clip's first is measured against the frame before
+         (src/motiondetect.c:211-218, 294-303) */
+      cb->detect[i] = (uint8_t)(fnum > 0);
+      cb->prevSlot[i] = i == 0 ? (cl->lastSlot >= 0 ? cl->lastSlot : cb->slot0) : cb->slot0 + i - 1;
+      cb->luma[i] = M->luma[c][fnum];
+      b->itemClip[b->nitems] = c;
+      b->itemFrame[b->nitems] = i;
+      b->nitems++;
+    }
+    if (n > 0) {
+      cl->next += n;
+      cl->lastSlot = cb->slot0 + n - 1;
+    }
+    M->n[c] = n;
+    M->rows[c] = cb->luma;
+    M->prevs[c] = cb->prevSlot;
+    M->dets[c] = cb->detect;
+  }
+  if (vsg_multi_submit(M->gpu, set, M->n, M->rows, M->linesize, M->prevs, M->dets) != 0) {
+    vs_log_error("vidstab_amd", "GPU multi-clip detect failed: %s\n", vsg_last_error());
+    return VS_ERROR;
+  }
+  b->active = 1;
+  return VS_OK;
+}
+
+static int batch_coarse(Multi* M, MBatch* b) {
+  if (vsg_multi_wait_coarse(M->gpu, b->set) != 0) goto fail;
+  for (int c = 0; c < M->nclips; c++)
+    if (vsg_wait_coarse(vsg_multi_clip(M->gpu, c), b->set, &b->cb[c].view) != 0) goto fail;
+  {
+    Work w = {M, b};
+    vs_pool_run(M->pool, b->nitems, coarse_fn, &w);
+  }
+  int* nj = M->n;
+  for (int c = 0; c < M->nclips; c++) {
+    MClipBatch* cb = &b->cb[c];
+    const int cap = M->clips[c].jobCap;
+    int n = 0;
+    for (int i = 0; i < cb->n; i++) {
+      const size_t at = (size_t)i * cap;
+      cb->jobStart[i] = n;
+      if (cb->jobCount[i] > 0 && (size_t)n != at) {
+        memmove(cb->jobs + n, cb->jobs + at, sizeof(VSGJob) * (size_t)cb->jobCount[i]);
+        memmove(cb->jobSel + n, cb->jobSel + at, sizeof(int) * (size_t)cb->jobCount[i]);
+      }
+      n += cb->jobCount[i];
+    }
+    cb->jobStart[cb->n] = n;
+    nj[c] = n;
+    M->jobs[c] = cb->jobs;
+  }
+  if (vsg_multi_submit_fine(M->gpu, b->set, nj, M->jobs) != 0) goto fail;
+  return VS_OK;
+fail:
+  vs_log_error("vidstab_amd", "GPU multi-clip detect failed: %s\n", vsg_last_error());
+  return VS_ERROR;
+}
+
+static int batch_fine(Multi* M, MBatch* b) {
+  if (vsg_multi_wait_fine(M->gpu, b->set) != 0) goto fail;
+  for (int c = 0; c < M->nclips; c++) {
+    if (vsg_wait_coarse(vsg_multi_clip(M->gpu, c), b->set, &b->cb[c].view) != 0 ||
+        vsg_wait_fine(vsg_multi_clip(M->gpu, c), b->set, &b->cb[c].res) != 0)
+      goto fail;
+  }
+  {
+    Work w = {M, b};
+    vs_pool_run(M->pool, b->nitems, fine_fn, &w);
+  }
+  b->active = 0;
+  return VS_OK;
+fail:
+  vs_log_error("vidstab_amd", "GPU multi-clip fine search failed: %s\n", vsg_last_error());
+  return VS_ERROR;
+}
+
+static void multi_free(Multi* M, MBatch* bs) {
+  if (bs) {
+    for (int k = 0; k < MULTI_INFLIGHT; k++) {
+      if (bs[k].cb) {
+        for (int c = 0; c < M->nclips; c++) {
+          if (bs[k].active)
+            for (int i = 0; i < bs[k].cb[c].n; i++) vs_vector_del(&bs[k].cb[c].coarse[i]);
+          free(bs[k].cb[c].jobs);
+          free(bs[k].cb[c].jobSel);
+        }
+        free(bs[k].cb);
+      }
+      free(bs[k].itemClip);
+      free(bs[k].itemFrame);
+    }
+    free(bs);
+  }
+  if (M->gpu) vsg_multi_destroy(M->gpu);
+  vs_pool_destroy(M->pool);
+  if (M->clips) {
+    for (int c = 0; c < M->nclips; c++) {
+      VSMotionDetect* md = &M->clips[c].md;
+      if (md->fieldscoarse.fields) vs_free(md->fieldscoarse.fields);
+      if (md->fieldsfine.fields) vs_free(md->fieldsfine.fields);
+    }
+    free(M->clips);
+  }
+  free(M->n);
+  free((void*)M->rows);
+  free((void*)M->prevs);
+  free((void*)M->dets);
+  free((void*)M->jobs);
+}
+
+VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameInfo* fis, int nclips,
+                               const int* nframes, const uint8_t* const* const* lumaDev, const int* linesize,
+                               LocalMotions* const* motions) {
+  if (!confs || !fis || nclips <= 0 || !nframes || !lumaDev || !linesize || !motions) return VS_ERROR;
+  Multi M;
+  memset(&M, 0, sizeof(M));
+  M.nclips = nclips;
+  M.B = VS_MIN(MULTI_MAX_B, VS_MAX(1, env_int("VIDSTAB_AMD_MULTI_BATCH", 8)));
+  M.motions = motions;
+  M.luma = lumaDev;
+  M.nframes = nframes;
+  M.linesize = linesize;
+  M.clips = (MClip*)calloc((size_t)nclips, sizeof(MClip));
+  M.n = (int*)calloc((size_t)nclips, sizeof(int));
+  M.rows = (const uint8_t* const**)calloc((size_t)nclips, sizeof(void*));
+  M.prevs = (const int**)calloc((size_t)nclips, sizeof(void*));
+  M.dets = (const uint8_t**)calloc((size_t)nclips, sizeof(void*));
+  M.jobs = (const VSGJob**)calloc((size_t)nclips, sizeof(void*));
+  MBatch* bs = (MBatch*)calloc(MULTI_INFLIGHT, sizeof(MBatch));
+  int* w = (int*)calloc((size_t)nclips, sizeof(int));
+  int* h = (int*)calloc((size_t)nclips, sizeof(int));
+  int* blur = (int*)calloc((size_t)nclips, sizeof(int));
+  VSGStageDesc* dc = (VSGStageDesc*)calloc((size_t)nclips, sizeof(VSGStageDesc));
+  VSGStageDesc* df = (VSGStageDesc*)calloc((size_t)nclips, sizeof(VSGStageDesc));
+  int ok = M.clips && M.n && M.rows && M.prevs && M.dets && M.jobs && bs && w && h && blur && dc && df;
+  int maxFrames = 0;
+  for (int c = 0; c < nclips && ok; c++) {
+    MClip* cl = &M.clips[c];
+    if (confs[c].virtualTripod > 0) {
+      vs_log_error(confs[c].modName, "the multi-clip form does not take virtualTripod\n");
+      ok = 0;
+      break;
+    }
+    if (nframes[c] < 0 || vsd_configure(&cl->md, &confs[c], &fis[c]) != VS_OK) {
+      ok = 0;
+      break;
+    }
+    cl->lastSlot = -1;
+    cl->jobCap = cl->md.fieldsfine.fieldNum;
+    w[c] = cl->md.fi.width;
+    h[c] = cl->md.fi.height;
+    blur[c] = vsd_blur_size(&cl->md);
+    dc[c] = vsd_stage_desc(&cl->md.fieldscoarse);
+    df[c] = vsd_stage_desc(&cl->md.fieldsfine);
+    maxFrames = VS_MAX(maxFrames, nframes[c]);
+  }
+  if (ok && vsg_multi_create(&M.gpu, env_int("VIDSTAB_AMD_DEVICE", 0), nclips, w, h, blur, dc, df, M.B) != 0) {
+    vs_log_error("vidstab_amd", "GPU multi-clip initialisation failed: %s\n", vsg_last_error());
+    ok = 0;
+  }
+  free(w);
+  free(h);
+  free(blur);
+  free(dc);
+  free(df);
+  if (ok) {
+    const long cores = sysconf(_SC_NPROCESSORS_ONLN);
+    M.pool = vs_pool_create(VS_MIN(16, VS_MAX(1, env_int("VIDSTAB_AMD_HOST_THREADS", (int)(cores * 0.8)))));
+    for (int k = 0; k < MULTI_INFLIGHT && ok; k++) {
+      bs[k].cb = (MClipBatch*)calloc((size_t)nclips, sizeof(MClipBatch));
+      bs[k].itemClip = (int*)malloc(sizeof(int) * (size_t)nclips * M.B);
+      bs[k].itemFrame = (int*)malloc(sizeof(int) * (size_t)nclips * M.B);
+      ok = bs[k].cb && bs[k].itemClip && bs[k].itemFrame;
+      for (int c = 0; c < nclips && ok; c++) {
+        const size_t cap = (size_t)M.B * M.clips[c].jobCap;
+        bs[k].cb[c].jobs = (VSGJob*)malloc(sizeof(VSGJob) * (cap ? cap : 1));
+        bs[k].cb[c].jobSel = (int*)malloc(sizeof(int) * (cap ? cap : 1));
+        ok = bs[k].cb[c].jobs && bs[k].cb[c].jobSel;
+      }
+    }
+  }
+  const int nb = (maxFrames + M.B - 1) / M.B;
+  int done = 0;
+  for (int k = 0; k < nb && k < 2 && ok; k++) ok = batch_submit(&M, &bs[k], k % VSG_NSETS) == VS_OK;
+  for (int k = 0; k < nb && ok; k++) {
+    MBatch* b = &bs[k % MULTI_INFLIGHT];
+    if (batch_coarse(&M, b) != VS_OK) {
+      ok = 0;
+      break;
+    }
+    if (k + 2 < nb && batch_submit(&M, &bs[(k + 2) % MULTI_INFLIGHT], (k + 2) % VSG_NSETS) != VS_OK) {
+      ok = 0;
+      break;
+    }
+    if (batch_fine(&M, b) != VS_OK) {
+      ok = 0;
+      break;
+    }
+    done = k + 1;
+  }
+  if (!ok) {
+    /* release what was produced: the first `done` batches of every clip */
+    for (int c = 0; c < nclips; c++) {
+      const int upto = VS_MIN(nframes[c], done * M.B);
+      for (int t = 0; t < upto; t++) vs_vector_del(&motions[c][t]);
+    }
+  }
+  multi_free(&M, bs);
+  return ok ? VS_OK : VS_ERROR;
+}

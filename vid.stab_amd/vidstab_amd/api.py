@@ -73,7 +73,8 @@ class VSVector(ctypes.Structure):
 
 class VSGStageDesc(ctypes.Structure):
     _fields_ = [("fieldSize", c_int), ("maxShift", c_int), ("stepSize", c_int),
-                ("nfields", c_int), ("fields", ctypes.POINTER(Field))]
+                ("nfields", c_int), ("fieldRows", c_int), ("maxFields", c_int),
+                ("contrastThreshold", ctypes.c_double), ("fields", ctypes.POINTER(Field))]
 
 
 class VSGJob(ctypes.Structure):
@@ -116,6 +117,10 @@ def lib():
         L.vsMotionDetectionCleanup.restype = None
         L.vs_vector_del.argtypes = [ctypes.POINTER(VSVector)]
         L.vsLocalMotionsArrayDel.argtypes = [ctypes.POINTER(VSVector), c_int]
+        L.vsMotionDetectMulti.argtypes = [ctypes.POINTER(VSMotionDetectConfig), ctypes.POINTER(VSFrameInfo),
+                                          c_int, ctypes.POINTER(c_int), c_void_p, ctypes.POINTER(c_int),
+                                          c_void_p]
+        L.vsMotionDetectMulti.restype = c_int
         L.vsLocalMotionsArrayDel.restype = None
         L.vs_vector_size.argtypes = [ctypes.POINTER(VSVector)]
         L.vs_vector_get.argtypes = [ctypes.POINTER(VSVector), c_int]
@@ -223,3 +228,45 @@ class Detector:
             self.close()
         except Exception:
             pass
+
+
+def detect_multi(clips, keep=True, mod=b"vidstab_amd"):
+    """The batched multi-clip form (vsMotionDetectMulti).
+
+    clips: sequence of dicts with width, height, shakiness, accuracy,
+    stepsize, mincontrast, ptrs (device addresses of the luma planes, one per
+    frame) and linesize.  Returns per clip the per-frame LocalMotions
+    (tuples, as motions_of) or, with keep=False, the per-frame counts."""
+    L = lib()
+    n = len(clips)
+    confs = (VSMotionDetectConfig * n)()
+    fis = (VSFrameInfo * n)()
+    nframes = (c_int * n)()
+    ls = (c_int * n)()
+    rows = (c_void_p * n)()
+    outs = (c_void_p * n)()
+    keepalive = []
+    for c, cl in enumerate(clips):
+        if not L.vsFrameInfoInit(ctypes.byref(fis[c]), cl["width"], cl["height"], PF_YUV420P):
+            raise ValueError("vsFrameInfoInit failed")
+        conf = L.vsMotionDetectGetDefaultConfig(mod)
+        conf.shakiness, conf.accuracy, conf.stepSize = cl["shakiness"], cl["accuracy"], cl["stepsize"]
+        conf.contrastThreshold = float(cl["mincontrast"])
+        confs[c] = conf
+        nf = len(cl["ptrs"])
+        nframes[c] = nf
+        ls[c] = cl["linesize"]
+        p = (c_void_p * max(nf, 1))(*cl["ptrs"])
+        v = (VSVector * max(nf, 1))()
+        keepalive += [p, v]
+        rows[c] = ctypes.cast(p, c_void_p)
+        outs[c] = ctypes.cast(v, c_void_p)
+    if L.vsMotionDetectMulti(confs, fis, n, nframes, rows, ls, outs) != 0:
+        raise RuntimeError("vsMotionDetectMulti failed: " + L.vsg_last_error().decode())
+    res = []
+    for c in range(n):
+        v = keepalive[2 * c + 1]
+        nf = nframes[c]
+        res.append([motions_of(v[t]) for t in range(nf)] if keep else [v[t].nelems for t in range(nf)])
+        L.vsLocalMotionsArrayDel(v, nf)
+    return res
