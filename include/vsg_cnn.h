@@ -1,0 +1,62 @@
+/*
+ * vsg_cnn.h -- C ABI of the proposal network's gfx950 kernels (learned
+ * detect form): BF16 3x3 convolutions on the matrix cores with FP32
+ * accumulation, their data / weight gradients, field pooling, loss and Adam.
+ * The learned form has no counterpart in the reference; these entry points
+ * serve its training driver (vidstab_amd/learned.py) and the learned detect
+ * path.  All pointers are device pointers; `stream` is a hipStream_t (NULL:
+ * the default stream).  Functions return 0 or a HIP error code.
+ *
+ * Activations: NHWC, C = 32, BF16 bit patterns, index ((n*H+y)*W+x)*32+c.
+ * Weights: FP32 [9 taps][32 in][32 out], tap = (dy+1)*3 + (dx+1); bias FP32 [32].
+ */
+#ifndef VSG_CNN_H
+#define VSG_CNN_H
+
+#include <stdint.h>
+#include "vidstab_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* MFMA B-operand packing of one layer (9*4*64*4 BF16 values): forward, or
+   (transposed != 0) flipped and transposed for the data gradient */
+VS_API int vsgc_pack_weights(const float* w, uint16_t* packed, int transposed, void* stream);
+
+/* out = act(conv3x3_dilated(in) + bias): mode 0 -> BF16 + ReLU into outb,
+   1 -> BF16 into outb, 2 -> FP32 into outf; padding = dilation; bias may be
+   NULL (the data gradient: conv of dz with the transposed packing) */
+VS_API int vsgc_conv(const uint16_t* in, int n, int h, int w, const uint16_t* packed, const float* bias,
+                     int dil, int mode, uint16_t* outb, float* outf, void* stream);
+
+/* dW += d(loss)/dW and dB += d(loss)/dB of a layer from its input and the
+   gradient dz at its pre-activation (accumulates; dB may be NULL) */
+VS_API int vsgc_conv_wgrad(const uint16_t* in, const uint16_t* dz, int n, int h, int w, int dil, float* dW,
+                           float* dB, void* stream);
+
+/* grad[i] = 0 where act[i] <= 0 (ReLU backward, in place) */
+VS_API int vsgc_relu_mask(uint16_t* grad, const uint16_t* act, long long count, void* stream);
+VS_API int vsgc_f32_to_bf16(const float* f, uint16_t* b, long long count, void* stream);
+
+/* channel `chan` of an NHWC-32 map (width/4 x height/4) = (4x4 mean - 128)/64 */
+VS_API int vsgc_downsample(const uint8_t* plane, int pitch, int width, int height, uint16_t* out, int chan,
+                           void* stream);
+
+/* windows: nf x {sample, x0, y0, x1, y1} (half-open) on the map;
+   pred[f] = mean of channels 0, 1 of the FP32 map over window f */
+VS_API int vsgc_field_pool(const float* map, int h, int w, const int* windows, int nf, float* pred, void* stream);
+/* dmap (FP32, zeroed by the caller) += dpred[f] / area over window f */
+VS_API int vsgc_field_pool_bwd(const float* dpred, int h, int w, const int* windows, int nf, float* dmap,
+                               void* stream);
+/* loss += sum_f weight[f] * |pred - label|^2 * inv; dpred = its gradient */
+VS_API int vsgc_field_loss(const float* pred, const float* label, const float* weight, int nf, float inv,
+                           float* dpred, float* loss, void* stream);
+/* one Adam step (bias-corrected, step >= 1) */
+VS_API int vsgc_adam(float* w, const float* g, float* m, float* v, int count, float lr, float b1, float b2,
+                     float eps, int step, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VSG_CNN_H */

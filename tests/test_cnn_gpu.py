@@ -1,0 +1,151 @@
+"""The proposal network's gfx950 kernels against the CPU oracle
+(oracle/cnn_oracle.py): MFMA convolution forward in all three output modes,
+the data gradient through the transposed packing, the weight and bias
+gradients, and one full forward + loss + backward of the 6-layer network.
+
+Tolerances: inputs and weights are BF16 on both sides, so every product is
+exact; the device sums in FP32 (relative error well below 1e-4 for these
+sizes) and rounds stored activations to BF16 (relative 2^-8).  Outputs
+stored in BF16 are compared within 2^-7 relative plus a small absolute term;
+FP32 results within 1e-3 of the largest magnitude.  The network gradients
+pass through five BF16 roundings of activations and of dz on the device;
+the oracle applies the same roundings, and the check is 3e-2 of the largest
+gradient of each layer."""
+import numpy as np
+import pytest
+
+import cnn_oracle as co
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev_bf16(a):
+    from vidstab_amd import learned
+    torch = _torch()
+    return torch.from_numpy(learned.bf16_bits(a)).to("cuda")
+
+
+def _host_f(t):
+    from vidstab_amd import learned
+    return learned.bits_to_float(t.cpu().numpy()).astype(np.float64)
+
+
+def _rand_map(rng, n, h, w, scale=1.0):
+    return co.bf16_round(rng.standard_normal((n, h, w, 32)).astype(np.float32) * scale)
+
+
+@pytest.mark.parametrize("dil", [1, 2, 5])
+def test_conv_forward_modes(dil):
+    from vidstab_amd import learned
+    torch = _torch()
+    L = learned.lib()
+    rng = np.random.default_rng(10 + dil)
+    n, h, w = 2, 13, 45
+    x = _rand_map(rng, n, h, w)
+    wt = co.bf16_round(rng.standard_normal((9, 32, 32)).astype(np.float32) * 0.1)
+    b = rng.standard_normal(32).astype(np.float32) * 0.1
+    ref = co.conv(x, wt, b.astype(np.float64), dil)
+    dx = _dev_bf16(x)
+    dw = torch.from_numpy(wt.astype(np.float32)).cuda()
+    db = torch.from_numpy(b).cuda()
+    pk = torch.empty(9 * 4 * 64 * 4, dtype=torch.int16, device="cuda")
+    st = learned._stream()
+    assert L.vsgc_pack_weights(learned._p(dw), learned._p(pk), 0, st) == 0
+    outf = torch.empty((n, h, w, 32), dtype=torch.float32, device="cuda")
+    assert L.vsgc_conv(learned._p(dx), n, h, w, learned._p(pk), learned._p(db), dil, 2, None,
+                       learned._p(outf), st) == 0
+    got = outf.cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(got - ref)) <= 1e-3 * np.max(np.abs(ref))
+    for mode, fn in ((0, lambda z: np.maximum(z, 0.0)), (1, lambda z: z)):
+        outb = torch.empty((n, h, w, 32), dtype=torch.int16, device="cuda")
+        assert L.vsgc_conv(learned._p(dx), n, h, w, learned._p(pk), learned._p(db), dil, mode,
+                           learned._p(outb), None, st) == 0
+        got = _host_f(outb)
+        want = fn(ref)
+        assert np.all(np.abs(got - want) <= 2.0 ** -7 * np.abs(want) + 1e-3), f"mode {mode}"
+
+
+@pytest.mark.parametrize("dil", [1, 3])
+def test_conv_gradients(dil):
+    from vidstab_amd import learned
+    torch = _torch()
+    L = learned.lib()
+    rng = np.random.default_rng(20 + dil)
+    n, h, w = 2, 11, 37
+    x = _rand_map(rng, n, h, w)
+    dz = _rand_map(rng, n, h, w, 0.5)
+    wt = co.bf16_round(rng.standard_normal((9, 32, 32)).astype(np.float32) * 0.1)
+    st = learned._stream()
+    # data gradient: convolution of dz with the flipped, transposed packing
+    dw = torch.from_numpy(wt.astype(np.float32)).cuda()
+    pkT = torch.empty(9 * 4 * 64 * 4, dtype=torch.int16, device="cuda")
+    assert L.vsgc_pack_weights(learned._p(dw), learned._p(pkT), 1, st) == 0
+    ddz = _dev_bf16(dz)
+    dxo = torch.empty((n, h, w, 32), dtype=torch.int16, device="cuda")
+    assert L.vsgc_conv(learned._p(ddz), n, h, w, learned._p(pkT), None, dil, 1, learned._p(dxo), None, st) == 0
+    want = co.conv_data_grad(dz, wt, dil)
+    got = _host_f(dxo)
+    assert np.all(np.abs(got - want) <= 2.0 ** -7 * np.abs(want) + 1e-3)
+    # weight and bias gradients
+    gW = torch.zeros((9, 32, 32), dtype=torch.float32, device="cuda")
+    gB = torch.zeros(32, dtype=torch.float32, device="cuda")
+    assert L.vsgc_conv_wgrad(learned._p(_dev_bf16(x)), learned._p(ddz), n, h, w, dil, learned._p(gW),
+                             learned._p(gB), st) == 0
+    wW, wB = co.conv_weight_grad(x, dz, dil)
+    assert np.max(np.abs(gW.cpu().numpy() - wW)) <= 1e-3 * np.max(np.abs(wW))
+    assert np.max(np.abs(gB.cpu().numpy() - wB)) <= 1e-3 * np.max(np.abs(wB))
+
+
+def test_network_loss_and_gradients_match_oracle():
+    """one training step's gradients of the 6-layer network vs the oracle run
+    with the device's BF16 rounding points"""
+    from vidstab_amd import learned
+    torch = _torch()
+    rng = np.random.default_rng(7)
+    n, h, w = 2, 24, 40
+    net = learned.ProposalNet(seed=5)
+    layers = net.layers_numpy()
+    layers = [(co.bf16_round(wk), bk.astype(np.float64), d) for (wk, bk, d) in layers]
+    x = np.zeros((n, h, w, 32))
+    x[..., :2] = co.bf16_round(rng.standard_normal((n, h, w, 2)).astype(np.float32))
+    windows = np.array([[0, 2, 3, 14, 15], [1, 20, 1, 39, 20], [0, 10, 10, 30, 23]], dtype=np.int32)
+    label = rng.standard_normal((3, 2)) * 0.5
+    weight = np.array([1.0, 1.0, 0.5])
+
+    # oracle with the device's rounding points: activations stored in BF16
+    acts = [x]
+    for k, (wk, bk, d) in enumerate(layers):
+        z = co.conv(acts[-1], wk, bk, d)
+        acts.append(co.bf16_round(np.maximum(z, 0.0)) if k + 1 < len(layers) else z)
+    pred = co.pool(acts[-1], windows)
+    loss, dpred = co.loss(pred, label, weight, 1.0 / 3.0)
+    dz = co.bf16_round(co.pool_grad(dpred, acts[-1].shape, windows))
+    want = [None] * len(layers)
+    for k in range(len(layers) - 1, -1, -1):
+        want[k] = co.conv_weight_grad(acts[k], dz, layers[k][2])
+        if k:
+            dz = co.bf16_round(co.conv_data_grad(dz, layers[k][0], layers[k][2])) * (acts[k] > 0)
+
+    xt = _dev_bf16(x)
+    acts_d, out = net.forward(xt)
+    wins = torch.from_numpy(windows).cuda()
+    lab = torch.from_numpy(label.astype(np.float32)).cuda()
+    wgt = torch.from_numpy(weight.astype(np.float32)).cuda()
+    net.zero_grad()
+    l_d, pred_d, dout = net.loss_grad(out, wins, lab, wgt)
+    net.backward(acts_d, dout)
+    torch.cuda.synchronize()
+    assert np.allclose(pred_d.cpu().numpy(), pred, rtol=2e-2, atol=2e-3)
+    assert abs(float(l_d.item()) - loss) <= 2e-2 * abs(loss) + 1e-5
+    for k in range(len(layers)):
+        gw = net.gw[k].cpu().numpy()
+        gb = net.gb[k].cpu().numpy()
+        scale = max(np.max(np.abs(want[k][0])), 1e-12)
+        assert np.max(np.abs(gw - want[k][0])) <= 3e-2 * scale, f"layer {k} dW"
+        bscale = max(np.max(np.abs(want[k][1])), 1e-12)
+        assert np.max(np.abs(gb - want[k][1])) <= 3e-2 * bscale, f"layer {k} dB"

@@ -1,0 +1,363 @@
+/*
+ * vsg_cnn.hip -- the proposal network of the learned detect form: a stack
+ * of 3x3 convolutions, 32 channels, BF16 activations and weights with FP32
+ * accumulation on the CDNA matrix cores (v_mfma_f32_32x32x8_bf16), and the
+ * kernels that train it on the device (data and weight gradients, field
+ * pooling and its loss, Adam).  Declared in vsg_cnn.h.
+ *
+ * Layouts.  Activations are NHWC with C = 32 in BF16 (raw 16-bit patterns):
+ * index ((n*H + y)*W + x)*32 + c.  Narrower layers (the 2-channel input, the
+ * 2-channel output) are carried as 32 channels with the rest zero, so one
+ * kernel shape serves every layer.  Master weights are FP32 [tap][ci][co]
+ * with tap = (dy+1)*3 + (dx+1); the convolution reads them pre-packed as the
+ * MFMA B operand: [tap][kstep][lane][4], lane l holding rows
+ * k = 4*(l/32) + 0..3 of column n = l%32 for input channels kstep*8 + k.
+ *
+ * The convolution is an implicit GEMM, M = 32 pixels of one image row,
+ * N = 32 output channels, K = 9 taps x 32 channels: each wave keeps the 36
+ * B fragments of a layer in registers and walks row tiles; the A fragment of
+ * a lane is 4 consecutive channels of one (shifted) pixel, one 8-byte load,
+ * zero outside the image (padding = dilation).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vsg_cnn.h"
+
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+/* round to nearest even (finite values) */
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+constexpr int kC = 32;
+
+/* ---- weight packing -------------------------------------------------------------
+   forward: B[k][n] = W[tap][ks*8+k][n]; data gradient (transposed, flipped):
+   B[k][n] = W[8-tap][n][ks*8+k] */
+__global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk, int transposed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;   /* over 9*4*64*4 */
+  if (i >= 9 * 4 * 64 * 4) return;
+  const int e = i & 3, lane = (i >> 2) & 63, ks = (i >> 8) & 3, tap = i >> 10;
+  const int k = ks * 8 + 4 * (lane >> 5) + e, n = lane & 31;
+  const float v = transposed ? w[((8 - tap) * kC + n) * kC + k] : w[(tap * kC + k) * kC + n];
+  pk[i] = f2bf(v);
+}
+
+/* ---- convolution ------------------------------------------------------------------
+   out = act(conv(in, W) + bias); mode: 0 bf16 + ReLU, 1 bf16, 2 fp32 */
+__global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, int N, int H, int W,
+                                              const uint16_t* __restrict__ pk, const float* __restrict__ bias,
+                                              int dil, int mode, uint16_t* __restrict__ outb,
+                                              float* __restrict__ outf) {
+  const int lane = threadIdx.x & 63;
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  v4s b[36];
+#pragma unroll
+  for (int q = 0; q < 36; ++q) b[q] = *reinterpret_cast<const v4s*>(pk + ((size_t)q * 64 + lane) * 4);
+  const int co = lane & 31, kq = lane >> 5;
+  const float bco = bias ? bias[co] : 0.0f;
+  const int tilesX = (W + 31) >> 5;
+  const int ntiles = N * H * tilesX;
+  for (int t = gw; t < ntiles; t += nw) {
+    const int xt = t % tilesX, r = t / tilesX, y = r % H, n = r / H;
+    const int x = xt * 32 + (lane & 31);
+    v16f acc = {};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int yy = y + (tap / 3 - 1) * dil, xx = x + (tap % 3 - 1) * dil;
+      const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+      const uint16_t* p = in + (((size_t)n * H + (ok ? yy : 0)) * W + (ok ? xx : 0)) * kC + 4 * kq;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        v4s a = {0, 0, 0, 0};
+        if (ok) a = *reinterpret_cast<const v4s*>(p + ks * 8);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[tap * 4 + ks], acc, 0, 0, 0);
+      }
+    }
+    const size_t rowBase = ((size_t)n * H + y) * W;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
+      const int px = xt * 32 + m;
+      if (px < W) {
+        float val = acc[v] + bco;
+        const size_t o = (rowBase + px) * kC + co;
+        if (mode == 2) {
+          outf[o] = val;
+        } else {
+          if (mode == 0) val = val > 0.0f ? val : 0.0f;
+          outb[o] = f2bf(val);
+        }
+      }
+    }
+  }
+}
+
+/* ---- weight gradient ----------------------------------------------------------------
+   dW[tap][ci][co] += sum_p in[p + tap*dil][ci] * dz[p][co] for taps
+   tap0 .. tap0+2: an MFMA GEMM with M = ci, N = co, K = pixels (8 per step). */
+__global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ in, const uint16_t* __restrict__ dz,
+                                               int N, int H, int W, int dil, int tap0, float* __restrict__ dW) {
+  __shared__ float red[3 * kC * kC];   /* the workgroup's partial sums */
+  for (int i = threadIdx.x; i < 3 * kC * kC; i += blockDim.x) red[i] = 0.0f;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  const int c = lane & 31, kq = lane >> 5;
+  const long long npix = (long long)N * H * W;
+  const long long ngroups = (npix + 7) / 8;
+  v16f acc[3] = {};
+  for (long long g = gw; g < ngroups; g += nw) {
+    v4s bfr;
+    int py[4], px[4], pn[4];
+    bool pv[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const long long p = g * 8 + 4 * kq + e;
+      pv[e] = p < npix;
+      const long long q = pv[e] ? p : 0;
+      px[e] = (int)(q % W);
+      const long long r = q / W;
+      py[e] = (int)(r % H);
+      pn[e] = (int)(r / H);
+      bfr[e] = pv[e] ? (short)dz[q * kC + c] : (short)0;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int tap = tap0 + j;
+      const int dy = (tap / 3 - 1) * dil, dx = (tap % 3 - 1) * dil;
+      v4s a;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int yy = py[e] + dy, xx = px[e] + dx;
+        const bool ok = pv[e] && yy >= 0 && yy < H && xx >= 0 && xx < W;
+        a[e] = ok ? (short)in[(((size_t)pn[e] * H + yy) * W + xx) * kC + c] : (short)0;
+      }
+      acc[j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bfr, acc[j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int ci = 8 * (v >> 2) + 4 * kq + (v & 3);
+      atomicAdd(&red[(j * kC + ci) * kC + c], acc[j][v]);
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * kC * kC; i += blockDim.x) atomicAdd(&dW[tap0 * kC * kC + i], red[i]);
+}
+
+/* dB[co] += sum_p dz[p][co]: each half-wave walks pixels, lane = channel */
+__global__ __launch_bounds__(256) void k_bgrad(const uint16_t* __restrict__ dz, long long npix, float* __restrict__ dB) {
+  const int c = threadIdx.x & 31;
+  const long long stride = (long long)gridDim.x * (blockDim.x >> 5);
+  float s = 0.0f;
+  for (long long p = blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5); p < npix; p += stride)
+    s += bf2f(dz[p * kC + c]);
+  atomicAdd(&dB[c], s);
+}
+
+/* grad *= (act > 0), bf16 in place */
+__global__ void k_relu_mask(uint16_t* __restrict__ g, const uint16_t* __restrict__ act, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !(bf2f(act[i]) > 0.0f)) g[i] = 0;
+}
+
+__global__ void k_f2bf(const float* __restrict__ f, uint16_t* __restrict__ b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = f2bf(f[i]);
+}
+
+/* ---- 4x4 box downsample of a luma plane into one channel of an NHWC-32 map:
+   (mean - 128) / 64 */
+__global__ void k_down4(const uint8_t* __restrict__ src, int pitch, int Wq, int Hq, uint16_t* __restrict__ out,
+                        int chan) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= Wq || y >= Hq) return;
+  const uint8_t* p = src + (size_t)(4 * y) * pitch + 4 * x;
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(p + (size_t)r * pitch);
+    s += (w & 0xff) + ((w >> 8) & 0xff) + ((w >> 16) & 0xff) + (w >> 24);
+  }
+  out[((size_t)y * Wq + x) * kC + chan] = f2bf(((float)s * (1.0f / 16.0f) - 128.0f) * (1.0f / 64.0f));
+}
+
+/* ---- field pooling: pred[f] = mean over the field's window of channels 0, 1
+   of an fp32 NHWC-32 map; one wave per field */
+__global__ void k_pool(const float* __restrict__ map, int H, int W, const int* __restrict__ win, int nf,
+                       float* __restrict__ pred) {
+  const int f = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (f >= nf) return;
+  const int n = win[5 * f];
+  const int x0 = max(win[5 * f + 1], 0), y0 = max(win[5 * f + 2], 0);
+  const int x1 = min(win[5 * f + 3], W), y1 = min(win[5 * f + 4], H);   /* clipped to the map */
+  const int w = x1 > x0 ? x1 - x0 : 0, cnt = y1 > y0 ? w * (y1 - y0) : 0;
+  float sx = 0.0f, sy = 0.0f;
+  for (int i = lane; i < cnt; i += 64) {
+    const int y = y0 + i / w, x = x0 + i % w;
+    const float* p = map + (((size_t)n * H + y) * W + x) * kC;
+    sx += p[0];
+    sy += p[1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sx += __shfl_xor(sx, off);
+    sy += __shfl_xor(sy, off);
+  }
+  if (lane == 0) {
+    pred[2 * f] = cnt ? sx / cnt : 0.0f;
+    pred[2 * f + 1] = cnt ? sy / cnt : 0.0f;
+  }
+}
+
+/* d map (fp32, zeroed by the caller) += dpred[f] / area over each window */
+__global__ void k_pool_bwd(const float* __restrict__ dpred, int H, int W, const int* __restrict__ win, int nf,
+                           float* __restrict__ dmap) {
+  const int f = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (f >= nf) return;
+  const int n = win[5 * f];
+  const int x0 = max(win[5 * f + 1], 0), y0 = max(win[5 * f + 2], 0);
+  const int x1 = min(win[5 * f + 3], W), y1 = min(win[5 * f + 4], H);   /* clipped to the map */
+  const int w = x1 > x0 ? x1 - x0 : 0, cnt = y1 > y0 ? w * (y1 - y0) : 0;
+  if (!cnt) return;
+  const float gx = dpred[2 * f] / cnt, gy = dpred[2 * f + 1] / cnt;
+  for (int i = lane; i < cnt; i += 64) {
+    const int y = y0 + i / w, x = x0 + i % w;
+    float* p = dmap + (((size_t)n * H + y) * W + x) * kC;
+    atomicAdd(p, gx);
+    atomicAdd(p + 1, gy);
+  }
+}
+
+/* mean squared error over valid fields: loss += sum w*(pred-label)^2 / nvalid,
+   dpred = 2 w (pred - label) / nvalid */
+__global__ void k_loss(const float* __restrict__ pred, const float* __restrict__ label,
+                       const float* __restrict__ weight, int nf, float inv, float* __restrict__ dpred,
+                       float* __restrict__ loss) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (i < 2 * nf) {
+    const float d = pred[i] - label[i], w = weight[i >> 1];
+    dpred[i] = 2.0f * w * d * inv;
+    l = w * d * d * inv;
+  }
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+  if ((threadIdx.x & 63) == 0 && l != 0.0f) atomicAdd(loss, l);
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+int blocks_for(long long n, int t) { return (int)((n + t - 1) / t); }
+
+int conv_grid(void) {
+  static int g = 0;
+  if (g == 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    g = cus * 4;   /* 16 waves per CU walking row tiles */
+  }
+  return g;
+}
+
+}  // namespace
+
+#define CNN_RET return (int)hipGetLastError()
+
+extern "C" int vsgc_pack_weights(const float* w, uint16_t* packed, int transposed, void* stream) {
+  hipLaunchKernelGGL(k_pack, dim3(blocks_for(9 * 4 * 64 * 4, 256)), dim3(256), 0, (hipStream_t)stream, w, packed,
+                     transposed);
+  CNN_RET;
+}
+
+extern "C" int vsgc_conv(const uint16_t* in, int n, int h, int w, const uint16_t* packed, const float* bias,
+                         int dil, int mode, uint16_t* outb, float* outf, void* stream) {
+  if (n <= 0 || h <= 0 || w <= 0 || dil < 1 || mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
+  if ((mode == 2 && !outf) || (mode != 2 && !outb)) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_conv, dim3(conv_grid()), dim3(256), 0, (hipStream_t)stream, in, n, h, w, packed, bias, dil,
+                     mode, outb, outf);
+  CNN_RET;
+}
+
+extern "C" int vsgc_conv_wgrad(const uint16_t* in, const uint16_t* dz, int n, int h, int w, int dil, float* dW,
+                               float* dB, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  for (int tap0 = 0; tap0 < 9; tap0 += 3)
+    hipLaunchKernelGGL(k_wgrad, dim3(128), dim3(256), 0, st, in, dz, n, h, w, dil, tap0, dW);
+  if (dB) hipLaunchKernelGGL(k_bgrad, dim3(256), dim3(256), 0, st, dz, (long long)n * h * w, dB);
+  CNN_RET;
+}
+
+extern "C" int vsgc_relu_mask(uint16_t* grad, const uint16_t* act, long long count, void* stream) {
+  hipLaunchKernelGGL(k_relu_mask, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream, grad, act, count);
+  CNN_RET;
+}
+
+extern "C" int vsgc_f32_to_bf16(const float* f, uint16_t* b, long long count, void* stream) {
+  hipLaunchKernelGGL(k_f2bf, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream, f, b, count);
+  CNN_RET;
+}
+
+extern "C" int vsgc_downsample(const uint8_t* plane, int pitch, int width, int height, uint16_t* out, int chan,
+                               void* stream) {
+  const int wq = width / 4, hq = height / 4;
+  if (wq <= 0 || hq <= 0 || chan < 0 || chan >= kC) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_down4, dim3(blocks_for(wq, 256), hq), dim3(256), 0, (hipStream_t)stream, plane, pitch, wq, hq,
+                     out, chan);
+  CNN_RET;
+}
+
+extern "C" int vsgc_field_pool(const float* map, int h, int w, const int* windows, int nf, float* pred,
+                               void* stream) {
+  if (nf <= 0) return 0;
+  hipLaunchKernelGGL(k_pool, dim3(blocks_for((long long)nf * 64, 256)), dim3(256), 0, (hipStream_t)stream, map, h, w,
+                     windows, nf, pred);
+  CNN_RET;
+}
+
+extern "C" int vsgc_field_pool_bwd(const float* dpred, int h, int w, const int* windows, int nf, float* dmap,
+                                   void* stream) {
+  if (nf <= 0) return 0;
+  hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for((long long)nf * 64, 256)), dim3(256), 0, (hipStream_t)stream, dpred,
+                     h, w, windows, nf, dmap);
+  CNN_RET;
+}
+
+extern "C" int vsgc_field_loss(const float* pred, const float* label, const float* weight, int nf, float inv,
+                               float* dpred, float* loss, void* stream) {
+  if (nf <= 0) return 0;
+  hipLaunchKernelGGL(k_loss, dim3(blocks_for(2LL * nf, 256)), dim3(256), 0, (hipStream_t)stream, pred, label, weight,
+                     nf, inv, dpred, loss);
+  CNN_RET;
+}
+
+extern "C" int vsgc_adam(float* w, const float* g, float* m, float* v, int count, float lr, float b1, float b2,
+                         float eps, int step, void* stream) {
+  const float c1 = 1.0f / (1.0f - powf(b1, (float)step)), c2 = 1.0f / (1.0f - powf(b2, (float)step));
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, count, lr,
+                     b1, b2, eps, c1, c2);
+  CNN_RET;
+}
