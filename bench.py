@@ -76,7 +76,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2,
-                    help="workload: 2 = 1080p, 3 = 4K, 4 = 64 clips of uneven size (batched multi-clip form)")
+                    help="workload: 2 = 1080p, 3 = 4K, 4 = 64 clips of uneven size (batched multi-clip "
+                         "form), 5 = 4K motion blur, plain vs learned-proposal form")
+    ap.add_argument("--weights", default=os.path.join(REPO, "vid.stab_amd", "learned", "proposal_c5.bin"))
     ap.add_argument("--frames", type=int, default=0, help="clip length (0: the workload's)")
     ap.add_argument("--cpu-frames", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
@@ -96,6 +98,8 @@ def main():
 
     if args.config == 4:
         return run_multi(args)
+    if args.config == 5:
+        return run_learned(args)
     clip = Clip(args.config, 0)
     spec = clip.spec
     nfr = args.frames or spec.nframes
@@ -239,6 +243,75 @@ def run_multi(args):
                                "360p-2160p, 30-120 frames, device-resident frames", "clips": nclips,
                    "frames": frames, "parallelism": "1 GPU"},
         "per_clip_sequential_fps": round(seq_fps, 2),
+    }
+    print(json.dumps(res), flush=True)
+
+
+def run_learned(args):
+    """4K motion-blur clip (config 5, clip 0, held out from training): the
+    plain form and the learned-proposal form on the same device-resident
+    frames; agreement with the full search per field and per frame transform"""
+    import numpy as np
+    import torch
+    from vidstab_amd import api
+    from vidstab_amd.synth import Clip
+
+    clip = Clip(5, 0)
+    s = clip.spec
+    nfr = args.frames or s.nframes
+    log(f"generating {nfr} frames {s.width}x{s.height}")
+    host = np.empty((nfr, s.height, s.width), dtype=np.uint8)
+    for t in range(nfr):
+        clip.render(t, out=host[t])
+    dev = torch.from_numpy(host).to("cuda:0")
+    torch.cuda.synchronize()
+    fb = s.width * s.height
+    ptrs = [dev.data_ptr() + t * fb for t in range(nfr)]
+    ls = [s.width] * nfr
+    kw = dict(shakiness=s.shakiness, accuracy=s.accuracy, stepsize=s.stepsize, mincontrast=s.mincontrast)
+
+    def timed(det):
+        for _ in range(args.warmup):
+            det.detect_device(ptrs, ls)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            det.detect_device(ptrs, ls)
+        torch.cuda.synchronize()
+        return nfr * args.steps / (time.perf_counter() - t0)
+
+    plain = api.Detector(s.width, s.height, **kw)
+    plain_fps = timed(plain)
+    ref = plain.detect_device(ptrs, ls, keep=True)
+    S = plain.md.fieldscoarse.fieldSize
+    plain.close()
+    lrn = api.Detector(s.width, s.height, **kw)
+    lrn.set_proposal(args.weights)
+    learned_fps = timed(lrn)
+    f0, b0 = lrn.proposal_stats()
+    got = lrn.detect_device(ptrs, ls, keep=True)
+    f1, b1 = lrn.proposal_stats()
+    lrn.close()
+    cn, ce, an, ae = api.compare_forms(ref, got, S)
+    errs = [api.transform_distance(s.width, s.height, api.simple_transform(s.width, s.height, p),
+                                   api.simple_transform(s.width, s.height, q)) for p, q in zip(ref[1:], got[1:])]
+    log(f"plain {plain_fps:.1f} fps, learned {learned_fps:.1f} fps ({learned_fps / plain_fps:.2f}x); fields equal "
+        f"{100.0 * ae / max(an, 1):.2f}% (coarse {100.0 * ce / max(cn, 1):.2f}%), fallbacks "
+        f"{100.0 * (b1 - b0) / max(f1 - f0, 1):.2f}%, mean transform error {np.mean(errs):.4f} px")
+    res = {
+        "metric": METRIC, "value": round(learned_fps, 2), "unit": "frames/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(nfr * 1000.0 / learned_fps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8+bf16",
+        "data": "synthetic",
+        "config": {"workload": f"learned-proposal detect, {s.width}x{s.height} YUV420p 8-bit, {nfr}-frame seeded "
+                               "value-noise clip with motion blur (config 5, clip 0, not in the training set), "
+                               "device-resident frames", "width": s.width, "height": s.height, "frames": nfr,
+                   "stepsize": s.stepsize, "parallelism": "1 GPU"},
+        "plain_fps": round(plain_fps, 2), "speedup_vs_plain": round(learned_fps / plain_fps, 3),
+        "fields_equal_pct": round(100.0 * ae / max(an, 1), 3),
+        "coarse_fields_equal_pct": round(100.0 * ce / max(cn, 1), 3),
+        "fallback_pct": round(100.0 * (b1 - b0) / max(f1 - f0, 1), 3),
+        "global_transform_err_px": round(float(np.mean(errs)), 5),
     }
     print(json.dumps(res), flush=True)
 

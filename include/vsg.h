@@ -125,6 +125,18 @@ VS_API void vsg_profile_enable(int on);
 VS_API void vsg_profile_get(VSGProfile* out);
 VS_API void vsg_profile_reset(void);
 
+/* ---- the learned-proposal form ------------------------------------------------
+   Attaches a proposal network to a context: from the next batch on, each
+   coarse field is searched only in the 5x5 window of 1-px shifts around the
+   network's proposal, and goes through the exact full search whenever the
+   window's minimum lies on its edge, exceeds `threshold` (match units:
+   SAD / field area), or the proposal is out of the search range.  weights:
+   per layer W[9][32][32] (tap, in, out) then B[32], FP32; dil: per layer
+   dilation.  Single-clip contexts only. */
+VS_API int vsg_set_proposal(VSGContext* ctx, int nlayers, const int* dil, const float* weights, float threshold);
+/* coarse fields decided so far, and how many of them needed the full search */
+VS_API int vsg_proposal_stats(VSGContext* ctx, long long* fields, long long* fallbacks);
+
 /* ---- the batched multi-clip form ---------------------------------------------
    One context per clip, all on one stream; the searches of all clips that
    share a plan run as one launch over their flattened job lists, and each

@@ -90,6 +90,7 @@ void vsg_ctx_free(VSGContext* c) {
   (void)hipSetDevice(c->device);
   for (VSGSet& s : c->set)
     if (s.stream) (void)hipStreamSynchronize(s.stream);
+  vsg_learned_free(c);
   for (VSGStage& s : c->st) {
     dfree(s.d_fx);
     dfree(s.d_fy);
@@ -422,6 +423,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   }
   s.prof = g_profOn.load() != 0;
   if (vsg_ctx_enqueue_front(c, set, n) != 0) return -1;
+  if (c->learned && vsg_learned_front(c, set, n, refSlot, stream) != 0) return -1;
   if (refSlot >= 0) {
     /* the frozen tripod plane: after the blur, before any search reads it */
     VSG_CHECK(hipMemcpyAsync(c->d_ref, c->d_blur + (size_t)refSlot * c->slotStride, c->slotStride,
@@ -438,8 +440,12 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   s.follower = -1;
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID0], stream));
   const int nc = c->st[0].nf;
-  VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
-  VSG_CHECK(vsg_launch_search(ga, stream));
+  if (c->learned) {
+    if (vsg_learned_coarse(c, set, n, stream) != 0) return -1;
+  } else {
+    VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
+    VSG_CHECK(vsg_launch_search(ga, stream));
+  }
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID1], stream));
   VSG_CHECK(hipEventRecord(s.searchDone, stream));
   if (vsg_ctx_enqueue_coarse_readback(c, set, n) != 0) return -1;
@@ -453,6 +459,7 @@ extern "C" int vsg_wait_coarse(VSGContext* c, int set, VSGBatchView* view) {
   VSGSet& s = c->set[set];
   VSG_CHECK(hipSetDevice(c->device));
   VSG_CHECK(hipEventSynchronize(s.done));
+  vsg_learned_account(c, set);
   if (s.prof) {
     s.prof = false;
     double fieldBytes = 0;

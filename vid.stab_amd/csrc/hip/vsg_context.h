@@ -61,6 +61,8 @@ struct VSGSet {
   bool prof = false, profF = false;
 };
 
+struct VSGLearned;   /* vsg_learned.hip */
+
 struct VSGContext {
   int device = 0;
   bool ownStreams = true;
@@ -84,6 +86,7 @@ struct VSGContext {
   size_t patchSlotStride = 0;
   uint8_t* d_minmax = nullptr;
   VSGSet set[VSG_NSETS];
+  VSGLearned* learned = nullptr;   /* proposal network of the learned form, or none */
 };
 
 /* creation / destruction shared by both ABIs (vsg_api.hip); a null `shared`
@@ -96,6 +99,12 @@ int vsg_ctx_sync(VSGContext* c);
 int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift);
 /* picks each stage's plan (vsg_tune.hip) */
 int vsg_ctx_tune(VSGContext* c);
+/* the learned form (vsg_learned.hip): quarter-resolution planes in the
+   front stage, the proposal coarse stage, its statistics, teardown */
+int vsg_learned_front(VSGContext* c, int set, int n, int refSlot, hipStream_t st);
+int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st);
+void vsg_learned_account(VSGContext* c, int set);
+void vsg_learned_free(VSGContext* c);
 void vsg_ctx_free(VSGContext* c);
 /* enqueue helpers (no host synchronisation) */
 int vsg_ctx_enqueue_front(VSGContext* c, int set, int n);      /* H2D descr., blur, fields, select */

@@ -1,0 +1,271 @@
+/*
+ * vsg_learned.hip -- the learned-proposal coarse stage of a context.
+ *
+ * With a proposal network attached (vsg_set_proposal), a batch's coarse
+ * stage becomes:
+ *   1. every frame's raw luma is 4x4-downsampled once, in the front stage,
+ *      into a per-slot quarter-resolution BF16 plane;
+ *   2. the network input of frame i pairs its previous frame's plane
+ *      (channel 0) with its own (channel 1), NHWC with 32 channels;
+ *   3. the 6-layer network (vsg_cnn.hip) runs over the batch, and each coarse
+ *      field's proposal is the mean of the two output channels over the
+ *      field's window (field size / 4 at the field centre / 4);
+ *   4. k_sad_window searches the 5x5 window of 1-px shifts around each
+ *      proposal; a minimum inside the window whose SAD is within the typical
+ *      residual of the exact search is the field's result;
+ *   5. every other field goes through the exact full search (the plain
+ *      coarse kernel) -- the fallback that keeps the form honest.
+ * The fine stage and everything on the host are unchanged.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "vsg_cnn.h"
+#include "vsg_context.h"
+
+namespace {
+
+#define LCHECK(call)                                      \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) return vsg_fail(#call, e_);     \
+  } while (0)
+
+constexpr int kC = 32;
+constexpr int kMaxLayers = 8;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+/* frame z of the batch -> quarter-resolution plane of its slot */
+__global__ void k_down_slots(const VSGBlurSrc* __restrict__ src, int Wq, int Hq, uint16_t* __restrict__ down,
+                             size_t planeStride) {
+  const VSGBlurSrc s = src[blockIdx.z];
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (!s.p || x >= Wq) return;
+  const uint8_t* p = s.p + (size_t)(4 * y) * s.pitch + 4 * x;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const uint8_t* q = p + (size_t)r * s.pitch;
+    sum += (uint32_t)q[0] + q[1] + q[2] + q[3];
+  }
+  down[blockIdx.z * planeStride + (size_t)y * Wq + x] = f2bf(((float)sum * (1.0f / 16.0f) - 128.0f) * (1.0f / 64.0f));
+}
+
+/* network input of frame z: channel 0 = previous frame's plane (the stored
+   reference plane for prev < 0), channel 1 = its own */
+__global__ void k_pair(const uint16_t* __restrict__ down, size_t planeStride, const uint16_t* __restrict__ downRef,
+                       const int* __restrict__ prevSlot, int slot0, int Wq, int Hq, uint16_t* __restrict__ in) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= Wq) return;
+  const int pv = prevSlot[z];
+  const uint16_t* a = pv < 0 ? downRef : down + (size_t)pv * planeStride;
+  const uint16_t* b = down + (size_t)(slot0 + z) * planeStride;
+  uint16_t* o = in + (((size_t)z * Hq + y) * Wq + x) * kC;
+  o[0] = a[(size_t)y * Wq + x];
+  o[1] = b[(size_t)y * Wq + x];
+}
+
+__global__ void k_zero_u32(unsigned* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+struct VSGLearned {
+  int nl = 0;
+  int dil[kMaxLayers] = {};
+  float thr = 0;
+  uint16_t* d_pk[kMaxLayers] = {};
+  float* d_bias[kMaxLayers] = {};
+  int Wq = 0, Hq = 0;
+  size_t planeStride = 0;
+  uint16_t* d_down = nullptr;      /* [slots][Hq][Wq] */
+  uint16_t* d_downRef = nullptr;   /* [Hq][Wq] */
+  uint16_t* d_in = nullptr;        /* [B][Hq][Wq][32], channels >= 2 stay zero */
+  uint16_t* d_act[2] = {};
+  float* d_out = nullptr;          /* [B][Hq][Wq][32] */
+  int* d_win = nullptr;            /* [B][nc][5] */
+  float* d_pred = nullptr;         /* [B][nc][2] */
+  VSGJob* d_fjobs[VSG_NSETS] = {};
+  unsigned* d_stats[VSG_NSETS] = {};
+  unsigned* h_stats[VSG_NSETS] = {};
+  long long fields = 0, fallbacks = 0;
+};
+
+void vsg_learned_free(VSGContext* c) {
+  VSGLearned* L = c->learned;
+  if (!L) return;
+  for (int k = 0; k < kMaxLayers; ++k) {
+    if (L->d_pk[k]) (void)hipFree(L->d_pk[k]);
+    if (L->d_bias[k]) (void)hipFree(L->d_bias[k]);
+  }
+  void* bufs[] = {L->d_down, L->d_downRef, L->d_in, L->d_act[0], L->d_act[1], L->d_out, L->d_win, L->d_pred};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (int s = 0; s < VSG_NSETS; ++s) {
+    if (L->d_fjobs[s]) (void)hipFree(L->d_fjobs[s]);
+    if (L->d_stats[s]) (void)hipFree(L->d_stats[s]);
+    if (L->h_stats[s]) (void)hipHostFree(L->h_stats[s]);
+  }
+  delete L;
+  c->learned = nullptr;
+}
+
+extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, const float* weights, float threshold) {
+  if (!c || nlayers < 1 || nlayers > kMaxLayers || !dil || !weights || !(threshold > 0.0f))
+    return vsg_fail("vsg_set_proposal", hipSuccess);
+  if (c->W < 64 || c->H < 64) return vsg_fail("vsg_set_proposal: frame too small", hipSuccess);
+  LCHECK(hipSetDevice(c->device));
+  if (vsg_ctx_sync(c) != 0) return -1;
+  vsg_learned_free(c);
+  VSGLearned* L = new VSGLearned();
+  c->learned = L;
+  L->nl = nlayers;
+  L->thr = threshold;
+  L->Wq = c->W / 4;
+  L->Hq = c->H / 4;
+  L->planeStride = (size_t)L->Wq * L->Hq;
+  const int B = c->B, nc = c->st[0].nf, slots = VSG_NSETS * B;
+  const size_t mapElems = (size_t)B * L->Hq * L->Wq * kC;
+  float* d_w = nullptr;
+  const size_t perLayer = 9 * kC * kC + kC;
+  int rc = 0;
+  do {
+    if (hipMalloc(&d_w, sizeof(float) * 9 * kC * kC) != hipSuccess) { rc = vsg_fail("proposal weights", hipErrorOutOfMemory); break; }
+    for (int k = 0; k < nlayers && rc == 0; ++k) {
+      L->dil[k] = dil[k];
+      if (dil[k] < 1) { rc = vsg_fail("vsg_set_proposal: dilation", hipSuccess); break; }
+      if (hipMalloc(&L->d_pk[k], sizeof(uint16_t) * 9 * 4 * 64 * 4) != hipSuccess ||
+          hipMalloc(&L->d_bias[k], sizeof(float) * kC) != hipSuccess ||
+          hipMemcpy(d_w, weights + k * perLayer, sizeof(float) * 9 * kC * kC, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(L->d_bias[k], weights + k * perLayer + 9 * kC * kC, sizeof(float) * kC, hipMemcpyHostToDevice) !=
+              hipSuccess ||
+          vsgc_pack_weights(d_w, L->d_pk[k], 0, nullptr) != 0) {
+        rc = vsg_fail("proposal weights", hipErrorOutOfMemory);
+        break;
+      }
+    }
+    if (rc) break;
+    if (hipMalloc(&L->d_down, sizeof(uint16_t) * L->planeStride * slots) != hipSuccess ||
+        hipMalloc(&L->d_downRef, sizeof(uint16_t) * L->planeStride) != hipSuccess ||
+        hipMalloc(&L->d_in, sizeof(uint16_t) * mapElems) != hipSuccess ||
+        hipMalloc(&L->d_act[0], sizeof(uint16_t) * mapElems) != hipSuccess ||
+        hipMalloc(&L->d_act[1], sizeof(uint16_t) * mapElems) != hipSuccess ||
+        hipMalloc(&L->d_out, sizeof(float) * mapElems) != hipSuccess ||
+        hipMalloc(&L->d_win, sizeof(int) * 5 * (size_t)B * nc) != hipSuccess ||
+        hipMalloc(&L->d_pred, sizeof(float) * 2 * (size_t)B * nc) != hipSuccess) {
+      rc = vsg_fail("proposal buffers", hipErrorOutOfMemory);
+      break;
+    }
+    if (hipMemset(L->d_in, 0, sizeof(uint16_t) * mapElems) != hipSuccess ||
+        hipMemset(L->d_down, 0, sizeof(uint16_t) * L->planeStride * slots) != hipSuccess ||
+        hipMemset(L->d_downRef, 0, sizeof(uint16_t) * L->planeStride) != hipSuccess) {
+      rc = vsg_fail("proposal buffers", hipErrorUnknown);
+      break;
+    }
+    /* windows: field size / 4 around the field centre / 4, per frame slot */
+    std::vector<int16_t> fx(nc), fy(nc);
+    if (hipMemcpy(fx.data(), c->st[0].d_fx, sizeof(int16_t) * nc, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(fy.data(), c->st[0].d_fy, sizeof(int16_t) * nc, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = vsg_fail("proposal windows", hipErrorUnknown);
+      break;
+    }
+    std::vector<int> win((size_t)5 * B * nc);
+    const int h = c->st[0].S / 8;
+    for (int z = 0; z < B; ++z)
+      for (int f = 0; f < nc; ++f) {
+        int* w = &win[5 * ((size_t)z * nc + f)];
+        const int cx = fx[f] / 4, cy = fy[f] / 4;
+        w[0] = z;
+        w[1] = cx - h < 0 ? 0 : cx - h;
+        w[2] = cy - h < 0 ? 0 : cy - h;
+        w[3] = cx + h > L->Wq ? L->Wq : cx + h;
+        w[4] = cy + h > L->Hq ? L->Hq : cy + h;
+      }
+    if (hipMemcpy(L->d_win, win.data(), sizeof(int) * win.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = vsg_fail("proposal windows", hipErrorUnknown);
+      break;
+    }
+    for (int s = 0; s < VSG_NSETS; ++s) {
+      if (hipMalloc(&L->d_fjobs[s], sizeof(VSGJob) * (size_t)B * nc) != hipSuccess ||
+          hipMalloc(&L->d_stats[s], sizeof(unsigned) * 2) != hipSuccess ||
+          hipHostMalloc(&L->h_stats[s], sizeof(unsigned) * 2, hipHostMallocDefault) != hipSuccess) {
+        rc = vsg_fail("proposal buffers", hipErrorOutOfMemory);
+        break;
+      }
+      L->h_stats[s][0] = L->h_stats[s][1] = 0;
+    }
+  } while (0);
+  if (d_w) (void)hipFree(d_w);
+  if (rc == 0 && hipDeviceSynchronize() != hipSuccess) rc = vsg_fail("vsg_set_proposal", hipErrorUnknown);
+  if (rc != 0) vsg_learned_free(c);
+  return rc;
+}
+
+extern "C" int vsg_proposal_stats(VSGContext* c, long long* fields, long long* fallbacks) {
+  if (!c || !c->learned) return vsg_fail("vsg_proposal_stats: no proposal network", hipSuccess);
+  if (fields) *fields = c->learned->fields;
+  if (fallbacks) *fallbacks = c->learned->fallbacks;
+  return 0;
+}
+
+/* front-stage part: quarter-resolution planes of the batch's frames */
+int vsg_learned_front(VSGContext* c, int set, int n, int refSlot, hipStream_t st) {
+  VSGLearned* L = c->learned;
+  const int slot0 = set * c->B;
+  hipLaunchKernelGGL(k_down_slots, dim3((L->Wq + 255) / 256, L->Hq, n), dim3(256), 0, st, c->set[set].d_src, L->Wq,
+                     L->Hq, L->d_down + (size_t)slot0 * L->planeStride, L->planeStride);
+  LCHECK(hipGetLastError());
+  if (refSlot >= 0)
+    LCHECK(hipMemcpyAsync(L->d_downRef, L->d_down + (size_t)refSlot * L->planeStride,
+                          sizeof(uint16_t) * L->planeStride, hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
+/* the coarse stage: network, proposals, window search, exact fallback */
+int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
+  VSGLearned* L = c->learned;
+  VSGSet& s = c->set[set];
+  const int slot0 = set * c->B, nc = c->st[0].nf;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 2);
+  hipLaunchKernelGGL(k_pair, dim3((L->Wq + 255) / 256, L->Hq, n), dim3(256), 0, st, L->d_down, L->planeStride,
+                     L->d_downRef, s.d_prev, slot0, L->Wq, L->Hq, L->d_in);
+  LCHECK(hipGetLastError());
+  const uint16_t* x = L->d_in;
+  for (int k = 0; k < L->nl; ++k) {
+    const bool last = k == L->nl - 1;
+    uint16_t* y = L->d_act[k & 1];
+    if (vsgc_conv(x, n, L->Hq, L->Wq, L->d_pk[k], L->d_bias[k], L->dil[k], last ? 2 : 0, last ? nullptr : y,
+                  last ? L->d_out : nullptr, st) != 0)
+      return vsg_fail("proposal network", hipErrorLaunchFailure);
+    x = y;
+  }
+  if (vsgc_field_pool(L->d_out, L->Hq, L->Wq, L->d_win, n * nc, L->d_pred, st) != 0)
+    return vsg_fail("proposal pooling", hipErrorLaunchFailure);
+  VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
+  const double maxSad = (double)L->thr * c->st[0].S * c->st[0].S;
+  LCHECK(vsg_launch_window(ga, L->d_pred, nc, (uint32_t)(maxSad < 4294967295.0 ? maxSad : 4294967295.0),
+                           L->d_fjobs[set], L->d_stats[set], st));
+  VSGSearchArgs fb = ga;
+  fb.jobs = L->d_fjobs[set];
+  LCHECK(vsg_launch_search(fb, st));
+  LCHECK(hipMemcpyAsync(L->h_stats[set], L->d_stats[set], sizeof(unsigned) * 2, hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+/* after the set's coarse read-back completed */
+void vsg_learned_account(VSGContext* c, int set) {
+  VSGLearned* L = c->learned;
+  if (!L) return;
+  L->fields += L->h_stats[set][0];
+  L->fallbacks += L->h_stats[set][1];
+  L->h_stats[set][0] = L->h_stats[set][1] = 0;
+}

@@ -66,7 +66,9 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-__device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned long long key, int job,
+/* L.sums[ti*5 + tj] = SAD of candidate (tx - r + ns*ti, ty - r + ns*tj) for
+   ti, tj < ni, summed by the whole workgroup; ends with a barrier */
+__device__ void level_sums(const VSGSearchArgs& a, const VSGJob& jb, int tx, int ty, int ns, int r, int ni,
                            RefineLds& L) {
   const int nthr = blockDim.x, nwaves = nthr >> 6;
   const int S = a.g.S, W4 = S >> 2;
@@ -77,6 +79,44 @@ __device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned lo
   const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
   const uint32_t* patch = reinterpret_cast<const uint32_t*>(a.patches + (size_t)jb.slot * a.patchSlotStride +
                                                             a.stageOff + (size_t)jb.field * S * S);
+  uint32_t acc[kNc];
+#pragma unroll
+  for (int c = 0; c < kNc; ++c) acc[c] = 0;
+  if (r0 < rowsPerPass) {
+    const uint8_t* colBase = prev + (fxc - S / 2 + tx - r + jb.ox) + 4 * w;
+    const int rowBase = fyc - S / 2 + ty - r + jb.oy;
+#pragma unroll 2
+    for (int y = r0; y < S; y += rowsPerPass) {
+      const uint32_t c = patch[y * W4 + w];
+#pragma unroll
+      for (int tj = 0; tj < kMaxNi; ++tj) {
+        if (tj < ni) {
+          const uint8_t* p = colBase + (size_t)(rowBase + y + ns * tj) * a.pitch;
+#pragma unroll
+          for (int ti = 0; ti < kMaxNi; ++ti)
+            if (ti < ni)
+              acc[ti * kMaxNi + tj] =
+                  __builtin_amdgcn_sad_u8(*reinterpret_cast<const u32_any*>(p + ns * ti), c, acc[ti * kMaxNi + tj]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kNc; ++c) {
+    const uint32_t v = wave_sum(acc[c]);
+    if (lane == 0) L.part[wave][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kNc) {
+    uint32_t t = 0;
+    for (int q = 0; q < nwaves; ++q) t += L.part[q][threadIdx.x];
+    L.sums[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+__device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned long long key, int job,
+                           RefineLds& L) {
   int ga, gb;
   vsg_unrank((int)(uint32_t)key, &ga, &gb);
   int tx = ga * a.g.step, ty = gb * a.g.step;
@@ -84,40 +124,7 @@ __device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned lo
   for (int st = a.g.step; st > 1; st /= 2) {
     const int ns = st / 2, r = st - ns;
     const int ni = (2 * r) / ns + 1;
-    uint32_t acc[kNc];
-#pragma unroll
-    for (int c = 0; c < kNc; ++c) acc[c] = 0;
-    if (r0 < rowsPerPass) {
-      const uint8_t* colBase = prev + (fxc - S / 2 + tx - r + jb.ox) + 4 * w;
-      const int rowBase = fyc - S / 2 + ty - r + jb.oy;
-#pragma unroll 2
-      for (int y = r0; y < S; y += rowsPerPass) {
-        const uint32_t c = patch[y * W4 + w];
-#pragma unroll
-        for (int tj = 0; tj < kMaxNi; ++tj) {
-          if (tj < ni) {
-            const uint8_t* p = colBase + (size_t)(rowBase + y + ns * tj) * a.pitch;
-#pragma unroll
-            for (int ti = 0; ti < kMaxNi; ++ti)
-              if (ti < ni)
-                acc[ti * kMaxNi + tj] =
-                    __builtin_amdgcn_sad_u8(*reinterpret_cast<const u32_any*>(p + ns * ti), c, acc[ti * kMaxNi + tj]);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < kNc; ++c) {
-      const uint32_t v = wave_sum(acc[c]);
-      if (lane == 0) L.part[wave][c] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < kNc) {
-      uint32_t t = 0;
-      for (int q = 0; q < nwaves; ++q) t += L.part[q][threadIdx.x];
-      L.sums[threadIdx.x] = t;
-    }
-    __syncthreads();
+    level_sums(a, jb, tx, ty, ns, r, ni, L);
     int btx = tx, bty = ty;
     for (int ti = 0; ti < ni; ++ti)
       for (int tj = 0; tj < ni; ++tj) {
@@ -234,6 +241,65 @@ __global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict
   const int local = unit - sg->unitBase;
   const int job = local / sg->a.tiles;
   search_wg<J, K, D, CW>(sg->a, job, local - job * sg->a.tiles, L);
+}
+
+/* ---- proposal window (learned form) -----------------------------------------------
+   Job j's field gets proposal p = round(4 * pred) (quarter-resolution
+   network output); the 5x5 window of 1-px shifts around p is searched
+   exhaustively, keeping the reference refinement's rule (the centre first,
+   then x outer / y inner, strict "<").  The result is accepted when the
+   minimum is inside the window and its SAD is at most maxSad; then it is
+   written to results[j] and the job is removed from the fallback list,
+   otherwise the job goes to the fallback list for the exact full search. */
+__global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float* __restrict__ pred, int nc,
+                                                    uint32_t maxSad, VSGJob* __restrict__ fallback,
+                                                    unsigned* __restrict__ stats) {
+  __shared__ RefineLds L;
+  const int job = blockIdx.x;
+  if (job >= a.njobs) return;
+  const VSGJob jb = a.jobs[job];
+  if (jb.field < 0) {
+    if (threadIdx.x == 0) fallback[job] = jb;
+    return;
+  }
+  const int frame = job / nc;
+  const float* pp = pred + 2 * ((size_t)frame * nc + jb.field);
+  const float fx = rintf(4.0f * pp[0]), fy = rintf(4.0f * pp[1]);
+  const float lim = (float)(a.g.n * a.g.step - 2);   /* the full search's reach, less the window */
+  bool accept = fx >= -lim && fx <= lim && fy >= -lim && fy <= lim;
+  int bx = 0, by = 0;
+  uint32_t minerr = 0;
+  if (accept) {
+    const int px = (int)fx, py = (int)fy;
+    level_sums(a, jb, px, py, 1, 2, 5, L);
+    bx = px;
+    by = py;
+    minerr = L.sums[2 * kMaxNi + 2];
+    for (int ti = 0; ti < 5; ++ti)
+      for (int tj = 0; tj < 5; ++tj) {
+        const uint32_t e = L.sums[ti * kMaxNi + tj];
+        if (!(ti == 2 && tj == 2) && e < minerr) {
+          minerr = e;
+          bx = px - 2 + ti;
+          by = py - 2 + tj;
+        }
+      }
+    accept = bx > px - 2 && bx < px + 2 && by > py - 2 && by < py + 2 && minerr <= maxSad;
+  }
+  if (threadIdx.x == 0) {
+    VSGJob f = jb;
+    if (accept) {
+      VSGResult res;
+      res.tx = (int16_t)bx;
+      res.ty = (int16_t)by;
+      res.minerror = minerr;
+      a.results[job] = res;
+      f.field = -1;
+    }
+    fallback[job] = f;
+    atomicAdd(&stats[0], 1u);
+    if (!accept) atomicAdd(&stats[1], 1u);
+  }
 }
 
 template <int J, int K, int D, int CW>
@@ -366,4 +432,12 @@ hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, 
   if (g.J == 4 && g.D == 1) return launch_multi_k<4, 1>(dSegs, nseg, totalUnits, g, st);
   if (g.J == 4 && g.D == 3) return launch_multi_k<4, 3>(dSegs, nseg, totalUnits, g, st);
   return launch_multi_k<1, 1>(dSegs, nseg, totalUnits, g, st);
+}
+
+hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int nc, uint32_t maxSad, VSGJob* fallback,
+                             unsigned* stats, hipStream_t st) {
+  if (a.njobs <= 0) return hipSuccess;
+  if (a.g.S > 256 || nc <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sad_window, dim3(a.njobs), dim3(256), 0, st, a, pred, nc, maxSad, fallback, stats);
+  return hipGetLastError();
 }

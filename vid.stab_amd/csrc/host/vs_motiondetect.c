@@ -640,6 +640,48 @@ static void del_fn(void* ctx, int i) {
   for (int k = 0; k < 16; k++) vs_vector_del(&m[16 * i + k]);
 }
 
+VS_API int vsMotionDetectSetProposal(VSMotionDetect* md, const char* weightsFile) {
+  MDState* s = state_find(md);
+  if (!s || !weightsFile) return VS_ERROR;
+  FILE* f = fopen(weightsFile, "rb");
+  if (!f) {
+    vs_log_error(md->conf.modName, "cannot open proposal weights %s\n", weightsFile);
+    return VS_ERROR;
+  }
+  char magic[4];
+  int32_t hdr[2], dils[8];
+  float thr = 0;
+  float* w = NULL;
+  int ok = fread(magic, 1, 4, f) == 4 && memcmp(magic, "VSGP", 4) == 0 && fread(hdr, 4, 2, f) == 2 &&
+           hdr[0] == 1 && hdr[1] >= 1 && hdr[1] <= 8;
+  const size_t perLayer = 9 * 32 * 32 + 32;
+  if (ok) ok = fread(dils, 4, (size_t)hdr[1], f) == (size_t)hdr[1] && fread(&thr, 4, 1, f) == 1;
+  if (ok) {
+    w = (float*)malloc(sizeof(float) * perLayer * (size_t)hdr[1]);
+    ok = w && fread(w, sizeof(float), perLayer * (size_t)hdr[1], f) == perLayer * (size_t)hdr[1];
+  }
+  fclose(f);
+  if (!ok) {
+    vs_log_error(md->conf.modName, "bad proposal weights file %s\n", weightsFile);
+    free(w);
+    return VS_ERROR;
+  }
+  const int rc = vsg_set_proposal(s->gpu, hdr[1], dils, w, thr);
+  free(w);
+  if (rc != 0) {
+    vs_log_error(md->conf.modName, "proposal network: %s\n", vsg_last_error());
+    return VS_ERROR;
+  }
+  vs_log_info(md->conf.modName, "learned proposals: %i layers, residual threshold %.3f\n", hdr[1], thr);
+  return VS_OK;
+}
+
+VS_API int vsMotionDetectProposalStats(const VSMotionDetect* md, long long* fields, long long* fallbacks) {
+  MDState* s = state_find(md);
+  if (!s) return VS_ERROR;
+  return vsg_proposal_stats(s->gpu, fields, fallbacks) == 0 ? VS_OK : VS_ERROR;
+}
+
 VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n) {
   if (!motions || n <= 0) return;
   const int full = n / 16;

@@ -122,6 +122,11 @@ def lib():
                                           c_void_p]
         L.vsMotionDetectMulti.restype = c_int
         L.vsLocalMotionsArrayDel.restype = None
+        L.vsSimpleMotionsToTransform.argtypes = [VSFrameInfo, ctypes.c_char_p, ctypes.POINTER(VSVector)]
+        L.vsSimpleMotionsToTransform.restype = VSTransform
+        L.vsMotionDetectSetProposal.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.c_char_p]
+        L.vsMotionDetectProposalStats.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.POINTER(ctypes.c_longlong),
+                                                  ctypes.POINTER(ctypes.c_longlong)]
         L.vs_vector_size.argtypes = [ctypes.POINTER(VSVector)]
         L.vs_vector_get.argtypes = [ctypes.POINTER(VSVector), c_int]
         L.vs_vector_get.restype = c_void_p
@@ -213,6 +218,17 @@ class Detector:
         self.L.vsLocalMotionsArrayDel(vecs, n)
         return out
 
+    def set_proposal(self, path):
+        """switch to the learned-proposal form (vsMotionDetectSetProposal)"""
+        if self.L.vsMotionDetectSetProposal(ctypes.byref(self.md), str(path).encode()) != 0:
+            raise RuntimeError("vsMotionDetectSetProposal failed: " + self.L.vsg_last_error().decode())
+
+    def proposal_stats(self):
+        f, b = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        if self.L.vsMotionDetectProposalStats(ctypes.byref(self.md), ctypes.byref(f), ctypes.byref(b)) != 0:
+            raise RuntimeError("no proposal network")
+        return f.value, b.value
+
     def blurred(self):
         out = np.zeros((self.height, self.width), dtype=np.uint8)
         if self.L.vsMotionDetectGetBlurred(ctypes.byref(self.md), out.ctypes.data, self.width) != 0:
@@ -270,3 +286,59 @@ def detect_multi(clips, keep=True, mod=b"vidstab_amd"):
         res.append([motions_of(v[t]) for t in range(nf)] if keep else [v[t].nelems for t in range(nf)])
         L.vsLocalMotionsArrayDel(v, nf)
     return res
+
+
+def simple_transform(width, height, motions):
+    """vsSimpleMotionsToTransform of a frame's LocalMotions (tuples as
+    motions_of returns them) -> (x, y, alpha)"""
+    L = lib()
+    fi = VSFrameInfo()
+    if not L.vsFrameInfoInit(ctypes.byref(fi), width, height, PF_YUV420P):
+        raise ValueError("vsFrameInfoInit failed")
+    n = len(motions)
+    lms = (LocalMotion * max(n, 1))()
+    ptrs = (c_void_p * max(n, 1))()
+    for i, (vx, vy, fx, fy, size, cbits, mbits) in enumerate(motions):
+        lm = lms[i]
+        lm.v.x, lm.v.y = vx, vy
+        lm.f.x, lm.f.y, lm.f.size = fx, fy, size
+        lm.contrast = float(np.array([cbits], dtype=np.uint64).view(np.float64)[0])
+        lm.match = float(np.array([mbits], dtype=np.uint64).view(np.float64)[0])
+        ptrs[i] = ctypes.cast(ctypes.byref(lm), c_void_p)
+    vec = VSVector()
+    vec.data = ctypes.cast(ptrs, ctypes.POINTER(c_void_p))
+    vec.buffersize = max(n, 1)
+    vec.nelems = n
+    t = L.vsSimpleMotionsToTransform(fi, b"vidstab_amd", ctypes.byref(vec))
+    return t.x, t.y, t.alpha
+
+
+def transform_distance(width, height, a, b):
+    """largest displacement difference (px) of the frame corners between two
+    (x, y, alpha) transforms about the frame centre"""
+    cx, cy = width / 2.0, height / 2.0
+    worst = 0.0
+    for qx, qy in ((0, 0), (width, 0), (0, height), (width, height)):
+        dx, dy = qx - cx, qy - cy
+        pa = (np.cos(a[2]) * dx - np.sin(a[2]) * dy + a[0], np.sin(a[2]) * dx + np.cos(a[2]) * dy + a[1])
+        pb = (np.cos(b[2]) * dx - np.sin(b[2]) * dy + b[0], np.sin(b[2]) * dx + np.cos(b[2]) * dy + b[1])
+        worst = max(worst, float(np.hypot(pa[0] - pb[0], pa[1] - pb[1])))
+    return worst
+
+
+def compare_forms(plain, learned, coarse_size):
+    """per-field agreement of two runs' LocalMotions on the same frames:
+    (coarse fields compared, coarse equal, all fields compared, all equal);
+    a field is identified by its position and size"""
+    cn = ce = an = ae = 0
+    for fp, fl in zip(plain, learned):
+        dp = {(m[2], m[3], m[4]): m for m in fp}
+        dl = {(m[2], m[3], m[4]): m for m in fl}
+        for key, m in dp.items():
+            same = dl.get(key) == m
+            an += 1
+            ae += same
+            if key[2] == coarse_size:
+                cn += 1
+                ce += same
+    return cn, ce, an, ae
