@@ -136,6 +136,9 @@ VS_API void vsg_profile_reset(void);
 VS_API int vsg_set_proposal(VSGContext* ctx, int nlayers, const int* dil, const float* weights, float threshold);
 /* coarse fields decided so far, and how many of them needed the full search */
 VS_API int vsg_proposal_stats(VSGContext* ctx, long long* fields, long long* fallbacks);
+/* the fallbacks by cause: proposal out of the search range, window minimum on
+   the window edge, minimum above the residual threshold */
+VS_API int vsg_proposal_fallbacks(VSGContext* ctx, long long* outOfRange, long long* edge, long long* residual);
 
 /* ---- the batched multi-clip form ---------------------------------------------
    One context per clip, all on one stream; the searches of all clips that

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "..", "vid.stab_amd", "learned", "proposal_c5.bin")
-NFRAMES = 24
+NFRAMES = 72   # several batches in flight at once
 
 
 def _say(msg):
@@ -67,6 +67,7 @@ def test_learned_form_agrees_with_full_search(clip4k):
     _say(f"[gpu] learned: {fields} coarse fields, {fallbacks} fallbacks; coarse equal {ce}/{cn}, "
          f"all equal {ae}/{an}, mean transform error {np.mean(errs):.4f} px")
     assert fields == cn
+    assert fallbacks <= 0.15 * fields
     assert ae >= 0.99 * an
     assert np.mean(errs) <= 0.05
 

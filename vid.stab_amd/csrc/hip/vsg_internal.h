@@ -97,7 +97,9 @@ hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st);
 
 /* the learned form's proposal window search over a coarse job list: accepted
    jobs get their result and field -1 in `fallback`, the others are copied
-   to `fallback` for the exact search; stats[0] += jobs, stats[1] += fallbacks */
+   to `fallback` for the exact search; stats[0] += jobs, stats[1] += fallbacks,
+   stats[2..4] += fallbacks out of range / window minimum on the edge / above
+   the residual threshold */
 hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int nc, uint32_t maxSad, VSGJob* fallback,
                              unsigned* stats, hipStream_t st);
 

@@ -97,7 +97,11 @@ struct VSGLearned {
   VSGJob* d_fjobs[VSG_NSETS] = {};
   unsigned* d_stats[VSG_NSETS] = {};
   unsigned* h_stats[VSG_NSETS] = {};
-  long long fields = 0, fallbacks = 0;
+  long long fields = 0, fallbacks = 0, why[3] = {0, 0, 0};
+  /* the network buffers above are shared by the slot sets, whose streams may
+     run concurrently: each set's network stage waits for the previous one */
+  hipEvent_t netDone = nullptr;
+  int netLastSet = -1;
 };
 
 void vsg_learned_free(VSGContext* c) {
@@ -110,6 +114,7 @@ void vsg_learned_free(VSGContext* c) {
   void* bufs[] = {L->d_down, L->d_downRef, L->d_in, L->d_act[0], L->d_act[1], L->d_out, L->d_win, L->d_pred};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (L->netDone) (void)hipEventDestroy(L->netDone);
   for (int s = 0; s < VSG_NSETS; ++s) {
     if (L->d_fjobs[s]) (void)hipFree(L->d_fjobs[s]);
     if (L->d_stats[s]) (void)hipFree(L->d_stats[s]);
@@ -194,14 +199,18 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
       rc = vsg_fail("proposal windows", hipErrorUnknown);
       break;
     }
+    if (hipEventCreateWithFlags(&L->netDone, hipEventDisableTiming) != hipSuccess) {
+      rc = vsg_fail("proposal event", hipErrorUnknown);
+      break;
+    }
     for (int s = 0; s < VSG_NSETS; ++s) {
       if (hipMalloc(&L->d_fjobs[s], sizeof(VSGJob) * (size_t)B * nc) != hipSuccess ||
-          hipMalloc(&L->d_stats[s], sizeof(unsigned) * 2) != hipSuccess ||
-          hipHostMalloc(&L->h_stats[s], sizeof(unsigned) * 2, hipHostMallocDefault) != hipSuccess) {
+          hipMalloc(&L->d_stats[s], sizeof(unsigned) * 5) != hipSuccess ||
+          hipHostMalloc(&L->h_stats[s], sizeof(unsigned) * 5, hipHostMallocDefault) != hipSuccess) {
         rc = vsg_fail("proposal buffers", hipErrorOutOfMemory);
         break;
       }
-      L->h_stats[s][0] = L->h_stats[s][1] = 0;
+      for (int q = 0; q < 5; ++q) L->h_stats[s][q] = 0;
     }
   } while (0);
   if (d_w) (void)hipFree(d_w);
@@ -214,6 +223,14 @@ extern "C" int vsg_proposal_stats(VSGContext* c, long long* fields, long long* f
   if (!c || !c->learned) return vsg_fail("vsg_proposal_stats: no proposal network", hipSuccess);
   if (fields) *fields = c->learned->fields;
   if (fallbacks) *fallbacks = c->learned->fallbacks;
+  return 0;
+}
+
+extern "C" int vsg_proposal_fallbacks(VSGContext* c, long long* outOfRange, long long* edge, long long* residual) {
+  if (!c || !c->learned) return vsg_fail("vsg_proposal_fallbacks: no proposal network", hipSuccess);
+  if (outOfRange) *outOfRange = c->learned->why[0];
+  if (edge) *edge = c->learned->why[1];
+  if (residual) *residual = c->learned->why[2];
   return 0;
 }
 
@@ -235,7 +252,8 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
   VSGLearned* L = c->learned;
   VSGSet& s = c->set[set];
   const int slot0 = set * c->B, nc = c->st[0].nf;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 2);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 5);
+  if (L->netLastSet >= 0 && L->netLastSet != set) LCHECK(hipStreamWaitEvent(st, L->netDone, 0));
   hipLaunchKernelGGL(k_pair, dim3((L->Wq + 255) / 256, L->Hq, n), dim3(256), 0, st, L->d_down, L->planeStride,
                      L->d_downRef, s.d_prev, slot0, L->Wq, L->Hq, L->d_in);
   LCHECK(hipGetLastError());
@@ -254,10 +272,13 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
   const double maxSad = (double)L->thr * c->st[0].S * c->st[0].S;
   LCHECK(vsg_launch_window(ga, L->d_pred, nc, (uint32_t)(maxSad < 4294967295.0 ? maxSad : 4294967295.0),
                            L->d_fjobs[set], L->d_stats[set], st));
+  /* the window search was the last reader of the shared network buffers */
+  LCHECK(hipEventRecord(L->netDone, st));
+  L->netLastSet = set;
   VSGSearchArgs fb = ga;
   fb.jobs = L->d_fjobs[set];
   LCHECK(vsg_launch_search(fb, st));
-  LCHECK(hipMemcpyAsync(L->h_stats[set], L->d_stats[set], sizeof(unsigned) * 2, hipMemcpyDeviceToHost, st));
+  LCHECK(hipMemcpyAsync(L->h_stats[set], L->d_stats[set], sizeof(unsigned) * 5, hipMemcpyDeviceToHost, st));
   return 0;
 }
 
@@ -267,5 +288,6 @@ void vsg_learned_account(VSGContext* c, int set) {
   if (!L) return;
   L->fields += L->h_stats[set][0];
   L->fallbacks += L->h_stats[set][1];
-  L->h_stats[set][0] = L->h_stats[set][1] = 0;
+  for (int q = 0; q < 3; ++q) L->why[q] += L->h_stats[set][2 + q];
+  for (int q = 0; q < 5; ++q) L->h_stats[set][q] = 0;
 }

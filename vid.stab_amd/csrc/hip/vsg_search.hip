@@ -266,7 +266,8 @@ __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float
   const float* pp = pred + 2 * ((size_t)frame * nc + jb.field);
   const float fx = rintf(4.0f * pp[0]), fy = rintf(4.0f * pp[1]);
   const float lim = (float)(a.g.n * a.g.step - 2);   /* the full search's reach, less the window */
-  bool accept = fx >= -lim && fx <= lim && fy >= -lim && fy <= lim;
+  const bool inRange = fx >= -lim && fx <= lim && fy >= -lim && fy <= lim;
+  bool accept = inRange, interior = false;
   int bx = 0, by = 0;
   uint32_t minerr = 0;
   if (accept) {
@@ -284,7 +285,8 @@ __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float
           by = py - 2 + tj;
         }
       }
-    accept = bx > px - 2 && bx < px + 2 && by > py - 2 && by < py + 2 && minerr <= maxSad;
+    interior = bx > px - 2 && bx < px + 2 && by > py - 2 && by < py + 2;
+    accept = interior && minerr <= maxSad;
   }
   if (threadIdx.x == 0) {
     VSGJob f = jb;
@@ -299,6 +301,9 @@ __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float
     fallback[job] = f;
     atomicAdd(&stats[0], 1u);
     if (!accept) atomicAdd(&stats[1], 1u);
+    if (!inRange) atomicAdd(&stats[2], 1u);
+    else if (!interior) atomicAdd(&stats[3], 1u);
+    else if (!accept) atomicAdd(&stats[4], 1u);
   }
 }
 

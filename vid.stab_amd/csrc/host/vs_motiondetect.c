@@ -679,7 +679,14 @@ VS_API int vsMotionDetectSetProposal(VSMotionDetect* md, const char* weightsFile
 VS_API int vsMotionDetectProposalStats(const VSMotionDetect* md, long long* fields, long long* fallbacks) {
   MDState* s = state_find(md);
   if (!s) return VS_ERROR;
-  return vsg_proposal_stats(s->gpu, fields, fallbacks) == 0 ? VS_OK : VS_ERROR;
+  if (vsg_proposal_stats(s->gpu, fields, fallbacks) != 0) return VS_ERROR;
+  if (getenv("VIDSTAB_AMD_PROPOSAL_LOG")) {
+    long long r = 0, e = 0, t = 0;
+    vsg_proposal_fallbacks(s->gpu, &r, &e, &t);
+    vs_log_info(md->conf.modName, "proposal fallbacks: %lld out of range, %lld on the window edge, %lld above "
+                "the residual threshold\n", r, e, t);
+  }
+  return VS_OK;
 }
 
 VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n) {
