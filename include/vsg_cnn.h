@@ -29,6 +29,18 @@ VS_API int vsgc_pack_weights(const float* w, uint16_t* packed, int transposed, v
    NULL (the data gradient: conv of dz with the transposed packing) */
 VS_API int vsgc_conv(const uint16_t* in, int n, int h, int w, const uint16_t* packed, const float* bias,
                      int dil, int mode, uint16_t* outb, float* outf, void* stream);
+/* as vsgc_conv, for an input whose channels >= inChannels are zero; with
+   inChannels <= 8 only the first 8 channels enter the MFMA chain */
+VS_API int vsgc_conv_k(const uint16_t* in, int n, int h, int w, const uint16_t* packed, const float* bias,
+                       int dil, int mode, int inChannels, uint16_t* outb, float* outf, void* stream);
+
+/* The same convolution on the gfx950 K = 16 matrix instruction
+   (v_mfma_f32_32x32x16_bf16), used by the learned detect path: weights
+   packed by vsgc_pack_weights16 (9*2*64*8 BF16 values, forward only);
+   inChannels <= 16 uses one K step per tap. */
+VS_API int vsgc_pack_weights16(const float* w, uint16_t* packed16, void* stream);
+VS_API int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* packed16, const float* bias,
+                       int dil, int mode, int inChannels, uint16_t* outb, float* outf, void* stream);
 
 /* dW += d(loss)/dW and dB += d(loss)/dB of a layer from its input and the
    gradient dz at its pre-activation (accumulates; dB may be NULL) */

@@ -61,6 +61,13 @@ def test_conv_forward_modes(dil):
                        learned._p(outf), st) == 0
     got = outf.cpu().numpy().astype(np.float64)
     assert np.max(np.abs(got - ref)) <= 1e-3 * np.max(np.abs(ref))
+    # the K = 16 instruction path (learned detect), fp32 and bf16 outputs
+    pk16 = torch.empty(9 * 2 * 64 * 8, dtype=torch.int16, device="cuda")
+    assert L.vsgc_pack_weights16(learned._p(dw), learned._p(pk16), st) == 0
+    out16 = torch.empty((n, h, w, 32), dtype=torch.float32, device="cuda")
+    assert L.vsgc_conv16(learned._p(dx), n, h, w, learned._p(pk16), learned._p(db), dil, 2, 32, None,
+                         learned._p(out16), st) == 0
+    assert np.max(np.abs(out16.cpu().numpy().astype(np.float64) - ref)) <= 1e-3 * np.max(np.abs(ref))
     for mode, fn in ((0, lambda z: np.maximum(z, 0.0)), (1, lambda z: z)):
         outb = torch.empty((n, h, w, 32), dtype=torch.int16, device="cuda")
         assert L.vsgc_conv(learned._p(dx), n, h, w, learned._p(pk), learned._p(db), dil, mode,
@@ -99,6 +106,31 @@ def test_conv_gradients(dil):
     wW, wB = co.conv_weight_grad(x, dz, dil)
     assert np.max(np.abs(gW.cpu().numpy() - wW)) <= 1e-3 * np.max(np.abs(wW))
     assert np.max(np.abs(gB.cpu().numpy() - wB)) <= 1e-3 * np.max(np.abs(wB))
+
+
+def test_narrow_input_paths_match_full_width():
+    """the first layer's 2-channel input: the 8- and 16-channel K paths equal
+    the full 32-channel computation"""
+    from vidstab_amd import learned
+    torch = _torch()
+    L = learned.lib()
+    rng = np.random.default_rng(31)
+    n, h, w, dil = 2, 9, 70, 1
+    x = np.zeros((n, h, w, 32))
+    x[..., :2] = co.bf16_round(rng.standard_normal((n, h, w, 2)).astype(np.float32))
+    wt = co.bf16_round(rng.standard_normal((9, 32, 32)).astype(np.float32) * 0.2)
+    ref = co.conv(x, wt, None, dil)
+    st = learned._stream()
+    dx = _dev_bf16(x)
+    dw = torch.from_numpy(wt.astype(np.float32)).cuda()
+    pk = torch.empty(9 * 4 * 64 * 4, dtype=torch.int16, device="cuda")
+    pk16 = torch.empty(9 * 2 * 64 * 8, dtype=torch.int16, device="cuda")
+    assert L.vsgc_pack_weights(learned._p(dw), learned._p(pk), 0, st) == 0
+    assert L.vsgc_pack_weights16(learned._p(dw), learned._p(pk16), st) == 0
+    for fn, packed, args in ((L.vsgc_conv_k, pk, 2), (L.vsgc_conv16, pk16, 2)):
+        out = torch.empty((n, h, w, 32), dtype=torch.float32, device="cuda")
+        assert fn(learned._p(dx), n, h, w, learned._p(packed), None, dil, 2, args, None, learned._p(out), st) == 0
+        assert np.max(np.abs(out.cpu().numpy().astype(np.float64) - ref)) <= 1e-3 * np.max(np.abs(ref))
 
 
 def test_network_loss_and_gradients_match_oracle():

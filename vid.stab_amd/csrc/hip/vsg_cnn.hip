@@ -28,6 +28,7 @@ namespace {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -53,7 +54,13 @@ __global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk, i
 }
 
 /* ---- convolution ------------------------------------------------------------------
-   out = act(conv(in, W) + bias); mode: 0 bf16 + ReLU, 1 bf16, 2 fp32 */
+   out = act(conv(in, W) + bias); mode: 0 bf16 + ReLU, 1 bf16, 2 fp32.
+   KS input-channel steps of 8: 4 for a 32-channel input, 1 when only
+   channels 0..7 can be non-zero (the network's 2-channel input), which
+   skips MFMAs whose A operand is all zero.  All A fragments of a tile are
+   loaded before its MFMA chain, so the loads of a tile overlap each other
+   instead of each waiting in front of its MFMA. */
+template <int KS>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, int N, int H, int W,
                                               const uint16_t* __restrict__ pk, const float* __restrict__ bias,
                                               int dil, int mode, uint16_t* __restrict__ outb,
@@ -61,9 +68,12 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, i
   const int lane = threadIdx.x & 63;
   const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nw = (gridDim.x * blockDim.x) >> 6;
-  v4s b[36];
+  v4s b[9 * KS];
 #pragma unroll
-  for (int q = 0; q < 36; ++q) b[q] = *reinterpret_cast<const v4s*>(pk + ((size_t)q * 64 + lane) * 4);
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      b[tap * KS + ks] = *reinterpret_cast<const v4s*>(pk + ((size_t)(tap * 4 + ks) * 64 + lane) * 4);
   const int co = lane & 31, kq = lane >> 5;
   const float bco = bias ? bias[co] : 0.0f;
   const int tilesX = (W + 31) >> 5;
@@ -71,19 +81,91 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, i
   for (int t = gw; t < ntiles; t += nw) {
     const int xt = t % tilesX, r = t / tilesX, y = r % H, n = r / H;
     const int x = xt * 32 + (lane & 31);
-    v16f acc = {};
+    v4s a[9 * KS];
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
       const int yy = y + (tap / 3 - 1) * dil, xx = x + (tap % 3 - 1) * dil;
       const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
       const uint16_t* p = in + (((size_t)n * H + (ok ? yy : 0)) * W + (ok ? xx : 0)) * kC + 4 * kq;
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        v4s a = {0, 0, 0, 0};
-        if (ok) a = *reinterpret_cast<const v4s*>(p + ks * 8);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[tap * 4 + ks], acc, 0, 0, 0);
+      for (int ks = 0; ks < KS; ++ks) {
+        v4s v = {0, 0, 0, 0};
+        if (ok) v = *reinterpret_cast<const v4s*>(p + ks * 8);
+        a[tap * KS + ks] = v;
       }
     }
+    v16f acc = {};
+#pragma unroll
+    for (int q = 0; q < 9 * KS; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[q], b[q], acc, 0, 0, 0);
+    const size_t rowBase = ((size_t)n * H + y) * W;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
+      const int px = xt * 32 + m;
+      if (px < W) {
+        float val = acc[v] + bco;
+        const size_t o = (rowBase + px) * kC + co;
+        if (mode == 2) {
+          outf[o] = val;
+        } else {
+          if (mode == 0) val = val > 0.0f ? val : 0.0f;
+          outb[o] = f2bf(val);
+        }
+      }
+    }
+  }
+}
+
+/* ---- convolution on the gfx950 K = 16 matrix instruction -----------------------------
+   v_mfma_f32_32x32x16_bf16: lane l holds 8 consecutive K values (rows
+   8*(l/32) + 0..7) of A row / B column l%32, so an A fragment is one 16-byte
+   load of 8 channels of one pixel; a tile takes 9 x KS MFMAs (KS = 2 for 32
+   input channels, 1 for at most 16).  B is packed [tap][ks][lane][8]. */
+__global__ void k_pack16(const float* __restrict__ w, uint16_t* __restrict__ pk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;   /* over 9*2*64*8 */
+  if (i >= 9 * 2 * 64 * 8) return;
+  const int e = i & 7, lane = (i >> 3) & 63, ks = (i >> 9) & 1, tap = i >> 10;
+  const int k = ks * 16 + 8 * (lane >> 5) + e, n = lane & 31;
+  pk[i] = f2bf(w[(tap * kC + k) * kC + n]);
+}
+
+template <int KS>
+__global__ __launch_bounds__(256) void k_conv16(const uint16_t* __restrict__ in, int N, int H, int W,
+                                                const uint16_t* __restrict__ pk, const float* __restrict__ bias,
+                                                int dil, int mode, uint16_t* __restrict__ outb,
+                                                float* __restrict__ outf) {
+  const int lane = threadIdx.x & 63;
+  const int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  v8bf b[9 * KS];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      b[tap * KS + ks] = *reinterpret_cast<const v8bf*>(pk + ((size_t)(tap * 2 + ks) * 64 + lane) * 8);
+  const int co = lane & 31, kq = lane >> 5;
+  const float bco = bias ? bias[co] : 0.0f;
+  const int tilesX = (W + 31) >> 5;
+  const int ntiles = N * H * tilesX;
+  for (int t = gw; t < ntiles; t += nw) {
+    const int xt = t % tilesX, r = t / tilesX, y = r % H, n = r / H;
+    const int x = xt * 32 + (lane & 31);
+    v8bf a[9 * KS];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int yy = y + (tap / 3 - 1) * dil, xx = x + (tap % 3 - 1) * dil;
+      const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+      const uint16_t* p = in + (((size_t)n * H + (ok ? yy : 0)) * W + (ok ? xx : 0)) * kC + 8 * kq;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        v8bf v = {};
+        if (ok) v = *reinterpret_cast<const v8bf*>(p + ks * 16);
+        a[tap * KS + ks] = v;
+      }
+    }
+    v16f acc = {};
+#pragma unroll
+    for (int q = 0; q < 9 * KS; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[q], b[q], acc, 0, 0, 0);
     const size_t rowBase = ((size_t)n * H + y) * W;
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
@@ -295,10 +377,19 @@ extern "C" int vsgc_pack_weights(const float* w, uint16_t* packed, int transpose
 
 extern "C" int vsgc_conv(const uint16_t* in, int n, int h, int w, const uint16_t* packed, const float* bias,
                          int dil, int mode, uint16_t* outb, float* outf, void* stream) {
+  return vsgc_conv_k(in, n, h, w, packed, bias, dil, mode, 32, outb, outf, stream);
+}
+
+extern "C" int vsgc_conv_k(const uint16_t* in, int n, int h, int w, const uint16_t* packed, const float* bias,
+                           int dil, int mode, int inChannels, uint16_t* outb, float* outf, void* stream) {
   if (n <= 0 || h <= 0 || w <= 0 || dil < 1 || mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
   if ((mode == 2 && !outf) || (mode != 2 && !outb)) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_conv, dim3(conv_grid()), dim3(256), 0, (hipStream_t)stream, in, n, h, w, packed, bias, dil,
-                     mode, outb, outf);
+  if (inChannels <= 8)
+    hipLaunchKernelGGL(k_conv<1>, dim3(conv_grid()), dim3(256), 0, (hipStream_t)stream, in, n, h, w, packed, bias,
+                       dil, mode, outb, outf);
+  else
+    hipLaunchKernelGGL(k_conv<4>, dim3(conv_grid()), dim3(256), 0, (hipStream_t)stream, in, n, h, w, packed, bias,
+                       dil, mode, outb, outf);
   CNN_RET;
 }
 
@@ -359,5 +450,23 @@ extern "C" int vsgc_adam(float* w, const float* g, float* m, float* v, int count
   const float c1 = 1.0f / (1.0f - powf(b1, (float)step)), c2 = 1.0f / (1.0f - powf(b2, (float)step));
   hipLaunchKernelGGL(k_adam, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, count, lr,
                      b1, b2, eps, c1, c2);
+  CNN_RET;
+}
+
+extern "C" int vsgc_pack_weights16(const float* w, uint16_t* packed, void* stream) {
+  hipLaunchKernelGGL(k_pack16, dim3(blocks_for(9 * 2 * 64 * 8, 256)), dim3(256), 0, (hipStream_t)stream, w, packed);
+  CNN_RET;
+}
+
+extern "C" int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* packed16, const float* bias,
+                           int dil, int mode, int inChannels, uint16_t* outb, float* outf, void* stream) {
+  if (n <= 0 || h <= 0 || w <= 0 || dil < 1 || mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
+  if ((mode == 2 && !outf) || (mode != 2 && !outb)) return (int)hipErrorInvalidValue;
+  if (inChannels <= 16)
+    hipLaunchKernelGGL(k_conv16<1>, dim3(conv_grid()), dim3(256), 0, (hipStream_t)stream, in, n, h, w, packed16,
+                       bias, dil, mode, outb, outf);
+  else
+    hipLaunchKernelGGL(k_conv16<2>, dim3(conv_grid()), dim3(256), 0, (hipStream_t)stream, in, n, h, w, packed16,
+                       bias, dil, mode, outb, outf);
   CNN_RET;
 }

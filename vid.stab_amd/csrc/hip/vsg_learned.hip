@@ -83,7 +83,7 @@ struct VSGLearned {
   int nl = 0;
   int dil[kMaxLayers] = {};
   float thr = 0;
-  uint16_t* d_pk[kMaxLayers] = {};
+  uint16_t* d_pk[kMaxLayers] = {};   /* K = 16 packings */
   float* d_bias[kMaxLayers] = {};
   int Wq = 0, Hq = 0;
   size_t planeStride = 0;
@@ -148,12 +148,12 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
     for (int k = 0; k < nlayers && rc == 0; ++k) {
       L->dil[k] = dil[k];
       if (dil[k] < 1) { rc = vsg_fail("vsg_set_proposal: dilation", hipSuccess); break; }
-      if (hipMalloc(&L->d_pk[k], sizeof(uint16_t) * 9 * 4 * 64 * 4) != hipSuccess ||
+      if (hipMalloc(&L->d_pk[k], sizeof(uint16_t) * 9 * 2 * 64 * 8) != hipSuccess ||
           hipMalloc(&L->d_bias[k], sizeof(float) * kC) != hipSuccess ||
           hipMemcpy(d_w, weights + k * perLayer, sizeof(float) * 9 * kC * kC, hipMemcpyHostToDevice) != hipSuccess ||
           hipMemcpy(L->d_bias[k], weights + k * perLayer + 9 * kC * kC, sizeof(float) * kC, hipMemcpyHostToDevice) !=
               hipSuccess ||
-          vsgc_pack_weights(d_w, L->d_pk[k], 0, nullptr) != 0) {
+          vsgc_pack_weights16(d_w, L->d_pk[k], nullptr) != 0) {
         rc = vsg_fail("proposal weights", hipErrorOutOfMemory);
         break;
       }
@@ -261,8 +261,8 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
   for (int k = 0; k < L->nl; ++k) {
     const bool last = k == L->nl - 1;
     uint16_t* y = L->d_act[k & 1];
-    if (vsgc_conv(x, n, L->Hq, L->Wq, L->d_pk[k], L->d_bias[k], L->dil[k], last ? 2 : 0, last ? nullptr : y,
-                  last ? L->d_out : nullptr, st) != 0)
+    if (vsgc_conv16(x, n, L->Hq, L->Wq, L->d_pk[k], L->d_bias[k], L->dil[k], last ? 2 : 0, k == 0 ? 2 : 32,
+                    last ? nullptr : y, last ? L->d_out : nullptr, st) != 0)
       return vsg_fail("proposal network", hipErrorLaunchFailure);
     x = y;
   }

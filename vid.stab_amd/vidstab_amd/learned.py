@@ -34,6 +34,9 @@ def lib():
         vp, i32, i64, f32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
         L.vsgc_pack_weights.argtypes = [vp, vp, i32, vp]
         L.vsgc_conv.argtypes = [vp, i32, i32, i32, vp, vp, i32, i32, vp, vp, vp]
+        L.vsgc_conv_k.argtypes = [vp, i32, i32, i32, vp, vp, i32, i32, i32, vp, vp, vp]
+        L.vsgc_pack_weights16.argtypes = [vp, vp, vp]
+        L.vsgc_conv16.argtypes = [vp, i32, i32, i32, vp, vp, i32, i32, i32, vp, vp, vp]
         L.vsgc_conv_wgrad.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, vp]
         L.vsgc_relu_mask.argtypes = [vp, vp, i64, vp]
         L.vsgc_f32_to_bf16.argtypes = [vp, vp, i64, vp]
@@ -160,14 +163,15 @@ class ProposalNet:
         out = None
         for k in range(self.nlayers):
             last = k == self.nlayers - 1
+            cin = 2 if k == 0 else C      # the input map carries frames t-1, t in channels 0, 1
             if last:
                 out = torch.empty((n, h, w, C), dtype=torch.float32, device=x.device)
-                _chk(L.vsgc_conv(_p(acts[-1]), n, h, w, _p(self.pk[k]), _p(self.b[k]), self.dils[k], 2, None,
-                                 _p(out), st), "conv")
+                _chk(L.vsgc_conv_k(_p(acts[-1]), n, h, w, _p(self.pk[k]), _p(self.b[k]), self.dils[k], 2, cin, None,
+                                   _p(out), st), "conv")
             else:
                 y = torch.empty((n, h, w, C), dtype=torch.int16, device=x.device)
-                _chk(L.vsgc_conv(_p(acts[-1]), n, h, w, _p(self.pk[k]), _p(self.b[k]), self.dils[k], 0, _p(y),
-                                 None, st), "conv")
+                _chk(L.vsgc_conv_k(_p(acts[-1]), n, h, w, _p(self.pk[k]), _p(self.b[k]), self.dils[k], 0, cin, _p(y),
+                                   None, st), "conv")
                 acts.append(y)
         return acts, out
 
