@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="clip length (0: the workload's)")
     ap.add_argument("--cpu-frames", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-4k", action="store_true", help="skip the 4K plain vs learned-proposal measurement")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-frame (PCIe-inclusive) rate")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 256 MiB device copy (known bytes) to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -162,6 +164,23 @@ def main():
     bk = 0
     blur_gbs = prof.work[bk] / (prof.ms[bk] / 1000.0) / 1e9 if prof.ms[bk] else 0.0
 
+    host_fps = None
+    if not args.no_host:
+        # the same clip handed over as host frames (vsMotionDetectionBatch):
+        # every plane crosses PCIe inside the timed pass
+        det = api.Detector(spec.width, spec.height, shakiness=spec.shakiness, accuracy=spec.accuracy,
+                           stepsize=spec.stepsize, mincontrast=spec.mincontrast)
+        frames = [host[t] for t in range(nfr)]
+        det.detect_batch(frames[:64])
+        t0 = time.perf_counter()
+        det.detect_batch(frames, keep=False)
+        host_fps = nfr / (time.perf_counter() - t0)
+        det.close()
+        log(f"host frames (PCIe-inclusive): {host_fps:.1f} frames/s")
+    del dev
+    forms = None
+    if not args.no_4k:
+        forms = measure_forms_4k(0, 1, 1, args.weights)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -182,6 +201,8 @@ def main():
         "blur_roofline": {"bound": "hbm", "achieved": round(blur_gbs, 1), "peak": HBM_PEAK, "unit": "GB/s",
                           "frac": round(blur_gbs / HBM_PEAK, 4)},
         "cpu_baseline": cpu,
+        "host_frames_fps": round(host_fps, 2) if host_fps else None,
+        "forms_4k": forms,
     }
     print(json.dumps(res), flush=True)
 
@@ -247,7 +268,7 @@ def run_multi(args):
     print(json.dumps(res), flush=True)
 
 
-def run_learned(args):
+def measure_forms_4k(nframes, steps, warmup, weights):
     """4K motion-blur clip (config 5, clip 0, held out from training): the
     plain form and the learned-proposal form on the same device-resident
     frames; agreement with the full search per field and per frame transform"""
@@ -258,12 +279,13 @@ def run_learned(args):
 
     clip = Clip(5, 0)
     s = clip.spec
-    nfr = args.frames or s.nframes
+    nfr = nframes or s.nframes
     log(f"generating {nfr} frames {s.width}x{s.height}")
     host = np.empty((nfr, s.height, s.width), dtype=np.uint8)
     for t in range(nfr):
         clip.render(t, out=host[t])
     dev = torch.from_numpy(host).to("cuda:0")
+    del host
     torch.cuda.synchronize()
     fb = s.width * s.height
     ptrs = [dev.data_ptr() + t * fb for t in range(nfr)]
@@ -271,14 +293,14 @@ def run_learned(args):
     kw = dict(shakiness=s.shakiness, accuracy=s.accuracy, stepsize=s.stepsize, mincontrast=s.mincontrast)
 
     def timed(det):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             det.detect_device(ptrs, ls)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             det.detect_device(ptrs, ls)
         torch.cuda.synchronize()
-        return nfr * args.steps / (time.perf_counter() - t0)
+        return nfr * steps / (time.perf_counter() - t0)
 
     plain = api.Detector(s.width, s.height, **kw)
     plain_fps = timed(plain)
@@ -286,33 +308,40 @@ def run_learned(args):
     S = plain.md.fieldscoarse.fieldSize
     plain.close()
     lrn = api.Detector(s.width, s.height, **kw)
-    lrn.set_proposal(args.weights)
+    lrn.set_proposal(weights)
     learned_fps = timed(lrn)
     f0, b0 = lrn.proposal_stats()
     got = lrn.detect_device(ptrs, ls, keep=True)
     f1, b1 = lrn.proposal_stats()
     lrn.close()
+    del dev
     cn, ce, an, ae = api.compare_forms(ref, got, S)
     errs = [api.transform_distance(s.width, s.height, api.simple_transform(s.width, s.height, p),
                                    api.simple_transform(s.width, s.height, q)) for p, q in zip(ref[1:], got[1:])]
-    log(f"plain {plain_fps:.1f} fps, learned {learned_fps:.1f} fps ({learned_fps / plain_fps:.2f}x); fields equal "
-        f"{100.0 * ae / max(an, 1):.2f}% (coarse {100.0 * ce / max(cn, 1):.2f}%), fallbacks "
-        f"{100.0 * (b1 - b0) / max(f1 - f0, 1):.2f}%, mean transform error {np.mean(errs):.4f} px")
+    out = {"workload": f"{s.width}x{s.height}, {nfr} frames of config 5 clip 0 (motion blur, not in the "
+                       "training set), device-resident", "frames": nfr,
+           "plain_fps": round(plain_fps, 2), "learned_fps": round(learned_fps, 2),
+           "speedup_vs_plain": round(learned_fps / plain_fps, 3),
+           "fields_equal_pct": round(100.0 * ae / max(an, 1), 3),
+           "coarse_fields_equal_pct": round(100.0 * ce / max(cn, 1), 3),
+           "fallback_pct": round(100.0 * (b1 - b0) / max(f1 - f0, 1), 3),
+           "global_transform_err_px": round(float(np.mean(errs)), 5)}
+    log(f"4K: plain {plain_fps:.1f} fps, learned {learned_fps:.1f} fps ({learned_fps / plain_fps:.2f}x); fields "
+        f"equal {out['fields_equal_pct']:.2f}%, fallbacks {out['fallback_pct']:.2f}%, mean transform error "
+        f"{out['global_transform_err_px']:.4f} px")
+    return out
+
+
+def run_learned(args):
+    f = measure_forms_4k(args.frames, args.steps, args.warmup, args.weights)
     res = {
-        "metric": METRIC, "value": round(learned_fps, 2), "unit": "frames/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(nfr * 1000.0 / learned_fps, 3),
+        "metric": METRIC, "value": f["learned_fps"], "unit": "frames/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(f["frames"] * 1000.0 / f["learned_fps"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8+bf16",
         "data": "synthetic",
-        "config": {"workload": f"learned-proposal detect, {s.width}x{s.height} YUV420p 8-bit, {nfr}-frame seeded "
-                               "value-noise clip with motion blur (config 5, clip 0, not in the training set), "
-                               "device-resident frames", "width": s.width, "height": s.height, "frames": nfr,
-                   "stepsize": s.stepsize, "parallelism": "1 GPU"},
-        "plain_fps": round(plain_fps, 2), "speedup_vs_plain": round(learned_fps / plain_fps, 3),
-        "fields_equal_pct": round(100.0 * ae / max(an, 1), 3),
-        "coarse_fields_equal_pct": round(100.0 * ce / max(cn, 1), 3),
-        "fallback_pct": round(100.0 * (b1 - b0) / max(f1 - f0, 1), 3),
-        "global_transform_err_px": round(float(np.mean(errs)), 5),
+        "config": {"workload": "learned-proposal detect, " + f["workload"], "parallelism": "1 GPU"},
     }
+    res.update({k: v for k, v in f.items() if k not in ("workload", "frames")})
     print(json.dumps(res), flush=True)
 
 

@@ -102,7 +102,7 @@ void vsg_ctx_free(VSGContext* c) {
     dfree(s.d_selIdx); dfree(s.d_selC); dfree(s.d_count); dfree(s.d_jobsC); dfree(s.d_jobsF);
     dfree(s.d_best); dfree(s.d_done); dfree(s.d_resC); dfree(s.d_resF); dfree(s.d_src); dfree(s.d_prev); dfree(s.d_detect);
     hfree(s.h_selIdx); hfree(s.h_selC); hfree(s.h_count); hfree(s.h_resC); hfree(s.h_jobsF);
-    hfree(s.h_resF); hfree(s.h_src); hfree(s.h_prev); hfree(s.h_detect);
+    hfree(s.h_resF); hfree(s.h_src); hfree(s.h_prev); hfree(s.h_detect); hfree(s.h_stage);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.doneF) (void)hipEventDestroy(s.doneF);
     if (s.planesReady) (void)hipEventDestroy(s.planesReady);
@@ -411,9 +411,18 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
       s.h_src[i].p = luma[i];
       s.h_src[i].pitch = linesize[i];
     } else {
-      uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * c->pitch * c->H;
-      VSG_CHECK(hipMemcpy2DAsync(dst, c->pitch, luma[i], linesize[i], c->W, c->H, hipMemcpyHostToDevice,
-                                 stream));
+      /* pageable host frame: rows into the set's pinned staging area (the
+         set's previous batch is complete), then one DMA of the plane */
+      const size_t plane = (size_t)c->pitch * c->H;
+      if (!s.h_stage) VSG_CHECK(halloc(&s.h_stage, plane * c->B));
+      uint8_t* stg = s.h_stage + plane * i;
+      if (linesize[i] == c->pitch) {
+        memcpy(stg, luma[i], plane - (c->pitch - c->W));
+      } else {
+        for (int y = 0; y < c->H; ++y) memcpy(stg + (size_t)y * c->pitch, luma[i] + (size_t)y * linesize[i], c->W);
+      }
+      uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * plane;
+      VSG_CHECK(hipMemcpyAsync(dst, stg, plane, hipMemcpyHostToDevice, stream));
       s.h_src[i].p = dst;
       s.h_src[i].pitch = c->pitch;
     }

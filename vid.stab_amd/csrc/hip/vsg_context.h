@@ -48,6 +48,7 @@ struct VSGSet {
   VSGBlurSrc* h_src = nullptr;
   int* h_prev = nullptr;
   uint8_t* h_detect = nullptr;
+  uint8_t* h_stage = nullptr;   /* pinned staging of host frames (B x pitch x H), allocated on first use */
   /* state */
   hipStream_t stream = nullptr;   /* all work of the set runs in order here */
   int n = 0;              /* frames in flight */

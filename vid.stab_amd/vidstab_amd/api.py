@@ -194,16 +194,14 @@ class Detector:
         self.L.vs_vector_del(ctypes.byref(vec))
         return out
 
-    def detect_batch(self, lumas):
+    def detect_batch(self, lumas, keep=True):
         lumas = [np.ascontiguousarray(x, dtype=np.uint8) for x in lumas]
         frames = (VSFrame * len(lumas))(*[self._frame(x) for x in lumas])
         vecs = (VSVector * len(lumas))()
         if self.L.vsMotionDetectionBatch(ctypes.byref(self.md), vecs, frames, len(lumas)) != 0:
             raise RuntimeError("vsMotionDetectionBatch failed: " + self.L.vsg_last_error().decode())
-        out = []
-        for v in vecs:
-            out.append(motions_of(v))
-            self.L.vs_vector_del(ctypes.byref(v))
+        out = [motions_of(v) for v in vecs] if keep else [v.nelems for v in vecs]
+        self.L.vsLocalMotionsArrayDel(vecs, len(lumas))
         return out
 
     def detect_device(self, ptrs, linesizes, keep=False):
