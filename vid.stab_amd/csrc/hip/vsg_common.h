@@ -62,6 +62,7 @@ typedef struct {
   int D;        /* words between a lane's horizontal candidates (m*step/4) */
   int CW;       /* curr words per column chunk */
   int wg;       /* threads per workgroup of the grid kernel (multiple of 64) */
+  int lds;      /* 1: the field patch is staged in LDS (VGPR operand), 0: scalar loads (SGPR operand) */
 } VSGGrid;
 
 #endif /* VSG_COMMON_H */

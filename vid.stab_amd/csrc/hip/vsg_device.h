@@ -18,6 +18,7 @@ struct VSGTile {
   const uint8_t* prev;     /* previous blurred plane */
   int pitch;               /* bytes per row of prev (multiple of 64) */
   vsg_cword_ptr patch;     /* the current field, S x S, 4-byte aligned */
+  const uint32_t* lpatch;  /* the same field staged in LDS (plans with an LDS patch) */
   int S, s, n, m, U, V;    /* field size, grid step, half width, lane layout */
   int fxc, fyc;            /* field centre */
   int ox, oy;              /* search offset */
@@ -33,7 +34,7 @@ struct VSGTile {
    wave-uniform scalar loads (SGPR operand of v_sad_u8).  Iterating over prev
    rows p, candidate k pairs prev row p with field row p - k*step.
    Returns the lane's best (sad << 32 | spiral rank), or ~0 for idle lanes. */
-template <int J, int K, int D, int CW>
+template <int J, int K, int D, int CW, bool LDSP>
 __device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int lid) {
   const int S = t.S, s = t.s, n = t.n, m = t.m;
   const int u = lid % t.U;
@@ -61,6 +62,7 @@ __device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int
     vsg_gword_ptr wp = (vsg_gword_ptr)(start & ~(uintptr_t)3);
     const uint32_t sh = (uint32_t)(start & 3);
     vsg_cword_ptr cbase = t.patch + CW * c;
+    const uint32_t* lbase = t.lpatch + CW * c;
     for (int p = 0; p < prows; ++p) {
       uint32_t raw[NP + 1];
 #pragma unroll
@@ -73,9 +75,10 @@ __device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int
         const int y = p - k * s;
         if (y >= 0 && y < S) {
           vsg_cword_ptr crow = cbase + y * wordsPerRow;
+          const uint32_t* lrow = lbase + y * wordsPerRow;
 #pragma unroll
           for (int w = 0; w < CW; ++w) {
-            const uint32_t cw = crow[w];
+            const uint32_t cw = LDSP ? lrow[w] : crow[w];
 #pragma unroll
             for (int j = 0; j < J; ++j) acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw, acc[k][j]);
           }

@@ -156,8 +156,9 @@ __device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned lo
    workgroup that finishes the job last (counted in done[job]) reads the
    winner and runs the refinement, then resets best / done for the next
    launch.  The refinement of early jobs overlaps the grid of later ones. */
-template <int J, int K, int D, int CW>
+template <int J, int K, int D, int CW, bool LDSP>
 __device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int tile, RefineLds& L) {
+  extern __shared__ uint32_t lpatch[];   /* LDSP: the field, S*S/4 words */
   const VSGJob jb = a.jobs[job];
   if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
   VSGTile t;
@@ -175,7 +176,18 @@ __device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int t
   t.prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
   t.patch = (vsg_cword_ptr)(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
                             (size_t)jb.field * t.S * t.S);
-  unsigned long long best = vsg_sad_lane<J, K, D, CW>(t, tile * blockDim.x + threadIdx.x);
+  t.lpatch = lpatch;
+  if (LDSP) {
+    /* the whole workgroup reads the same field words: stage them once, and
+       every lane then reads them by LDS broadcast instead of scalar loads
+       that miss a scalar cache shared by many resident fields */
+    const uint4* src = reinterpret_cast<const uint4*>(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
+                                                      (size_t)jb.field * t.S * t.S);
+    uint4* dst = reinterpret_cast<uint4*>(lpatch);
+    for (int i = threadIdx.x; i < t.S * t.S / 16; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+  }
+  unsigned long long best = vsg_sad_lane<J, K, D, CW, LDSP>(t, tile * blockDim.x + threadIdx.x);
   best = vsg_wave_min_u64(best);
   const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   if (a.tiles == 1) {
@@ -209,13 +221,13 @@ __device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int t
   refine_job(a, jb, L.key, job, L);
 }
 
-template <int J, int K, int D, int CW>
+template <int J, int K, int D, int CW, bool LDSP>
 __global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
   __shared__ RefineLds L;
   const int unit = xcd_unit(blockIdx.x, a.chunk, a.njobs * a.tiles);
   if (unit < 0) return;
   const int job = unit / a.tiles;
-  search_wg<J, K, D, CW>(a, job, unit - job * a.tiles, L);
+  search_wg<J, K, D, CW, LDSP>(a, job, unit - job * a.tiles, L);
 }
 
 /* The multi-clip form: one launch over the flattened units of several
@@ -223,7 +235,7 @@ __global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
    shape fits all).  Segment s owns units [unitBase, unitBase + njobs*tiles);
    a workgroup finds its segment by binary search over the (wave-uniform)
    table. */
-template <int J, int K, int D, int CW>
+template <int J, int K, int D, int CW, bool LDSP>
 __global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict__ segs, int nseg, int totalUnits,
                                                    int chunk) {
   __shared__ RefineLds L;
@@ -240,7 +252,7 @@ __global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict
   const VSGMultiSeg* sg = segs + lo;
   const int local = unit - sg->unitBase;
   const int job = local / sg->a.tiles;
-  search_wg<J, K, D, CW>(sg->a, job, local - job * sg->a.tiles, L);
+  search_wg<J, K, D, CW, LDSP>(sg->a, job, local - job * sg->a.tiles, L);
 }
 
 /* ---- proposal window (learned form) -----------------------------------------------
@@ -310,7 +322,10 @@ __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float
 template <int J, int K, int D, int CW>
 hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
   const int nb = xcd_blocks(a.njobs * a.tiles, a.chunk);
-  hipLaunchKernelGGL((k_sad_search<J, K, D, CW>), dim3(nb), dim3(a.g.wg), 0, st, a);
+  if (a.g.lds)
+    hipLaunchKernelGGL((k_sad_search<J, K, D, CW, true>), dim3(nb), dim3(a.g.wg), (size_t)a.g.S * a.g.S, st, a);
+  else
+    hipLaunchKernelGGL((k_sad_search<J, K, D, CW, false>), dim3(nb), dim3(a.g.wg), 0, st, a);
   return hipGetLastError();
 }
 
@@ -336,22 +351,26 @@ hipError_t launch_grid_k(const VSGSearchArgs& a, hipStream_t st) {
 }
 
 template <int J, int K, int D, int CW>
-hipError_t launch_multi(const VSGMultiSeg* segs, int nseg, int totalUnits, int wg, hipStream_t st) {
+hipError_t launch_multi(const VSGMultiSeg* segs, int nseg, int totalUnits, int wg, int lds, int S, hipStream_t st) {
   const int chunk = 32;
   const int nb = xcd_blocks(totalUnits, chunk);
-  hipLaunchKernelGGL((k_sad_multi<J, K, D, CW>), dim3(nb), dim3(wg), 0, st, segs, nseg, totalUnits, chunk);
+  if (lds)
+    hipLaunchKernelGGL((k_sad_multi<J, K, D, CW, true>), dim3(nb), dim3(wg), (size_t)S * S, st, segs, nseg,
+                       totalUnits, chunk);
+  else
+    hipLaunchKernelGGL((k_sad_multi<J, K, D, CW, false>), dim3(nb), dim3(wg), 0, st, segs, nseg, totalUnits, chunk);
   return hipGetLastError();
 }
 
 template <int J, int K, int D>
 hipError_t launch_multi_cw(const VSGMultiSeg* segs, int nseg, int units, const VSGGrid& g, hipStream_t st) {
   switch (g.CW) {
-    case 28: return launch_multi<J, K, D, 28>(segs, nseg, units, g.wg, st);
-    case 20: return launch_multi<J, K, D, 20>(segs, nseg, units, g.wg, st);
-    case 16: return launch_multi<J, K, D, 16>(segs, nseg, units, g.wg, st);
-    case 12: return launch_multi<J, K, D, 12>(segs, nseg, units, g.wg, st);
-    case 8: return launch_multi<J, K, D, 8>(segs, nseg, units, g.wg, st);
-    default: return launch_multi<J, K, D, 4>(segs, nseg, units, g.wg, st);
+    case 28: return launch_multi<J, K, D, 28>(segs, nseg, units, g.wg, g.lds, g.S, st);
+    case 20: return launch_multi<J, K, D, 20>(segs, nseg, units, g.wg, g.lds, g.S, st);
+    case 16: return launch_multi<J, K, D, 16>(segs, nseg, units, g.wg, g.lds, g.S, st);
+    case 12: return launch_multi<J, K, D, 12>(segs, nseg, units, g.wg, g.lds, g.S, st);
+    case 8: return launch_multi<J, K, D, 8>(segs, nseg, units, g.wg, g.lds, g.S, st);
+    default: return launch_multi<J, K, D, 4>(segs, nseg, units, g.wg, g.lds, g.S, st);
   }
 }
 
@@ -383,6 +402,7 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K) {
   const int lcm4 = 4 * step / gcd_int(4, step);
   const int D = lcm4 / 4;
   g->K = (K == 6 || K == 8) ? K : 4;
+  g->lds = 0;
   /* the widest column chunk (<= 28 words) that divides a field row: one prev
      row segment load then feeds J*K*CW v_sad_u8 */
   static const int kCW[] = {28, 20, 16, 12, 8, 4};

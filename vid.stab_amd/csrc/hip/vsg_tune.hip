@@ -4,7 +4,10 @@
  * A plan fixes how a job's (2n+1)^2 candidate grid is spread over lanes:
  * K candidate rows per lane (4, 6 or 8; J columns follow from the step)
  * and with it the workgroup size and workgroups per job; the column chunk CW
- * follows from the field size.  Every plan computes the same
+ * follows from the field size.  For fields up to 128 px a plan may also
+ * stage the field in LDS (read by broadcast into a VGPR operand) instead of
+ * reading it with scalar loads (SGPR operand), which trades scalar-cache
+ * misses for LDS reads.  Every plan computes the same
  * argmin, so the choice only moves time.  For each stage of a new context the
  * tuner times every plan on the device over one full batch of that stage's
  * jobs (all fields of maxBatch frames, zero offsets, the context's own
@@ -30,7 +33,8 @@ constexpr int kPlans[3] = {4, 6, 8};
 
 typedef std::tuple<int, int, int, int, int, int> TuneKey;   /* device, S, step, maxShift, B, nf */
 std::mutex g_tuneMu;
-std::map<TuneKey, int> g_tuned;
+std::map<TuneKey, int> g_tuned;   /* K + 16 * lds */
+constexpr int kMaxLdsField = 128;
 
 int env_int(const char* name, int def) {
   const char* v = getenv(name);
@@ -44,12 +48,13 @@ int env_int(const char* name, int def) {
     if (e_ != hipSuccess) return vsg_fail(#call, e_);     \
   } while (0)
 
-/* times plan K on stage k (ms, best of 2 after a warm-up) */
-int time_plan(VSGContext* c, int k, int K, const VSGJob* dJobs, int njobs, float* ms) {
+/* times plan (K, lds) on stage k (ms, best of 2 after a warm-up) */
+int time_plan(VSGContext* c, int k, int K, int lds, const VSGJob* dJobs, int njobs, float* ms) {
   VSGStage& S = c->st[k];
   VSGSet& s = c->set[0];
   VSGStage trial = S;
   trial.tiles = vsg_plan_grid(&trial.g, S.S, S.step, S.maxShift, K);
+  trial.g.lds = lds;
   VSGSearchArgs a = vsg_ctx_search_args(c, k, dJobs, njobs, s.d_best, s.d_done, k == 0 ? s.d_resC : s.d_resF);
   a.g = trial.g;
   a.tiles = trial.tiles;
@@ -95,18 +100,23 @@ int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift) {
 
 int vsg_ctx_tune(VSGContext* c) {
   const int forced = env_int("VIDSTAB_AMD_SEARCH_K", 0);
+  const int forcedLds = env_int("VIDSTAB_AMD_SEARCH_LDS", -1);
   const bool timing = env_int("VIDSTAB_AMD_AUTOTUNE", 1) != 0;
   for (int k = 0; k < 2; ++k) {
     VSGStage& S = c->st[k];
-    int K = 0;
+    int K = 0, lds = 0;
     if (forced == 4 || forced == 6 || forced == 8) {
       K = forced;
+      lds = forcedLds == 1 && S.S <= kMaxLdsField;
     } else if (timing) {
       const TuneKey key(c->device, S.S, S.step, S.maxShift, c->B, S.nf);
       {
         std::lock_guard<std::mutex> lk(g_tuneMu);
         auto it = g_tuned.find(key);
-        if (it != g_tuned.end()) K = it->second;
+        if (it != g_tuned.end()) {
+          K = it->second & 15;
+          lds = it->second >> 4;
+        }
       }
       if (K == 0) {
         /* one full batch of the stage's jobs on set 0's buffers */
@@ -124,20 +134,28 @@ int vsg_ctx_tune(VSGContext* c) {
         VSGJob* dJobs = k == 0 ? c->set[0].d_jobsC : c->set[0].d_jobsF;
         TUNE_CHECK(hipMemcpy(dJobs, jobs.data(), sizeof(VSGJob) * (size_t)njobs, hipMemcpyHostToDevice));
         float bestMs = 0;
-        for (int q = 0; q < 3; ++q) {
-          float ms = 0;
-          if (time_plan(c, k, kPlans[q], dJobs, njobs, &ms) != 0) return -1;
-          S.tuneMs[q] = ms;
-          if (K == 0 || ms < bestMs) {
-            bestMs = ms;
-            K = kPlans[q];
+        for (int l = 0; l < 2; ++l) {
+          if (l == 1 && (S.S > kMaxLdsField || forcedLds == 0)) break;
+          if (l == 0 && forcedLds == 1 && S.S <= kMaxLdsField) continue;
+          for (int q = 0; q < 3; ++q) {
+            float ms = 0;
+            if (time_plan(c, k, kPlans[q], l, dJobs, njobs, &ms) != 0) return -1;
+            S.tuneMs[3 * l + q] = ms;
+            if (K == 0 || ms < bestMs) {
+              bestMs = ms;
+              K = kPlans[q];
+              lds = l;
+            }
           }
         }
         std::lock_guard<std::mutex> lk(g_tuneMu);
-        g_tuned[key] = K;
+        g_tuned[key] = K + 16 * lds;
       }
     }
-    if (K != 0) S.tiles = vsg_plan_grid(&S.g, S.S, S.step, S.maxShift, K);
+    if (K != 0) {
+      S.tiles = vsg_plan_grid(&S.g, S.S, S.step, S.maxShift, K);
+      S.g.lds = lds;
+    }
   }
   return 0;
 }
@@ -146,12 +164,13 @@ extern "C" int vsg_get_plan(VSGContext* c, int stage, int* out, float* tuneMs) {
   if (!c || stage < 0 || stage > 1 || !out) return vsg_fail("vsg_get_plan", hipSuccess);
   const VSGStage& S = c->st[stage];
   if (tuneMs)
-    for (int q = 0; q < 3; ++q) tuneMs[q] = S.tuneMs[q];
+    for (int q = 0; q < 6; ++q) tuneMs[q] = S.tuneMs[q];
   out[0] = S.g.J;
   out[1] = S.g.K;
   out[2] = S.g.D;
   out[3] = S.g.CW;
   out[4] = S.g.wg;
   out[5] = S.tiles;
+  out[6] = S.g.lds;
   return 0;
 }

@@ -255,7 +255,7 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
     return VS_ERROR;
   }
   s->key = md;
-  s->maxBatch = VS_MIN(MD_MAX_BATCH, VS_MAX(1, env_int("VIDSTAB_AMD_MAX_BATCH", 32)));
+  s->maxBatch = VS_MIN(MD_MAX_BATCH, VS_MAX(1, env_int("VIDSTAB_AMD_MAX_BATCH", 64)));
   s->lastSlot = -1;
   const int device = env_int("VIDSTAB_AMD_DEVICE", 0);
   VSGStageDesc dc = vsd_stage_desc(&md->fieldscoarse), df = vsd_stage_desc(&md->fieldsfine);
@@ -276,13 +276,14 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
   nth = VS_MIN(16, VS_MAX(1, env_int("VIDSTAB_AMD_HOST_THREADS", nth)));
   s->pool = vs_pool_create(nth);
   for (int k = 0; k < 2; k++) {
-    int pl[6];
-    float tm[3];
+    int pl[7];
+    float tm[6];
     if (vsg_get_plan(s->gpu, k, pl, tm) == 0)
       vs_log_info(md->conf.modName,
-                  "%s search plan: J=%i K=%i D=%i CW=%i workgroup %i x %i per field "
-                  "(batch timed at K=4/6/8: %.3f/%.3f/%.3f ms)\n",
-                  k == 0 ? "coarse" : "fine", pl[0], pl[1], pl[2], pl[3], pl[4], pl[5], tm[0], tm[1], tm[2]);
+                  "%s search plan: J=%i K=%i D=%i CW=%i workgroup %i x %i per field, field %s "
+                  "(batch timed at K=4/6/8: scalar %.3f/%.3f/%.3f ms, LDS %.3f/%.3f/%.3f ms)\n",
+                  k == 0 ? "coarse" : "fine", pl[0], pl[1], pl[2], pl[3], pl[4], pl[5], pl[6] ? "in LDS" : "scalar",
+                  tm[0], tm[1], tm[2], tm[3], tm[4], tm[5]);
   }
   vs_log_info(md->conf.modName, "GPU: HIP device %i, batch %i, %i host threads\n", device, s->maxBatch, nth);
   state_insert(s);
