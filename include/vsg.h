@@ -84,11 +84,18 @@ typedef struct {
 } VSGBatchView;
 
 /* Creates the device state for width x height frames on `device`: blur box
-   `blurSize` (< 2: plain copy), the coarse and fine grids, two sets of
+   `blurSize` (< 2: plain copy), the coarse and fine grids, VSG_NSETS sets of
    maxBatch slots.  Replaces the buffers of vsMotionDetectInit
-   (src/motiondetect.c:104-164). */
+   (src/motiondetect.c:104-164).  Frames are planar; their luma is searched. */
 VS_API int vsg_create(VSGContext** ctx, int device, int width, int height, int blurSize,
                       const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch);
+/* The same for frames of bytesPerPixel bytes a pixel: 1 is vsg_create; 3 or 4
+   are packed RGB24 / BGR24 / RGBA frames, whose plane 0 (width*bytesPerPixel
+   bytes a row) is searched unblurred with the packed rules
+   (src/motiondetect.c:197-201, 370-377, 591-660).  Packed frames need
+   blurSize < 2. */
+VS_API int vsg_create_fmt(VSGContext** ctx, int device, int width, int height, int bytesPerPixel,
+                          int blurSize, const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch);
 VS_API void vsg_destroy(VSGContext* ctx);
 
 /* Enqueues a batch of n <= maxBatch frames into `set`: upload (host luma) or

@@ -50,7 +50,7 @@ class OrState(ctypes.Structure):
                 ("curr", ctypes.c_void_p), ("prev", ctypes.c_void_p), ("tmp", ctypes.c_void_p),
                 ("hasSeenOneFrame", ctypes.c_int), ("frameNum", ctypes.c_int),
                 ("nCoarse", ctypes.c_int), ("nFineAll", ctypes.c_int), ("nFineKept", ctypes.c_int),
-                ("lastT", VSTransform), ("lastMeanMatch", ctypes.c_double)]
+                ("lastT", VSTransform), ("lastMeanMatch", ctypes.c_double), ("bpp", ctypes.c_int)]
 
 
 _lib = None
@@ -64,6 +64,17 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.or_new.restype = ctypes.POINTER(OrState)
         L.or_new.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double, ctypes.c_int]
+        L.or_new_packed.restype = ctypes.POINTER(OrState)
+        L.or_new_packed.argtypes = [ctypes.c_int] * 6 + [ctypes.c_double, ctypes.c_int]
+        L.or_contrast_packed.restype = ctypes.c_double
+        L.or_contrast_packed.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5
+        L.or_sad_packed.restype = ctypes.c_uint32
+        L.or_sad_packed.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 7
+        L.or_field_search_packed.restype = ctypes.c_int
+        L.or_field_search_packed.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(OrFields), ctypes.POINTER(Field), ctypes.c_int,
+                                             ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint32)]
         L.or_delete.argtypes = [ctypes.POINTER(OrState)]
         L.or_detect.restype = ctypes.c_int
         L.or_detect.argtypes = [ctypes.POINTER(OrState), ctypes.c_void_p, ctypes.c_int,
@@ -108,14 +119,19 @@ def motions_to_list(arr, n):
 
 
 class Detector:
-    """One vsMotionDetection stream on the CPU oracle."""
+    """One vsMotionDetection stream on the CPU oracle.  bpp 1: planar luma;
+    3 or 4: packed RGB24 / BGR24 / RGBA frames (plane 0, width*bpp bytes a row)."""
 
     def __init__(self, width, height, shakiness=5, accuracy=15, stepsize=6,
-                 mincontrast=0.25, tripod=0):
+                 mincontrast=0.25, tripod=0, bpp=1):
         self._L = lib()
-        self.width, self.height = width, height
-        self.s = self._L.or_new(width, height, shakiness, accuracy, stepsize,
-                                float(mincontrast), tripod)
+        self.width, self.height, self.bpp = width, height, bpp
+        if bpp == 1:
+            self.s = self._L.or_new(width, height, shakiness, accuracy, stepsize,
+                                    float(mincontrast), tripod)
+        else:
+            self.s = self._L.or_new_packed(width, height, bpp, shakiness, accuracy, stepsize,
+                                           float(mincontrast), tripod)
         if not self.s:
             raise RuntimeError("or_new failed")
         st = self.s.contents
@@ -123,15 +139,16 @@ class Detector:
         self.buf = (LocalMotion * self.maxout)()
 
     def detect(self, luma):
-        """luma: uint8 array (H, stride) -> list of motion tuples"""
+        """luma (or packed plane 0): uint8 array (H, stride) -> list of motion tuples"""
         luma = np.ascontiguousarray(luma, dtype=np.uint8)
         n = self._L.or_detect(self.s, luma.ctypes.data, luma.shape[1], self.buf, self.maxout)
         return motions_to_list(self.buf, n)
 
     def blurred(self):
+        """the working plane (packed frames: the unblurred copy, width*bpp bytes a row)"""
         p = self._L.or_blurred(self.s)
         return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint8)),
-                                     shape=(self.height, self.width)).copy()
+                                     shape=(self.height, self.width * self.bpp)).copy()
 
     @property
     def state(self):
@@ -155,7 +172,7 @@ class RefDetector:
     through the same C API.  Used as the CPU baseline and as a cross-check."""
 
     def __init__(self, width, height, shakiness=5, accuracy=15, stepsize=6, mincontrast=0.25,
-                 tripod=0, threads=0):
+                 tripod=0, threads=0, pformat=None):
         from vidstab_amd import api  # struct layouts only (identical ABI)
         if not os.path.exists(REF_PATH):
             raise RuntimeError(f"reference build missing: {REF_PATH}")
@@ -174,7 +191,8 @@ class RefDetector:
         ctypes.c_int.in_dll(L, "vs_log_level").value = 0
         self.L = L
         self.fi = api.VSFrameInfo()
-        L.vsFrameInfoInit(ctypes.byref(self.fi), width, height, api.PF_YUV420P)
+        self.packed = pformat is not None and pformat > api.PF_PACKED
+        L.vsFrameInfoInit(ctypes.byref(self.fi), width, height, pformat if self.packed else api.PF_YUV420P)
         self.md = api.VSMotionDetect()
         conf = L.vsMotionDetectGetDefaultConfig(b"ref")
         conf.shakiness, conf.accuracy, conf.stepSize = shakiness, accuracy, stepsize
@@ -191,8 +209,9 @@ class RefDetector:
         fr = api.VSFrame()
         fr.data[0] = luma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         fr.linesize[0] = luma.shape[1]
-        fr.data[1] = fr.data[2] = self._chroma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
-        fr.linesize[1] = fr.linesize[2] = self.width // 2
+        if not self.packed:
+            fr.data[1] = fr.data[2] = self._chroma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            fr.linesize[1] = fr.linesize[2] = self.width // 2
         vec = api.VSVector()
         self.L.vsMotionDetection(ctypes.byref(self.md), ctypes.byref(vec), ctypes.byref(fr))
         out = api.motions_of(vec) if keep else vec.nelems

@@ -1,6 +1,6 @@
 /*
  * vsoracle.c -- CPU restatement of vid.stab's detect pass (vsMotionDetection)
- * for the planar formats.
+ * for the planar formats and the packed RGB24 / BGR24 / RGBA formats.
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the GPU library.  Nothing in the
  * product links this file.
@@ -77,7 +77,26 @@ int or_init(OrState* s, int width, int height, int shakiness, int accuracy, int 
                       fieldSizeFine, contrastThreshold / 2))
     return -1;
   s->blurSize = or_blur_size(stepSize, width, height);
+  s->bpp = 1;
   size_t n = (size_t)width * height;
+  s->curr = (uint8_t*)calloc(n, 1);
+  s->prev = (uint8_t*)calloc(n, 1);
+  s->tmp = (uint8_t*)calloc(n, 1);
+  return (s->curr && s->prev && s->tmp) ? 0 : -1;
+}
+
+/* src/motiondetect.c:83-165 for PF_RGB24 / PF_BGR24 / PF_RGBA: the same field
+   geometry, no blur (:197-201), planes of width*bpp bytes a row */
+int or_init_packed(OrState* s, int width, int height, int bpp, int shakiness, int accuracy, int stepSize,
+                   double contrastThreshold, int virtualTripod) {
+  if (bpp != 3 && bpp != 4) return -1;
+  if (or_init(s, width, height, shakiness, accuracy, stepSize, contrastThreshold, virtualTripod) != 0) return -1;
+  s->bpp = bpp;
+  s->blurSize = 1;
+  size_t n = (size_t)width * height * bpp;
+  free(s->curr);
+  free(s->prev);
+  free(s->tmp);
   s->curr = (uint8_t*)calloc(n, 1);
   s->prev = (uint8_t*)calloc(n, 1);
   s->tmp = (uint8_t*)calloc(n, 1);
@@ -299,6 +318,84 @@ int or_field_search(const uint8_t* cur, const uint8_t* prev, int w, int h, int s
   return 1;
 }
 
+/* ---- packed frames: src/motiondetect.c:370-377, 389-407, 591-660, 971-996 -- */
+
+/* mean Michelson contrast of the three colour channels (a fourth byte is left
+   out).  The reference's -ffast-math build computes the mean as
+   ((c0 + c1) + c2) * (1/3): a multiplication by the rounded reciprocal, not a
+   division by 3 (read from its x86 code built here from its own sources). */
+double or_contrast_packed(const uint8_t* img, int stride, int bpp, int x, int y, int size) {
+  double c[3];
+  int s2 = size / 2;
+  for (int ch = 0; ch < 3; ch++) {
+    int lo = 255, hi = 0;
+    for (int j = 0; j < size; j++) {
+      const uint8_t* p = img + (size_t)(y - s2 + j) * stride + (size_t)(x - s2) * bpp + ch;
+      for (int k = 0; k < size; k++) {
+        int v = p[(size_t)k * bpp];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+      }
+    }
+    c[ch] = or_contrast(lo, hi);
+  }
+  return ((c[0] + c[1]) + c[2]) * (1.0 / 3.0);
+}
+
+/* SAD over size rows of size*bpp bytes; the column is scaled by bpp, the row
+   offset is the byte stride */
+static uint32_t or_sad_packed_thr(const uint8_t* cur, const uint8_t* prev, int stride, int bpp, int fx,
+                                  int fy, int size, int dx, int dy, uint32_t thr) {
+  int s2 = size / 2, rb = size * bpp;
+  const uint8_t* p1 = cur + (size_t)(fy - s2) * stride + (size_t)(fx - s2) * bpp;
+  const uint8_t* p2 = prev + (size_t)(fy - s2 + dy) * stride + (ptrdiff_t)(fx - s2 + dx) * bpp;
+  uint32_t sum = 0;
+  for (int j = 0; j < size; j++) {
+    for (int k = 0; k < rb; k++) sum += (uint32_t)abs((int)p1[k] - (int)p2[k]);
+    if (sum > thr) break;
+    p1 += stride;
+    p2 += stride;
+  }
+  return sum;
+}
+
+uint32_t or_sad_packed(const uint8_t* cur, const uint8_t* prev, int stride, int bpp, int fx, int fy,
+                       int size, int dx, int dy) {
+  return or_sad_packed_thr(cur, prev, stride, bpp, fx, fy, size, dx, dy, UINT_MAX);
+}
+
+/* calcFieldTransPacked: the unshifted position first, then the raster
+   i = -maxShift .. maxShift by stepSize (x outer, y inner, (0,0) skipped),
+   then 1-px shifts within stepSize-1 of the best, strict "<" throughout;
+   reject at maxShift + stepSize - 1 on either axis */
+int or_field_search_packed(const uint8_t* cur, const uint8_t* prev, int stride, int bpp,
+                           const OrFields* fs, const Field* f, int ox, int oy,
+                           int* tx_out, int* ty_out, uint32_t* minerror_out) {
+  int stepSize = fs->stepSize, maxShift = fs->maxShift;
+  int tx = 0, ty = 0;
+  uint32_t minerror = or_sad_packed_thr(cur, prev, stride, bpp, f->x, f->y, f->size, ox, oy, UINT_MAX);
+  for (int i = -maxShift; i <= maxShift; i += stepSize)
+    for (int j = -maxShift; j <= maxShift; j += stepSize) {
+      if (i == 0 && j == 0) continue;
+      uint32_t e = or_sad_packed_thr(cur, prev, stride, bpp, f->x, f->y, f->size, i + ox, j + oy, minerror);
+      if (e < minerror) { minerror = e; tx = i; ty = j; }
+    }
+  if (stepSize > 1) {
+    int txc = tx, tyc = ty, r = stepSize - 1;
+    for (int i = txc - r; i <= txc + r; i++)
+      for (int j = tyc - r; j <= tyc + r; j++) {
+        if (i == txc && j == tyc) continue;
+        uint32_t e = or_sad_packed_thr(cur, prev, stride, bpp, f->x, f->y, f->size, i + ox, j + oy, minerror);
+        if (e < minerror) { minerror = e; tx = i; ty = j; }
+      }
+  }
+  *tx_out = tx;
+  *ty_out = ty;
+  *minerror_out = minerror;
+  if (abs(tx) >= maxShift + stepSize - 1 || abs(ty) >= maxShift + stepSize - 1) return 0;
+  return 1;
+}
+
 /* ---- statistics: src/transformtype.c:445-459, 513-534 ---------------------- */
 
 static int cmp_d(const void* a, const void* b) {
@@ -450,8 +547,11 @@ static int or_calc_trans_fields(OrState* s, const OrFields* fs, const VSTransfor
     }
     int tx, ty;
     uint32_t me;
-    if (!or_field_search(s->curr, s->prev, s->width, s->height, s->width, fs, f, ox, oy, &tx, &ty, &me))
-      continue;
+    int ok = s->bpp == 1
+                 ? or_field_search(s->curr, s->prev, s->width, s->height, s->width, fs, f, ox, oy, &tx, &ty, &me)
+                 : or_field_search_packed(s->curr, s->prev, s->width * s->bpp, s->bpp, fs, f, ox, oy, &tx, &ty,
+                                          &me);
+    if (!ok) continue;
     LocalMotion* lm = &slots[q].lm;
     memset(lm, 0, sizeof(*lm));
     lm->f = *f;
@@ -471,8 +571,8 @@ static int or_calc_trans_fields(OrState* s, const OrFields* fs, const VSTransfor
 }
 
 int or_detect(OrState* s, const uint8_t* luma, int stride, LocalMotion* out, int maxout) {
-  const int W = s->width, H = s->height;
-  or_boxblur(s->curr, luma, W, H, stride, s->blurSize);
+  const int W = s->width, H = s->height, bpp = s->bpp;
+  or_boxblur(s->curr, luma, W * bpp, H, stride, bpp == 1 ? s->blurSize : 1);
   int tripodLeadIn = s->virtualTripod >= 1 && s->frameNum < s->virtualTripod;
   int total = 0;
   s->nCoarse = s->nFineAll = s->nFineKept = 0;
@@ -482,12 +582,20 @@ int or_detect(OrState* s, const uint8_t* luma, int stride, LocalMotion* out, int
     for (int i = 0; i < s->coarse.fieldNum; i++) {
       int mn, mx;
       const Field* f = &s->coarse.fields[i];
+      if (bpp != 1) {
+        cc[i] = or_contrast_packed(s->curr, W * bpp, bpp, f->x, f->y, f->size);
+        continue;
+      }
       or_minmax(s->curr, W, f->x, f->y, f->size, &mn, &mx);
       cc[i] = or_contrast(mn, mx);
     }
     for (int i = 0; i < s->fine.fieldNum; i++) {
       int mn, mx;
       const Field* f = &s->fine.fields[i];
+      if (bpp != 1) {
+        cf[i] = or_contrast_packed(s->curr, W * bpp, bpp, f->x, f->y, f->size);
+        continue;
+      }
       or_minmax(s->curr, W, f->x, f->y, f->size, &mn, &mx);
       cf[i] = or_contrast(mn, mx);
     }
@@ -520,7 +628,7 @@ int or_detect(OrState* s, const uint8_t* luma, int stride, LocalMotion* out, int
     s->hasSeenOneFrame = 1;
   }
   if (s->virtualTripod < 1 || s->frameNum < s->virtualTripod)
-    memcpy(s->prev, s->curr, (size_t)W * H);
+    memcpy(s->prev, s->curr, (size_t)W * H * bpp);
   s->frameNum++;
   return total;
 }
@@ -531,6 +639,18 @@ OrState* or_new(int width, int height, int shakiness, int accuracy, int stepSize
   OrState* s = (OrState*)calloc(1, sizeof(OrState));
   if (!s) return NULL;
   if (or_init(s, width, height, shakiness, accuracy, stepSize, contrastThreshold, virtualTripod) != 0) {
+    or_free(s);
+    free(s);
+    return NULL;
+  }
+  return s;
+}
+
+OrState* or_new_packed(int width, int height, int bpp, int shakiness, int accuracy, int stepSize,
+                       double contrastThreshold, int virtualTripod) {
+  OrState* s = (OrState*)calloc(1, sizeof(OrState));
+  if (!s) return NULL;
+  if (or_init_packed(s, width, height, bpp, shakiness, accuracy, stepSize, contrastThreshold, virtualTripod) != 0) {
     or_free(s);
     free(s);
     return NULL;

@@ -34,10 +34,14 @@ typedef struct {
   int nCoarse, nFineAll, nFineKept;
   VSTransform lastT;
   double lastMeanMatch;
+  int bpp;                    /* 1: planar luma; 3 or 4: packed RGB24/BGR24/RGBA plane 0 */
 } OrState;
 
 int  or_init(OrState* s, int width, int height, int shakiness, int accuracy, int stepSize,
              double contrastThreshold, int virtualTripod);
+/* packed frames: bpp 3 or 4, curr/prev hold width*bpp bytes a row, unblurred */
+int  or_init_packed(OrState* s, int width, int height, int bpp, int shakiness, int accuracy, int stepSize,
+                    double contrastThreshold, int virtualTripod);
 void or_free(OrState* s);
 /* one vsMotionDetection step on a luma plane; writes up to maxout motions and
    returns their number (coarse ++ cleaned fine) */
@@ -57,12 +61,21 @@ int      or_select(const OrFields* fs, const double* contrast, int* idx, double*
 int      or_field_search(const uint8_t* cur, const uint8_t* prev, int w, int h, int stride,
                          const OrFields* fs, const Field* f, int ox, int oy,
                          int* tx, int* ty, uint32_t* minerror);
+/* packed building blocks (src/motiondetect.c:370-377, 591-660, 971-996) */
+double   or_contrast_packed(const uint8_t* img, int stride, int bpp, int x, int y, int size);
+uint32_t or_sad_packed(const uint8_t* cur, const uint8_t* prev, int stride, int bpp, int fx, int fy,
+                       int size, int dx, int dy);
+int      or_field_search_packed(const uint8_t* cur, const uint8_t* prev, int stride, int bpp,
+                                const OrFields* fs, const Field* f, int ox, int oy,
+                                int* tx, int* ty, uint32_t* minerror);
 double   or_cleanmean(double* ds, int len, double* mn, double* mx);
 VSTransform or_simple_transform(int w, int h, const LocalMotion* lms, int n);
 void     or_transform_point(const VSTransform* t, int w, int h, int x, int y, int* ox, int* oy);
 
 OrState*       or_new(int width, int height, int shakiness, int accuracy, int stepSize,
                       double contrastThreshold, int virtualTripod);
+OrState*       or_new_packed(int width, int height, int bpp, int shakiness, int accuracy, int stepSize,
+                             double contrastThreshold, int virtualTripod);
 void           or_delete(OrState* s);
 const uint8_t* or_blurred(const OrState* s);
 

@@ -39,6 +39,7 @@ typedef struct {
   int tripod;
   int stride_pad;
   double mincontrast;    /* <0: from the spec */
+  int pformat;           /* 0: PF_YUV420P; else a packed format (vsyn_pack of the luma) */
 } Case;
 
 static void run_case(FILE* out, const Case* cs, int first) {
@@ -57,7 +58,8 @@ static void run_case(FILE* out, const Case* cs, int first) {
   vsyn_open(&sp2, &clip, path);
 
   VSFrameInfo fi;
-  vsFrameInfoInit(&fi, sp.width, sp.height, PF_YUV420P);
+  vsFrameInfoInit(&fi, sp.width, sp.height, cs->pformat ? (VSPixelFormat)cs->pformat : PF_YUV420P);
+  const int bpp = cs->pformat ? fi.bytesPerPixel : 1;
   VSMotionDetect md;
   memset(&md, 0, sizeof(md));
   VSMotionDetectConfig conf = vsMotionDetectGetDefaultConfig("golden");
@@ -79,6 +81,9 @@ static void run_case(FILE* out, const Case* cs, int first) {
   md.serializationMode = BINARY_SERIALIZATION_MODE;
 
   int ls = sp.width + cs->stride_pad;
+  const int pls = sp.width * bpp + cs->stride_pad;   /* packed frames: plane 0 stride */
+  uint8_t* P = cs->pformat ? (uint8_t*)malloc((size_t)pls * sp.height) : NULL;
+  if (P) memset(P, 0x5A, (size_t)pls * sp.height);
   uint8_t* Y = (uint8_t*)malloc((size_t)ls * sp.height);
   uint8_t* U = (uint8_t*)malloc((size_t)(sp.width / 2) * (sp.height / 2));
   uint8_t* V = (uint8_t*)malloc((size_t)(sp.width / 2) * (sp.height / 2));
@@ -86,11 +91,12 @@ static void run_case(FILE* out, const Case* cs, int first) {
   memset(U, 128, (size_t)(sp.width / 2) * (sp.height / 2));
   memset(V, 128, (size_t)(sp.width / 2) * (sp.height / 2));
 
-  fprintf(out, "%s{\"name\":\"%s\",\"config\":%d,\"clip\":%d,\"width\":%d,\"height\":%d,"
+  fprintf(out, "%s{\"name\":\"%s\",\"pformat\":%d,\"bpp\":%d,\"config\":%d,\"clip\":%d,\"width\":%d,\"height\":%d,"
                "\"stride\":%d,\"shakiness\":%d,\"accuracy\":%d,\"stepsize\":%d,"
                "\"mincontrast\":\"%016" PRIx64 "\",\"tripod\":%d,\"md_conf\":[%d,%d,%d],"
                "\"coarse\":[%d,%d,%d,%d],\"fine\":[%d,%d,%d,%d],\"frames\":[",
-          first ? "" : ",\n", cs->name, cs->config, cs->clip, sp.width, sp.height, ls,
+          first ? "" : ",\n", cs->name, cs->pformat, bpp, cs->config, cs->clip, sp.width, sp.height,
+          cs->pformat ? pls : ls,
           sp.shakiness, sp.accuracy, sp.stepsize, dbits(sp.mincontrast), cs->tripod,
           md.conf.shakiness, md.conf.accuracy, md.conf.stepSize,
           md.fieldscoarse.fieldSize, md.fieldscoarse.maxShift, md.fieldscoarse.fieldNum,
@@ -100,13 +106,18 @@ static void run_case(FILE* out, const Case* cs, int first) {
     vsyn_render(&clip, path, t, Y, ls);
     VSFrame fr;
     memset(&fr, 0, sizeof(fr));
-    fr.data[0] = Y; fr.linesize[0] = ls;
-    fr.data[1] = U; fr.linesize[1] = sp.width / 2;
-    fr.data[2] = V; fr.linesize[2] = sp.width / 2;
+    if (P) {
+      vsyn_pack(Y, sp.width, sp.height, ls, bpp, P, pls);
+      fr.data[0] = P; fr.linesize[0] = pls;
+    } else {
+      fr.data[0] = Y; fr.linesize[0] = ls;
+      fr.data[1] = U; fr.linesize[1] = sp.width / 2;
+      fr.data[2] = V; fr.linesize[2] = sp.width / 2;
+    }
     LocalMotions lms;
     vsMotionDetection(&md, &lms, &fr);
-    uint64_t hin = vsyn_hash_plane(Y, sp.width, sp.height, ls);
-    uint64_t hbl = vsyn_hash_plane(md.curr.data[0], sp.width, sp.height, md.curr.linesize[0]);
+    uint64_t hin = P ? vsyn_hash_plane(P, sp.width * bpp, sp.height, pls) : vsyn_hash_plane(Y, sp.width, sp.height, ls);
+    uint64_t hbl = vsyn_hash_plane(md.curr.data[0], sp.width * bpp, sp.height, md.curr.linesize[0]);
     vsWriteToFile(&md, fb, &lms);
     md.serializationMode = ASCII_SERIALIZATION_MODE;
     vsWriteToFile(&md, fa, &lms);
@@ -135,14 +146,14 @@ static void run_case(FILE* out, const Case* cs, int first) {
   vsMotionDetectionCleanup(&md);
   vsyn_clip_free(&clip);
   free(path);
-  free(Y); free(U); free(V);
+  free(Y); free(U); free(V); free(P);
   fprintf(stderr, "case %s done (%d frames)\n", cs->name, nfr);
 }
 
 int main(int argc, char** argv) {
   const char* path = argc > 1 ? argv[1] : "detect_golden.json";
   vs_log_level = 0;
-  static const Case cases[] = {
+  static const Case planar[] = {
     /* name          cfg clip  w     h    nfr step acc shak tripod pad  contrast */
     { "c1_full",      1, 0,    0,    0,   60,  0,   0,  0,   0,    0,  -1 },
     { "c1_stride",    1, 1,    0,    0,   8,   0,   0,  0,   0,    40, -1 },
@@ -160,12 +171,28 @@ int main(int argc, char** argv) {
     { "c4_clip2",     4, 2,    0,    0,   4,   0,   0,  0,   0,    0,  -1 },
     { "c5_4kblur",    5, 0,    0,    0,   2,   0,   0,  0,   0,    0,  -1 },
   };
+  /* packed RGB24 / BGR24 / RGBA frames: vsyn_pack of the rendered luma */
+  static const Case packed[] = {
+    /* name            cfg clip  w     h    nfr step acc shak tripod pad  contrast  format */
+    { "p_rgb24_c1",      1, 20,   0,    0,   8,   0,   0,  0,   0,    0,  -1, PF_RGB24 },
+    { "p_bgr24_pad",     1, 21,   0,    0,   5,   0,   0,  0,   0,    40, -1, PF_BGR24 },
+    { "p_rgba_c1",       1, 22,   0,    0,   6,   0,   0,  0,   0,    0,  -1, PF_RGBA },
+    { "p_rgb24_tripod",  1, 23,   0,    0,   6,   0,   0,  0,   2,    0,  -1, PF_RGB24 },
+    { "p_rgb24_step4",   1, 24,   0,    0,   5,   4,   0,  0,   0,    0,  -1, PF_RGB24 },
+    { "p_rgba_step8acc6",1, 25,   0,    0,   5,   8,   6,  0,   0,    0,  -1, PF_RGBA },
+    { "p_rgb24_step1",   1, 26,   0,    0,   3,   1,   0,  0,   0,    0,  -1, PF_RGB24 },
+    { "p_rgba_odd",      1, 27,   502,  298, 5,   5,   0,  0,   0,    12, -1, PF_RGBA },
+    { "p_rgb24_1080p",   2, 20,   0,    0,   3,   0,   0,  0,   0,    0,  -1, PF_RGB24 },
+  };
+  const int usePacked = argc > 2 && strcmp(argv[2], "packed") == 0;
+  const Case* cases = usePacked ? packed : planar;
+  const size_t ncases = usePacked ? sizeof(packed) / sizeof(packed[0]) : sizeof(planar) / sizeof(planar[0]);
   FILE* out = fopen(path, "w");
   if (!out) return 1;
   fprintf(out, "{\"source\":\"reference vsMotionDetection (oracle/_ref/libvsref.so)\",\"cases\":[\n");
-  int only = argc > 2 ? atoi(argv[2]) : -1;
+  int only = argc > 3 ? atoi(argv[3]) : -1;
   int first = 1;
-  for (size_t i = 0; i < sizeof(cases) / sizeof(cases[0]); i++) {
+  for (size_t i = 0; i < ncases; i++) {
     if (only >= 0 && (int)i != only) continue;
     run_case(out, &cases[i], first);
     first = 0;

@@ -2,13 +2,15 @@
  * vsg_api.hip -- the thin C ABI over the detect kernels (declared in vsg.h):
  * device buffers, the per-set HIP streams of one detector, uploads and read-backs.
  *
- * HBM layout of one context (pitch = width rounded up to 64 bytes; VSG_NSETS
- * sets of B = maxBatch slots, global slot = set*B + i):
- *   raw[3B]       uploaded luma planes, pitch x H (host-frame batches only)
- *   blur[3B], ref blurred planes, pitch x (H + padRows); the pad rows keep the
- *                 grid kernel's idle-candidate reads inside the buffer
+ * HBM layout of one context (pitch = width * bytes per pixel rounded up to
+ * 64 bytes; VSG_NSETS sets of B = maxBatch slots, global slot = set*B + i):
+ *   raw[3B]       uploaded planes, pitch x H (host-frame batches only): the
+ *                 luma, or plane 0 of a packed RGB24 / BGR24 / RGBA frame
+ *   blur[3B], ref blurred luma (packed: an unblurred copy), pitch x
+ *                 (H + padRows); the pad rows keep the grid kernel's
+ *                 idle-candidate reads inside the buffer
  *   patches[3B]   an aligned, dense copy of every coarse and fine field
- *   minmax[3B]    8-bit (min, max) per field
+ *   minmax[3B]    8-bit (min, max) per field (packed: per colour channel)
  *   per set       selection lists, coarse/fine job lists, best[] keys, results
  *
  * Each set has its own stream, so one batch's search can fill the tail of the
@@ -139,8 +141,10 @@ static hipError_t halloc(T** p, size_t n) {
   return hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1), hipHostMallocDefault);
 }
 
-int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize, const VSGStageDesc* coarse,
+int vsg_ctx_init(VSGContext* c, int device, int width, int height, int bpp, int blurSize, const VSGStageDesc* coarse,
                  const VSGStageDesc* fine, int B, const hipStream_t* shared) {
+  if (bpp != 1 && bpp != 3 && bpp != 4) return vsg_fail("vsg_create: bytes per pixel", hipSuccess);
+  if (bpp != 1 && blurSize >= 2) return vsg_fail("vsg_create: packed frames are not blurred", hipSuccess);
   c->device = device;
   VSG_CHECK(hipSetDevice(device));
   c->ownStreams = shared == nullptr;
@@ -152,7 +156,10 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
   }
   c->W = width;
   c->H = height;
-  c->pitch = (width + 63) & ~63;
+  c->bpp = bpp;
+  c->Wb = width * bpp;
+  c->nmm = bpp == 1 ? 2 : 6;
+  c->pitch = (c->Wb + 63) & ~63;
   c->blurSize = blurSize;
   c->B = B;
   const VSGStageDesc* sd[2] = {coarse, fine};
@@ -173,12 +180,17 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
         S.rows < 1 || S.rows + 1 > 256)
       return vsg_fail("vsg_create: field grid", hipSuccess);
     maxStep = S.step > maxStep ? S.step : maxStep;
-    S.tiles = vsg_plan_model(&S.g, S.S, S.step, S.maxShift);
-    S.candsPerJob = (2.0 * S.g.n + 1.0) * (2.0 * S.g.n + 1.0);
+    S.tiles = vsg_plan_model(&S.g, S.S, S.step, S.maxShift, bpp);
+    S.candsPerJob = (double)S.g.G * S.g.G;
     int rc = 0;
-    for (int st = S.step; st > 1; st /= 2) {
-      const int ns = st / 2, r = st - ns, ni = (2 * r) / ns + 1;
-      rc += ni * ni - ((r % ns) == 0 ? 1 : 0);
+    if (bpp == 1) {
+      for (int st = S.step; st > 1; st /= 2) {
+        const int ns = st / 2, r = st - ns, ni = (2 * r) / ns + 1;
+        rc += ni * ni - ((r % ns) == 0 ? 1 : 0);
+      }
+    } else {
+      /* the unshifted position, then (2r+1)^2 - 1 shifts, r = step - 1 */
+      rc = 1 + (2 * S.step - 1) * (2 * S.step - 1) - 1;
     }
     S.refinePerJob = rc;
     S.patchOff = patchBytes;
@@ -191,9 +203,9 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
       ax.push_back(f.x);
       ay.push_back(f.y);
       asz.push_back(f.size);
-      poff.push_back((uint32_t)(patchBytes + (size_t)i * S.S * S.S));
+      poff.push_back((uint32_t)(patchBytes + (size_t)i * S.S * S.S * bpp));
     }
-    patchBytes += (size_t)S.nf * S.S * S.S;
+    patchBytes += (size_t)S.nf * S.S * S.S * bpp;
     VSG_CHECK(dalloc(&S.d_fx, S.nf));
     VSG_CHECK(dalloc(&S.d_fy, S.nf));
     VSG_CHECK(hipMemcpy(S.d_fx, fx.data(), sizeof(int16_t) * S.nf, hipMemcpyHostToDevice));
@@ -234,7 +246,7 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
   }
   c->patchSlotStride = (patchBytes + 255) & ~(size_t)255;
   VSG_CHECK(dalloc(&c->d_patches, (size_t)slots * c->patchSlotStride));
-  VSG_CHECK(dalloc(&c->d_minmax, (size_t)slots * c->nfTotal * 2));
+  VSG_CHECK(dalloc(&c->d_minmax, (size_t)slots * c->nfTotal * c->nmm));
   const int nc = c->st[0].nf, nf = c->st[1].nf, nmax = nc > nf ? nc : nf;
   for (VSGSet& s : c->set) {
     VSG_CHECK(dalloc(&s.d_selIdx, (size_t)B * c->nfTotal));
@@ -273,14 +285,14 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize,
   return vsg_ctx_tune(c);
 }
 
-extern "C" int vsg_create(VSGContext** out, int device, int width, int height, int blurSize,
-                          const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch) {
+extern "C" int vsg_create_fmt(VSGContext** out, int device, int width, int height, int bytesPerPixel,
+                              int blurSize, const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch) {
   if (!out || !coarse || !fine || width <= 0 || height <= 0 || maxBatch <= 0)
     return vsg_fail("vsg_create", hipSuccess);
   *out = nullptr;
   VSGContext* c = new (std::nothrow) VSGContext();
   if (!c) return vsg_fail("vsg_create: out of host memory", hipSuccess);
-  if (vsg_ctx_init(c, device, width, height, blurSize, coarse, fine, maxBatch, nullptr) != 0) {
+  if (vsg_ctx_init(c, device, width, height, bytesPerPixel, blurSize, coarse, fine, maxBatch, nullptr) != 0) {
     char keep[sizeof(g_err)];
     memcpy(keep, g_err, sizeof(keep));
     free_ctx(c);
@@ -289,6 +301,11 @@ extern "C" int vsg_create(VSGContext** out, int device, int width, int height, i
   }
   *out = c;
   return 0;
+}
+
+extern "C" int vsg_create(VSGContext** out, int device, int width, int height, int blurSize,
+                          const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch) {
+  return vsg_create_fmt(out, device, width, height, 1, blurSize, coarse, fine, maxBatch);
 }
 
 /* ---- batches ------------------------------------------------------------------ */
@@ -329,8 +346,10 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n) {
   VSG_CHECK(hipMemcpyAsync(s.d_prev, s.h_prev, sizeof(int) * n, hipMemcpyHostToDevice, stream));
   VSG_CHECK(hipMemcpyAsync(s.d_detect, s.h_detect, n, hipMemcpyHostToDevice, stream));
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR0], stream));
+  /* packed frames are not blurred (src/motiondetect.c:197-201): their blurSize
+     is below 2, which copies Wb = W*bpp bytes a row */
   VSG_CHECK(vsg_launch_blur(s.d_src, n, c->d_blur + (size_t)slot0 * c->slotStride, c->slotStride, c->pitch,
-                            c->W, c->H, c->blurSize, c->d_tmp, stream));
+                            c->Wb, c->H, c->blurSize, c->d_tmp, stream));
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR1], stream));
   VSGFieldArgs fa;
   fa.blurred = c->d_blur + (size_t)slot0 * c->slotStride;
@@ -343,13 +362,16 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n) {
   fa.nfields = c->nfTotal;
   fa.patches = c->d_patches + (size_t)slot0 * c->patchSlotStride;
   fa.patchSlotStride = c->patchSlotStride;
-  fa.minmax = c->d_minmax + (size_t)slot0 * c->nfTotal * 2;
+  fa.minmax = c->d_minmax + (size_t)slot0 * c->nfTotal * c->nmm;
   fa.src = s.d_src;
+  fa.bpp = c->bpp;
+  fa.nmm = c->nmm;
   VSG_CHECK(vsg_launch_fields(fa, n, stream));
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_FIELDS1], stream));
   VSGSelectArgs sa;
   sa.minmax = c->d_minmax;
   sa.nfTotal = c->nfTotal;
+  sa.nmm = c->nmm;
   sa.ctab = c->d_ctab;
   for (int k = 0; k < 2; ++k) {
     sa.st[k].nf = c->st[k].nf;
@@ -399,7 +421,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   VSG_CHECK(hipEventSynchronize(s.done));
   VSG_CHECK(hipEventSynchronize(s.doneF));
   for (int i = 0; i < n; ++i) {
-    if (linesize[i] < c->W) return vsg_fail("vsg_submit: linesize < width", hipSuccess);
+    if (linesize[i] < c->Wb) return vsg_fail("vsg_submit: linesize < width * bytes per pixel", hipSuccess);
     if (prevSlot[i] >= VSG_NSETS * c->B) return vsg_fail("vsg_submit: prevSlot", hipSuccess);
   }
   /* the set's planes are overwritten below; the batch that followed the set's
@@ -417,9 +439,9 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
       if (!s.h_stage) VSG_CHECK(halloc(&s.h_stage, plane * c->B));
       uint8_t* stg = s.h_stage + plane * i;
       if (linesize[i] == c->pitch) {
-        memcpy(stg, luma[i], plane - (c->pitch - c->W));
+        memcpy(stg, luma[i], plane - (c->pitch - c->Wb));
       } else {
-        for (int y = 0; y < c->H; ++y) memcpy(stg + (size_t)y * c->pitch, luma[i] + (size_t)y * linesize[i], c->W);
+        for (int y = 0; y < c->H; ++y) memcpy(stg + (size_t)y * c->pitch, luma[i] + (size_t)y * linesize[i], c->Wb);
       }
       uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * plane;
       VSG_CHECK(hipMemcpyAsync(dst, stg, plane, hipMemcpyHostToDevice, stream));
@@ -472,13 +494,13 @@ extern "C" int vsg_wait_coarse(VSGContext* c, int set, VSGBatchView* view) {
   if (s.prof) {
     s.prof = false;
     double fieldBytes = 0;
-    for (const VSGStage& S : c->st) fieldBytes += (double)S.nf * S.S * S.S;
+    for (const VSGStage& S : c->st) fieldBytes += (double)S.nf * S.S * S.S * c->bpp;
     int ncoarse = 0;
     for (int i = 0; i < s.n; ++i) ncoarse += s.h_count[2 * i];
-    prof_add(VSG_K_BLUR, s.ev[VSG_EV_BLUR0], s.ev[VSG_EV_BLUR1], 2.0 * c->W * c->H * s.n);
+    prof_add(VSG_K_BLUR, s.ev[VSG_EV_BLUR0], s.ev[VSG_EV_BLUR1], 2.0 * c->Wb * c->H * s.n);
     prof_add(VSG_K_FIELDS, s.ev[VSG_EV_BLUR1], s.ev[VSG_EV_FIELDS1], 2.0 * fieldBytes * s.n);
     prof_add(VSG_K_SELECT, s.ev[VSG_EV_FIELDS1], s.ev[VSG_EV_SELECT1], (double)c->nfTotal * s.n);
-    const double px = (double)c->st[0].S * c->st[0].S;
+    const double px = (double)c->st[0].S * c->st[0].S * c->bpp;
     prof_add(VSG_K_GRID_COARSE, s.ev[VSG_EV_GRID0], s.ev[VSG_EV_GRID1],
              ncoarse * (c->st[0].candsPerJob + c->st[0].refinePerJob) * px);
   }
@@ -529,7 +551,7 @@ extern "C" int vsg_wait_fine(VSGContext* c, int set, const VSGResult** results) 
   if (s.profF) {
     s.profF = false;
     const VSGStage& S = c->st[1];
-    const double px = (double)S.S * S.S;
+    const double px = (double)S.S * S.S * c->bpp;
     prof_add(VSG_K_GRID_FINE, s.evf[VSG_EVF_0], s.evf[VSG_EVF_GRID1],
              s.njobsF * (S.candsPerJob + S.refinePerJob) * px);
   }
@@ -538,10 +560,10 @@ extern "C" int vsg_wait_fine(VSGContext* c, int set, const VSGResult** results) 
 }
 
 extern "C" int vsg_read_blurred(VSGContext* c, int slot, uint8_t* dst, int dstStride) {
-  if (!c || slot >= VSG_NSETS * c->B || !dst || dstStride < c->W) return vsg_fail("vsg_read_blurred", hipSuccess);
+  if (!c || slot >= VSG_NSETS * c->B || !dst || dstStride < c->Wb) return vsg_fail("vsg_read_blurred", hipSuccess);
   VSG_CHECK(hipSetDevice(c->device));
   const uint8_t* src = slot < 0 ? c->d_ref : c->d_blur + (size_t)slot * c->slotStride;
   if (vsg_ctx_sync(c) != 0) return -1;
-  VSG_CHECK(hipMemcpy2D(dst, dstStride, src, c->pitch, c->W, c->H, hipMemcpyDeviceToHost));
+  VSG_CHECK(hipMemcpy2D(dst, dstStride, src, c->pitch, c->Wb, c->H, hipMemcpyDeviceToHost));
   return 0;
 }

@@ -51,15 +51,24 @@ VSG_HD uint32_t vsg_div_magic(int size) {
   return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)size - 1) / (uint64_t)size);
 }
 
-/* grid geometry of one search stage */
+/* grid geometry of one search stage.  Candidate (a, b), a and b in
+   [alo, alo + G), is the shift (org + a*step, org + b*step) px.
+   Planar frames: the spiral's square, alo = -n, org = 0, ranked by vsg_rank.
+   Packed frames (bpp 3 or 4): the raster i = -maxShift, -maxShift + step, ...
+   <= maxShift (src/motiondetect.c:618-619), alo = 0, org = -maxShift, ranked
+   1 + a*G + b (x outer, y inner); rank 0 is the unshifted position, which the
+   reference evaluates first (:613). */
 typedef struct {
   int S;        /* field edge, multiple of 16 */
   int step;     /* grid step in px */
-  int n;        /* grid half width: maxShift / step */
+  int n;        /* planar: grid half width, maxShift / step */
+  int alo, G;   /* first grid index and grid points per axis */
+  int org;      /* shift of grid index 0 in px */
+  int bpp;      /* bytes per pixel: 1 planar luma, 3 or 4 packed */
   int m;        /* grid columns between a lane's horizontal candidates */
   int U, V;     /* lanes per job along x and y */
   int J, K;     /* candidates per lane along x and y */
-  int D;        /* words between a lane's horizontal candidates (m*step/4) */
+  int D;        /* words between a lane's horizontal candidates (m*step*bpp/4) */
   int CW;       /* curr words per column chunk */
   int wg;       /* threads per workgroup of the grid kernel (multiple of 64) */
   int lds;      /* 1: the field patch is staged in LDS (VGPR operand), 0: scalar loads (SGPR operand) */

@@ -69,6 +69,9 @@ struct VSGContext {
   bool ownStreams = true;
   int lastSet = -1;       /* set of the latest batch */
   int W = 0, H = 0, pitch = 0, rowsAlloc = 0;
+  int bpp = 1;            /* bytes per pixel: 1 planar luma, 3 or 4 packed */
+  int Wb = 0;             /* bytes per row of a plane: W * bpp */
+  int nmm = 2;            /* min/max bytes per field (6: three packed channels) */
   size_t slotStride = 0;
   int blurSize = 0;
   int B = 0;
@@ -92,12 +95,12 @@ struct VSGContext {
 
 /* creation / destruction shared by both ABIs (vsg_api.hip); a null `shared`
    creates one private stream per set, else set k runs on shared[k] */
-int vsg_ctx_init(VSGContext* c, int device, int width, int height, int blurSize, const VSGStageDesc* coarse,
+int vsg_ctx_init(VSGContext* c, int device, int width, int height, int bpp, int blurSize, const VSGStageDesc* coarse,
                  const VSGStageDesc* fine, int maxBatch, const hipStream_t* shared);
 /* all streams of the context idle */
 int vsg_ctx_sync(VSGContext* c);
 /* plan with the lowest modelled cost; returns workgroups per job */
-int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift);
+int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift, int bpp);
 /* picks each stage's plan (vsg_tune.hip) */
 int vsg_ctx_tune(VSGContext* c);
 /* the learned form (vsg_learned.hip): quarter-resolution planes in the

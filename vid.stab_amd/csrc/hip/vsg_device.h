@@ -17,35 +17,43 @@ typedef const __attribute__((address_space(1))) uint32_t* vsg_gword_ptr;   /* gl
 struct VSGTile {
   const uint8_t* prev;     /* previous blurred plane */
   int pitch;               /* bytes per row of prev (multiple of 64) */
-  vsg_cword_ptr patch;     /* the current field, S x S, 4-byte aligned */
+  vsg_cword_ptr patch;     /* the current field, S rows of S*bpp bytes, 4-byte aligned */
   const uint32_t* lpatch;  /* the same field staged in LDS (plans with an LDS patch) */
-  int S, s, n, m, U, V;    /* field size, grid step, half width, lane layout */
+  int S, s, m, U, V;       /* field size, grid step, lane layout */
+  int alo, G, org, bpp;    /* grid indices [alo, alo+G), shift org + index*s, bytes per pixel */
   int fxc, fyc;            /* field centre */
   int ox, oy;              /* search offset */
 };
 
+/* rank of grid point (a, b) in the reference's visit order */
+__device__ __forceinline__ int vsg_grid_rank(const VSGTile& t, int a, int b) {
+  return t.bpp == 1 ? vsg_rank(a, b) : 1 + a * t.G + b;
+}
+
 /* ---- SAD grid search, one lane ------------------------------------------------
    Lane lid = (u, v) of a job owns the candidates a = a0 + m*j (j < J) and
-   b = b0 + k (k < K) of the (2n+1)^2 grid, a0 = -n + (u % m) + m*J*(u / m),
-   b0 = -n + K*v.  m*step is a multiple of 4, so the J candidates of a lane
-   sit D = m*step/4 words apart in the same prev row: one load of a prev row
-   segment feeds J*K candidates, each prev word is shifted into the field's
-   alignment once (v_alignbyte), and the current field's words come in as
-   wave-uniform scalar loads (SGPR operand of v_sad_u8).  Iterating over prev
-   rows p, candidate k pairs prev row p with field row p - k*step.
-   Returns the lane's best (sad << 32 | spiral rank), or ~0 for idle lanes. */
+   b = b0 + k (k < K) of the G x G grid, a0 = alo + (u % m) + m*J*(u / m),
+   b0 = alo + K*v.  m*step*bpp is a multiple of 4, so the J candidates of a
+   lane sit D = m*step*bpp/4 words apart in the same prev row: one load of a
+   prev row segment feeds J*K candidates, each prev word is shifted into the
+   field's alignment once (v_alignbyte), and the current field's words come in
+   as wave-uniform scalar loads (SGPR operand of v_sad_u8).  Iterating over
+   prev rows p, candidate k pairs prev row p with field row p - k*step.  A
+   packed pixel is bpp consecutive bytes; the SAD runs over all of them
+   (src/motiondetect.c:984-990).
+   Returns the lane's best (sad << 32 | rank), or ~0 for idle lanes. */
 template <int J, int K, int D, int CW, bool LDSP>
 __device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int lid) {
-  const int S = t.S, s = t.s, n = t.n, m = t.m;
+  const int S = t.S, s = t.s, m = t.m, ahi = t.alo + t.G - 1;
   const int u = lid % t.U;
   int v = lid / t.U;
   const bool laneOk = v < t.V;
   if (!laneOk) v = t.V - 1;
-  const int a0 = -n + (u % m) + m * J * (u / m);
-  const int b0 = -n + K * v;
-  const int prow0 = t.fyc - S / 2 + b0 * s + t.oy;
-  const int pcol0 = t.fxc - S / 2 + a0 * s + t.ox;
-  const int wordsPerRow = S >> 2;
+  const int a0 = t.alo + (u % m) + m * J * (u / m);
+  const int b0 = t.alo + K * v;
+  const int prow0 = t.fyc - S / 2 + t.org + b0 * s + t.oy;
+  const int pcol0 = (t.fxc - S / 2 + t.org + a0 * s + t.ox) * t.bpp;
+  const int wordsPerRow = (S * t.bpp) >> 2;
   const int pitchWords = t.pitch >> 2;
 
   uint32_t acc[K][J];
@@ -94,8 +102,8 @@ __device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       const int ga = a0 + m * j, gb = b0 + k;
-      if (laneOk && ga <= n && gb <= n) {
-        const unsigned long long key = ((unsigned long long)acc[k][j] << 32) | (uint32_t)vsg_rank(ga, gb);
+      if (laneOk && ga <= ahi && gb <= ahi) {
+        const unsigned long long key = ((unsigned long long)acc[k][j] << 32) | (uint32_t)vsg_grid_rank(t, ga, gb);
         best = key < best ? key : best;
       }
     }

@@ -24,7 +24,10 @@ struct VSGBlurSrc {
 hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t slotStride,
                            int dstPitch, int W, int H, int size, uint8_t* tmp, hipStream_t st);
 
-/* per-field min/max plus the aligned copy of the field ("patch") */
+/* per-field min/max plus the aligned copy of the field ("patch").  Planar
+   frames: (min, max) of the luma; packed frames: (min, max) of each of the
+   three colour channels, the fourth byte of a 4-byte pixel left out
+   (src/motiondetect.c:370-377) */
 struct VSGFieldArgs {
   const uint8_t* blurred;
   size_t slotStride;
@@ -36,8 +39,10 @@ struct VSGFieldArgs {
   int nfields;
   uint8_t* patches;
   size_t patchSlotStride;
-  uint8_t* minmax;            /* [slot][nfields][2] */
+  uint8_t* minmax;            /* [slot][nfields][nmm] */
   const VSGBlurSrc* src;      /* per slot of the launch; p == null: idle slot (skipped) */
+  int bpp;                    /* bytes per pixel */
+  int nmm;                    /* min/max bytes per field: 2 (planar) or 6 (packed) */
 };
 hipError_t vsg_launch_fields(const VSGFieldArgs& a, int nslots, hipStream_t st);
 
@@ -52,7 +57,7 @@ struct VSGSearchArgs {
   const uint8_t* ref;
   int pitch;
   const uint8_t* patches;       /* patch of (slot, field): patches + slot*patchSlotStride
-                                   + stageOff + field*S*S */
+                                   + stageOff + field*S*S*bpp */
   size_t patchSlotStride;
   size_t stageOff;
   VSGGrid g;
@@ -72,8 +77,9 @@ struct VSGSelectStage {
   double ct;         /* contrast threshold */
 };
 struct VSGSelectArgs {
-  const uint8_t* minmax;     /* [slot][nfTotal][2] */
+  const uint8_t* minmax;     /* [slot][nfTotal][nmm] */
   int nfTotal;
+  int nmm;                   /* 2: one (min, max) per field; 6: one per colour channel */
   const double* ctab;        /* contrast of (min, max): ctab[min*256 + max] */
   VSGSelectStage st[2];
   int slot0;                 /* first slot of the batch */
@@ -86,9 +92,12 @@ struct VSGSelectArgs {
 };
 hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st);
 
-/* fills J, D, CW, m, U, V, wg for a stage with K candidate rows per lane
-   (4, 6 or 8); returns the workgroups per job */
-int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K);
+/* fills the grid and J, D, CW, m, U, V, wg for a stage with K candidate rows
+   per lane (4, 6 or 8) on frames of bpp bytes per pixel (1: planar luma);
+   returns the workgroups per job */
+int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K, int bpp);
+/* bytes of one field's patch: S rows of S*bpp bytes */
+static inline size_t vsg_patch_bytes(const VSGGrid& g) { return (size_t)g.S * g.S * g.bpp; }
 /* modelled instruction count of one job under a plan (for ranking plans) */
 double vsg_plan_cost(const VSGGrid& g, int tiles);
 /* grid search (argmin into best[]) fused with the refinement, which the

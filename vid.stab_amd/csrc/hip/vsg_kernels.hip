@@ -150,6 +150,57 @@ __device__ __forceinline__ uint32_t load_u32_unaligned(const uint8_t* p) {
   return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(a & 3));
 }
 
+/* packed pixels: min/max of each colour channel.  Byte b of field-row word w
+   is byte 4w + b of a row that starts on a pixel, channel (4w + b) % bpp */
+__device__ void fields_packed(const VSGFieldArgs& a, int f, int slot, int lane) {
+  const int S = a.fsz[f], bpp = a.bpp;
+  const int words = (S * bpp) >> 2;
+  const int left = (a.fx[f] - S / 2) * bpp, top = a.fy[f] - S / 2;
+  const uint8_t* img = a.blurred + (size_t)slot * a.slotStride;
+  uint32_t* patch = reinterpret_cast<uint32_t*>(a.patches + (size_t)slot * a.patchSlotStride + a.patchOff[f]);
+  uint32_t lo0 = 255, lo1 = 255, lo2 = 255, hi0 = 0, hi1 = 0, hi2 = 0;
+  for (int i = lane; i < S * words; i += 64) {
+    const int y = i / words, w = i - y * words;
+    const uint32_t v = load_u32_unaligned(img + (size_t)(top + y) * a.pitch + left + 4 * w);
+    patch[i] = v;
+    const int c0 = (4 * w) % bpp;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const uint32_t x = (v >> (8 * b)) & 0xff;
+      int ch = c0 + b;
+      ch = ch >= bpp ? ch - bpp : ch;
+      ch = ch >= bpp ? ch - bpp : ch;
+      if (ch == 0) {
+        lo0 = min(lo0, x);
+        hi0 = max(hi0, x);
+      } else if (ch == 1) {
+        lo1 = min(lo1, x);
+        hi1 = max(hi1, x);
+      } else if (ch == 2) {
+        lo2 = min(lo2, x);
+        hi2 = max(hi2, x);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo0 = min(lo0, (uint32_t)__shfl_xor((int)lo0, off));
+    hi0 = max(hi0, (uint32_t)__shfl_xor((int)hi0, off));
+    lo1 = min(lo1, (uint32_t)__shfl_xor((int)lo1, off));
+    hi1 = max(hi1, (uint32_t)__shfl_xor((int)hi1, off));
+    lo2 = min(lo2, (uint32_t)__shfl_xor((int)lo2, off));
+    hi2 = max(hi2, (uint32_t)__shfl_xor((int)hi2, off));
+  }
+  if (lane == 0) {
+    uint8_t* mm = a.minmax + ((size_t)slot * a.nfields + f) * 6;
+    mm[0] = (uint8_t)lo0;
+    mm[1] = (uint8_t)hi0;
+    mm[2] = (uint8_t)lo1;
+    mm[3] = (uint8_t)hi1;
+    mm[4] = (uint8_t)lo2;
+    mm[5] = (uint8_t)hi2;
+  }
+}
+
 /* one wave per (field, slot) */
 __global__ __launch_bounds__(256) void k_fields(VSGFieldArgs a) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -157,6 +208,10 @@ __global__ __launch_bounds__(256) void k_fields(VSGFieldArgs a) {
   if (f >= a.nfields) return;
   const int slot = blockIdx.y;
   if (a.src && !a.src[slot].p) return;   /* idle frame slot */
+  if (a.bpp != 1) {
+    fields_packed(a, f, slot, lane);
+    return;
+  }
   const int S = a.fsz[f];
   const int words = S >> 2;
   const int left = a.fx[f] - S / 2, top = a.fy[f] - S / 2;
@@ -192,7 +247,10 @@ __global__ __launch_bounds__(256) void k_fields(VSGFieldArgs a) {
    whole array with the picks zeroed keeps equal contrasts in index order
    again, so the same rule applies.  Contrast doubles come from a host-built
    table of (max-min)/(max+min+0.1) over all 65536 (min, max) pairs, so no
-   device double division is involved. */
+   device double division is involved.  A packed field's contrast is the mean
+   of its three channels' (src/motiondetect.c:370-377), which the reference's
+   -ffast-math build evaluates as ((c0 + c1) + c2) * (1/3); the same two
+   IEEE additions and one multiplication by the rounded 1/3 run here. */
 constexpr int kSelectThreads = 512;
 
 __global__ __launch_bounds__(kSelectThreads) void k_select(VSGSelectArgs a) {
@@ -215,9 +273,15 @@ __global__ __launch_bounds__(kSelectThreads) void k_select(VSGSelectArgs a) {
   }
   double* c = reinterpret_cast<double*>(lds);
   int* pick = reinterpret_cast<int*>(c + n);
-  const uint8_t* mm = a.minmax + ((size_t)(a.slot0 + f) * a.nfTotal + st.mmOff) * 2;
+  const int nmm = a.nmm;
+  const uint8_t* mm = a.minmax + ((size_t)(a.slot0 + f) * a.nfTotal + st.mmOff) * nmm;
   for (int i = threadIdx.x; i < n; i += kSelectThreads) {
-    const double v = a.ctab[mm[2 * i] * 256 + mm[2 * i + 1]];
+    const uint8_t* q = mm + (size_t)nmm * i;
+    double v = a.ctab[q[0] * 256 + q[1]];
+    if (nmm == 6) {
+      const double sum = (v + a.ctab[q[2] * 256 + q[3]]) + a.ctab[q[4] * 256 + q[5]];
+      v = __dmul_rn(sum, 1.0 / 3.0);
+    }
     c[i] = v < st.ct ? 0.0 : v;
     pick[i] = 0;
   }

@@ -127,6 +127,7 @@ void vsg_learned_free(VSGContext* c) {
 extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, const float* weights, float threshold) {
   if (!c || nlayers < 1 || nlayers > kMaxLayers || !dil || !weights || !(threshold > 0.0f))
     return vsg_fail("vsg_set_proposal", hipSuccess);
+  if (c->bpp != 1) return vsg_fail("vsg_set_proposal: the learned form takes planar frames", hipSuccess);
   if (c->W < 64 || c->H < 64) return vsg_fail("vsg_set_proposal: frame too small", hipSuccess);
   LCHECK(hipSetDevice(c->device));
   if (vsg_ctx_sync(c) != 0) return -1;

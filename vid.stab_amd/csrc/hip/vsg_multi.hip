@@ -206,7 +206,7 @@ extern "C" int vsg_multi_create(VSGMulti** out, int device, int nclips, const in
       break;
     }
     m->ctx.push_back(x);
-    rc = vsg_ctx_init(x, device, width[c], height[c], blurSize[c], &coarse[c], &fine[c], maxBatch, shared);
+    rc = vsg_ctx_init(x, device, width[c], height[c], 1, blurSize[c], &coarse[c], &fine[c], maxBatch, shared);
   }
   /* group clips by plan, per stage */
   for (int k = 0; k < 2 && rc == 0; ++k)

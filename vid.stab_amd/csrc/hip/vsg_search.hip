@@ -66,37 +66,40 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-/* L.sums[ti*5 + tj] = SAD of candidate (tx - r + ns*ti, ty - r + ns*tj) for
-   ti, tj < ni, summed by the whole workgroup; ends with a barrier */
-__device__ void level_sums(const VSGSearchArgs& a, const VSGJob& jb, int tx, int ty, int ns, int r, int ni,
+/* L.sums[ti*5 + tj] = SAD of candidate (x0 + ns*ti, y0 + ns*tj) for ti < nx,
+   tj < ny (<= 5), summed by the whole workgroup; ends with a barrier.  A
+   field row is S*bpp bytes; the workgroup has at least that many words'
+   threads (the plan guarantees nthr >= S*bpp/4). */
+__device__ void level_sums(const VSGSearchArgs& a, const VSGJob& jb, int x0, int y0, int ns, int nx, int ny,
                            RefineLds& L) {
   const int nthr = blockDim.x, nwaves = nthr >> 6;
-  const int S = a.g.S, W4 = S >> 2;
-  const int rowsPerPass = nthr / W4;          /* S <= 256, nthr >= 64 */
+  const int S = a.g.S, bpp = a.g.bpp, W4 = (S * bpp) >> 2;
+  const int rowsPerPass = nthr / W4;
   const int w = threadIdx.x % W4, r0 = threadIdx.x / W4;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int fxc = a.fx[jb.field], fyc = a.fy[jb.field];
   const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
   const uint32_t* patch = reinterpret_cast<const uint32_t*>(a.patches + (size_t)jb.slot * a.patchSlotStride +
-                                                            a.stageOff + (size_t)jb.field * S * S);
+                                                            a.stageOff + (size_t)jb.field * S * S * bpp);
+  const int nsb = ns * bpp;
   uint32_t acc[kNc];
 #pragma unroll
   for (int c = 0; c < kNc; ++c) acc[c] = 0;
   if (r0 < rowsPerPass) {
-    const uint8_t* colBase = prev + (fxc - S / 2 + tx - r + jb.ox) + 4 * w;
-    const int rowBase = fyc - S / 2 + ty - r + jb.oy;
+    const uint8_t* colBase = prev + (fxc - S / 2 + x0 + jb.ox) * bpp + 4 * w;
+    const int rowBase = fyc - S / 2 + y0 + jb.oy;
 #pragma unroll 2
     for (int y = r0; y < S; y += rowsPerPass) {
       const uint32_t c = patch[y * W4 + w];
 #pragma unroll
       for (int tj = 0; tj < kMaxNi; ++tj) {
-        if (tj < ni) {
+        if (tj < ny) {
           const uint8_t* p = colBase + (size_t)(rowBase + y + ns * tj) * a.pitch;
 #pragma unroll
           for (int ti = 0; ti < kMaxNi; ++ti)
-            if (ti < ni)
+            if (ti < nx)
               acc[ti * kMaxNi + tj] =
-                  __builtin_amdgcn_sad_u8(*reinterpret_cast<const u32_any*>(p + ns * ti), c, acc[ti * kMaxNi + tj]);
+                  __builtin_amdgcn_sad_u8(*reinterpret_cast<const u32_any*>(p + nsb * ti), c, acc[ti * kMaxNi + tj]);
         }
       }
     }
@@ -115,8 +118,63 @@ __device__ void level_sums(const VSGSearchArgs& a, const VSGJob& jb, int tx, int
   __syncthreads();
 }
 
+/* Packed frames (src/motiondetect.c:613-645): the unshifted position first
+   (rank 0, ahead of every grid point), then one level of 1-px shifts within
+   r = step-1 of the grid winner, x outer / y inner, centre skipped, strict
+   "<".  The (2r+1)^2 square is summed in blocks of at most 5 x 5; the
+   sequential scan's winner is the minimum of (sad, visit position), with the
+   grid winner at position 0. */
+__device__ void refine_packed(const VSGSearchArgs& a, const VSGJob& jb, unsigned long long key, int job,
+                              RefineLds& L) {
+  const int step = a.g.step;
+  level_sums(a, jb, 0, 0, 1, 1, 1, L);
+  const unsigned long long k0 = (unsigned long long)L.sums[0] << 32;
+  key = k0 < key ? k0 : key;
+  __syncthreads();
+  const int q = (int)(uint32_t)key;
+  int tx = 0, ty = 0;
+  if (q > 0) {
+    tx = a.g.org + ((q - 1) / a.g.G) * step;
+    ty = a.g.org + ((q - 1) % a.g.G) * step;
+  }
+  unsigned long long best = key & 0xFFFFFFFF00000000ull;
+  const int r = step - 1, ni = 2 * r + 1;
+  if (r > 0) {
+    for (int bi = 0; bi < ni; bi += kMaxNi)
+      for (int bj = 0; bj < ni; bj += kMaxNi) {
+        const int nx = min(kMaxNi, ni - bi), ny = min(kMaxNi, ni - bj);
+        level_sums(a, jb, tx - r + bi, ty - r + bj, 1, nx, ny, L);
+        for (int ti = 0; ti < nx; ++ti)
+          for (int tj = 0; tj < ny; ++tj) {
+            const int gi = bi + ti, gj = bj + tj;
+            if (gi == r && gj == r) continue;
+            const unsigned long long kk =
+                ((unsigned long long)L.sums[ti * kMaxNi + tj] << 32) | (uint32_t)(1 + gi * ni + gj);
+            best = kk < best ? kk : best;
+          }
+        __syncthreads();   /* sums[] is rewritten by the next block */
+      }
+  }
+  const int p = (int)(uint32_t)best;
+  if (p > 0) {
+    tx += (p - 1) / ni - r;
+    ty += (p - 1) % ni - r;
+  }
+  if (threadIdx.x == 0) {
+    VSGResult res;
+    res.tx = (int16_t)tx;
+    res.ty = (int16_t)ty;
+    res.minerror = (uint32_t)(best >> 32);
+    a.results[job] = res;
+  }
+}
+
 __device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned long long key, int job,
                            RefineLds& L) {
+  if (a.g.bpp != 1) {
+    refine_packed(a, jb, key, job, L);
+    return;
+  }
   int ga, gb;
   vsg_unrank((int)(uint32_t)key, &ga, &gb);
   int tx = ga * a.g.step, ty = gb * a.g.step;
@@ -124,7 +182,7 @@ __device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned lo
   for (int st = a.g.step; st > 1; st /= 2) {
     const int ns = st / 2, r = st - ns;
     const int ni = (2 * r) / ns + 1;
-    level_sums(a, jb, tx, ty, ns, r, ni, L);
+    level_sums(a, jb, tx - r, ty - r, ns, ni, ni, L);
     int btx = tx, bty = ty;
     for (int ti = 0; ti < ni; ++ti)
       for (int tj = 0; tj < ni; ++tj) {
@@ -158,13 +216,16 @@ __device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned lo
    launch.  The refinement of early jobs overlaps the grid of later ones. */
 template <int J, int K, int D, int CW, bool LDSP>
 __device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int tile, RefineLds& L) {
-  extern __shared__ uint32_t lpatch[];   /* LDSP: the field, S*S/4 words */
+  extern __shared__ uint32_t lpatch[];   /* LDSP: the field, S*S*bpp/4 words */
   const VSGJob jb = a.jobs[job];
   if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
   VSGTile t;
   t.S = a.g.S;
   t.s = a.g.step;
-  t.n = a.g.n;
+  t.alo = a.g.alo;
+  t.G = a.g.G;
+  t.org = a.g.org;
+  t.bpp = a.g.bpp;
   t.m = a.g.m;
   t.U = a.g.U;
   t.V = a.g.V;
@@ -174,17 +235,18 @@ __device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int t
   t.oy = jb.oy;
   t.pitch = a.pitch;
   t.prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
+  const size_t patchBytes = (size_t)t.S * t.S * t.bpp;
   t.patch = (vsg_cword_ptr)(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
-                            (size_t)jb.field * t.S * t.S);
+                            (size_t)jb.field * patchBytes);
   t.lpatch = lpatch;
   if (LDSP) {
     /* the whole workgroup reads the same field words: stage them once, and
        every lane then reads them by LDS broadcast instead of scalar loads
        that miss a scalar cache shared by many resident fields */
     const uint4* src = reinterpret_cast<const uint4*>(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
-                                                      (size_t)jb.field * t.S * t.S);
+                                                      (size_t)jb.field * patchBytes);
     uint4* dst = reinterpret_cast<uint4*>(lpatch);
-    for (int i = threadIdx.x; i < t.S * t.S / 16; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < (int)(patchBytes / 16); i += blockDim.x) dst[i] = src[i];
     __syncthreads();
   }
   unsigned long long best = vsg_sad_lane<J, K, D, CW, LDSP>(t, tile * blockDim.x + threadIdx.x);
@@ -284,7 +346,7 @@ __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float
   uint32_t minerr = 0;
   if (accept) {
     const int px = (int)fx, py = (int)fy;
-    level_sums(a, jb, px, py, 1, 2, 5, L);
+    level_sums(a, jb, px - 2, py - 2, 1, 5, 5, L);
     bx = px;
     by = py;
     minerr = L.sums[2 * kMaxNi + 2];
@@ -323,7 +385,7 @@ template <int J, int K, int D, int CW>
 hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
   const int nb = xcd_blocks(a.njobs * a.tiles, a.chunk);
   if (a.g.lds)
-    hipLaunchKernelGGL((k_sad_search<J, K, D, CW, true>), dim3(nb), dim3(a.g.wg), (size_t)a.g.S * a.g.S, st, a);
+    hipLaunchKernelGGL((k_sad_search<J, K, D, CW, true>), dim3(nb), dim3(a.g.wg), vsg_patch_bytes(a.g), st, a);
   else
     hipLaunchKernelGGL((k_sad_search<J, K, D, CW, false>), dim3(nb), dim3(a.g.wg), 0, st, a);
   return hipGetLastError();
@@ -351,11 +413,12 @@ hipError_t launch_grid_k(const VSGSearchArgs& a, hipStream_t st) {
 }
 
 template <int J, int K, int D, int CW>
-hipError_t launch_multi(const VSGMultiSeg* segs, int nseg, int totalUnits, int wg, int lds, int S, hipStream_t st) {
+hipError_t launch_multi(const VSGMultiSeg* segs, int nseg, int totalUnits, int wg, int lds, size_t patchBytes,
+                        hipStream_t st) {
   const int chunk = 32;
   const int nb = xcd_blocks(totalUnits, chunk);
   if (lds)
-    hipLaunchKernelGGL((k_sad_multi<J, K, D, CW, true>), dim3(nb), dim3(wg), (size_t)S * S, st, segs, nseg,
+    hipLaunchKernelGGL((k_sad_multi<J, K, D, CW, true>), dim3(nb), dim3(wg), patchBytes, st, segs, nseg,
                        totalUnits, chunk);
   else
     hipLaunchKernelGGL((k_sad_multi<J, K, D, CW, false>), dim3(nb), dim3(wg), 0, st, segs, nseg, totalUnits, chunk);
@@ -365,12 +428,12 @@ hipError_t launch_multi(const VSGMultiSeg* segs, int nseg, int totalUnits, int w
 template <int J, int K, int D>
 hipError_t launch_multi_cw(const VSGMultiSeg* segs, int nseg, int units, const VSGGrid& g, hipStream_t st) {
   switch (g.CW) {
-    case 28: return launch_multi<J, K, D, 28>(segs, nseg, units, g.wg, g.lds, g.S, st);
-    case 20: return launch_multi<J, K, D, 20>(segs, nseg, units, g.wg, g.lds, g.S, st);
-    case 16: return launch_multi<J, K, D, 16>(segs, nseg, units, g.wg, g.lds, g.S, st);
-    case 12: return launch_multi<J, K, D, 12>(segs, nseg, units, g.wg, g.lds, g.S, st);
-    case 8: return launch_multi<J, K, D, 8>(segs, nseg, units, g.wg, g.lds, g.S, st);
-    default: return launch_multi<J, K, D, 4>(segs, nseg, units, g.wg, g.lds, g.S, st);
+    case 28: return launch_multi<J, K, D, 28>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
+    case 20: return launch_multi<J, K, D, 20>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
+    case 16: return launch_multi<J, K, D, 16>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
+    case 12: return launch_multi<J, K, D, 12>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
+    case 8: return launch_multi<J, K, D, 8>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
+    default: return launch_multi<J, K, D, 4>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
   }
 }
 
@@ -394,28 +457,40 @@ int gcd_int(int x, int y) {
 
 }  // namespace
 
-int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K) {
+int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K, int bpp) {
   g->S = S;
   g->step = step;
+  g->bpp = bpp;
   g->n = maxShift / step;
-  const int G = 2 * g->n + 1;
-  const int lcm4 = 4 * step / gcd_int(4, step);
+  if (bpp == 1) {
+    g->G = 2 * g->n + 1;
+    g->alo = -g->n;
+    g->org = 0;
+  } else {
+    g->G = 2 * maxShift / step + 1;
+    g->alo = 0;
+    g->org = -maxShift;
+  }
+  const int G = g->G;
+  const int stepBytes = step * bpp;
+  const int lcm4 = 4 * stepBytes / gcd_int(4, stepBytes);
   const int D = lcm4 / 4;
   g->K = (K == 6 || K == 8) ? K : 4;
   g->lds = 0;
   /* the widest column chunk (<= 28 words) that divides a field row: one prev
      row segment load then feeds J*K*CW v_sad_u8 */
+  const int rowWords = S * bpp / 4;
   static const int kCW[] = {28, 20, 16, 12, 8, 4};
   g->CW = 4;
   for (int cw : kCW)
-    if ((S / 4) % cw == 0) {
+    if (rowWords % cw == 0) {
       g->CW = cw;
       break;
     }
   if (D == 1 || D == 3) {
     g->J = 4;
     g->D = D;
-    g->m = lcm4 / step;
+    g->m = lcm4 / stepBytes;
   } else {
     g->J = 1;   /* any step: one column per lane, no horizontal sharing */
     g->D = 1;
@@ -428,6 +503,9 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K) {
      whole waves, so idle lanes stay below one wave */
   const int lanes = g->U * g->V;
   g->wg = lanes >= 256 ? 256 : 64 * ((lanes + 63) / 64);
+  /* the refinement gives each word of a field row its own thread */
+  const int rowThreads = 64 * ((rowWords + 63) / 64);
+  if (g->wg < rowThreads) g->wg = rowThreads;
   return (lanes + g->wg - 1) / g->wg;
 }
 
@@ -436,7 +514,7 @@ double vsg_plan_cost(const VSGGrid& g, int tiles) {
      alignbytes and a few address / scalar-load slots; per lane the J*K*S*CW
      v_sad_u8 of each column chunk; times the lanes the job occupies */
   const int NP = g.CW + g.D * (g.J - 1);
-  const int chunks = (g.S / 4) / g.CW;
+  const int chunks = (g.S * g.bpp / 4) / g.CW;
   const double rows = g.S + (g.K - 1) * g.step;
   const double perLane = chunks * (rows * (2.0 * NP + 5.0) + (double)g.K * g.J * g.S * g.CW);
   return perLane * (double)tiles * g.wg;
@@ -444,7 +522,8 @@ double vsg_plan_cost(const VSGGrid& g, int tiles) {
 
 hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
-  if (a.chunk <= 0 || a.g.S > 256 || a.g.wg < 64 || a.g.wg > 256) return hipErrorInvalidValue;
+  if (a.chunk <= 0 || a.g.S > 256 || a.g.wg < 64 || a.g.wg > 256 || a.g.S * a.g.bpp > 4 * a.g.wg)
+    return hipErrorInvalidValue;
   const VSGGrid& g = a.g;
   if (g.J == 4 && g.D == 1) return launch_grid_k<4, 1>(a, st);
   if (g.J == 4 && g.D == 3) return launch_grid_k<4, 3>(a, st);
@@ -453,7 +532,7 @@ hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st) {
 
 hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, const VSGGrid& g, hipStream_t st) {
   if (totalUnits <= 0 || nseg <= 0) return hipSuccess;
-  if (!dSegs || g.S > 256 || g.wg < 64 || g.wg > 256) return hipErrorInvalidValue;
+  if (!dSegs || g.S > 256 || g.wg < 64 || g.wg > 256 || g.S * g.bpp > 4 * g.wg) return hipErrorInvalidValue;
   if (g.J == 4 && g.D == 1) return launch_multi_k<4, 1>(dSegs, nseg, totalUnits, g, st);
   if (g.J == 4 && g.D == 3) return launch_multi_k<4, 3>(dSegs, nseg, totalUnits, g, st);
   return launch_multi_k<1, 1>(dSegs, nseg, totalUnits, g, st);
@@ -462,7 +541,7 @@ hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, 
 hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int nc, uint32_t maxSad, VSGJob* fallback,
                              unsigned* stats, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
-  if (a.g.S > 256 || nc <= 0) return hipErrorInvalidValue;
+  if (a.g.S > 256 || nc <= 0 || a.g.bpp != 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_sad_window, dim3(a.njobs), dim3(256), 0, st, a, pred, nc, maxSad, fallback, stats);
   return hipGetLastError();
 }

@@ -31,10 +31,10 @@ namespace {
 
 constexpr int kPlans[3] = {4, 6, 8};
 
-typedef std::tuple<int, int, int, int, int, int> TuneKey;   /* device, S, step, maxShift, B, nf */
+typedef std::tuple<int, int, int, int, int, int, int> TuneKey;   /* device, S, step, maxShift, B, nf, bpp */
 std::mutex g_tuneMu;
 std::map<TuneKey, int> g_tuned;   /* K + 16 * lds */
-constexpr int kMaxLdsField = 128;
+constexpr int kMaxLdsPatch = 128 * 128 * 3;   /* bytes of a field staged in LDS */
 
 int env_int(const char* name, int def) {
   const char* v = getenv(name);
@@ -53,7 +53,7 @@ int time_plan(VSGContext* c, int k, int K, int lds, const VSGJob* dJobs, int njo
   VSGStage& S = c->st[k];
   VSGSet& s = c->set[0];
   VSGStage trial = S;
-  trial.tiles = vsg_plan_grid(&trial.g, S.S, S.step, S.maxShift, K);
+  trial.tiles = vsg_plan_grid(&trial.g, S.S, S.step, S.maxShift, K, c->bpp);
   trial.g.lds = lds;
   VSGSearchArgs a = vsg_ctx_search_args(c, k, dJobs, njobs, s.d_best, s.d_done, k == 0 ? s.d_resC : s.d_resF);
   a.g = trial.g;
@@ -82,12 +82,12 @@ int time_plan(VSGContext* c, int k, int K, int lds, const VSGJob* dJobs, int njo
 
 }  // namespace
 
-int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift) {
+int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift, int bpp) {
   int bestTiles = 0;
   double bestCost = 0;
   for (int K : kPlans) {
     VSGGrid t;
-    const int tiles = vsg_plan_grid(&t, S, step, maxShift, K);
+    const int tiles = vsg_plan_grid(&t, S, step, maxShift, K, bpp);
     const double cost = vsg_plan_cost(t, tiles);
     if (bestTiles == 0 || cost < bestCost) {
       bestCost = cost;
@@ -105,11 +105,12 @@ int vsg_ctx_tune(VSGContext* c) {
   for (int k = 0; k < 2; ++k) {
     VSGStage& S = c->st[k];
     int K = 0, lds = 0;
+    const bool ldsOk = S.S <= 128 && (size_t)S.S * S.S * c->bpp <= (size_t)kMaxLdsPatch;
     if (forced == 4 || forced == 6 || forced == 8) {
       K = forced;
-      lds = forcedLds == 1 && S.S <= kMaxLdsField;
+      lds = forcedLds == 1 && ldsOk;
     } else if (timing) {
-      const TuneKey key(c->device, S.S, S.step, S.maxShift, c->B, S.nf);
+      const TuneKey key(c->device, S.S, S.step, S.maxShift, c->B, S.nf, c->bpp);
       {
         std::lock_guard<std::mutex> lk(g_tuneMu);
         auto it = g_tuned.find(key);
@@ -135,8 +136,8 @@ int vsg_ctx_tune(VSGContext* c) {
         TUNE_CHECK(hipMemcpy(dJobs, jobs.data(), sizeof(VSGJob) * (size_t)njobs, hipMemcpyHostToDevice));
         float bestMs = 0;
         for (int l = 0; l < 2; ++l) {
-          if (l == 1 && (S.S > kMaxLdsField || forcedLds == 0)) break;
-          if (l == 0 && forcedLds == 1 && S.S <= kMaxLdsField) continue;
+          if (l == 1 && (!ldsOk || forcedLds == 0)) break;
+          if (l == 0 && forcedLds == 1 && ldsOk) continue;
           for (int q = 0; q < 3; ++q) {
             float ms = 0;
             if (time_plan(c, k, kPlans[q], l, dJobs, njobs, &ms) != 0) return -1;
@@ -153,7 +154,7 @@ int vsg_ctx_tune(VSGContext* c) {
       }
     }
     if (K != 0) {
-      S.tiles = vsg_plan_grid(&S.g, S.S, S.step, S.maxShift, K);
+      S.tiles = vsg_plan_grid(&S.g, S.S, S.step, S.maxShift, K, c->bpp);
       S.g.lds = lds;
     }
   }

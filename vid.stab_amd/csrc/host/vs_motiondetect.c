@@ -3,7 +3,7 @@
  * (reference: src/motiondetect.c:60-305, 314-358, 752-807).
  *
  * Per batch of frames (one frame for vsMotionDetection):
- *   GPU  (vsg_submit)      upload, box blur, per-field min/max, field
+ *   GPU  (vsg_submit)      upload, box blur (packed RGB: a copy), per-field min/max, field
  *                          selection of both grids, coarse grid search and
  *                          refinement for every field of every frame
  *   host (after the wait)  coarse LocalMotions, vsSimpleMotionsToTransform
@@ -157,6 +157,8 @@ static int init_fields(VSMotionDetect* md, VSMotionDetectFields* fs, int size, i
 }
 
 int vsd_blur_size(const VSMotionDetect* md) {
+  /* packed frames are searched unblurred (src/motiondetect.c:197-201) */
+  if (md->fi.pFormat > PF_PACKED) return 1;
   int size = md->conf.stepSize;
   if (size < 2) return 1;
   size = (size / 2) * 2 + 1;
@@ -184,11 +186,6 @@ int vsd_configure(VSMotionDetect* md, const VSMotionDetectConfig* conf, const VS
   md->fieldsfine.fields = NULL;
   if (fi->pFormat <= PF_NONE || fi->pFormat == PF_PACKED || fi->pFormat >= PF_NUMBER) {
     vs_log_warn(md->conf.modName, "unsupported Pixel Format (%i)\n", md->fi.pFormat);
-    return VS_ERROR;
-  }
-  if (fi->pFormat > PF_PACKED) {
-    vs_log_error(md->conf.modName, "packed pixel formats are not supported by the GPU detector (%i)\n",
-                 md->fi.pFormat);
     return VS_ERROR;
   }
   if (md->conf.stepSize < 1) {
@@ -259,8 +256,9 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
   s->lastSlot = -1;
   const int device = env_int("VIDSTAB_AMD_DEVICE", 0);
   VSGStageDesc dc = vsd_stage_desc(&md->fieldscoarse), df = vsd_stage_desc(&md->fieldsfine);
-  if (vsg_create(&s->gpu, device, md->fi.width, md->fi.height, vsd_blur_size(md), &dc, &df,
-                 s->maxBatch) != 0) {
+  const int bpp = md->fi.pFormat > PF_PACKED ? md->fi.bytesPerPixel : 1;
+  if (vsg_create_fmt(&s->gpu, device, md->fi.width, md->fi.height, bpp, vsd_blur_size(md), &dc, &df,
+                     s->maxBatch) != 0) {
     vs_log_error(md->conf.modName, "GPU initialisation failed: %s\n", vsg_last_error());
     state_free(s);
     free_fields(md);
@@ -302,12 +300,19 @@ VS_API void vsMotionDetectionCleanup(VSMotionDetect* md) {
 
 /* ---- per-frame host steps ------------------------------------------------------ */
 
-/* src/motiondetect.c:573-585 */
-static int make_motion(const VSMotionDetectFields* fs, const Field* f, const VSGResult* r, int ox,
+/* src/motiondetect.c:573-585 (planar) and :648-658 (packed).  The planar
+   rule reads stepSize after the refinement halved it to 1, hence the
+   asymmetric bound; the packed one uses the stage's step on both axes. */
+static int make_motion(const VSMotionDetectFields* fs, int packed, const Field* f, const VSGResult* r, int ox,
                        int oy, double contrast, LocalMotion* lm) {
-  int st = fs->stepSize;
-  while (st > 1) st /= 2;
-  if (abs(r->tx) >= fs->maxShift + st - 1 || abs(r->ty) >= fs->maxShift + st) return 0;
+  if (packed) {
+    const int lim = fs->maxShift + fs->stepSize - 1;
+    if (abs(r->tx) >= lim || abs(r->ty) >= lim) return 0;
+  } else {
+    int st = fs->stepSize;
+    while (st > 1) st /= 2;
+    if (abs(r->tx) >= fs->maxShift + st - 1 || abs(r->ty) >= fs->maxShift + st) return 0;
+  }
   memset(lm, 0, sizeof(*lm));
   lm->f = *f;
   lm->v.x = (int16_t)(r->tx + ox);
@@ -333,7 +338,7 @@ int vsd_frame_coarse(const VSMotionDetect* md, const VSGBatchView* v, int i, int
   const VSGResult* res = v->coarse + (size_t)i * v->ncoarse;
   for (int q = 0; q < v->count[2 * i]; q++) {
     LocalMotion lm;
-    if (make_motion(fc, &fc->fields[idx[q]], &res[q], 0, 0, con[q], &lm))
+    if (make_motion(fc, md->fi.pFormat > PF_PACKED, &fc->fields[idx[q]], &res[q], 0, 0, con[q], &lm))
       vs_vector_append_dup(coarse, &lm, sizeof(LocalMotion));
   }
   if (vs_vector_size(coarse) < 1) {
@@ -379,7 +384,8 @@ void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, Loca
     const double* fcon = v->contrast + (size_t)i * v->stride + v->ncoarse;
     for (int k = 0; k < njobs; k++) {
       LocalMotion lm;
-      if (!make_motion(ff, &ff->fields[jobs[k].field], &res[k], jobs[k].ox, jobs[k].oy, fcon[jobSel[k]], &lm))
+      if (!make_motion(ff, md->fi.pFormat > PF_PACKED, &ff->fields[jobs[k].field], &res[k], jobs[k].ox, jobs[k].oy,
+                       fcon[jobSel[k]], &lm))
         continue;
       if (lm_match_better(&meanMatch, &lm)) vs_vector_append_dup(&fine, &lm, sizeof(LocalMotion));
     }

@@ -276,6 +276,11 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
       ok = 0;
       break;
     }
+    if (cl->md.fi.pFormat > PF_PACKED) {
+      vs_log_error(confs[c].modName, "the multi-clip form takes planar frames\n");
+      ok = 0;
+      break;
+    }
     cl->lastSlot = -1;
     cl->jobCap = cl->md.fieldsfine.fieldNum;
     w[c] = cl->md.fi.width;

@@ -312,6 +312,18 @@ void vsyn_render(const VSynClip* c, const VSynMotion* mo, int t, uint8_t* dst, i
   }
 }
 
+void vsyn_pack(const uint8_t* luma, int w, int h, int lstride, int bpp, uint8_t* dst, int dstride) {
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const uint8_t v = luma[(size_t)y * lstride + x];
+      uint8_t* d = dst + (size_t)y * dstride + (size_t)x * bpp;
+      d[0] = v;
+      d[1] = (uint8_t)(7 * v);
+      d[2] = (uint8_t)(255 - v);
+      if (bpp > 3) d[3] = (uint8_t)(v ^ 0x5A);
+    }
+}
+
 uint64_t vsyn_hash_plane(const uint8_t* p, int w, int h, int stride) {
   uint64_t hsh = 0xCBF29CE484222325ull;
   for (int y = 0; y < h; ++y)

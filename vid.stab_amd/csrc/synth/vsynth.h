@@ -84,6 +84,11 @@ int vsyn_spec(int config, int clip, VSynSpec* s);
 /* build the clip and its motion path (path must hold s->nframes entries) */
 int vsyn_open(const VSynSpec* s, VSynClip* c, VSynMotion* path);
 
+/* a packed frame (bpp 3 or 4 bytes a pixel) from a rendered luma plane, one
+   distinct per-pixel map per byte: Y, (7*Y) mod 256, 255 - Y, Y xor 0x5A.
+   Every channel moves with the content but has its own min/max. */
+void vsyn_pack(const uint8_t* luma, int w, int h, int lstride, int bpp, uint8_t* dst, int dstride);
+
 /* FNV-1a over a plane (row by row), for fixtures */
 uint64_t vsyn_hash_plane(const uint8_t* p, int w, int h, int stride);
 

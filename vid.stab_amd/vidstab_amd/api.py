@@ -173,15 +173,19 @@ class Detector:
         if L.vsMotionDetectInit(ctypes.byref(self.md), ctypes.byref(conf), ctypes.byref(self.fi)) != 0:
             raise RuntimeError("vsMotionDetectInit failed: " + L.vsg_last_error().decode())
         self.width, self.height = width, height
+        self.packed = pformat > PF_PACKED
+        self.bpp = self.fi.bytesPerPixel if self.packed else 1
         self._chroma = np.full((height // 2) * (width // 2), 128, dtype=np.uint8)
 
     def _frame(self, luma):
+        """luma: the luma plane, or plane 0 of a packed frame (width*bpp bytes a row)"""
         fr = VSFrame()
         fr.data[0] = luma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         fr.linesize[0] = luma.shape[1]
-        fr.data[1] = self._chroma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
-        fr.data[2] = fr.data[1]
-        fr.linesize[1] = fr.linesize[2] = self.width // 2
+        if not self.packed:
+            fr.data[1] = self._chroma.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            fr.data[2] = fr.data[1]
+            fr.linesize[1] = fr.linesize[2] = self.width // 2
         return fr
 
     def detect(self, luma):
@@ -228,8 +232,10 @@ class Detector:
         return f.value, b.value
 
     def blurred(self):
-        out = np.zeros((self.height, self.width), dtype=np.uint8)
-        if self.L.vsMotionDetectGetBlurred(ctypes.byref(self.md), out.ctypes.data, self.width) != 0:
+        """the last working plane: blurred luma, or a packed frame's unblurred copy"""
+        wb = self.width * self.bpp
+        out = np.zeros((self.height, wb), dtype=np.uint8)
+        if self.L.vsMotionDetectGetBlurred(ctypes.byref(self.md), out.ctypes.data, wb) != 0:
             raise RuntimeError("vsMotionDetectGetBlurred failed")
         return out
 

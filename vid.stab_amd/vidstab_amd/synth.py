@@ -107,6 +107,23 @@ class Clip:
             pass
 
 
+def pack(luma, bpp, stride=None, width=None):
+    """packed frame (H, stride) of bpp bytes a pixel from a luma plane: bytes
+    Y, (7*Y) mod 256, 255 - Y and (bpp 4) Y xor 0x5A, as vsyn_pack writes"""
+    w = width or luma.shape[1]
+    y = np.asarray(luma, dtype=np.uint8)[:, :w]
+    h = y.shape[0]
+    stride = stride or w * bpp
+    out = np.full((h, stride), 0x5A, dtype=np.uint8)
+    px = out[:, :w * bpp].reshape(h, w, bpp)
+    px[..., 0] = y
+    px[..., 1] = (y.astype(np.uint32) * 7).astype(np.uint8)
+    px[..., 2] = 255 - y
+    if bpp > 3:
+        px[..., 3] = y ^ 0x5A
+    return out
+
+
 def hash_plane(a, width=None):
     a = np.ascontiguousarray(a, dtype=np.uint8)
     w = width or a.shape[1]
