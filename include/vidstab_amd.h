@@ -26,12 +26,14 @@ extern "C" {
 VS_API int vsMotionDetectionBatch(VSMotionDetect* md, LocalMotions* motions, VSFrame* frames, int n);
 
 /* as vsMotionDetectionBatch, but the luma planes are already in device memory
-   of the detector's GPU (lumaDev[i], linesize[i] bytes per row) */
+   of the detector's GPU (lumaDev[i], linesize[i] bytes per row).  No show
+   overlays are drawn into device frames. */
 VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions,
                                    const uint8_t* const* lumaDev, const int* linesize, int n);
 
 /* copies the last blurred plane of the detector (the reference's md->curr
-   luma) into dst; for parity checks */
+   luma; for packed RGB24 / BGR24 / RGBA frames the unblurred plane 0,
+   width * bytesPerPixel bytes a row) into dst; for parity checks */
 VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride);
 
 /* The learned-proposal form: loads a proposal network (the weights file

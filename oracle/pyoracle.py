@@ -50,7 +50,8 @@ class OrState(ctypes.Structure):
                 ("curr", ctypes.c_void_p), ("prev", ctypes.c_void_p), ("tmp", ctypes.c_void_p),
                 ("hasSeenOneFrame", ctypes.c_int), ("frameNum", ctypes.c_int),
                 ("nCoarse", ctypes.c_int), ("nFineAll", ctypes.c_int), ("nFineKept", ctypes.c_int),
-                ("lastT", VSTransform), ("lastMeanMatch", ctypes.c_double), ("bpp", ctypes.c_int)]
+                ("lastT", VSTransform), ("lastMeanMatch", ctypes.c_double), ("bpp", ctypes.c_int),
+                ("show", ctypes.c_int)]
 
 
 _lib = None
@@ -123,7 +124,7 @@ class Detector:
     3 or 4: packed RGB24 / BGR24 / RGBA frames (plane 0, width*bpp bytes a row)."""
 
     def __init__(self, width, height, shakiness=5, accuracy=15, stepsize=6,
-                 mincontrast=0.25, tripod=0, bpp=1):
+                 mincontrast=0.25, tripod=0, bpp=1, show=0):
         self._L = lib()
         self.width, self.height, self.bpp = width, height, bpp
         if bpp == 1:
@@ -135,11 +136,16 @@ class Detector:
         if not self.s:
             raise RuntimeError("or_new failed")
         st = self.s.contents
+        st.show = show
         self.maxout = st.coarse.fieldNum + st.fine.fieldNum + 1
         self.buf = (LocalMotion * self.maxout)()
 
     def detect(self, luma):
-        """luma (or packed plane 0): uint8 array (H, stride) -> list of motion tuples"""
+        """luma (or packed plane 0): uint8 array (H, stride) -> list of motion tuples.
+        With show > 0 the overlays are drawn into the array (it must be
+        C-contiguous uint8, as the reference draws into the caller's frame)."""
+        if self.s.contents.show:
+            assert luma.dtype == np.uint8 and luma.flags.c_contiguous
         luma = np.ascontiguousarray(luma, dtype=np.uint8)
         n = self._L.or_detect(self.s, luma.ctypes.data, luma.shape[1], self.buf, self.maxout)
         return motions_to_list(self.buf, n)

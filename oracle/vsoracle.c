@@ -520,6 +520,86 @@ void or_transform_point(const VSTransform* t, int w, int h, int x, int y, int* o
   *oy = (int16_t)(int)Y;
 }
 
+/* ---- overlays: src/motiondetect.c:271-286, 816-952 --------------------------
+   Grey levels into plane 0 (every colour byte of a packed pixel, never a
+   fourth one), clipped to the frame. */
+
+static void or_run(uint8_t* img, int stride, int w, int h, int bpp, int x, int y, int len, int vertical,
+                   uint8_t c) {
+  int nch = bpp > 3 ? 3 : bpp;
+  if (vertical) {
+    if (x < 0 || x >= w) return;
+    int y0 = OR_MAX(y, 0), y1 = OR_MIN(y + len, h);
+    for (int yy = y0; yy < y1; yy++)
+      for (int k = 0; k < nch; k++) img[(size_t)yy * stride + (size_t)x * bpp + k] = c;
+  } else {
+    if (y < 0 || y >= h) return;
+    int x0 = OR_MAX(x, 0), x1 = OR_MIN(x + len, w);
+    for (int xx = x0; xx < x1; xx++)
+      for (int k = 0; k < nch; k++) img[(size_t)y * stride + (size_t)xx * bpp + k] = c;
+  }
+}
+
+/* filled box of sx x sy centred at (x, y) */
+static void or_fill(uint8_t* img, int stride, int w, int h, int bpp, int x, int y, int sx, int sy, uint8_t c) {
+  for (int j = 0; j < sy; j++) or_run(img, stride, w, h, bpp, x - sx / 2, y - sy / 2 + j, sx, 0, c);
+}
+
+/* outline: top and bottom rows of sx, left and right columns of sy */
+static void or_frame(uint8_t* img, int stride, int w, int h, int bpp, int x, int y, int sx, int sy, uint8_t c) {
+  or_run(img, stride, w, h, bpp, x - sx / 2, y - sy / 2, sx, 0, c);
+  or_run(img, stride, w, h, bpp, x - sx / 2, y + sy / 2, sx, 0, c);
+  or_run(img, stride, w, h, bpp, x - sx / 2, y - sy / 2, sy, 1, c);
+  or_run(img, stride, w, h, bpp, x + sx / 2, y - sy / 2, sy, 1, c);
+}
+
+/* the reference's drawLine from the field centre (whose x, y it overwrites
+   with the end point when it swaps the ends of a leftward or upward
+   axis-parallel line) to the end point; the diagonal case steps rows with
+   x = (int)((a.x - horlen/2) + m*dy), the order the reference's -ffast-math
+   build evaluates it in */
+static void or_arrow(uint8_t* img, int stride, int w, int h, int bpp, Field* a, int bx, int by, int thick,
+                     uint8_t c) {
+  int dx = bx - a->x, dy = by - a->y;
+  if (dy == 0) {
+    if (dx < 0) { a->x = (int16_t)bx; a->y = (int16_t)by; dx = -dx; }
+    for (int r = -thick / 2; r <= thick / 2; r++) or_run(img, stride, w, h, bpp, a->x, a->y + r, dx + 1, 0, c);
+  } else if (dx == 0) {
+    if (dy < 0) { a->x = (int16_t)bx; a->y = (int16_t)by; dy = -dy; }
+    for (int r = -thick / 2; r <= thick / 2; r++) or_run(img, stride, w, h, bpp, a->x + r, a->y, dy + 1, 1, c);
+  } else {
+    double m = (double)dx / (double)dy;
+    int horlen = (int)((double)thick + fabs(m));
+    double base = (double)(a->x - horlen / 2);
+    for (int k = 0; k <= abs(dy); k++) {
+      int step = dy < 0 ? -k : k;
+      int x = (int)(base + m * (double)step);
+      or_run(img, stride, w, h, bpp, x, a->y + step, horlen + 1, 0, c);
+    }
+  }
+}
+
+static void or_draw(const OrState* s, uint8_t* img, int stride, LocalMotion* co, int nc, LocalMotion* fi, int nf) {
+  const int w = s->width, h = s->height, bpp = s->bpp;
+  const int scan = s->coarse.maxShift;
+  if (s->show > 1)
+    for (int i = 0; i < nc; i++)
+      or_frame(img, stride, w, h, bpp, co[i].f.x, co[i].f.y, co[i].f.size + 2 * scan, co[i].f.size + 2 * scan, 80);
+  for (int i = 0; i < nc; i++) or_fill(img, stride, w, h, bpp, co[i].f.x, co[i].f.y, co[i].f.size, co[i].f.size, 40);
+  for (int i = 0; i < nf; i++) or_frame(img, stride, w, h, bpp, fi[i].f.x, fi[i].f.y, fi[i].f.size, fi[i].f.size, 40);
+  for (int pass = 0; pass < 2; pass++) {
+    LocalMotion* lm = pass ? fi : co;
+    int n = pass ? nf : nc;
+    uint8_t color = pass ? 64 : 180;
+    for (int i = 0; i < n; i++) {
+      int ex = lm[i].f.x + lm[i].v.x, ey = lm[i].f.y + lm[i].v.y;
+      or_fill(img, stride, w, h, bpp, lm[i].f.x, lm[i].f.y, 5, 5, 0);
+      or_fill(img, stride, w, h, bpp, ex, ey, 5, 5, 250);
+      or_arrow(img, stride, w, h, bpp, &lm[i].f, ex, ey, 3, color);
+    }
+  }
+}
+
 /* ---- the per-frame driver: src/motiondetect.c:191-305 --------------------- */
 
 typedef struct { LocalMotion lm; int ok; } OrSlot;
@@ -570,7 +650,7 @@ static int or_calc_trans_fields(OrState* s, const OrFields* fs, const VSTransfor
   return n;
 }
 
-int or_detect(OrState* s, const uint8_t* luma, int stride, LocalMotion* out, int maxout) {
+int or_detect(OrState* s, uint8_t* luma, int stride, LocalMotion* out, int maxout) {
   const int W = s->width, H = s->height, bpp = s->bpp;
   or_boxblur(s->curr, luma, W * bpp, H, stride, bpp == 1 ? s->blurSize : 1);
   int tripodLeadIn = s->virtualTripod >= 1 && s->frameNum < s->virtualTripod;
@@ -615,6 +695,7 @@ int or_detect(OrState* s, const uint8_t* luma, int stride, LocalMotion* out, int
       for (int i = 0; i < nf; i++)
         if (fi[i].match <= meanMatch) fi[nk++] = fi[i];
     }
+    if (s->show) or_draw(s, luma, stride, co, nc, fi, nk);
     for (int i = 0; i < nc && total < maxout; i++) out[total++] = co[i];
     for (int i = 0; i < nk && total < maxout; i++) out[total++] = fi[i];
     s->nCoarse = nc;

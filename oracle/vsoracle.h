@@ -35,6 +35,7 @@ typedef struct {
   VSTransform lastT;
   double lastMeanMatch;
   int bpp;                    /* 1: planar luma; 3 or 4: packed RGB24/BGR24/RGBA plane 0 */
+  int show;                   /* 1 / 2: overlays drawn into the frame handed to or_detect */
 } OrState;
 
 int  or_init(OrState* s, int width, int height, int shakiness, int accuracy, int stepSize,
@@ -43,9 +44,11 @@ int  or_init(OrState* s, int width, int height, int shakiness, int accuracy, int
 int  or_init_packed(OrState* s, int width, int height, int bpp, int shakiness, int accuracy, int stepSize,
                     double contrastThreshold, int virtualTripod);
 void or_free(OrState* s);
-/* one vsMotionDetection step on a luma plane; writes up to maxout motions and
-   returns their number (coarse ++ cleaned fine) */
-int  or_detect(OrState* s, const uint8_t* luma, int stride, LocalMotion* out, int maxout);
+/* one vsMotionDetection step on a luma plane (packed: plane 0); writes up to
+   maxout motions and returns their number (coarse ++ cleaned fine).  With
+   s->show > 0 the overlays are drawn into luma, as the reference draws into
+   the caller's frame. */
+int  or_detect(OrState* s, uint8_t* luma, int stride, LocalMotion* out, int maxout);
 
 /* building blocks, exposed for unit tests */
 int      or_blur_size(int stepSize, int width, int height);

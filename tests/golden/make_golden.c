@@ -40,6 +40,7 @@ typedef struct {
   int stride_pad;
   double mincontrast;    /* <0: from the spec */
   int pformat;           /* 0: PF_YUV420P; else a packed format (vsyn_pack of the luma) */
+  int show;              /* conf.show: overlays drawn into the caller's frame */
 } Case;
 
 static void run_case(FILE* out, const Case* cs, int first) {
@@ -68,6 +69,7 @@ static void run_case(FILE* out, const Case* cs, int first) {
   conf.stepSize = sp.stepsize;
   conf.contrastThreshold = sp.mincontrast;
   conf.virtualTripod = cs->tripod;
+  conf.show = cs->show;
   conf.numThreads = 1;
   if (vsMotionDetectInit(&md, &conf, &fi) != VS_OK) { fprintf(stderr, "init failed\n"); exit(1); }
 
@@ -93,11 +95,11 @@ static void run_case(FILE* out, const Case* cs, int first) {
 
   fprintf(out, "%s{\"name\":\"%s\",\"pformat\":%d,\"bpp\":%d,\"config\":%d,\"clip\":%d,\"width\":%d,\"height\":%d,"
                "\"stride\":%d,\"shakiness\":%d,\"accuracy\":%d,\"stepsize\":%d,"
-               "\"mincontrast\":\"%016" PRIx64 "\",\"tripod\":%d,\"md_conf\":[%d,%d,%d],"
+               "\"mincontrast\":\"%016" PRIx64 "\",\"tripod\":%d,\"show\":%d,\"md_conf\":[%d,%d,%d],"
                "\"coarse\":[%d,%d,%d,%d],\"fine\":[%d,%d,%d,%d],\"frames\":[",
           first ? "" : ",\n", cs->name, cs->pformat, bpp, cs->config, cs->clip, sp.width, sp.height,
           cs->pformat ? pls : ls,
-          sp.shakiness, sp.accuracy, sp.stepsize, dbits(sp.mincontrast), cs->tripod,
+          sp.shakiness, sp.accuracy, sp.stepsize, dbits(sp.mincontrast), cs->tripod, cs->show,
           md.conf.shakiness, md.conf.accuracy, md.conf.stepSize,
           md.fieldscoarse.fieldSize, md.fieldscoarse.maxShift, md.fieldscoarse.fieldNum,
           md.fieldscoarse.maxFields, md.fieldsfine.fieldSize, md.fieldsfine.maxShift,
@@ -115,16 +117,19 @@ static void run_case(FILE* out, const Case* cs, int first) {
       fr.data[2] = V; fr.linesize[2] = sp.width / 2;
     }
     LocalMotions lms;
-    vsMotionDetection(&md, &lms, &fr);
     uint64_t hin = P ? vsyn_hash_plane(P, sp.width * bpp, sp.height, pls) : vsyn_hash_plane(Y, sp.width, sp.height, ls);
+    vsMotionDetection(&md, &lms, &fr);
+    /* with show > 0 the reference draws into the frame it was handed */
+    uint64_t hdr = P ? vsyn_hash_plane(P, sp.width * bpp, sp.height, pls) : vsyn_hash_plane(Y, sp.width, sp.height, ls);
     uint64_t hbl = vsyn_hash_plane(md.curr.data[0], sp.width * bpp, sp.height, md.curr.linesize[0]);
     vsWriteToFile(&md, fb, &lms);
     md.serializationMode = ASCII_SERIALIZATION_MODE;
     vsWriteToFile(&md, fa, &lms);
     md.serializationMode = BINARY_SERIALIZATION_MODE;
     int n = vs_vector_size(&lms);
-    fprintf(out, "%s\n {\"in\":\"%016" PRIx64 "\",\"blur\":\"%016" PRIx64 "\",\"lm\":[",
-            t ? "," : "", hin, hbl);
+    fprintf(out, "%s\n {\"in\":\"%016" PRIx64 "\",\"drawn\":\"%016" PRIx64 "\",\"blur\":\"%016" PRIx64
+                 "\",\"lm\":[",
+            t ? "," : "", hin, hdr, hbl);
     for (int i = 0; i < n; i++) {
       LocalMotion* m = (LocalMotion*)vs_vector_get(&lms, i);
       fprintf(out, "%s[%d,%d,%d,%d,%d,\"%016" PRIx64 "\",\"%016" PRIx64 "\"]", i ? "," : "",
@@ -184,9 +189,23 @@ int main(int argc, char** argv) {
     { "p_rgba_odd",      1, 27,   502,  298, 5,   5,   0,  0,   0,    12, -1, PF_RGBA },
     { "p_rgb24_1080p",   2, 20,   0,    0,   3,   0,   0,  0,   0,    0,  -1, PF_RGB24 },
   };
+  /* show = 1 / 2: the overlays the reference draws into the caller's frame */
+  static const Case shows[] = {
+    /* name            cfg clip  w     h    nfr step acc shak tripod pad  contrast  format    show */
+    { "s_c1_show1",      1, 60,   0,    0,   6,   0,   0,  0,   0,    0,  -1, 0,        1 },
+    { "s_c1_show2",      1, 61,   0,    0,   6,   0,   0,  0,   0,    24, -1, 0,        2 },
+    { "s_c1_step3_show2",1, 62,   0,    0,   4,   3,   0,  0,   0,    0,  -1, 0,        2 },
+    { "s_odd_show2",     1, 63,   502,  298, 5,   0,   0,  0,   0,    0,  -1, 0,        2 },
+    { "s_rgba_show2",    1, 64,   0,    0,   5,   0,   0,  0,   0,    0,  -1, PF_RGBA,  2 },
+    { "s_bgr24_show1",   1, 65,   0,    0,   5,   0,   0,  0,   0,    12, -1, PF_BGR24, 1 },
+    { "s_1080p_show2",   2, 60,   0,    0,   3,   0,   0,  0,   0,    0,  -1, 0,        2 },
+  };
   const int usePacked = argc > 2 && strcmp(argv[2], "packed") == 0;
-  const Case* cases = usePacked ? packed : planar;
-  const size_t ncases = usePacked ? sizeof(packed) / sizeof(packed[0]) : sizeof(planar) / sizeof(planar[0]);
+  const int useShow = argc > 2 && strcmp(argv[2], "show") == 0;
+  const Case* cases = useShow ? shows : usePacked ? packed : planar;
+  const size_t ncases = useShow     ? sizeof(shows) / sizeof(shows[0])
+                        : usePacked ? sizeof(packed) / sizeof(packed[0])
+                                    : sizeof(planar) / sizeof(planar[0]);
   FILE* out = fopen(path, "w");
   if (!out) return 1;
   fprintf(out, "{\"source\":\"reference vsMotionDetection (oracle/_ref/libvsref.so)\",\"cases\":[\n");

@@ -12,10 +12,12 @@ gcc -O2 -std=gnu11 -ffp-contract=off -fno-fast-math -fopenmp \
     -o "$REPO/oracle/_ref/make_golden" "$HERE/make_golden.c" \
     "$REPO/vid.stab_amd/csrc/synth/vsynth.c" \
     -L"$REPO/oracle/_ref" -Wl,-rpath,"$REPO/oracle/_ref" -lvsref -lm
-# make_golden.sh [planar|packed [case index]]: one fixture set, or both
+# make_golden.sh [planar|packed|show [case index]]: one fixture set, or all
 case "$1" in
   planar) "$REPO/oracle/_ref/make_golden" "$HERE/detect_golden.json" planar $2 ;;
   packed) "$REPO/oracle/_ref/make_golden" "$HERE/packed_golden.json" packed $2 ;;
+  show) "$REPO/oracle/_ref/make_golden" "$HERE/show_golden.json" show $2 ;;
   *) "$REPO/oracle/_ref/make_golden" "$HERE/detect_golden.json" planar
-     "$REPO/oracle/_ref/make_golden" "$HERE/packed_golden.json" packed ;;
+     "$REPO/oracle/_ref/make_golden" "$HERE/packed_golden.json" packed
+     "$REPO/oracle/_ref/make_golden" "$HERE/show_golden.json" show ;;
 esac

@@ -27,9 +27,18 @@ VSGStageDesc vsd_stage_desc(const VSMotionDetectFields* fs);
 int vsd_frame_coarse(const VSMotionDetect* md, const VSGBatchView* v, int i, int fnum, int slot, int prevSlot,
                      LocalMotions* coarse, VSTransform* offset, VSGJob* jobs, int* jobSel);
 
-/* Frame i: fine results of its njobs jobs -> the cleaned fine motions, then
-   out = coarse ++ fine (the elements of coarse move into out).  Reads md only. */
+/* Frame i: fine results of its njobs jobs -> the cleaned fine motions; with
+   md->conf.show > 0 and a frame to draw into (plane 0, linesize bytes a
+   row; null: none), the overlays; then out = coarse ++ fine (the elements of
+   coarse move into out).  Reads md only. */
 void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
-                    const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, LocalMotions* out);
+                    const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, uint8_t* draw,
+                    int drawLinesize, LocalMotions* out);
+
+/* the show overlays of one frame (src/motiondetect.c:271-286, 816-952) into
+   plane 0 of the caller's frame; may rewrite the field position of motions
+   whose line the reference draws from its end (vs_draw.c) */
+void vsd_draw_overlays(const VSMotionDetect* md, uint8_t* img, int linesize, LocalMotions* coarse,
+                       LocalMotions* fine);
 
 #endif /* VS_DETECT_INTERNAL_H */

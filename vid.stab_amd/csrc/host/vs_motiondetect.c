@@ -371,7 +371,8 @@ int vsd_frame_coarse(const VSMotionDetect* md, const VSGBatchView* v, int i, int
 }
 
 void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
-                    const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, LocalMotions* out) {
+                    const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, uint8_t* draw,
+                    int drawLinesize, LocalMotions* out) {
   const VSMotionDetectFields* ff = &md->fieldsfine;
   LocalMotions fine;
   vs_vector_init(&fine, 0);
@@ -390,6 +391,7 @@ void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, Loca
       if (lm_match_better(&meanMatch, &lm)) vs_vector_append_dup(&fine, &lm, sizeof(LocalMotion));
     }
   }
+  if (draw) vsd_draw_overlays(md, draw, drawLinesize, coarse, &fine);
   *out = vs_vector_concat(coarse, &fine);
   vs_vector_fini(coarse);
   vs_vector_fini(&fine);
@@ -400,6 +402,8 @@ void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, Loca
 typedef struct {
   int set, n, slot0;
   int fnum0;                   /* frame number of the batch's first frame */
+  uint8_t* draw[MD_MAX_BATCH]; /* show > 0: plane 0 of the caller's frames (null: no overlays) */
+  int drawLs[MD_MAX_BATCH];
   int prevSlot[MD_MAX_BATCH];
   uint8_t detect[MD_MAX_BATCH];
   LocalMotions* out;
@@ -417,7 +421,7 @@ typedef struct {
 /* decides, for the next n frames, which are measured and against which plane
    (src/motiondetect.c:211-218, 294-303), and submits them */
 static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t* const* luma,
-                        const int* linesize, int onDevice, int n, LocalMotions* out) {
+                        const int* linesize, int onDevice, uint8_t* const* draw, int n, LocalMotions* out) {
   const int tripod = md->conf.virtualTripod;
   b->set = s->nextSet;
   s->nextSet = (s->nextSet + 1) % VSG_NSETS;
@@ -436,6 +440,8 @@ static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t*
     /* tripod: every measured frame compares against frame tripod-1, kept as
        the stored reference plane; otherwise against the frame before */
     b->prevSlot[i] = tripod >= 1 ? -1 : (i == 0 ? s->lastSlot : b->slot0 + i - 1);
+    b->draw[i] = draw ? draw[i] : NULL;
+    b->drawLs[i] = linesize[i];
     if (tripod >= 1 && fnum == tripod - 1) refSlot = b->slot0 + i;
   }
   if (vsg_submit(s->gpu, b->set, n, luma, linesize, onDevice, b->prevSlot, b->detect, refSlot) != 0) {
@@ -479,7 +485,7 @@ static void frame_fine_fn(void* ctx, int i) {
   }
   const int j0 = b->jobStart[i];
   vsd_frame_fine(w->md, w->v, i, &b->coarse[i], b->jobs + j0, b->jobSel + j0, w->res + j0,
-                 b->jobStart[i + 1] - j0, &b->out[i]);
+                 b->jobStart[i + 1] - j0, b->draw[i], b->drawLs[i], &b->out[i]);
 }
 
 /* coarse results -> coarse LocalMotions, transform, fine jobs; submits them */
@@ -538,7 +544,7 @@ static int batch_fine(VSMotionDetect* md, MDState* s, Batch* b) {
    fine search), and batch k+1 waits with its coarse results. */
 #define MD_INFLIGHT 3
 static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_t* const* luma,
-                         const int* linesize, int onDevice, int n) {
+                         const int* linesize, int onDevice, uint8_t* const* draw, int n) {
   if (!md || md->initialized != 2 || !motions || n < 0) return VS_ERROR;
   MDState* s = state_find(md);
   if (!s) {
@@ -566,7 +572,8 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
   int ok = 1, done = 0;
   for (int k = 0; k < nb && k < 2 && ok; k++) {
     const int f0 = k * B, cnt = VS_MIN(B, n - f0);
-    ok = batch_submit(md, s, &bs[k], luma + f0, linesize + f0, onDevice, cnt, motions + f0) == VS_OK;
+    ok = batch_submit(md, s, &bs[k], luma + f0, linesize + f0, onDevice, draw ? draw + f0 : NULL, cnt,
+                      motions + f0) == VS_OK;
   }
   for (int k = 0; k < nb && ok; k++) {
     Batch* b = &bs[k % MD_INFLIGHT];
@@ -576,8 +583,8 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
     }
     if (k + 2 < nb) {
       const int f0 = (k + 2) * B, cnt = VS_MIN(B, n - f0);
-      if (batch_submit(md, s, &bs[(k + 2) % MD_INFLIGHT], luma + f0, linesize + f0, onDevice, cnt,
-                       motions + f0) != VS_OK) {
+      if (batch_submit(md, s, &bs[(k + 2) % MD_INFLIGHT], luma + f0, linesize + f0, onDevice,
+                       draw ? draw + f0 : NULL, cnt, motions + f0) != VS_OK) {
         ok = 0;
         break;
       }
@@ -604,27 +611,32 @@ VS_API int vsMotionDetection(VSMotionDetect* md, LocalMotions* motions, VSFrame*
   if (!md || !frame || md->initialized != 2) return VS_ERROR;
   md->currorig = *frame;
   const uint8_t* luma = frame->data[0];
+  uint8_t* draw = frame->data[0];
   int ls = frame->linesize[0];
-  return detect_frames(md, motions, &luma, &ls, 0, 1);
+  return detect_frames(md, motions, &luma, &ls, 0, md->conf.show > 0 ? &draw : NULL, 1);
 }
 
 VS_API int vsMotionDetectionBatch(VSMotionDetect* md, LocalMotions* motions, VSFrame* frames, int n) {
   if (!md || !frames || n < 0) return VS_ERROR;
   if (n == 0) return VS_OK;
   const uint8_t** luma = (const uint8_t**)malloc(sizeof(uint8_t*) * (size_t)n);
+  uint8_t** draw = (uint8_t**)malloc(sizeof(uint8_t*) * (size_t)n);
   int* ls = (int*)malloc(sizeof(int) * (size_t)n);
-  if (!luma || !ls) {
+  if (!luma || !draw || !ls) {
     free(luma);
+    free(draw);
     free(ls);
     return VS_ERROR;
   }
   for (int i = 0; i < n; i++) {
     luma[i] = frames[i].data[0];
+    draw[i] = frames[i].data[0];
     ls[i] = frames[i].linesize[0];
   }
   md->currorig = frames[n - 1];
-  int r = detect_frames(md, motions, luma, ls, 0, n);
+  int r = detect_frames(md, motions, luma, ls, 0, md->conf.show > 0 ? draw : NULL, n);
   free(luma);
+  free(draw);
   free(ls);
   return r;
 }
@@ -632,7 +644,8 @@ VS_API int vsMotionDetectionBatch(VSMotionDetect* md, LocalMotions* motions, VSF
 VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions, const uint8_t* const* lumaDev,
                                    const int* linesize, int n) {
   if (n == 0) return VS_OK;
-  return detect_frames(md, motions, lumaDev, linesize, 1, n);
+  /* device-resident frames get no overlays: show applies to host frames */
+  return detect_frames(md, motions, lumaDev, linesize, 1, NULL, n);
 }
 
 /* bulk release on a library-wide pool: the vectors are independent */

@@ -112,7 +112,7 @@ static void fine_fn(void* ctx, int k) {
   }
   const int j0 = cb->jobStart[i];
   vsd_frame_fine(&cl->md, &cb->view, i, &cb->coarse[i], cb->jobs + j0, cb->jobSel + j0, cb->res + j0,
-                 cb->jobStart[i + 1] - j0, out);
+                 cb->jobStart[i + 1] - j0, NULL, 0, out);
 }
 
 /* next frames of every clip into set `set` */

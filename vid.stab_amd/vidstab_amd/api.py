@@ -158,7 +158,7 @@ class Detector:
     """One vsMotionDetect stream on the GPU, driven through the C API."""
 
     def __init__(self, width, height, shakiness=5, accuracy=15, stepsize=6, mincontrast=0.25,
-                 tripod=0, pformat=PF_YUV420P, mod=b"vidstab_amd"):
+                 tripod=0, pformat=PF_YUV420P, mod=b"vidstab_amd", show=0):
         L = lib()
         self.L = L
         self.fi = VSFrameInfo()
@@ -170,9 +170,11 @@ class Detector:
         conf.shakiness, conf.accuracy, conf.stepSize = shakiness, accuracy, stepsize
         conf.contrastThreshold = float(mincontrast)
         conf.virtualTripod = tripod
+        conf.show = show
         if L.vsMotionDetectInit(ctypes.byref(self.md), ctypes.byref(conf), ctypes.byref(self.fi)) != 0:
             raise RuntimeError("vsMotionDetectInit failed: " + L.vsg_last_error().decode())
         self.width, self.height = width, height
+        self.show = show
         self.packed = pformat > PF_PACKED
         self.bpp = self.fi.bytesPerPixel if self.packed else 1
         self._chroma = np.full((height // 2) * (width // 2), 128, dtype=np.uint8)
@@ -189,6 +191,10 @@ class Detector:
         return fr
 
     def detect(self, luma):
+        """one vsMotionDetection call; with show > 0 the overlays are drawn into
+        luma, which must then be a C-contiguous uint8 array"""
+        if self.show:
+            assert luma.dtype == np.uint8 and luma.flags.c_contiguous
         luma = np.ascontiguousarray(luma, dtype=np.uint8)
         fr = self._frame(luma)
         vec = VSVector()
@@ -199,6 +205,8 @@ class Detector:
         return out
 
     def detect_batch(self, lumas, keep=True):
+        if self.show:
+            assert all(x.dtype == np.uint8 and x.flags.c_contiguous for x in lumas)
         lumas = [np.ascontiguousarray(x, dtype=np.uint8) for x in lumas]
         frames = (VSFrame * len(lumas))(*[self._frame(x) for x in lumas])
         vecs = (VSVector * len(lumas))()
