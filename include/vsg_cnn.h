@@ -37,10 +37,18 @@ VS_API int vsgc_conv_k(const uint16_t* in, int n, int h, int w, const uint16_t* 
 /* The same convolution on the gfx950 K = 16 matrix instruction
    (v_mfma_f32_32x32x16_bf16), used by the learned detect path: weights
    packed by vsgc_pack_weights16 (9*2*64*8 BF16 values, forward only);
-   inChannels <= 16 uses one K step per tap. */
+   inChannels <= 16 uses one K step per tap.  Dilations up to 8 run the
+   LDS-tiled kernel (4 output rows spaced by the dilation per wave), larger
+   ones the direct kernel; both accumulate the taps in the same order, so
+   their results are identical. */
 VS_API int vsgc_pack_weights16(const float* w, uint16_t* packed16, void* stream);
 VS_API int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* packed16, const float* bias,
                        int dil, int mode, int inChannels, uint16_t* outb, float* outf, void* stream);
+/* vsgc_conv16 on the direct kernel (one 32-px row tile per wave, every A
+   fragment loaded from global memory) at any dilation */
+VS_API int vsgc_conv16_direct(const uint16_t* in, int n, int h, int w, const uint16_t* packed16,
+                              const float* bias, int dil, int mode, int inChannels, uint16_t* outb, float* outf,
+                              void* stream);
 
 /* dW += d(loss)/dW and dB += d(loss)/dB of a layer from its input and the
    gradient dz at its pre-activation (accumulates; dB may be NULL) */

@@ -133,6 +133,42 @@ def test_narrow_input_paths_match_full_width():
         assert np.max(np.abs(out.cpu().numpy().astype(np.float64) - ref)) <= 1e-3 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("dil,cin,mode", [(1, 32, 0), (2, 32, 0), (4, 32, 1), (8, 32, 2), (1, 2, 0), (3, 32, 0)])
+def test_tiled_conv_equals_direct_kernel(dil, cin, mode):
+    """the LDS-tiled kernel (4 rows spaced by the dilation per wave, several
+    column bands and row tiles, image edges inside tiles) accumulates the taps
+    in the direct kernel's order: outputs are bit-identical"""
+    from vidstab_amd import learned
+    torch = _torch()
+    L = learned.lib()
+    rng = np.random.default_rng(40 + dil + cin)
+    n, h, w = 3, 37, 300
+    x = np.zeros((n, h, w, 32))
+    x[..., :cin] = co.bf16_round(rng.standard_normal((n, h, w, cin)).astype(np.float32))
+    wt = co.bf16_round(rng.standard_normal((9, 32, 32)).astype(np.float32) * 0.2)
+    b = rng.standard_normal(32).astype(np.float32) * 0.1
+    st = learned._stream()
+    dx = _dev_bf16(x)
+    pk16 = torch.empty(9 * 2 * 64 * 8, dtype=torch.int16, device="cuda")
+    assert L.vsgc_pack_weights16(learned._p(torch.from_numpy(wt.astype(np.float32)).cuda()), learned._p(pk16), st) == 0
+    db = torch.from_numpy(b).cuda()
+    outs = []
+    for fn in (L.vsgc_conv16, L.vsgc_conv16_direct):
+        if mode == 2:
+            o = torch.full((n, h, w, 32), 7.0, dtype=torch.float32, device="cuda")
+            assert fn(learned._p(dx), n, h, w, learned._p(pk16), learned._p(db), dil, 2, cin, None, learned._p(o), st) == 0
+        else:
+            o = torch.full((n, h, w, 32), 7, dtype=torch.int16, device="cuda")
+            assert fn(learned._p(dx), n, h, w, learned._p(pk16), learned._p(db), dil, mode, cin, learned._p(o), None,
+                      st) == 0
+        outs.append(o.cpu().numpy())
+    assert outs[0].view(np.uint8).tobytes() == outs[1].view(np.uint8).tobytes()
+    # and against the oracle
+    ref = co.conv(x, wt, b.astype(np.float64), dil)
+    if mode == 2:
+        assert np.max(np.abs(outs[0].astype(np.float64) - ref)) <= 1e-3 * np.max(np.abs(ref))
+
+
 def test_network_loss_and_gradients_match_oracle():
     """one training step's gradients of the 6-layer network vs the oracle run
     with the device's BF16 rounding points"""

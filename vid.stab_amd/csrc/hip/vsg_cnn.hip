@@ -185,6 +185,113 @@ __global__ __launch_bounds__(256) void k_conv16(const uint16_t* __restrict__ in,
   }
 }
 
+/* ---- tiled convolution (inference) -----------------------------------------------------
+   A workgroup of 4 waves computes R output rows y0 + k*dil (k < R) of a
+   128-px column band of one image; wave w takes the 32 px x0 + 32w + (0..31).
+   The R + 2 input rows those rows read (y0 + (j-1)*dil, j < R + 2) and the
+   band plus a dil-px halo on either side are staged in LDS once, so the
+   workgroup reads each input pixel from HBM about (R+2)/R times instead of
+   once per tap.  Rows spaced by the dilation make the vertical taps of
+   different output rows read the same input row: an A fragment read from LDS
+   (one ds_read_b128: 8 channels of one pixel per lane) feeds the MFMAs of up
+   to three output rows, (R+2)*3*KS reads for R*9*KS MFMAs.
+   LDS pixel p (64 bytes, 4 chunks of 8 channels) keeps chunk c at position
+   c ^ ((p >> 1) & 3): the 8 lanes a ds_read_b128 serves together then read 8
+   distinct 16-byte bank groups. */
+constexpr int kTileW = 128;
+constexpr int kTileR = 4;
+constexpr int kMaxTileDil = 8;
+
+__device__ __forceinline__ int lds_chunk(int p, int c) { return p * 64 + 16 * (c ^ ((p >> 1) & 3)); }
+
+template <int KS, int R>
+__global__ __launch_bounds__(256) void k_conv16t(const uint16_t* __restrict__ in, int N, int H, int W,
+                                                 const uint16_t* __restrict__ pk, const float* __restrict__ bias,
+                                                 int dil, int mode, uint16_t* __restrict__ outb,
+                                                 float* __restrict__ outf, int rowTiles, int xTiles) {
+  extern __shared__ __align__(16) uint8_t tile[];
+  const int Wt = kTileW + 2 * dil;
+  int b = blockIdx.x;
+  const int xt = b % xTiles;
+  b /= xTiles;
+  const int ph = b % dil;
+  b /= dil;
+  const int rt = b % rowTiles;
+  const int n = b / rowTiles;
+  const int y0 = rt * R * dil + ph;
+  const int x0 = xt * kTileW;
+
+  /* stage: chunk c (8 channels, 16 bytes) of pixel p of row j; zeros outside
+     the image are the convolution's padding */
+  constexpr int NCH = 2 * KS;
+  const int total = (R + 2) * Wt * NCH;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int c = i % NCH, q = i / NCH;
+    const int px = q % Wt, j = q / Wt;
+    const int y = y0 + (j - 1) * dil, x = x0 - dil + px;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (y >= 0 && y < H && x >= 0 && x < W)
+      v = *reinterpret_cast<const uint4*>(in + (((size_t)n * H + y) * W + x) * kC + 8 * c);
+    *reinterpret_cast<uint4*>(tile + (size_t)j * Wt * 64 + lds_chunk(px, c)) = v;
+  }
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5;
+  v8bf bq[9 * KS];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      bq[tap * KS + ks] = *reinterpret_cast<const v8bf*>(pk + ((size_t)(tap * 2 + ks) * 64 + lane) * 8);
+  const int co = lane & 31;
+  const float bco = bias ? bias[co] : 0.0f;
+  __syncthreads();
+
+  v16f acc[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) acc[k] = v16f{};
+  const int pxBase = wave * 32 + (lane & 31);   /* staged pixel of tap dx = -dil */
+#pragma unroll
+  for (int j = 0; j < R + 2; ++j) {
+#pragma unroll
+    for (int tx = 0; tx < 3; ++tx) {
+      const int px = pxBase + tx * dil;
+      const uint8_t* row = tile + (size_t)j * Wt * 64;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const v8bf a = *reinterpret_cast<const v8bf*>(row + lds_chunk(px, 2 * ks + half));
+#pragma unroll
+        for (int ty = 0; ty < 3; ++ty) {
+          const int k = j - ty;   /* output row k reads input row k + ty */
+          if (k >= 0 && k < R)
+            acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[(ty * 3 + tx) * KS + ks], acc[k], 0, 0, 0);
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int y = y0 + k * dil;
+    if (y >= H) break;
+    const size_t rowBase = ((size_t)n * H + y) * W;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int m = 8 * (v >> 2) + 4 * half + (v & 3);
+      const int px = x0 + wave * 32 + m;
+      if (px < W) {
+        float val = acc[k][v] + bco;
+        const size_t o = (rowBase + px) * kC + co;
+        if (mode == 2) {
+          outf[o] = val;
+        } else {
+          if (mode == 0) val = val > 0.0f ? val : 0.0f;
+          outb[o] = f2bf(val);
+        }
+      }
+    }
+  }
+}
+
 /* ---- weight gradient ----------------------------------------------------------------
    dW[tap][ci][co] += sum_p in[p + tap*dil][ci] * dz[p][co] for taps
    tap0 .. tap0+2: an MFMA GEMM with M = ci, N = co, K = pixels (8 per step). */
@@ -460,6 +567,28 @@ extern "C" int vsgc_pack_weights16(const float* w, uint16_t* packed, void* strea
 
 extern "C" int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* packed16, const float* bias,
                            int dil, int mode, int inChannels, uint16_t* outb, float* outf, void* stream) {
+  if (n <= 0 || h <= 0 || w <= 0 || dil < 1 || mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
+  if ((mode == 2 && !outf) || (mode != 2 && !outb)) return (int)hipErrorInvalidValue;
+  if (dil <= kMaxTileDil) {
+    const int rowTiles = (h + kTileR * dil - 1) / (kTileR * dil);
+    const int xTiles = (w + kTileW - 1) / kTileW;
+    const long long blocks = (long long)n * rowTiles * dil * xTiles;
+    if (blocks > 0x7FFFFFFFLL) return (int)hipErrorInvalidValue;
+    const size_t lds = (size_t)(kTileR + 2) * (kTileW + 2 * dil) * 64;
+    if (inChannels <= 16)
+      hipLaunchKernelGGL((k_conv16t<1, kTileR>), dim3((unsigned)blocks), dim3(256), lds, (hipStream_t)stream, in, n,
+                         h, w, packed16, bias, dil, mode, outb, outf, rowTiles, xTiles);
+    else
+      hipLaunchKernelGGL((k_conv16t<2, kTileR>), dim3((unsigned)blocks), dim3(256), lds, (hipStream_t)stream, in, n,
+                         h, w, packed16, bias, dil, mode, outb, outf, rowTiles, xTiles);
+    CNN_RET;
+  }
+  return vsgc_conv16_direct(in, n, h, w, packed16, bias, dil, mode, inChannels, outb, outf, stream);
+}
+
+extern "C" int vsgc_conv16_direct(const uint16_t* in, int n, int h, int w, const uint16_t* packed16,
+                                  const float* bias, int dil, int mode, int inChannels, uint16_t* outb, float* outf,
+                                  void* stream) {
   if (n <= 0 || h <= 0 || w <= 0 || dil < 1 || mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
   if ((mode == 2 && !outf) || (mode != 2 && !outb)) return (int)hipErrorInvalidValue;
   if (inChannels <= 16)

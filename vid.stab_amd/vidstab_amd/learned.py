@@ -37,6 +37,7 @@ def lib():
         L.vsgc_conv_k.argtypes = [vp, i32, i32, i32, vp, vp, i32, i32, i32, vp, vp, vp]
         L.vsgc_pack_weights16.argtypes = [vp, vp, vp]
         L.vsgc_conv16.argtypes = [vp, i32, i32, i32, vp, vp, i32, i32, i32, vp, vp, vp]
+        L.vsgc_conv16_direct.argtypes = [vp, i32, i32, i32, vp, vp, i32, i32, i32, vp, vp, vp]
         L.vsgc_conv_wgrad.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, vp]
         L.vsgc_relu_mask.argtypes = [vp, vp, i64, vp]
         L.vsgc_f32_to_bf16.argtypes = [vp, vp, i64, vp]
