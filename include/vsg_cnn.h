@@ -13,6 +13,7 @@
 #ifndef VSG_CNN_H
 #define VSG_CNN_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include "vidstab_api.h"
 
@@ -38,12 +39,22 @@ VS_API int vsgc_conv_k(const uint16_t* in, int n, int h, int w, const uint16_t* 
    (v_mfma_f32_32x32x16_bf16), used by the learned detect path: weights
    packed by vsgc_pack_weights16 (9*2*64*8 BF16 values, forward only);
    inChannels <= 16 uses one K step per tap.  Dilations up to 8 run the
-   LDS-tiled kernel (4 output rows spaced by the dilation per wave), larger
-   ones the direct kernel; both accumulate the taps in the same order, so
-   their results are identical. */
+   LDS-tiled kernel (4 output rows spaced by the dilation per workgroup),
+   larger ones the direct kernel; both accumulate the taps in the same order,
+   so their results are identical.  mode 3 (tiled dilations only) writes FP32
+   channels 0 and 1 alone, as an NHWC-2 map [n][h][w][2]. */
 VS_API int vsgc_pack_weights16(const float* w, uint16_t* packed16, void* stream);
 VS_API int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* packed16, const float* bias,
                        int dil, int mode, int inChannels, uint16_t* outb, float* outf, void* stream);
+/* vsgc_conv16 of the network's first layer with its input built on the fly
+   from quarter-resolution BF16 planes (planeStride elements apart, each
+   h x w): sample z has channel 0 = plane prevIdx[z] (refPlane when
+   prevIdx[z] < 0; prevIdx is a device array of n ints), channel 1 = plane
+   curIdx0 + z and every other channel zero -- the result equals vsgc_conv16
+   with inChannels = 2 on that NHWC-32 map.  dil <= 8. */
+VS_API int vsgc_conv16_pair(const uint16_t* planes, size_t planeStride, const uint16_t* refPlane,
+                            const int* prevIdx, int curIdx0, int n, int h, int w, const uint16_t* packed16,
+                            const float* bias, int dil, int mode, uint16_t* outb, float* outf, void* stream);
 /* vsgc_conv16 on the direct kernel (one 32-px row tile per wave, every A
    fragment loaded from global memory) at any dilation */
 VS_API int vsgc_conv16_direct(const uint16_t* in, int n, int h, int w, const uint16_t* packed16,
@@ -66,6 +77,8 @@ VS_API int vsgc_downsample(const uint8_t* plane, int pitch, int width, int heigh
 /* windows: nf x {sample, x0, y0, x1, y1} (half-open) on the map;
    pred[f] = mean of channels 0, 1 of the FP32 map over window f */
 VS_API int vsgc_field_pool(const float* map, int h, int w, const int* windows, int nf, float* pred, void* stream);
+/* the same over an FP32 NHWC-2 map (vsgc_conv16 mode 3) */
+VS_API int vsgc_field_pool2(const float* map, int h, int w, const int* windows, int nf, float* pred, void* stream);
 /* dmap (FP32, zeroed by the caller) += dpred[f] / area over window f */
 VS_API int vsgc_field_pool_bwd(const float* dpred, int h, int w, const int* windows, int nf, float* dmap,
                                void* stream);

@@ -133,7 +133,8 @@ def test_narrow_input_paths_match_full_width():
         assert np.max(np.abs(out.cpu().numpy().astype(np.float64) - ref)) <= 1e-3 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("dil,cin,mode", [(1, 32, 0), (2, 32, 0), (4, 32, 1), (8, 32, 2), (1, 2, 0), (3, 32, 0)])
+@pytest.mark.parametrize("dil,cin,mode", [(1, 32, 0), (2, 32, 0), (4, 32, 1), (8, 32, 2), (1, 2, 0), (3, 32, 0),
+                                          (5, 16, 1), (7, 32, 0)])
 def test_tiled_conv_equals_direct_kernel(dil, cin, mode):
     """the LDS-tiled kernel (4 rows spaced by the dilation per wave, several
     column bands and row tiles, image edges inside tiles) accumulates the taps
@@ -167,6 +168,89 @@ def test_tiled_conv_equals_direct_kernel(dil, cin, mode):
     ref = co.conv(x, wt, b.astype(np.float64), dil)
     if mode == 2:
         assert np.max(np.abs(outs[0].astype(np.float64) - ref)) <= 1e-3 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("dil,mode", [(1, 0), (1, 3), (2, 1), (8, 2)])
+def test_pair_input_layer_equals_conv_on_the_built_map(dil, mode):
+    """vsgc_conv16_pair builds channel 0 (a previous plane, or the reference
+    plane for a negative index) and channel 1 (the sample's own plane) in the
+    staging step; its outputs equal vsgc_conv16 with inChannels = 2 on the
+    NHWC-32 map holding those planes, bit for bit, in every output mode"""
+    from vidstab_amd import learned
+    torch = _torch()
+    L = learned.lib()
+    P = learned._p
+    rng = np.random.default_rng(70 + dil + mode)
+    n, h, w, nplanes = 4, 29, 261, 7
+    planes = co.bf16_round(rng.standard_normal((nplanes, h, w)).astype(np.float32))
+    ref_plane = co.bf16_round(rng.standard_normal((h, w)).astype(np.float32))
+    prev = np.array([-1, 2, 6, 0], dtype=np.int32)
+    cur0 = 3
+    x = np.zeros((n, h, w, 32), np.float32)
+    for z in range(n):
+        x[z, :, :, 0] = ref_plane if prev[z] < 0 else planes[prev[z]]
+        x[z, :, :, 1] = planes[cur0 + z]
+    wt = co.bf16_round(rng.standard_normal((9, 32, 32)).astype(np.float32) * 0.2)
+    b = rng.standard_normal(32).astype(np.float32) * 0.1
+    st = learned._stream()
+    pk16 = torch.empty(9 * 2 * 64 * 8, dtype=torch.int16, device="cuda")
+    assert L.vsgc_pack_weights16(P(torch.from_numpy(wt.astype(np.float32)).cuda()), P(pk16), st) == 0
+    db = torch.from_numpy(b).cuda()
+    dplanes = torch.from_numpy(learned.bf16_bits(planes)).cuda()
+    dref = torch.from_numpy(learned.bf16_bits(ref_plane)).cuda()
+    dprev = torch.from_numpy(prev).cuda()
+    dx = _dev_bf16(x)
+    oc = {0: 32, 1: 32, 2: 32, 3: 2}[mode]
+    outs = []
+    for pair in (True, False):
+        if mode >= 2:
+            o = torch.full((n, h, w, oc), 7.0, dtype=torch.float32, device="cuda")
+            ob, of = None, P(o)
+        else:
+            o = torch.full((n, h, w, oc), 7, dtype=torch.int16, device="cuda")
+            ob, of = P(o), None
+        if pair:
+            rc = L.vsgc_conv16_pair(P(dplanes), h * w, P(dref), P(dprev), cur0, n, h, w, P(pk16), P(db), dil, mode,
+                                    ob, of, st)
+        else:
+            rc = L.vsgc_conv16(P(dx), n, h, w, P(pk16), P(db), dil, mode, 2, ob, of, st)
+        assert rc == 0
+        outs.append(o.cpu().numpy())
+    assert outs[0].view(np.uint8).tobytes() == outs[1].view(np.uint8).tobytes()
+    ref = co.conv(x.astype(np.float64), wt, b.astype(np.float64), dil)
+    if mode >= 2:
+        assert np.max(np.abs(outs[0].astype(np.float64) - ref[..., :oc])) <= 1e-3 * np.max(np.abs(ref))
+
+
+def test_two_channel_output_and_pooling():
+    """mode 3 writes channels 0 and 1 of mode 2's FP32 output, and
+    vsgc_field_pool2 over it equals vsgc_field_pool over the 32-channel map"""
+    from vidstab_amd import learned
+    torch = _torch()
+    L = learned.lib()
+    P = learned._p
+    rng = np.random.default_rng(91)
+    n, h, w = 2, 40, 150
+    x = _rand_map(rng, n, h, w)
+    wt = co.bf16_round(rng.standard_normal((9, 32, 32)).astype(np.float32) * 0.1)
+    b = rng.standard_normal(32).astype(np.float32) * 0.1
+    st = learned._stream()
+    pk16 = torch.empty(9 * 2 * 64 * 8, dtype=torch.int16, device="cuda")
+    assert L.vsgc_pack_weights16(P(torch.from_numpy(wt.astype(np.float32)).cuda()), P(pk16), st) == 0
+    db = torch.from_numpy(b).cuda()
+    dx = _dev_bf16(x)
+    o32 = torch.empty((n, h, w, 32), dtype=torch.float32, device="cuda")
+    o2 = torch.empty((n, h, w, 2), dtype=torch.float32, device="cuda")
+    assert L.vsgc_conv16(P(dx), n, h, w, P(pk16), P(db), 4, 2, 32, None, P(o32), st) == 0
+    assert L.vsgc_conv16(P(dx), n, h, w, P(pk16), P(db), 4, 3, 32, None, P(o2), st) == 0
+    assert (o32[..., :2].cpu().numpy().view(np.uint32) == o2.cpu().numpy().view(np.uint32)).all()
+    win = np.array([[0, 3, 2, 40, 30], [1, 100, 20, 150, 40], [1, -5, -5, 7, 9], [0, 60, 10, 61, 11]], np.int32)
+    dwin = torch.from_numpy(win).cuda()
+    p32 = torch.empty((4, 2), dtype=torch.float32, device="cuda")
+    p2 = torch.empty((4, 2), dtype=torch.float32, device="cuda")
+    assert L.vsgc_field_pool(P(o32), h, w, P(dwin), 4, P(p32), st) == 0
+    assert L.vsgc_field_pool2(P(o2), h, w, P(dwin), 4, P(p2), st) == 0
+    assert (p32.cpu().numpy().view(np.uint32) == p2.cpu().numpy().view(np.uint32)).all()
 
 
 def test_network_loss_and_gradients_match_oracle():

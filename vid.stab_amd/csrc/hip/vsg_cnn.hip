@@ -191,59 +191,137 @@ __global__ __launch_bounds__(256) void k_conv16(const uint16_t* __restrict__ in,
    The R + 2 input rows those rows read (y0 + (j-1)*dil, j < R + 2) and the
    band plus a dil-px halo on either side are staged in LDS once, so the
    workgroup reads each input pixel from HBM about (R+2)/R times instead of
-   once per tap.  Rows spaced by the dilation make the vertical taps of
-   different output rows read the same input row: an A fragment read from LDS
-   (one ds_read_b128: 8 channels of one pixel per lane) feeds the MFMAs of up
-   to three output rows, (R+2)*3*KS reads for R*9*KS MFMAs.
+   once per tap.  Every staging load of a thread is issued before the first
+   LDS write, so a workgroup waits for one memory latency, not one per load.
+   Rows spaced by the dilation make the vertical taps of different output rows
+   read the same input row: an A fragment read from LDS (one ds_read_b128:
+   8 channels of one pixel per lane) feeds the MFMAs of up to three output
+   rows, (R+2)*3*KS reads for R*9*KS MFMAs.
    LDS pixel p (64 bytes, 4 chunks of 8 channels) keeps chunk c at position
    c ^ ((p >> 1) & 3): the 8 lanes a ds_read_b128 serves together then read 8
-   distinct 16-byte bank groups. */
+   distinct 16-byte bank groups.
+   BF16 outputs go back through LDS (the staged rows are dead by then) and
+   leave as 16-byte stores of 8 channels; FP32 outputs are stored directly.
+   With PAIR the input is not an NHWC map: channel 0 of sample z is the
+   quarter-resolution plane prevIdx[z] (or refPlane when it is negative),
+   channel 1 the plane curIdx0 + z, every other channel zero (the network's
+   first layer, KS = 1).
+   Block b runs logical tile (b mod 8) * ceil(tiles / 8) + b / 8: blocks on
+   one XCD take consecutive row tiles of one band, whose staged rows overlap,
+   so the overlap is served by that XCD's L2. */
 constexpr int kTileW = 128;
 constexpr int kTileR = 4;
 constexpr int kMaxTileDil = 8;
+constexpr int kRowPad = kTileW + 2 * kMaxTileDil;   /* staged row stride in pixels, a multiple of 16 */
 
 __device__ __forceinline__ int lds_chunk(int p, int c) { return p * 64 + 16 * (c ^ ((p >> 1) & 3)); }
 
-template <int KS, int R>
-__global__ __launch_bounds__(256) void k_conv16t(const uint16_t* __restrict__ in, int N, int H, int W,
-                                                 const uint16_t* __restrict__ pk, const float* __restrict__ bias,
-                                                 int dil, int mode, uint16_t* __restrict__ outb,
-                                                 float* __restrict__ outf, int rowTiles, int xTiles) {
+/* read by staging loads that fall outside the image (the zero padding): the
+   loads need no select afterwards, so none of them waits before the last */
+__device__ __align__(16) uint32_t kZeroPad[4] = {0, 0, 0, 0};
+
+struct ConvTileArgs {
+  const uint16_t* in;          /* NHWC-32 input (PAIR: unused) */
+  const uint16_t* planes;      /* PAIR: quarter-resolution planes, planeStride elements apart */
+  size_t planeStride;
+  const uint16_t* refPlane;
+  const int* prevIdx;
+  int curIdx0;
+  int N, H, W;
+  const uint16_t* pk;
+  const float* bias;
+  int dil, mode;
+  uint16_t* outb;
+  float* outf;
+  int rowTiles, xTiles, tiles, perXcd;
+};
+
+/* the staged tile (55 KB) allows two workgroups per CU, so two waves per
+   SIMD is the occupancy to plan registers for */
+template <int KS, bool PAIR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv16t(const ConvTileArgs a) {
+  constexpr int R = kTileR;
   extern __shared__ __align__(16) uint8_t tile[];
+  const int lt = (int)(blockIdx.x & 7) * a.perXcd + (int)(blockIdx.x >> 3);
+  if (lt >= a.tiles) return;
+  const int dil = a.dil, H = a.H, W = a.W;
   const int Wt = kTileW + 2 * dil;
-  int b = blockIdx.x;
-  const int xt = b % xTiles;
-  b /= xTiles;
+  int b = lt;
+  const int rt = b % a.rowTiles;
+  b /= a.rowTiles;
+  const int xt = b % a.xTiles;
+  b /= a.xTiles;
   const int ph = b % dil;
-  b /= dil;
-  const int rt = b % rowTiles;
-  const int n = b / rowTiles;
+  const int n = b / dil;
   const int y0 = rt * R * dil + ph;
   const int x0 = xt * kTileW;
 
-  /* stage: chunk c (8 channels, 16 bytes) of pixel p of row j; zeros outside
-     the image are the convolution's padding */
-  constexpr int NCH = 2 * KS;
-  const int total = (R + 2) * Wt * NCH;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int c = i % NCH, q = i / NCH;
-    const int px = q % Wt, j = q / Wt;
-    const int y = y0 + (j - 1) * dil, x = x0 - dil + px;
-    uint4 v = make_uint4(0, 0, 0, 0);
-    if (y >= 0 && y < H && x >= 0 && x < W)
-      v = *reinterpret_cast<const uint4*>(in + (((size_t)n * H + y) * W + x) * kC + 8 * c);
-    *reinterpret_cast<uint4*>(tile + (size_t)j * Wt * 64 + lds_chunk(px, c)) = v;
-  }
-
+  /* the layer's B fragments (weights) and bias, loaded with the staging loads */
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5;
   v8bf bq[9 * KS];
 #pragma unroll
   for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
-      bq[tap * KS + ks] = *reinterpret_cast<const v8bf*>(pk + ((size_t)(tap * 2 + ks) * 64 + lane) * 8);
+      bq[tap * KS + ks] = *reinterpret_cast<const v8bf*>(a.pk + ((size_t)(tap * 2 + ks) * 64 + lane) * 8);
   const int co = lane & 31;
-  const float bco = bias ? bias[co] : 0.0f;
+  const float bco = a.bias ? a.bias[co] : 0.0f;
+
+  /* stage R + 2 input rows of Wt pixels; zeros outside the image are the
+     convolution's padding */
+  if constexpr (PAIR) {
+    constexpr int PPR = (kTileW + 2 * kMaxTileDil + 255) / 256;
+    const int pv = a.prevIdx[n];
+    const uint16_t* p0 = pv < 0 ? a.refPlane : a.planes + (size_t)pv * a.planeStride;
+    const uint16_t* p1 = a.planes + (size_t)(a.curIdx0 + n) * a.planeStride;
+    uint32_t v[(R + 2) * PPR];
+#pragma unroll
+    for (int j = 0; j < R + 2; ++j) {
+      const int y = y0 + (j - 1) * dil;
+      const bool rowOk = y >= 0 && y < H;
+#pragma unroll
+      for (int u = 0; u < PPR; ++u) {
+        const int px = threadIdx.x + 256 * u, x = x0 - dil + px;
+        /* every load is unconditional (outside the image it reads kZeroPad),
+           so all of them are in flight together */
+        const bool ok = rowOk && px < Wt && x >= 0 && x < W;
+        const size_t o = (size_t)y * W + x;
+        const uint16_t* q0 = ok ? p0 + o : reinterpret_cast<const uint16_t*>(kZeroPad);
+        const uint16_t* q1 = ok ? p1 + o : reinterpret_cast<const uint16_t*>(kZeroPad);
+        v[j * PPR + u] = (uint32_t)*q0 | ((uint32_t)*q1 << 16);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < R + 2; ++j)
+#pragma unroll
+      for (int u = 0; u < PPR; ++u) {
+        const int px = threadIdx.x + 256 * u;
+        if (px < Wt) {
+          uint8_t* row = tile + (size_t)j * kRowPad * 64;
+          *reinterpret_cast<uint4*>(row + lds_chunk(px, 0)) = make_uint4(v[j * PPR + u], 0, 0, 0);
+          *reinterpret_cast<uint4*>(row + lds_chunk(px, 1)) = make_uint4(0, 0, 0, 0);
+        }
+      }
+  } else {
+    /* LDS DMA (global_load_lds_dwordx4): one instruction moves 16 pixels of a
+       row, lane l loading the chunk that the swizzle puts at its LDS slot
+       l * 16 (pixel l / 4, position l % 4); nothing passes through VGPRs,
+       so every load of a wave is in flight at once */
+    constexpr int NG = kRowPad / 16;
+    for (int t = wave; t < (R + 2) * NG; t += 4) {
+      const int j = t / NG, g = t - j * NG;
+      const int y = y0 + (j - 1) * dil;
+      const int p = g * 16 + (lane >> 2);
+      const int c = (lane & 3) ^ ((p >> 1) & 3);
+      const int x = x0 - dil + p;
+      const bool ok = y >= 0 && y < H && p < Wt && x >= 0 && x < W;
+      const uint16_t* q =
+          ok ? a.in + (((size_t)n * H + y) * W + x) * kC + 8 * c : reinterpret_cast<const uint16_t*>(kZeroPad);
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)q,
+                                       (__attribute__((address_space(3))) void*)(tile + (j * kRowPad + g * 16) * 64),
+                                       16, 0, 0);
+    }
+  }
   __syncthreads();
 
   v16f acc[R];
@@ -252,40 +330,66 @@ __global__ __launch_bounds__(256) void k_conv16t(const uint16_t* __restrict__ in
   const int pxBase = wave * 32 + (lane & 31);   /* staged pixel of tap dx = -dil */
 #pragma unroll
   for (int j = 0; j < R + 2; ++j) {
+    const uint8_t* row = tile + (size_t)j * kRowPad * 64;
+    v8bf af[3 * KS];
 #pragma unroll
-    for (int tx = 0; tx < 3; ++tx) {
-      const int px = pxBase + tx * dil;
-      const uint8_t* row = tile + (size_t)j * Wt * 64;
+    for (int tx = 0; tx < 3; ++tx)
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const v8bf a = *reinterpret_cast<const v8bf*>(row + lds_chunk(px, 2 * ks + half));
+      for (int ks = 0; ks < KS; ++ks)
+        af[tx * KS + ks] = *reinterpret_cast<const v8bf*>(row + lds_chunk(pxBase + tx * dil, 2 * ks + half));
+#pragma unroll
+    for (int tx = 0; tx < 3; ++tx)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
         for (int ty = 0; ty < 3; ++ty) {
           const int k = j - ty;   /* output row k reads input row k + ty */
           if (k >= 0 && k < R)
-            acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[(ty * 3 + tx) * KS + ks], acc[k], 0, 0, 0);
+            acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[tx * KS + ks], bq[(ty * 3 + tx) * KS + ks], acc[k], 0,
+                                                              0, 0);
         }
-      }
-    }
   }
 
+  if (a.mode <= 1) {
+    /* BF16 through LDS: row k of the tile at k * kTileW * 64 bytes */
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int y = y0 + k * dil;
-    if (y >= H) break;
-    const size_t rowBase = ((size_t)n * H + y) * W;
+    for (int k = 0; k < R; ++k)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int m = 8 * (v >> 2) + 4 * half + (v & 3);
-      const int px = x0 + wave * 32 + m;
-      if (px < W) {
+      for (int v = 0; v < 16; ++v) {
+        const int pl = wave * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
         float val = acc[k][v] + bco;
-        const size_t o = (rowBase + px) * kC + co;
-        if (mode == 2) {
-          outf[o] = val;
-        } else {
-          if (mode == 0) val = val > 0.0f ? val : 0.0f;
-          outb[o] = f2bf(val);
+        if (a.mode == 0) val = val > 0.0f ? val : 0.0f;
+        *reinterpret_cast<uint16_t*>(tile + k * kTileW * 64 + lds_chunk(pl, co >> 3) + 2 * (co & 7)) = f2bf(val);
+      }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int y = y0 + k * dil;
+      if (y >= H) break;
+      uint16_t* orow = a.outb + ((size_t)n * H + y) * W * kC;
+#pragma unroll
+      for (int u = 0; u < kTileW * 4 / 256; ++u) {
+        const int i = threadIdx.x + 256 * u;
+        const int pl = i >> 2, c = i & 3, x = x0 + pl;
+        if (x < W)
+          *reinterpret_cast<uint4*>(orow + (size_t)x * kC + 8 * c) =
+              *reinterpret_cast<const uint4*>(tile + k * kTileW * 64 + lds_chunk(pl, c));
+      }
+    }
+  } else {
+    /* FP32: every channel (mode 2, NHWC-32) or channels 0, 1 (mode 3, NHWC-2) */
+    const int oc = a.mode == 2 ? kC : 2;
+    if (co < oc) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int y = y0 + k * dil;
+        if (y >= H) break;
+        const size_t rowBase = ((size_t)n * H + y) * W;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int px = x0 + wave * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+          if (px < W) a.outf[(rowBase + px) * oc + co] = acc[k][v] + bco;
         }
       }
     }
@@ -385,8 +489,8 @@ __global__ void k_down4(const uint8_t* __restrict__ src, int pitch, int Wq, int 
 }
 
 /* ---- field pooling: pred[f] = mean over the field's window of channels 0, 1
-   of an fp32 NHWC-32 map; one wave per field */
-__global__ void k_pool(const float* __restrict__ map, int H, int W, const int* __restrict__ win, int nf,
+   of an fp32 NHWC map of cs channels; one wave per field */
+__global__ void k_pool(const float* __restrict__ map, int H, int W, int cs, const int* __restrict__ win, int nf,
                        float* __restrict__ pred) {
   const int f = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (f >= nf) return;
@@ -397,7 +501,7 @@ __global__ void k_pool(const float* __restrict__ map, int H, int W, const int* _
   float sx = 0.0f, sy = 0.0f;
   for (int i = lane; i < cnt; i += 64) {
     const int y = y0 + i / w, x = x0 + i % w;
-    const float* p = map + (((size_t)n * H + y) * W + x) * kC;
+    const float* p = map + (((size_t)n * H + y) * W + x) * cs;
     sx += p[0];
     sy += p[1];
   }
@@ -532,7 +636,15 @@ extern "C" int vsgc_field_pool(const float* map, int h, int w, const int* window
                                void* stream) {
   if (nf <= 0) return 0;
   hipLaunchKernelGGL(k_pool, dim3(blocks_for((long long)nf * 64, 256)), dim3(256), 0, (hipStream_t)stream, map, h, w,
-                     windows, nf, pred);
+                     kC, windows, nf, pred);
+  CNN_RET;
+}
+
+extern "C" int vsgc_field_pool2(const float* map, int h, int w, const int* windows, int nf, float* pred,
+                                void* stream) {
+  if (nf <= 0) return 0;
+  hipLaunchKernelGGL(k_pool, dim3(blocks_for((long long)nf * 64, 256)), dim3(256), 0, (hipStream_t)stream, map, h, w,
+                     2, windows, nf, pred);
   CNN_RET;
 }
 
@@ -565,25 +677,79 @@ extern "C" int vsgc_pack_weights16(const float* w, uint16_t* packed, void* strea
   CNN_RET;
 }
 
+namespace {
+
+/* geometry of a tiled launch; false when the grid would not fit */
+bool tiled_grid(ConvTileArgs& a, unsigned& blocks, size_t& lds) {
+  a.rowTiles = (a.H + kTileR * a.dil - 1) / (kTileR * a.dil);
+  a.xTiles = (a.W + kTileW - 1) / kTileW;
+  const long long tiles = (long long)a.N * a.rowTiles * a.dil * a.xTiles;
+  if (tiles > 0x7FFFFFF0LL) return false;
+  a.tiles = (int)tiles;
+  a.perXcd = (a.tiles + 7) / 8;
+  blocks = 8u * (unsigned)a.perXcd;
+  lds = (size_t)(kTileR + 2) * kRowPad * 64;
+  return true;
+}
+
+}  // namespace
+
 extern "C" int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* packed16, const float* bias,
                            int dil, int mode, int inChannels, uint16_t* outb, float* outf, void* stream) {
-  if (n <= 0 || h <= 0 || w <= 0 || dil < 1 || mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
-  if ((mode == 2 && !outf) || (mode != 2 && !outb)) return (int)hipErrorInvalidValue;
+  if (n <= 0 || h <= 0 || w <= 0 || dil < 1 || mode < 0 || mode > 3) return (int)hipErrorInvalidValue;
+  if ((mode >= 2 && !outf) || (mode < 2 && !outb)) return (int)hipErrorInvalidValue;
   if (dil <= kMaxTileDil) {
-    const int rowTiles = (h + kTileR * dil - 1) / (kTileR * dil);
-    const int xTiles = (w + kTileW - 1) / kTileW;
-    const long long blocks = (long long)n * rowTiles * dil * xTiles;
-    if (blocks > 0x7FFFFFFFLL) return (int)hipErrorInvalidValue;
-    const size_t lds = (size_t)(kTileR + 2) * (kTileW + 2 * dil) * 64;
+    ConvTileArgs a = {};
+    a.in = in;
+    a.N = n;
+    a.H = h;
+    a.W = w;
+    a.pk = packed16;
+    a.bias = bias;
+    a.dil = dil;
+    a.mode = mode;
+    a.outb = outb;
+    a.outf = outf;
+    unsigned blocks;
+    size_t lds;
+    if (!tiled_grid(a, blocks, lds)) return (int)hipErrorInvalidValue;
     if (inChannels <= 16)
-      hipLaunchKernelGGL((k_conv16t<1, kTileR>), dim3((unsigned)blocks), dim3(256), lds, (hipStream_t)stream, in, n,
-                         h, w, packed16, bias, dil, mode, outb, outf, rowTiles, xTiles);
+      hipLaunchKernelGGL((k_conv16t<1, false>), dim3(blocks), dim3(256), lds, (hipStream_t)stream, a);
     else
-      hipLaunchKernelGGL((k_conv16t<2, kTileR>), dim3((unsigned)blocks), dim3(256), lds, (hipStream_t)stream, in, n,
-                         h, w, packed16, bias, dil, mode, outb, outf, rowTiles, xTiles);
+      hipLaunchKernelGGL((k_conv16t<2, false>), dim3(blocks), dim3(256), lds, (hipStream_t)stream, a);
     CNN_RET;
   }
+  if (mode == 3) return (int)hipErrorInvalidValue;
   return vsgc_conv16_direct(in, n, h, w, packed16, bias, dil, mode, inChannels, outb, outf, stream);
+}
+
+extern "C" int vsgc_conv16_pair(const uint16_t* planes, size_t planeStride, const uint16_t* refPlane,
+                                const int* prevIdx, int curIdx0, int n, int h, int w, const uint16_t* packed16,
+                                const float* bias, int dil, int mode, uint16_t* outb, float* outf, void* stream) {
+  if (!planes || !prevIdx || n <= 0 || h <= 0 || w <= 0 || dil < 1 || dil > kMaxTileDil || mode < 0 || mode > 3 ||
+      curIdx0 < 0 || planeStride < (size_t)h * w)
+    return (int)hipErrorInvalidValue;
+  if ((mode >= 2 && !outf) || (mode < 2 && !outb)) return (int)hipErrorInvalidValue;
+  ConvTileArgs a = {};
+  a.planes = planes;
+  a.planeStride = planeStride;
+  a.refPlane = refPlane ? refPlane : planes;
+  a.prevIdx = prevIdx;
+  a.curIdx0 = curIdx0;
+  a.N = n;
+  a.H = h;
+  a.W = w;
+  a.pk = packed16;
+  a.bias = bias;
+  a.dil = dil;
+  a.mode = mode;
+  a.outb = outb;
+  a.outf = outf;
+  unsigned blocks;
+  size_t lds;
+  if (!tiled_grid(a, blocks, lds)) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_conv16t<1, true>), dim3(blocks), dim3(256), lds, (hipStream_t)stream, a);
+  CNN_RET;
 }
 
 extern "C" int vsgc_conv16_direct(const uint16_t* in, int n, int h, int w, const uint16_t* packed16,

@@ -5,11 +5,12 @@
  * stage becomes:
  *   1. every frame's raw luma is 4x4-downsampled once, in the front stage,
  *      into a per-slot quarter-resolution BF16 plane;
- *   2. the network input of frame i pairs its previous frame's plane
- *      (channel 0) with its own (channel 1), NHWC with 32 channels;
- *   3. the 6-layer network (vsg_cnn.hip) runs over the batch, and each coarse
- *      field's proposal is the mean of the two output channels over the
- *      field's window (field size / 4 at the field centre / 4);
+ *   2. the network's first layer reads frame i's previous plane (channel 0)
+ *      and its own (channel 1) straight from those planes;
+ *   3. the 6-layer network (vsg_cnn.hip) runs over the batch, its last layer
+ *      writing only the two displacement channels, and each coarse field's
+ *      proposal is their mean over the field's window (field size / 4 at
+ *      the field centre / 4);
  *   4. k_sad_window searches the 5x5 window of 1-px shifts around each
  *      proposal; a minimum inside the window whose SAD is within the typical
  *      residual of the exact search is the field's result;
@@ -58,20 +59,6 @@ __global__ void k_down_slots(const VSGBlurSrc* __restrict__ src, int Wq, int Hq,
   down[blockIdx.z * planeStride + (size_t)y * Wq + x] = f2bf(((float)sum * (1.0f / 16.0f) - 128.0f) * (1.0f / 64.0f));
 }
 
-/* network input of frame z: channel 0 = previous frame's plane (the stored
-   reference plane for prev < 0), channel 1 = its own */
-__global__ void k_pair(const uint16_t* __restrict__ down, size_t planeStride, const uint16_t* __restrict__ downRef,
-                       const int* __restrict__ prevSlot, int slot0, int Wq, int Hq, uint16_t* __restrict__ in) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
-  if (x >= Wq) return;
-  const int pv = prevSlot[z];
-  const uint16_t* a = pv < 0 ? downRef : down + (size_t)pv * planeStride;
-  const uint16_t* b = down + (size_t)(slot0 + z) * planeStride;
-  uint16_t* o = in + (((size_t)z * Hq + y) * Wq + x) * kC;
-  o[0] = a[(size_t)y * Wq + x];
-  o[1] = b[(size_t)y * Wq + x];
-}
-
 __global__ void k_zero_u32(unsigned* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -89,9 +76,8 @@ struct VSGLearned {
   size_t planeStride = 0;
   uint16_t* d_down = nullptr;      /* [slots][Hq][Wq] */
   uint16_t* d_downRef = nullptr;   /* [Hq][Wq] */
-  uint16_t* d_in = nullptr;        /* [B][Hq][Wq][32], channels >= 2 stay zero */
-  uint16_t* d_act[2] = {};
-  float* d_out = nullptr;          /* [B][Hq][Wq][32] */
+  uint16_t* d_act[2] = {};         /* [B][Hq][Wq][32] */
+  float* d_out = nullptr;          /* [B][Hq][Wq][2] */
   int* d_win = nullptr;            /* [B][nc][5] */
   float* d_pred = nullptr;         /* [B][nc][2] */
   VSGJob* d_fjobs[VSG_NSETS] = {};
@@ -111,7 +97,7 @@ void vsg_learned_free(VSGContext* c) {
     if (L->d_pk[k]) (void)hipFree(L->d_pk[k]);
     if (L->d_bias[k]) (void)hipFree(L->d_bias[k]);
   }
-  void* bufs[] = {L->d_down, L->d_downRef, L->d_in, L->d_act[0], L->d_act[1], L->d_out, L->d_win, L->d_pred};
+  void* bufs[] = {L->d_down, L->d_downRef, L->d_act[0], L->d_act[1], L->d_out, L->d_win, L->d_pred};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (L->netDone) (void)hipEventDestroy(L->netDone);
@@ -148,7 +134,9 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
     if (hipMalloc(&d_w, sizeof(float) * 9 * kC * kC) != hipSuccess) { rc = vsg_fail("proposal weights", hipErrorOutOfMemory); break; }
     for (int k = 0; k < nlayers && rc == 0; ++k) {
       L->dil[k] = dil[k];
-      if (dil[k] < 1) { rc = vsg_fail("vsg_set_proposal: dilation", hipSuccess); break; }
+      /* the tiled convolution (dilations 1..8) runs the fused first layer and
+         the two-channel last layer */
+      if (dil[k] < 1 || dil[k] > 8) { rc = vsg_fail("vsg_set_proposal: dilation outside 1..8", hipSuccess); break; }
       if (hipMalloc(&L->d_pk[k], sizeof(uint16_t) * 9 * 2 * 64 * 8) != hipSuccess ||
           hipMalloc(&L->d_bias[k], sizeof(float) * kC) != hipSuccess ||
           hipMemcpy(d_w, weights + k * perLayer, sizeof(float) * 9 * kC * kC, hipMemcpyHostToDevice) != hipSuccess ||
@@ -162,17 +150,15 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
     if (rc) break;
     if (hipMalloc(&L->d_down, sizeof(uint16_t) * L->planeStride * slots) != hipSuccess ||
         hipMalloc(&L->d_downRef, sizeof(uint16_t) * L->planeStride) != hipSuccess ||
-        hipMalloc(&L->d_in, sizeof(uint16_t) * mapElems) != hipSuccess ||
         hipMalloc(&L->d_act[0], sizeof(uint16_t) * mapElems) != hipSuccess ||
         hipMalloc(&L->d_act[1], sizeof(uint16_t) * mapElems) != hipSuccess ||
-        hipMalloc(&L->d_out, sizeof(float) * mapElems) != hipSuccess ||
+        hipMalloc(&L->d_out, sizeof(float) * 2 * (size_t)B * L->Hq * L->Wq) != hipSuccess ||
         hipMalloc(&L->d_win, sizeof(int) * 5 * (size_t)B * nc) != hipSuccess ||
         hipMalloc(&L->d_pred, sizeof(float) * 2 * (size_t)B * nc) != hipSuccess) {
       rc = vsg_fail("proposal buffers", hipErrorOutOfMemory);
       break;
     }
-    if (hipMemset(L->d_in, 0, sizeof(uint16_t) * mapElems) != hipSuccess ||
-        hipMemset(L->d_down, 0, sizeof(uint16_t) * L->planeStride * slots) != hipSuccess ||
+    if (hipMemset(L->d_down, 0, sizeof(uint16_t) * L->planeStride * slots) != hipSuccess ||
         hipMemset(L->d_downRef, 0, sizeof(uint16_t) * L->planeStride) != hipSuccess) {
       rc = vsg_fail("proposal buffers", hipErrorUnknown);
       break;
@@ -255,19 +241,19 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
   const int slot0 = set * c->B, nc = c->st[0].nf;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 5);
   if (L->netLastSet >= 0 && L->netLastSet != set) LCHECK(hipStreamWaitEvent(st, L->netDone, 0));
-  hipLaunchKernelGGL(k_pair, dim3((L->Wq + 255) / 256, L->Hq, n), dim3(256), 0, st, L->d_down, L->planeStride,
-                     L->d_downRef, s.d_prev, slot0, L->Wq, L->Hq, L->d_in);
-  LCHECK(hipGetLastError());
-  const uint16_t* x = L->d_in;
+  const uint16_t* x = nullptr;
   for (int k = 0; k < L->nl; ++k) {
     const bool last = k == L->nl - 1;
-    uint16_t* y = L->d_act[k & 1];
-    if (vsgc_conv16(x, n, L->Hq, L->Wq, L->d_pk[k], L->d_bias[k], L->dil[k], last ? 2 : 0, k == 0 ? 2 : 32,
-                    last ? nullptr : y, last ? L->d_out : nullptr, st) != 0)
-      return vsg_fail("proposal network", hipErrorLaunchFailure);
+    uint16_t* y = last ? nullptr : L->d_act[k & 1];
+    float* yf = last ? L->d_out : nullptr;
+    const int mode = last ? 3 : 0;
+    const int rc = k == 0 ? vsgc_conv16_pair(L->d_down, L->planeStride, L->d_downRef, s.d_prev, slot0, n, L->Hq, L->Wq,
+                                             L->d_pk[k], L->d_bias[k], L->dil[k], mode, y, yf, st)
+                          : vsgc_conv16(x, n, L->Hq, L->Wq, L->d_pk[k], L->d_bias[k], L->dil[k], mode, 32, y, yf, st);
+    if (rc != 0) return vsg_fail("proposal network", hipErrorLaunchFailure);
     x = y;
   }
-  if (vsgc_field_pool(L->d_out, L->Hq, L->Wq, L->d_win, n * nc, L->d_pred, st) != 0)
+  if (vsgc_field_pool2(L->d_out, L->Hq, L->Wq, L->d_win, n * nc, L->d_pred, st) != 0)
     return vsg_fail("proposal pooling", hipErrorLaunchFailure);
   VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
   const double maxSad = (double)L->thr * c->st[0].S * c->st[0].S;
