@@ -311,9 +311,11 @@ def measure_forms_4k(nframes, steps, warmup, weights):
     lrn = api.Detector(s.width, s.height, **kw)
     lrn.set_proposal(weights)
     learned_fps = timed(lrn)
-    f0, b0 = lrn.proposal_stats()
+    f0, b0 = lrn.proposal_stats(0)
+    g0, c0 = lrn.proposal_stats(1)
     got = lrn.detect_device(ptrs, ls, keep=True)
-    f1, b1 = lrn.proposal_stats()
+    f1, b1 = lrn.proposal_stats(0)
+    g1, c1 = lrn.proposal_stats(1)
     lrn.close()
     del dev
     cn, ce, an, ae = api.compare_forms(ref, got, S)
@@ -325,10 +327,13 @@ def measure_forms_4k(nframes, steps, warmup, weights):
            "speedup_vs_plain": round(learned_fps / plain_fps, 3),
            "fields_equal_pct": round(100.0 * ae / max(an, 1), 3),
            "coarse_fields_equal_pct": round(100.0 * ce / max(cn, 1), 3),
+           "fine_fields_equal_pct": round(100.0 * (ae - ce) / max(an - cn, 1), 3),
            "fallback_pct": round(100.0 * (b1 - b0) / max(f1 - f0, 1), 3),
+           "fine_fallback_pct": round(100.0 * (c1 - c0) / max(g1 - g0, 1), 3),
            "global_transform_err_px": round(float(np.mean(errs)), 5)}
     log(f"4K: plain {plain_fps:.1f} fps, learned {learned_fps:.1f} fps ({learned_fps / plain_fps:.2f}x); fields "
-        f"equal {out['fields_equal_pct']:.2f}%, fallbacks {out['fallback_pct']:.2f}%, mean transform error "
+        f"equal {out['fields_equal_pct']:.2f}% (fine {out['fine_fields_equal_pct']:.2f}%), fallbacks coarse "
+        f"{out['fallback_pct']:.2f}% / fine {out['fine_fallback_pct']:.2f}%, mean transform error "
         f"{out['global_transform_err_px']:.4f} px")
     return out
 

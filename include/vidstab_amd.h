@@ -40,13 +40,17 @@ VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int 
    written by the project's trainer: magic "VSGP", int32 version 1, int32
    layers, int32 dilations[layers], float32 residual threshold, then per
    layer float32 W[9][32][32] and B[32], little-endian) and switches the
-   detector's coarse stage to network proposals refined in a +-2 px SAD
-   window, with the exact full search as the fallback for every field whose
-   window minimum is on the edge or above the threshold.  The fine stage and
-   everything after it are unchanged. */
+   detector's coarse and fine stages to network proposals refined in a
+   +-2 px SAD window, with the exact full search as the fallback for every
+   field whose window minimum is on the edge or above the threshold
+   (VIDSTAB_AMD_PROPOSAL_FINE=0 keeps the exact fine search).  Everything
+   after the searches is unchanged. */
 VS_API int vsMotionDetectSetProposal(VSMotionDetect* md, const char* weightsFile);
 /* coarse fields decided by the learned form so far, and the fallbacks */
 VS_API int vsMotionDetectProposalStats(const VSMotionDetect* md, long long* fields, long long* fallbacks);
+/* the same for stage 0 (coarse) or 1 (fine) */
+VS_API int vsMotionDetectProposalStageStats(const VSMotionDetect* md, int stage, long long* fields,
+                                            long long* fallbacks);
 
 /* The batched multi-clip form: the detect pass of nclips independent clips
    at once, frames already in device memory.  Clip c (config confs[c], frame

@@ -137,15 +137,22 @@ VS_API void vsg_profile_reset(void);
    coarse field is searched only in the 5x5 window of 1-px shifts around the
    network's proposal, and goes through the exact full search whenever the
    window's minimum lies on its edge, exceeds `threshold` (match units:
-   SAD / field area), or the proposal is out of the search range.  weights:
-   per layer W[9][32][32] (tap, in, out) then B[32], FP32; dil: per layer
-   dilation.  Single-clip contexts only. */
+   SAD / field area), or the proposal is out of the search range.  Fine
+   fields are searched the same way around the network's displacement less
+   the job's offset, unless VIDSTAB_AMD_PROPOSAL_FINE=0.  weights: per layer
+   W[9][32][32] (tap, in, out) then B[32], FP32; dil: per layer dilation
+   (1..8).  Single-clip contexts only. */
 VS_API int vsg_set_proposal(VSGContext* ctx, int nlayers, const int* dil, const float* weights, float threshold);
 /* coarse fields decided so far, and how many of them needed the full search */
 VS_API int vsg_proposal_stats(VSGContext* ctx, long long* fields, long long* fallbacks);
-/* the fallbacks by cause: proposal out of the search range, window minimum on
-   the window edge, minimum above the residual threshold */
-VS_API int vsg_proposal_fallbacks(VSGContext* ctx, long long* outOfRange, long long* edge, long long* residual);
+/* the same for stage 0 (coarse) or 1 (fine: zero while fine proposals are
+   switched off by VIDSTAB_AMD_PROPOSAL_FINE=0) */
+VS_API int vsg_proposal_stage_stats(VSGContext* ctx, int stage, long long* fields, long long* fallbacks);
+/* the fallbacks of stage 0 (coarse) or 1 (fine) by cause: proposal out of
+   the search range, window minimum on the window edge, minimum above the
+   residual threshold */
+VS_API int vsg_proposal_fallbacks(VSGContext* ctx, int stage, long long* outOfRange, long long* edge,
+                                  long long* residual);
 
 /* ---- the batched multi-clip form ---------------------------------------------
    One context per clip, all on one stream; the searches of all clips that

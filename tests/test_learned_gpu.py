@@ -1,8 +1,9 @@
 """The learned-proposal form on a held-out 4K motion-blur clip (config 5,
 clip 0; the trainer uses clips 100 and up).
 
-The exact full search stays the judge: the learned form's LocalMotions are
-compared field by field with the plain form's (which the fixtures pin to the
+The exact full search stays the judge: the learned form's LocalMotions (its
+coarse and fine fields both come from window searches around the network's
+displacements) are compared field by field with the plain form's (which the fixtures pin to the
 reference), and the simple global transform of every frame is compared in
 pixels at the frame corners.  The ablation runs the same method with an
 untrained network (all-zero weights: every proposal is "no motion") and
@@ -51,24 +52,30 @@ def _run_learned(c, path):
     det = api.Detector(s.width, s.height, **c["kw"])
     det.set_proposal(path)
     got = det.detect_device(c["ptrs"], [s.width] * NFRAMES, keep=True)
-    stats = det.proposal_stats()
+    stats = det.proposal_stats(0)
+    fine = det.proposal_stats(1)
     det.close()
-    return got, stats
+    return got, stats, fine
 
 
 def test_learned_form_agrees_with_full_search(clip4k):
     assert os.path.exists(WEIGHTS), "trained weights missing"
-    got, (fields, fallbacks) = _run_learned(clip4k, WEIGHTS)
+    got, (fields, fallbacks), (ffields, ffallbacks) = _run_learned(clip4k, WEIGHTS)
     cn, ce, an, ae = api.compare_forms(clip4k["plain"], got, clip4k["S"])
     s = clip4k["spec"]
     errs = [api.transform_distance(s.width, s.height, api.simple_transform(s.width, s.height, p),
                                    api.simple_transform(s.width, s.height, q))
             for p, q in zip(clip4k["plain"][1:], got[1:])]
-    _say(f"[gpu] learned: {fields} coarse fields, {fallbacks} fallbacks; coarse equal {ce}/{cn}, "
-         f"all equal {ae}/{an}, mean transform error {np.mean(errs):.4f} px")
+    _say(f"[gpu] learned: {fields} coarse fields, {fallbacks} fallbacks; {ffields} fine fields, {ffallbacks} "
+         f"fallbacks; coarse equal {ce}/{cn}, all equal {ae}/{an}, mean transform error {np.mean(errs):.4f} px")
     assert fields == cn
     assert fallbacks <= 0.15 * fields
+    # every fine field searched went through the window search (kept or not
+    # by the cleanup afterwards), and most windows were accepted
+    assert ffields >= an - cn > 0
+    assert ffallbacks <= 0.25 * ffields
     assert ae >= 0.99 * an
+    assert (ae - ce) >= 0.99 * (an - cn)
     assert np.mean(errs) <= 0.05
 
 
@@ -80,8 +87,8 @@ def test_ablation_untrained_network(clip4k, tmp_path):
     zero = [(np.zeros_like(w), np.zeros_like(b), d) for (w, b, d) in layers]
     zpath = tmp_path / "zero.bin"
     learned.save_weights(str(zpath), zero, thr)
-    got0, (f0, b0) = _run_learned(clip4k, str(zpath))
-    got1, (f1, b1) = _run_learned(clip4k, WEIGHTS)
+    got0, (f0, b0), _ = _run_learned(clip4k, str(zpath))
+    got1, (f1, b1), _ = _run_learned(clip4k, WEIGHTS)
     c0 = api.compare_forms(clip4k["plain"], got0, clip4k["S"])
     c1 = api.compare_forms(clip4k["plain"], got1, clip4k["S"])
     _say(f"[gpu] ablation: untrained {b0}/{f0} fallbacks, equal {c0[3]}/{c0[2]}; "

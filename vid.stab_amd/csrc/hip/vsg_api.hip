@@ -490,7 +490,7 @@ extern "C" int vsg_wait_coarse(VSGContext* c, int set, VSGBatchView* view) {
   VSGSet& s = c->set[set];
   VSG_CHECK(hipSetDevice(c->device));
   VSG_CHECK(hipEventSynchronize(s.done));
-  vsg_learned_account(c, set);
+  vsg_learned_account(c, set, 0);
   if (s.prof) {
     s.prof = false;
     double fieldBytes = 0;
@@ -534,8 +534,12 @@ extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* 
   VSG_CHECK(hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, s.stream));
   s.profF = g_profOn.load() != 0;
   if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_0], s.stream));
-  VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
-  VSG_CHECK(vsg_launch_search(fa, s.stream));
+  if (vsg_learned_fine_enabled(c)) {
+    if (vsg_learned_fine(c, set, njobs, s.stream) != 0) return -1;
+  } else {
+    VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
+    VSG_CHECK(vsg_launch_search(fa, s.stream));
+  }
   if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_GRID1], s.stream));
   VSG_CHECK(hipEventRecord(s.searchDone, s.stream));
   VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, s.stream));
@@ -548,6 +552,7 @@ extern "C" int vsg_wait_fine(VSGContext* c, int set, const VSGResult** results) 
   VSGSet& s = c->set[set];
   VSG_CHECK(hipSetDevice(c->device));
   VSG_CHECK(hipEventSynchronize(s.doneF));
+  if (s.njobsF > 0) vsg_learned_account(c, set, 1);
   if (s.profF) {
     s.profF = false;
     const VSGStage& S = c->st[1];

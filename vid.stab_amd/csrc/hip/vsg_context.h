@@ -104,10 +104,13 @@ int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift, int bpp);
 /* picks each stage's plan (vsg_tune.hip) */
 int vsg_ctx_tune(VSGContext* c);
 /* the learned form (vsg_learned.hip): quarter-resolution planes in the
-   front stage, the proposal coarse stage, its statistics, teardown */
+   front stage, the proposal coarse and fine stages, their statistics,
+   teardown */
 int vsg_learned_front(VSGContext* c, int set, int n, int refSlot, hipStream_t st);
 int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st);
-void vsg_learned_account(VSGContext* c, int set);
+bool vsg_learned_fine_enabled(const VSGContext* c);
+int vsg_learned_fine(VSGContext* c, int set, int njobs, hipStream_t st);
+void vsg_learned_account(VSGContext* c, int set, int stage);
 void vsg_learned_free(VSGContext* c);
 void vsg_ctx_free(VSGContext* c);
 /* enqueue helpers (no host synchronisation) */

@@ -104,13 +104,14 @@ double vsg_plan_cost(const VSGGrid& g, int tiles);
    last workgroup of each job runs (best[] -> results) */
 hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st);
 
-/* the learned form's proposal window search over a coarse job list: accepted
-   jobs get their result and field -1 in `fallback`, the others are copied
-   to `fallback` for the exact search; stats[0] += jobs, stats[1] += fallbacks,
-   stats[2..4] += fallbacks out of range / window minimum on the edge / above
-   the residual threshold */
-hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int nc, uint32_t maxSad, VSGJob* fallback,
-                             unsigned* stats, hipStream_t st);
+/* the learned form's proposal window search over a job list (either stage):
+   pred holds (dx, dy) per (frame of the batch, field), npred fields per
+   frame, frame = job slot - slot0; accepted jobs get their result and
+   field -1 in `fallback`, the others are copied to `fallback` for the exact
+   search; stats[0] += jobs, stats[1] += fallbacks, stats[2..4] += fallbacks
+   out of range / window minimum on the edge / above the residual threshold */
+hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npred, int slot0, uint32_t maxSad,
+                             VSGJob* fallback, unsigned* stats, hipStream_t st);
 
 /* one clip's stage inside a multi-clip search launch */
 struct VSGMultiSeg {

@@ -16,7 +16,13 @@
  *      residual of the exact search is the field's result;
  *   5. every other field goes through the exact full search (the plain
  *      coarse kernel) -- the fallback that keeps the form honest.
- * The fine stage and everything on the host are unchanged.
+ * The fine stage works the same way: the network output is also pooled over
+ * every fine field's window when the network runs, and the fine search of
+ * a field becomes a 5x5 window search around that displacement less the
+ * field's search offset (which the host derives from the coarse result),
+ * with the exact fine search as the fallback.  VIDSTAB_AMD_PROPOSAL_FINE=0
+ * keeps the exact fine search for every field.  Everything on the host is
+ * unchanged.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -66,6 +72,9 @@ __global__ void k_zero_u32(unsigned* p, int n) {
 
 }  // namespace
 
+struct VSGLearned;
+static uint32_t max_sad(const VSGLearned* L, int S);
+
 struct VSGLearned {
   int nl = 0;
   int dil[kMaxLayers] = {};
@@ -80,15 +89,26 @@ struct VSGLearned {
   float* d_out = nullptr;          /* [B][Hq][Wq][2] */
   int* d_win = nullptr;            /* [B][nc][5] */
   float* d_pred = nullptr;         /* [B][nc][2] */
+  bool fine = true;                /* proposals for the fine stage too */
+  int* d_winF = nullptr;           /* [B][nff][5] */
+  float* d_predF[VSG_NSETS] = {};  /* [B][nff][2]: kept per set until its fine stage */
   VSGJob* d_fjobs[VSG_NSETS] = {};
-  unsigned* d_stats[VSG_NSETS] = {};
+  VSGJob* d_fjobsF[VSG_NSETS] = {};
+  unsigned* d_stats[VSG_NSETS] = {};   /* 5 counters per stage: coarse, then fine */
   unsigned* h_stats[VSG_NSETS] = {};
-  long long fields = 0, fallbacks = 0, why[3] = {0, 0, 0};
+  long long fields[2] = {0, 0}, fallbacks[2] = {0, 0}, why[2][3] = {{0, 0, 0}, {0, 0, 0}};
   /* the network buffers above are shared by the slot sets, whose streams may
      run concurrently: each set's network stage waits for the previous one */
   hipEvent_t netDone = nullptr;
   int netLastSet = -1;
 };
+
+/* the acceptance bound of a window minimum: the residual threshold (a match
+   value, SAD per pixel) times the field area */
+static uint32_t max_sad(const VSGLearned* L, int S) {
+  const double m = (double)L->thr * S * S;
+  return (uint32_t)(m < 4294967295.0 ? m : 4294967295.0);
+}
 
 void vsg_learned_free(VSGContext* c) {
   VSGLearned* L = c->learned;
@@ -97,12 +117,14 @@ void vsg_learned_free(VSGContext* c) {
     if (L->d_pk[k]) (void)hipFree(L->d_pk[k]);
     if (L->d_bias[k]) (void)hipFree(L->d_bias[k]);
   }
-  void* bufs[] = {L->d_down, L->d_downRef, L->d_act[0], L->d_act[1], L->d_out, L->d_win, L->d_pred};
+  void* bufs[] = {L->d_down, L->d_downRef, L->d_act[0], L->d_act[1], L->d_out, L->d_win, L->d_pred, L->d_winF};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (L->netDone) (void)hipEventDestroy(L->netDone);
   for (int s = 0; s < VSG_NSETS; ++s) {
     if (L->d_fjobs[s]) (void)hipFree(L->d_fjobs[s]);
+    if (L->d_fjobsF[s]) (void)hipFree(L->d_fjobsF[s]);
+    if (L->d_predF[s]) (void)hipFree(L->d_predF[s]);
     if (L->d_stats[s]) (void)hipFree(L->d_stats[s]);
     if (L->h_stats[s]) (void)hipHostFree(L->h_stats[s]);
   }
@@ -125,7 +147,9 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
   L->Wq = c->W / 4;
   L->Hq = c->H / 4;
   L->planeStride = (size_t)L->Wq * L->Hq;
-  const int B = c->B, nc = c->st[0].nf, slots = VSG_NSETS * B;
+  const int B = c->B, nc = c->st[0].nf, nff = c->st[1].nf, slots = VSG_NSETS * B;
+  const char* fineEnv = getenv("VIDSTAB_AMD_PROPOSAL_FINE");
+  L->fine = !(fineEnv && atoi(fineEnv) == 0);
   const size_t mapElems = (size_t)B * L->Hq * L->Wq * kC;
   float* d_w = nullptr;
   const size_t perLayer = 9 * kC * kC + kC;
@@ -154,7 +178,8 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
         hipMalloc(&L->d_act[1], sizeof(uint16_t) * mapElems) != hipSuccess ||
         hipMalloc(&L->d_out, sizeof(float) * 2 * (size_t)B * L->Hq * L->Wq) != hipSuccess ||
         hipMalloc(&L->d_win, sizeof(int) * 5 * (size_t)B * nc) != hipSuccess ||
-        hipMalloc(&L->d_pred, sizeof(float) * 2 * (size_t)B * nc) != hipSuccess) {
+        hipMalloc(&L->d_pred, sizeof(float) * 2 * (size_t)B * nc) != hipSuccess ||
+        hipMalloc(&L->d_winF, sizeof(int) * 5 * (size_t)B * nff) != hipSuccess) {
       rc = vsg_fail("proposal buffers", hipErrorOutOfMemory);
       break;
     }
@@ -163,41 +188,47 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
       rc = vsg_fail("proposal buffers", hipErrorUnknown);
       break;
     }
-    /* windows: field size / 4 around the field centre / 4, per frame slot */
-    std::vector<int16_t> fx(nc), fy(nc);
-    if (hipMemcpy(fx.data(), c->st[0].d_fx, sizeof(int16_t) * nc, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(fy.data(), c->st[0].d_fy, sizeof(int16_t) * nc, hipMemcpyDeviceToHost) != hipSuccess) {
-      rc = vsg_fail("proposal windows", hipErrorUnknown);
-      break;
-    }
-    std::vector<int> win((size_t)5 * B * nc);
-    const int h = c->st[0].S / 8;
-    for (int z = 0; z < B; ++z)
-      for (int f = 0; f < nc; ++f) {
-        int* w = &win[5 * ((size_t)z * nc + f)];
-        const int cx = fx[f] / 4, cy = fy[f] / 4;
-        w[0] = z;
-        w[1] = cx - h < 0 ? 0 : cx - h;
-        w[2] = cy - h < 0 ? 0 : cy - h;
-        w[3] = cx + h > L->Wq ? L->Wq : cx + h;
-        w[4] = cy + h > L->Hq ? L->Hq : cy + h;
+    /* windows: field size / 4 around the field centre / 4, per frame slot,
+       for the coarse fields (d_win) and the fine fields (d_winF) */
+    for (int stage = 0; stage < 2 && rc == 0; ++stage) {
+      const VSGStage& S = c->st[stage];
+      std::vector<int16_t> fx(S.nf), fy(S.nf);
+      if (hipMemcpy(fx.data(), S.d_fx, sizeof(int16_t) * S.nf, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(fy.data(), S.d_fy, sizeof(int16_t) * S.nf, hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = vsg_fail("proposal windows", hipErrorUnknown);
+        break;
       }
-    if (hipMemcpy(L->d_win, win.data(), sizeof(int) * win.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      rc = vsg_fail("proposal windows", hipErrorUnknown);
-      break;
+      std::vector<int> win((size_t)5 * B * S.nf);
+      const int h = S.S / 8;
+      for (int z = 0; z < B; ++z)
+        for (int f = 0; f < S.nf; ++f) {
+          int* w = &win[5 * ((size_t)z * S.nf + f)];
+          const int cx = fx[f] / 4, cy = fy[f] / 4;
+          w[0] = z;
+          w[1] = cx - h < 0 ? 0 : cx - h;
+          w[2] = cy - h < 0 ? 0 : cy - h;
+          w[3] = cx + h > L->Wq ? L->Wq : cx + h;
+          w[4] = cy + h > L->Hq ? L->Hq : cy + h;
+        }
+      if (hipMemcpy(stage ? L->d_winF : L->d_win, win.data(), sizeof(int) * win.size(), hipMemcpyHostToDevice) !=
+          hipSuccess)
+        rc = vsg_fail("proposal windows", hipErrorUnknown);
     }
+    if (rc) break;
     if (hipEventCreateWithFlags(&L->netDone, hipEventDisableTiming) != hipSuccess) {
       rc = vsg_fail("proposal event", hipErrorUnknown);
       break;
     }
     for (int s = 0; s < VSG_NSETS; ++s) {
       if (hipMalloc(&L->d_fjobs[s], sizeof(VSGJob) * (size_t)B * nc) != hipSuccess ||
-          hipMalloc(&L->d_stats[s], sizeof(unsigned) * 5) != hipSuccess ||
-          hipHostMalloc(&L->h_stats[s], sizeof(unsigned) * 5, hipHostMallocDefault) != hipSuccess) {
+          hipMalloc(&L->d_fjobsF[s], sizeof(VSGJob) * (size_t)B * nff) != hipSuccess ||
+          hipMalloc(&L->d_predF[s], sizeof(float) * 2 * (size_t)B * nff) != hipSuccess ||
+          hipMalloc(&L->d_stats[s], sizeof(unsigned) * 10) != hipSuccess ||
+          hipHostMalloc(&L->h_stats[s], sizeof(unsigned) * 10, hipHostMallocDefault) != hipSuccess) {
         rc = vsg_fail("proposal buffers", hipErrorOutOfMemory);
         break;
       }
-      for (int q = 0; q < 5; ++q) L->h_stats[s][q] = 0;
+      for (int q = 0; q < 10; ++q) L->h_stats[s][q] = 0;
     }
   } while (0);
   if (d_w) (void)hipFree(d_w);
@@ -207,17 +238,24 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
 }
 
 extern "C" int vsg_proposal_stats(VSGContext* c, long long* fields, long long* fallbacks) {
-  if (!c || !c->learned) return vsg_fail("vsg_proposal_stats: no proposal network", hipSuccess);
-  if (fields) *fields = c->learned->fields;
-  if (fallbacks) *fallbacks = c->learned->fallbacks;
+  return vsg_proposal_stage_stats(c, 0, fields, fallbacks);
+}
+
+extern "C" int vsg_proposal_stage_stats(VSGContext* c, int stage, long long* fields, long long* fallbacks) {
+  if (!c || !c->learned || stage < 0 || stage > 1)
+    return vsg_fail("vsg_proposal_stage_stats: no proposal network", hipSuccess);
+  if (fields) *fields = c->learned->fields[stage];
+  if (fallbacks) *fallbacks = c->learned->fallbacks[stage];
   return 0;
 }
 
-extern "C" int vsg_proposal_fallbacks(VSGContext* c, long long* outOfRange, long long* edge, long long* residual) {
-  if (!c || !c->learned) return vsg_fail("vsg_proposal_fallbacks: no proposal network", hipSuccess);
-  if (outOfRange) *outOfRange = c->learned->why[0];
-  if (edge) *edge = c->learned->why[1];
-  if (residual) *residual = c->learned->why[2];
+extern "C" int vsg_proposal_fallbacks(VSGContext* c, int stage, long long* outOfRange, long long* edge,
+                                      long long* residual) {
+  if (!c || !c->learned || stage < 0 || stage > 1)
+    return vsg_fail("vsg_proposal_fallbacks: no proposal network", hipSuccess);
+  if (outOfRange) *outOfRange = c->learned->why[stage][0];
+  if (edge) *edge = c->learned->why[stage][1];
+  if (residual) *residual = c->learned->why[stage][2];
   return 0;
 }
 
@@ -239,7 +277,7 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
   VSGLearned* L = c->learned;
   VSGSet& s = c->set[set];
   const int slot0 = set * c->B, nc = c->st[0].nf;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 5);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 10);
   if (L->netLastSet >= 0 && L->netLastSet != set) LCHECK(hipStreamWaitEvent(st, L->netDone, 0));
   const uint16_t* x = nullptr;
   for (int k = 0; k < L->nl; ++k) {
@@ -253,12 +291,11 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
     if (rc != 0) return vsg_fail("proposal network", hipErrorLaunchFailure);
     x = y;
   }
-  if (vsgc_field_pool2(L->d_out, L->Hq, L->Wq, L->d_win, n * nc, L->d_pred, st) != 0)
+  if (vsgc_field_pool2(L->d_out, L->Hq, L->Wq, L->d_win, n * nc, L->d_pred, st) != 0 ||
+      (L->fine && vsgc_field_pool2(L->d_out, L->Hq, L->Wq, L->d_winF, n * c->st[1].nf, L->d_predF[set], st) != 0))
     return vsg_fail("proposal pooling", hipErrorLaunchFailure);
   VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
-  const double maxSad = (double)L->thr * c->st[0].S * c->st[0].S;
-  LCHECK(vsg_launch_window(ga, L->d_pred, nc, (uint32_t)(maxSad < 4294967295.0 ? maxSad : 4294967295.0),
-                           L->d_fjobs[set], L->d_stats[set], st));
+  LCHECK(vsg_launch_window(ga, L->d_pred, nc, slot0, max_sad(L, c->st[0].S), L->d_fjobs[set], L->d_stats[set], st));
   /* the window search was the last reader of the shared network buffers */
   LCHECK(hipEventRecord(L->netDone, st));
   L->netLastSet = set;
@@ -269,12 +306,30 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
   return 0;
 }
 
-/* after the set's coarse read-back completed */
-void vsg_learned_account(VSGContext* c, int set) {
+bool vsg_learned_fine_enabled(const VSGContext* c) { return c->learned && c->learned->fine; }
+
+/* the fine stage over the host's job list: window search around each fine
+   field's pooled displacement less its offset, exact search for the rest */
+int vsg_learned_fine(VSGContext* c, int set, int njobs, hipStream_t st) {
+  VSGLearned* L = c->learned;
+  VSGSet& s = c->set[set];
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set] + 5, 5);
+  VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
+  LCHECK(vsg_launch_window(fa, L->d_predF[set], c->st[1].nf, set * c->B, max_sad(L, c->st[1].S), L->d_fjobsF[set],
+                           L->d_stats[set] + 5, st));
+  fa.jobs = L->d_fjobsF[set];
+  LCHECK(vsg_launch_search(fa, st));
+  LCHECK(hipMemcpyAsync(L->h_stats[set] + 5, L->d_stats[set] + 5, sizeof(unsigned) * 5, hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+/* after the set's coarse (stage 0) or fine (stage 1) read-back completed */
+void vsg_learned_account(VSGContext* c, int set, int stage) {
   VSGLearned* L = c->learned;
   if (!L) return;
-  L->fields += L->h_stats[set][0];
-  L->fallbacks += L->h_stats[set][1];
-  for (int q = 0; q < 3; ++q) L->why[q] += L->h_stats[set][2 + q];
-  for (int q = 0; q < 5; ++q) L->h_stats[set][q] = 0;
+  unsigned* h = L->h_stats[set] + 5 * stage;
+  L->fields[stage] += h[0];
+  L->fallbacks[stage] += h[1];
+  for (int q = 0; q < 3; ++q) L->why[stage][q] += h[2 + q];
+  for (int q = 0; q < 5; ++q) h[q] = 0;
 }

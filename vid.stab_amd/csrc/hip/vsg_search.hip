@@ -318,15 +318,22 @@ __global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict
 }
 
 /* ---- proposal window (learned form) -----------------------------------------------
-   Job j's field gets proposal p = round(4 * pred) (quarter-resolution
-   network output); the 5x5 window of 1-px shifts around p is searched
-   exhaustively, keeping the reference refinement's rule (the centre first,
-   then x outer / y inner, strict "<").  The result is accepted when the
-   minimum is inside the window and its SAD is at most maxSad; then it is
-   written to results[j] and the job is removed from the fallback list,
-   otherwise the job goes to the fallback list for the exact full search. */
-__global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float* __restrict__ pred, int nc,
-                                                    uint32_t maxSad, VSGJob* __restrict__ fallback,
+   Job j's field gets proposal p = round(4 * pred) - (ox, oy): the network's
+   displacement (quarter-resolution units) of the field, less the job's
+   search offset, as a shift of the stage's search.  pred is indexed by the
+   job's frame in the batch (slot - slot0) and its field, npred fields per
+   frame.  The 5x5 window of 1-px shifts around p is searched exhaustively,
+   keeping the reference refinement's rule (the centre first, then x outer /
+   y inner, strict "<"); while its minimum lies on the window's edge the
+   window moves to that minimum (a descent, kWindowSteps windows at most).
+   The result is accepted when the minimum is inside the window and its SAD
+   is at most maxSad; then it is written to results[j] and the job is
+   removed from the fallback list, otherwise the job goes to the fallback
+   list for the exact full search. */
+constexpr int kWindowSteps = 4;
+
+__global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float* __restrict__ pred, int npred,
+                                                    int slot0, uint32_t maxSad, VSGJob* __restrict__ fallback,
                                                     unsigned* __restrict__ stats) {
   __shared__ RefineLds L;
   const int job = blockIdx.x;
@@ -336,32 +343,40 @@ __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float
     if (threadIdx.x == 0) fallback[job] = jb;
     return;
   }
-  const int frame = job / nc;
-  const float* pp = pred + 2 * ((size_t)frame * nc + jb.field);
-  const float fx = rintf(4.0f * pp[0]), fy = rintf(4.0f * pp[1]);
+  const float* pp = pred + 2 * ((size_t)(jb.slot - slot0) * npred + jb.field);
+  const float fx = rintf(4.0f * pp[0]) - (float)jb.ox, fy = rintf(4.0f * pp[1]) - (float)jb.oy;
   const float lim = (float)(a.g.n * a.g.step - 2);   /* the full search's reach, less the window */
-  const bool inRange = fx >= -lim && fx <= lim && fy >= -lim && fy <= lim;
-  bool accept = inRange, interior = false;
+  bool inRange = fx >= -lim && fx <= lim && fy >= -lim && fy <= lim, interior = false;
   int bx = 0, by = 0;
   uint32_t minerr = 0;
-  if (accept) {
-    const int px = (int)fx, py = (int)fy;
-    level_sums(a, jb, px - 2, py - 2, 1, 5, 5, L);
-    bx = px;
-    by = py;
-    minerr = L.sums[2 * kMaxNi + 2];
-    for (int ti = 0; ti < 5; ++ti)
-      for (int tj = 0; tj < 5; ++tj) {
-        const uint32_t e = L.sums[ti * kMaxNi + tj];
-        if (!(ti == 2 && tj == 2) && e < minerr) {
-          minerr = e;
-          bx = px - 2 + ti;
-          by = py - 2 + tj;
+  if (inRange) {
+    /* a minimum on the window's edge moves the window there, up to
+       kWindowSteps windows in all; every thread sees the same sums, so the
+       walk is uniform across the workgroup */
+    int px = (int)fx, py = (int)fy;
+    for (int it = 0; it < kWindowSteps; ++it) {
+      level_sums(a, jb, px - 2, py - 2, 1, 5, 5, L);
+      bx = px;
+      by = py;
+      minerr = L.sums[2 * kMaxNi + 2];
+      for (int ti = 0; ti < 5; ++ti)
+        for (int tj = 0; tj < 5; ++tj) {
+          const uint32_t e = L.sums[ti * kMaxNi + tj];
+          if (!(ti == 2 && tj == 2) && e < minerr) {
+            minerr = e;
+            bx = px - 2 + ti;
+            by = py - 2 + tj;
+          }
         }
-      }
-    interior = bx > px - 2 && bx < px + 2 && by > py - 2 && by < py + 2;
-    accept = interior && minerr <= maxSad;
+      interior = bx > px - 2 && bx < px + 2 && by > py - 2 && by < py + 2;
+      if (interior) break;
+      px = bx;
+      py = by;
+      inRange = px >= -lim && px <= lim && py >= -lim && py <= lim;
+      if (!inRange) break;
+    }
   }
+  const bool accept = inRange && interior && minerr <= maxSad;
   if (threadIdx.x == 0) {
     VSGJob f = jb;
     if (accept) {
@@ -538,10 +553,10 @@ hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, 
   return launch_multi_k<1, 1>(dSegs, nseg, totalUnits, g, st);
 }
 
-hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int nc, uint32_t maxSad, VSGJob* fallback,
-                             unsigned* stats, hipStream_t st) {
+hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npred, int slot0, uint32_t maxSad,
+                             VSGJob* fallback, unsigned* stats, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
-  if (a.g.S > 256 || nc <= 0 || a.g.bpp != 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_sad_window, dim3(a.njobs), dim3(256), 0, st, a, pred, nc, maxSad, fallback, stats);
+  if (a.g.S > 256 || npred <= 0 || slot0 < 0 || a.g.bpp != 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sad_window, dim3(a.njobs), dim3(256), 0, st, a, pred, npred, slot0, maxSad, fallback, stats);
   return hipGetLastError();
 }

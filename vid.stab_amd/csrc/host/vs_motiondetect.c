@@ -696,15 +696,24 @@ VS_API int vsMotionDetectSetProposal(VSMotionDetect* md, const char* weightsFile
   return VS_OK;
 }
 
+VS_API int vsMotionDetectProposalStageStats(const VSMotionDetect* md, int stage, long long* fields,
+                                            long long* fallbacks) {
+  MDState* s = state_find(md);
+  if (!s) return VS_ERROR;
+  return vsg_proposal_stage_stats(s->gpu, stage, fields, fallbacks) == 0 ? VS_OK : VS_ERROR;
+}
+
 VS_API int vsMotionDetectProposalStats(const VSMotionDetect* md, long long* fields, long long* fallbacks) {
   MDState* s = state_find(md);
   if (!s) return VS_ERROR;
   if (vsg_proposal_stats(s->gpu, fields, fallbacks) != 0) return VS_ERROR;
   if (getenv("VIDSTAB_AMD_PROPOSAL_LOG")) {
-    long long r = 0, e = 0, t = 0;
-    vsg_proposal_fallbacks(s->gpu, &r, &e, &t);
-    vs_log_info(md->conf.modName, "proposal fallbacks: %lld out of range, %lld on the window edge, %lld above "
-                "the residual threshold\n", r, e, t);
+    for (int stage = 0; stage < 2; ++stage) {
+      long long r = 0, e = 0, t = 0;
+      vsg_proposal_fallbacks(s->gpu, stage, &r, &e, &t);
+      vs_log_info(md->conf.modName, "%s proposal fallbacks: %lld out of range, %lld on the window edge, %lld "
+                  "above the residual threshold\n", stage ? "fine" : "coarse", r, e, t);
+    }
   }
   return VS_OK;
 }

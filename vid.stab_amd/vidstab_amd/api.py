@@ -127,6 +127,9 @@ def lib():
         L.vsMotionDetectSetProposal.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.c_char_p]
         L.vsMotionDetectProposalStats.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.POINTER(ctypes.c_longlong),
                                                   ctypes.POINTER(ctypes.c_longlong)]
+        L.vsMotionDetectProposalStageStats.argtypes = [ctypes.POINTER(VSMotionDetect), c_int,
+                                                       ctypes.POINTER(ctypes.c_longlong),
+                                                       ctypes.POINTER(ctypes.c_longlong)]
         L.vs_vector_size.argtypes = [ctypes.POINTER(VSVector)]
         L.vs_vector_get.argtypes = [ctypes.POINTER(VSVector), c_int]
         L.vs_vector_get.restype = c_void_p
@@ -233,9 +236,11 @@ class Detector:
         if self.L.vsMotionDetectSetProposal(ctypes.byref(self.md), str(path).encode()) != 0:
             raise RuntimeError("vsMotionDetectSetProposal failed: " + self.L.vsg_last_error().decode())
 
-    def proposal_stats(self):
+    def proposal_stats(self, stage=0):
+        """(fields decided by the learned form, fallbacks) of stage 0 (coarse) or 1 (fine)"""
         f, b = ctypes.c_longlong(0), ctypes.c_longlong(0)
-        if self.L.vsMotionDetectProposalStats(ctypes.byref(self.md), ctypes.byref(f), ctypes.byref(b)) != 0:
+        if self.L.vsMotionDetectProposalStageStats(ctypes.byref(self.md), stage, ctypes.byref(f),
+                                                   ctypes.byref(b)) != 0:
             raise RuntimeError("no proposal network")
         return f.value, b.value
 
