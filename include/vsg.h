@@ -154,6 +154,21 @@ VS_API int vsg_proposal_stage_stats(VSGContext* ctx, int stage, long long* field
 VS_API int vsg_proposal_fallbacks(VSGContext* ctx, int stage, long long* outOfRange, long long* edge,
                                   long long* residual);
 
+/* ---- second pass: per-frame global transforms ------------------------------------
+   vsMotionsToTransform (src/localmotion2transform.c:213-283) of nframes
+   frames at once, with the similarity model (the transform config's fov 0,
+   no lens estimate), on HIP device `device`.  Host arrays: lm holds
+   4 int16 per motion (field x, field y, motion x, motion y) and match one
+   double per motion, frames back to back, count[f] motions in frame f.
+   out[f] gets the frame's transform (zoom kept only with smoothZoom != 0;
+   extra = 1 for an empty frame or a residual above 100); residual[f] and
+   rounds[f] (gradient-descent rounds, 1..3) may be NULL.  The device's sin
+   and cos may differ from glibc's in the last bit, so results match the
+   reference to a tolerance.  Allocates and frees its own device memory. */
+VS_API int vsg_motions_to_transform(int device, int width, int height, const int16_t* lm, const double* match,
+                                    const int* count, int nframes, int smoothZoom, VSTransform* out,
+                                    double* residual, int* rounds);
+
 /* ---- the batched multi-clip form ---------------------------------------------
    One context per clip, all on one stream; the searches of all clips that
    share a plan run as one launch over their flattened job lists, and each

@@ -104,6 +104,10 @@ def lib():
                                           ctypes.POINTER(LocalMotion), ctypes.c_int]
         L.or_transform_point.argtypes = [ctypes.POINTER(VSTransform)] + [ctypes.c_int] * 4 + \
             [ctypes.POINTER(ctypes.c_int)] * 2
+        L.or_motions_to_transform.restype = VSTransform
+        L.or_motions_to_transform.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(LocalMotion), ctypes.c_int,
+                                              ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -117,6 +121,28 @@ def motions_to_list(arr, n):
         t = np.array([m.match], dtype=np.float64).view(np.uint64)[0]
         out.append((m.v.x, m.v.y, m.f.x, m.f.y, m.f.size, int(c), int(t)))
     return out
+
+
+def local_motions(motions):
+    """motion tuples (vx, vy, fx, fy, size, contrast_bits, match_bits) -> LocalMotion array"""
+    arr = (LocalMotion * max(1, len(motions)))()
+    for i, m in enumerate(motions):
+        arr[i].v.x, arr[i].v.y = m[0], m[1]
+        arr[i].f.x, arr[i].f.y, arr[i].f.size = m[2], m[3], m[4]
+        arr[i].contrast = float(np.array([m[5]], dtype=np.uint64).view(np.float64)[0])
+        arr[i].match = float(np.array([m[6]], dtype=np.uint64).view(np.float64)[0])
+    return arr
+
+
+def motions_to_transform(width, height, motions, smooth_zoom=0):
+    """the second pass's vsMotionsToTransform of one frame (similarity model):
+    (VSTransform, residual, gradient-descent rounds)"""
+    L = lib()
+    arr = local_motions(motions)
+    res, rounds = ctypes.c_double(0.0), ctypes.c_int(0)
+    t = L.or_motions_to_transform(width, height, arr, len(motions), smooth_zoom, ctypes.byref(res),
+                                  ctypes.byref(rounds))
+    return t, res.value, rounds.value
 
 
 class Detector:

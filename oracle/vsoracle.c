@@ -746,3 +746,146 @@ void or_delete(OrState* s) {
 }
 
 const uint8_t* or_blurred(const OrState* s) { return s->curr; }
+
+/* ---- second pass: vsMotionsToTransform, similarity model (fov 0) ----------------------
+   src/localmotion2transform.c:139-283 as the reference's -ffast-math build
+   evaluates it (read from its x86 code compiled here):
+   - calcTransformQuality (:139-171): prepare_transform_fov with f = 0, i.e.
+     z = 1 + zoom*0.01, zcos = cos(alpha)*z, zsin = z*sin(alpha) (one sincos);
+     transform_vec_double's order as in or_transform_point; the error sum is
+     sequential (transform_vec_double is a call the loop cannot vectorize
+     across); the result is (|alpha|*0.2 + |zoom|*0.002) + error/num, the
+     divisions by 5 and 500 turned into products with the rounded reciprocals;
+   - mean (src/transformtype.c:401-408): two interleaved partial sums
+     (even and odd indices, from +0.0) added, then the odd tail; stddev
+     (:421-428) sequential;
+   - disableFields (:180-211): thresh = sigma*stddevs + mu, strict ">";
+   - vsGradientDescent (:288-328): h = +1e-6 on odd sweeps, -1e-6 on even
+     ones; the gradient is a true division by h; x2 = x + grad*step over all
+     four parameters; step *= 1.2 on success, *= 0.5 otherwise;
+   - intMean (:173-177): exact (a sum of integers in double). */
+static double or_mean2(const double* ds, int len) {
+  double sum = 0.0;
+  if (len == 1) {
+    sum = 0.0 + ds[0];
+  } else if (len > 1) {
+    double even = 0.0, odd = 0.0;
+    for (int p = 0; p < len / 2; p++) {
+      even += ds[2 * p];
+      odd += ds[2 * p + 1];
+    }
+    sum = odd + even;
+    if (len & 1) sum += ds[len - 1];
+  }
+  return sum / (double)len;
+}
+
+static double or_stddev(const double* ds, int len, double mu) {
+  double sum = 0.0;
+  for (int i = 0; i < len; i++) sum += (ds[i] - mu) * (ds[i] - mu);
+  return sqrt(sum / (double)len);
+}
+
+static int or_disable_fields(double* mask, const double* q, int len, double stddevs) {
+  const double mu = or_mean2(q, len);
+  const double thresh = or_stddev(q, len, mu) * stddevs + mu;
+  int cnt = 0;
+  for (int i = 0; i < len; i++)
+    if (q[i] > thresh) {
+      mask[i] = -1.0;
+      cnt++;
+    }
+  return cnt;
+}
+
+static double or_quality(const double p[4], const LocalMotion* lms, int n, double* miss, int w, int h) {
+  double sn, cs;
+  sincos(p[2], &sn, &cs);
+  const double z = p[3] * 0.01 + 1.0;
+  const double zcos = cs * z, zsin = z * sn;
+  const double c_x = (double)(w / 2), c_y = (double)(h / 2);
+  double err = 0.0;
+  int num = 1;
+  for (int i = 0; i < n; i++) {
+    if (miss[i] >= 0) {
+      const double vx = (double)lms[i].f.x, vy = (double)lms[i].f.y;
+      const double rx = vx - c_x, ry = vy - c_y;
+      const double X = (rx * zcos + zsin * ry) + (p[0] + c_x);
+      const double Y = ((c_x - vx) * zsin + ry * zcos) + (p[1] + c_y);
+      const double dx = (X - vx) - (double)lms[i].v.x, dy = (Y - vy) - (double)lms[i].v.y;
+      const double e = dy * dy + dx * dx;
+      miss[i] = e;
+      err += e;
+      num++;
+    }
+  }
+  const double reg = fabs(p[2]) * 0.2 + fabs(p[3]) * 0.002;
+  return reg + (n > 0 ? err / (double)num : 0.0);
+}
+
+static double or_gradient_descent(double x[4], const LocalMotion* lms, int n, double* miss, int w, int h) {
+  double step[4] = {0.2, 0.2, 0.00005, 0.1};
+  double v = or_quality(x, lms, n, miss, w, h);
+  for (int i = 0; i < 16 * 4 && v > 0.01; i++) {
+    const int k = i % 4;
+    const double hh = ((i / 4) % 2) ? 1e-6 : -1e-6;
+    double x2[4] = {x[0], x[1], x[2], x[3]};
+    x2[k] = hh + x2[k];
+    double v2 = or_quality(x2, lms, n, miss, w, h);
+    double grad[4] = {0.0, 0.0, 0.0, 0.0};
+    grad[k] = (v - v2) / hh;
+    for (int j = 0; j < 4; j++) x2[j] = x[j] + grad[j] * step[k];
+    v2 = or_quality(x2, lms, n, miss, w, h);
+    if (v2 < v) {
+      for (int j = 0; j < 4; j++) x[j] = x2[j];
+      v = v2;
+      step[k] *= 1.2;
+    } else {
+      step[k] *= 0.5;
+    }
+  }
+  return v;
+}
+
+VSTransform or_motions_to_transform(int w, int h, const LocalMotion* lms, int n, int smoothZoom, double* residual,
+                                    int* rounds) {
+  VSTransform t;
+  memset(&t, 0, sizeof(t));
+  if (residual) *residual = 0.0;
+  if (rounds) *rounds = 0;
+  if (n <= 0) {
+    t.extra = 1;
+    return t;
+  }
+  double sx = 0.0, sy = 0.0;
+  for (int i = 0; i < n; i++) {
+    sx += (double)lms[i].v.x;
+    sy += (double)lms[i].v.y;
+  }
+  double* miss = (double*)calloc((size_t)n, sizeof(double));
+  double* q = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; i++) q[i] = lms[i].match;
+  or_disable_fields(miss, q, n, 1.5);
+  free(q);
+  double params[4] = {sx / (double)n, sy / (double)n, 0.0, 0.0};
+  double res = 0.0;
+  int k;
+  for (k = 0; k < 3; k++) {
+    double x[4] = {params[0], params[1], params[2], params[3]};
+    res = or_gradient_descent(x, lms, n, miss, w, h);
+    for (int j = 0; j < 4; j++) params[j] = x[j];
+    if ((k == 0 && res > 0.1) || (k == 1 && res > 20))
+      or_disable_fields(miss, miss, n, 1.0);
+    else
+      break;
+  }
+  free(miss);
+  t.x = params[0];
+  t.y = params[1];
+  t.alpha = params[2];
+  t.zoom = smoothZoom ? params[3] : 0.0;
+  t.extra = res > 100 ? 1 : 0;
+  if (residual) *residual = res;
+  if (rounds) *rounds = k + 1;
+  return t;
+}

@@ -74,6 +74,11 @@ int      or_field_search_packed(const uint8_t* cur, const uint8_t* prev, int str
 double   or_cleanmean(double* ds, int len, double* mn, double* mx);
 VSTransform or_simple_transform(int w, int h, const LocalMotion* lms, int n);
 void     or_transform_point(const VSTransform* t, int w, int h, int x, int y, int* ox, int* oy);
+/* second pass: vsMotionsToTransform of one frame's LocalMotions with the
+   similarity model (VSTransformConfig.fov = 0, no lens); residual and the
+   number of gradient-descent rounds are optional outputs */
+VSTransform or_motions_to_transform(int w, int h, const LocalMotion* lms, int n, int smoothZoom, double* residual,
+                                    int* rounds);
 
 OrState*       or_new(int width, int height, int shakiness, int accuracy, int stepSize,
                       double contrastThreshold, int virtualTripod);

@@ -9,6 +9,10 @@
  * records per frame: a hash of the input luma (pins the generator), a hash of
  * the reference's blurred plane (md->curr), every LocalMotion with the exact
  * bits of its doubles, and hashes of the binary and ASCII .trf streams.
+ * In "transforms" mode it runs the planar cases' detection again and records
+ * the reference's second-pass vsMotionsToTransform of every frame's
+ * LocalMotions (default transform config: fov 0, no lens; smoothZoom 1 so
+ * the fitted zoom is kept).
  */
 #define _GNU_SOURCE 1
 #include <inttypes.h>
@@ -16,8 +20,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "localmotion2transform.h"
 #include "motiondetect.h"
 #include "serialize.h"
+#include "transform.h"
 #include "vsynth.h"
 
 static uint64_t fnv(const void* p, size_t n, uint64_t h) {
@@ -155,6 +161,69 @@ static void run_case(FILE* out, const Case* cs, int first) {
   fprintf(stderr, "case %s done (%d frames)\n", cs->name, nfr);
 }
 
+/* the reference's vsMotionsToTransform of each frame of a planar case */
+static void run_case_transforms(FILE* out, const Case* cs, int first) {
+  VSynSpec sp;
+  vsyn_spec(cs->config, cs->clip, &sp);
+  if (cs->width) { sp.width = cs->width; sp.height = cs->height; }
+  if (cs->stepsize) sp.stepsize = cs->stepsize;
+  if (cs->accuracy) sp.accuracy = cs->accuracy;
+  if (cs->shakiness) sp.shakiness = cs->shakiness;
+  if (cs->mincontrast >= 0) sp.mincontrast = cs->mincontrast;
+  int nfr = cs->nframes < sp.nframes ? cs->nframes : sp.nframes;
+  int total = sp.nframes > nfr ? sp.nframes : nfr;
+  VSynClip clip;
+  VSynMotion* path = (VSynMotion*)calloc((size_t)total, sizeof(VSynMotion));
+  VSynSpec sp2 = sp; sp2.nframes = total;
+  vsyn_open(&sp2, &clip, path);
+  VSFrameInfo fi;
+  vsFrameInfoInit(&fi, sp.width, sp.height, PF_YUV420P);
+  VSMotionDetect md;
+  memset(&md, 0, sizeof(md));
+  VSMotionDetectConfig conf = vsMotionDetectGetDefaultConfig("golden");
+  conf.shakiness = sp.shakiness;
+  conf.accuracy = sp.accuracy;
+  conf.stepSize = sp.stepsize;
+  conf.contrastThreshold = sp.mincontrast;
+  conf.virtualTripod = cs->tripod;
+  conf.numThreads = 1;
+  if (vsMotionDetectInit(&md, &conf, &fi) != VS_OK) { fprintf(stderr, "init failed\n"); exit(1); }
+  VSTransformData td;
+  memset(&td, 0, sizeof(td));
+  td.fiSrc = fi;
+  td.conf = vsTransformGetDefaultConfig("golden");
+  td.conf.smoothZoom = 1;
+  int ls = sp.width + cs->stride_pad;
+  uint8_t* Y = (uint8_t*)malloc((size_t)ls * sp.height);
+  uint8_t* U = (uint8_t*)malloc((size_t)(sp.width / 2) * (sp.height / 2));
+  uint8_t* V = (uint8_t*)malloc((size_t)(sp.width / 2) * (sp.height / 2));
+  memset(Y, 0x5A, (size_t)ls * sp.height);
+  memset(U, 128, (size_t)(sp.width / 2) * (sp.height / 2));
+  memset(V, 128, (size_t)(sp.width / 2) * (sp.height / 2));
+  fprintf(out, "%s{\"name\":\"%s\",\"width\":%d,\"height\":%d,\"frames\":[", first ? "" : ",\n", cs->name,
+          sp.width, sp.height);
+  for (int t = 0; t < nfr; t++) {
+    vsyn_render(&clip, path, t, Y, ls);
+    VSFrame fr;
+    memset(&fr, 0, sizeof(fr));
+    fr.data[0] = Y; fr.linesize[0] = ls;
+    fr.data[1] = U; fr.linesize[1] = sp.width / 2;
+    fr.data[2] = V; fr.linesize[2] = sp.width / 2;
+    LocalMotions lms;
+    vsMotionDetection(&md, &lms, &fr);
+    VSTransform tr = vsMotionsToTransform(&td, &lms, NULL);
+    fprintf(out, "%s[\"%016" PRIx64 "\",\"%016" PRIx64 "\",\"%016" PRIx64 "\",\"%016" PRIx64 "\",%d]",
+            t ? "," : "", dbits(tr.x), dbits(tr.y), dbits(tr.alpha), dbits(tr.zoom), tr.extra);
+    vs_vector_del(&lms);
+  }
+  fprintf(out, "]}");
+  vsMotionDetectionCleanup(&md);
+  vsyn_clip_free(&clip);
+  free(path);
+  free(Y); free(U); free(V);
+  fprintf(stderr, "transforms of case %s done (%d frames)\n", cs->name, nfr);
+}
+
 int main(int argc, char** argv) {
   const char* path = argc > 1 ? argv[1] : "detect_golden.json";
   vs_log_level = 0;
@@ -200,6 +269,16 @@ int main(int argc, char** argv) {
     { "s_bgr24_show1",   1, 65,   0,    0,   5,   0,   0,  0,   0,    12, -1, PF_BGR24, 1 },
     { "s_1080p_show2",   2, 60,   0,    0,   3,   0,   0,  0,   0,    0,  -1, 0,        2 },
   };
+  const int useTransforms = argc > 2 && strcmp(argv[2], "transforms") == 0;
+  if (useTransforms) {
+    FILE* out = fopen(path, "w");
+    if (!out) return 1;
+    fprintf(out, "{\"source\":\"reference vsMotionsToTransform (oracle/_ref/libvsref.so)\",\"cases\":[\n");
+    for (size_t i = 0; i < sizeof(planar) / sizeof(planar[0]); i++) run_case_transforms(out, &planar[i], i == 0);
+    fprintf(out, "\n]}\n");
+    fclose(out);
+    return 0;
+  }
   const int usePacked = argc > 2 && strcmp(argv[2], "packed") == 0;
   const int useShow = argc > 2 && strcmp(argv[2], "show") == 0;
   const Case* cases = useShow ? shows : usePacked ? packed : planar;

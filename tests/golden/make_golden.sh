@@ -1,5 +1,6 @@
 #!/bin/sh
-# Regenerates tests/golden/detect_golden.json from the reference sources.
+# Regenerates the golden fixtures (detect, packed, show, transforms) from the
+# reference sources.
 # Needs /root/reference (this container only); the GPU box never runs it.
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -12,12 +13,14 @@ gcc -O2 -std=gnu11 -ffp-contract=off -fno-fast-math -fopenmp \
     -o "$REPO/oracle/_ref/make_golden" "$HERE/make_golden.c" \
     "$REPO/vid.stab_amd/csrc/synth/vsynth.c" \
     -L"$REPO/oracle/_ref" -Wl,-rpath,"$REPO/oracle/_ref" -lvsref -lm
-# make_golden.sh [planar|packed|show [case index]]: one fixture set, or all
+# make_golden.sh [planar|packed|show [case index]|transforms]: one fixture set, or all
 case "$1" in
   planar) "$REPO/oracle/_ref/make_golden" "$HERE/detect_golden.json" planar $2 ;;
   packed) "$REPO/oracle/_ref/make_golden" "$HERE/packed_golden.json" packed $2 ;;
   show) "$REPO/oracle/_ref/make_golden" "$HERE/show_golden.json" show $2 ;;
+  transforms) "$REPO/oracle/_ref/make_golden" "$HERE/transforms_golden.json" transforms ;;
   *) "$REPO/oracle/_ref/make_golden" "$HERE/detect_golden.json" planar
      "$REPO/oracle/_ref/make_golden" "$HERE/packed_golden.json" packed
-     "$REPO/oracle/_ref/make_golden" "$HERE/show_golden.json" show ;;
+     "$REPO/oracle/_ref/make_golden" "$HERE/show_golden.json" show
+     "$REPO/oracle/_ref/make_golden" "$HERE/transforms_golden.json" transforms ;;
 esac

@@ -124,6 +124,8 @@ def lib():
         L.vsLocalMotionsArrayDel.restype = None
         L.vsSimpleMotionsToTransform.argtypes = [VSFrameInfo, ctypes.c_char_p, ctypes.POINTER(VSVector)]
         L.vsSimpleMotionsToTransform.restype = VSTransform
+        L.vsMotionsToTransformBatch.argtypes = [ctypes.POINTER(VSFrameInfo), ctypes.POINTER(VSVector), c_int, c_int,
+                                                ctypes.POINTER(VSTransform)]
         L.vsMotionDetectSetProposal.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.c_char_p]
         L.vsMotionDetectProposalStats.argtypes = [ctypes.POINTER(VSMotionDetect), ctypes.POINTER(ctypes.c_longlong),
                                                   ctypes.POINTER(ctypes.c_longlong)]
@@ -328,6 +330,54 @@ def simple_transform(width, height, motions):
     vec.nelems = n
     t = L.vsSimpleMotionsToTransform(fi, b"vidstab_amd", ctypes.byref(vec))
     return t.x, t.y, t.alpha
+
+
+def _vector_of(motions, keep):
+    """a VSVector over LocalMotion structs built from motion tuples; the
+    backing arrays go into `keep`"""
+    n = len(motions)
+    lms = (LocalMotion * max(n, 1))()
+    ptrs = (c_void_p * max(n, 1))()
+    for i, (vx, vy, fx, fy, size, cbits, mbits) in enumerate(motions):
+        lm = lms[i]
+        lm.v.x, lm.v.y = vx, vy
+        lm.f.x, lm.f.y, lm.f.size = fx, fy, size
+        lm.contrast = float(np.array([cbits], dtype=np.uint64).view(np.float64)[0])
+        lm.match = float(np.array([mbits], dtype=np.uint64).view(np.float64)[0])
+        ptrs[i] = ctypes.cast(ctypes.byref(lm), c_void_p)
+    keep.append((lms, ptrs))
+    vec = VSVector()
+    vec.data = ctypes.cast(ptrs, ctypes.POINTER(c_void_p))
+    vec.buffersize = max(n, 1)
+    vec.nelems = n
+    return vec
+
+
+class ClipMotions:
+    """a clip's LocalMotions as the C API's LocalMotions vectors (one per
+    frame), built once from motion tuples"""
+
+    def __init__(self, frames):
+        self.n = len(frames)
+        self._keep = []
+        self.vecs = (VSVector * max(self.n, 1))()
+        for f, motions in enumerate(frames):
+            self.vecs[f] = _vector_of(motions, self._keep)
+
+
+def motions_to_transforms(width, height, frames, smooth_zoom=0):
+    """the second pass's per-frame global transforms of a clip on the GPU
+    (vsMotionsToTransformBatch): frames = one list of motion tuples per frame,
+    or a ClipMotions -> list of (x, y, alpha, zoom, extra)"""
+    L = lib()
+    fi = VSFrameInfo()
+    if not L.vsFrameInfoInit(ctypes.byref(fi), width, height, PF_YUV420P):
+        raise ValueError("vsFrameInfoInit failed")
+    clip = frames if isinstance(frames, ClipMotions) else ClipMotions(frames)
+    out = (VSTransform * max(clip.n, 1))()
+    if L.vsMotionsToTransformBatch(ctypes.byref(fi), clip.vecs, clip.n, smooth_zoom, out) != 0:
+        raise RuntimeError("vsMotionsToTransformBatch failed: " + L.vsg_last_error().decode())
+    return [(t.x, t.y, t.alpha, t.zoom, t.extra) for t in out[:clip.n]]
 
 
 def transform_distance(width, height, a, b):
