@@ -70,6 +70,53 @@ def cpu_baseline(spec, frames, nsample, threads):
                       f"frame, best of 3, {threads} OpenMP threads"}
 
 
+def measure_network_layers(weights, n=32, h=540, w=960, reps=5):
+    """the proposal network's layers one at a time (HIP events on torch's
+    stream): the first layer reading the quarter-resolution planes, the
+    32-channel layers, the two-channel FP32 last layer, over n frames"""
+    import numpy as np
+    import torch
+    from vidstab_amd import learned
+    L, st, P = learned.lib(), learned._stream(), learned._p
+    layers, _ = learned.load_weights(weights)
+    planes = torch.randint(-16000, 16000, (n + 1, h, w), dtype=torch.int16, device="cuda") & 0x3FFF
+    prev = torch.arange(-1, n - 1, dtype=torch.int32, device="cuda")
+    acts = [torch.empty((n, h, w, 32), dtype=torch.int16, device="cuda") for _ in range(2)]
+    out2 = torch.empty((n, h, w, 2), dtype=torch.float32, device="cuda")
+    packed = []
+    for wt, b, d in layers:
+        pk = torch.empty(9 * 2 * 64 * 8, dtype=torch.int16, device="cuda")
+        if L.vsgc_pack_weights16(P(torch.from_numpy(np.ascontiguousarray(wt, np.float32)).cuda()), P(pk), st) != 0:
+            raise RuntimeError("weight packing failed")
+        packed.append((pk, torch.from_numpy(np.ascontiguousarray(b, np.float32)).cuda(), d))
+    res = []
+    for k, (pk, b, d) in enumerate(packed):
+        last = k == len(packed) - 1
+        mode = 3 if last else 0
+        if k == 0:
+            run = lambda: L.vsgc_conv16_pair(P(planes), h * w, P(planes), P(prev), 1, n, h, w, P(pk), P(b), d, mode,
+                                             None if last else P(acts[0]), P(out2) if last else None, st)
+        else:
+            src, dst = acts[(k - 1) & 1], acts[k & 1]
+            run = lambda: L.vsgc_conv16(P(src), n, h, w, P(pk), P(b), d, mode, 32, None if last else P(dst),
+                                        P(out2) if last else None, st)
+        if run() != 0:
+            raise RuntimeError(f"layer {k} failed")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        flop = 2.0 * 9 * 32 * (16 if k == 0 else 32) * n * h * w   # MFMA work issued (K = 16 or 32 per tap)
+        res.append({"layer": k, "dilation": d, "ms": round(ms, 4), "tflops": round(flop / ms / 1e9, 1)})
+        log(f"network layer {k} (dilation {d}): {ms:.3f} ms per {n} frames of {w}x{h}, {flop / ms / 1e9:.1f} TFLOP/s")
+    log(f"network: {sum(r['ms'] for r in res):.3f} ms per {n} frames")
+    return res
+
+
 def measure_second_pass(spec, ptrs, ls):
     """the second pass's per-frame transforms (vsMotionsToTransformBatch, one
     launch for the clip, host LocalMotions in and transforms out) beside the
@@ -119,6 +166,8 @@ def main():
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K plain vs learned-proposal measurement")
     ap.add_argument("--no-host", action="store_true", help="skip the host-frame (PCIe-inclusive) rate")
     ap.add_argument("--no-second-pass", action="store_true", help="skip the second-pass transform measurement")
+    ap.add_argument("--net-layers", action="store_true",
+                    help="only time the proposal network's layers one at a time (32 frames of 960x540)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 256 MiB device copy (known bytes) to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -133,6 +182,9 @@ def main():
     from vidstab_amd import api
     from vidstab_amd.synth import Clip
 
+    if args.net_layers:
+        print(json.dumps({"network_layers": measure_network_layers(args.weights)}), flush=True)
+        return
     if args.config == 4:
         return run_multi(args)
     if args.config == 5:

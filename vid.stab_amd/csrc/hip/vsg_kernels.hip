@@ -34,28 +34,51 @@ __global__ __launch_bounds__(256) void k_blur(const VSGBlurSrc* __restrict__ src
   uint8_t* out = dst + (size_t)blockIdx.z * slotStride;
   const int rowsIn = kBlurTH + 2 * s2;
   const int colsIn = kBlurTW + 2 * s2;
-  const int inPitch = (colsIn + 3) & ~3;
+  const int inPitch = (colsIn + 15) & ~15;
   uint8_t* sIn = lds;
   uint8_t* sHor = lds + rowsIn * inPitch;
 
-  /* stage the input tile: interior tiles with dword loads (any source
-     alignment: gfx950 global loads take byte-aligned addresses), border tiles
-     byte by byte with edge replication */
-  const int nw = inPitch >> 2;
-  if (x0 - s2 >= 0 && x0 - s2 + inPitch <= W && y0 - s2 >= 0 && y0 + kBlurTH + s2 <= H) {
-    typedef uint32_t __attribute__((aligned(1))) u32_any;
-    const uint8_t* base = src.p + (size_t)(y0 - s2) * src.pitch + (x0 - s2);
-    for (int i = threadIdx.x; i < rowsIn * nw; i += 256) {
-      const int r = i / nw, q = i - r * nw;
-      *reinterpret_cast<uint32_t*>(sIn + r * inPitch + 4 * q) =
-          *reinterpret_cast<const u32_any*>(base + (size_t)r * src.pitch + 4 * q);
+  /* stage the input tile in 16-byte chunks (any source alignment: gfx950
+     global loads take byte-aligned addresses).  Rows outside the plane are
+     clamped to the first / last row -- the reference's edge replication --
+     and every load of a batch is issued before the first LDS write; a chunk
+     that leaves the plane on the left or right is then rebuilt byte by byte
+     with clamped columns. */
+  typedef uint4 __attribute__((aligned(1))) u128_any;
+  const int cpr = inPitch >> 4, chunks = rowsIn * cpr;
+  for (int c0 = 0; c0 < chunks; c0 += 4 * 256) {
+    uint4 v[4];
+    int gxs[4];
+    const uint8_t* rows[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + u * 256 + (int)threadIdx.x;
+      const int cc = c < chunks ? c : 0;
+      const int r = cc / cpr, q = cc - r * cpr;
+      const int gy = min(max(y0 - s2 + r, 0), H - 1);
+      gxs[u] = x0 - s2 + 16 * q;
+      rows[u] = src.p + (size_t)gy * src.pitch;
+      v[u] = *reinterpret_cast<const u128_any*>(rows[u] + min(max(gxs[u], 0), W - 16));
     }
-  } else {
-    for (int i = threadIdx.x; i < rowsIn * colsIn; i += 256) {
-      int r = i / colsIn, c = i - r * colsIn;
-      int gy = min(max(y0 - s2 + r, 0), H - 1);
-      int gx = min(max(x0 - s2 + c, 0), W - 1);
-      sIn[r * inPitch + c] = src.p[(size_t)gy * src.pitch + gx];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + u * 256 + (int)threadIdx.x;
+      if (c < chunks) {
+        if (gxs[u] < 0 || gxs[u] + 16 > W) {
+          uint32_t w[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb)
+              x |= (uint32_t)rows[u][min(max(gxs[u] + 4 * k + bb, 0), W - 1)] << (8 * bb);
+            w[k] = x;
+          }
+          v[u] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        const int r = c / cpr, q = c - r * cpr;
+        *reinterpret_cast<uint4*>(sIn + r * inPitch + 16 * q) = v[u];
+      }
     }
   }
   __syncthreads();
@@ -212,19 +235,39 @@ __global__ __launch_bounds__(256) void k_fields(VSGFieldArgs a) {
     fields_packed(a, f, slot, lane);
     return;
   }
+  /* the field in 16-byte chunks (field sizes are multiples of 16; patches
+     start at multiples of S*S bytes), four chunks of a lane in flight at once */
+  typedef uint4 __attribute__((aligned(1))) u128_any;
   const int S = a.fsz[f];
-  const int words = S >> 2;
+  const int cpr = S >> 4, total = S * cpr;
   const int left = a.fx[f] - S / 2, top = a.fy[f] - S / 2;
   const uint8_t* img = a.blurred + (size_t)slot * a.slotStride;
-  uint32_t* patch = reinterpret_cast<uint32_t*>(a.patches + (size_t)slot * a.patchSlotStride + a.patchOff[f]);
+  uint4* patch = reinterpret_cast<uint4*>(a.patches + (size_t)slot * a.patchSlotStride + a.patchOff[f]);
   uint32_t lo = 255, hi = 0;
-  for (int i = lane; i < S * words; i += 64) {
-    int y = i / words, w = i - y * words;
-    uint32_t v = load_u32_unaligned(img + (size_t)(top + y) * a.pitch + left + 4 * w);
-    patch[i] = v;
-    uint32_t b0 = v & 0xff, b1 = (v >> 8) & 0xff, b2 = (v >> 16) & 0xff, b3 = v >> 24;
-    lo = min(lo, min(min(b0, b1), min(b2, b3)));
-    hi = max(hi, max(max(b0, b1), max(b2, b3)));
+  for (int i0 = 0; i0 < total; i0 += 4 * 64) {
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * 64 + lane;
+      const int ic = i < total ? i : 0;
+      const int y = ic / cpr, c = ic - y * cpr;
+      v[u] = *reinterpret_cast<const u128_any*>(img + (size_t)(top + y) * a.pitch + left + 16 * c);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * 64 + lane;
+      if (i < total) {
+        patch[i] = v[u];
+        const uint32_t w4[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t w = w4[k];
+          const uint32_t b0 = w & 0xff, b1 = (w >> 8) & 0xff, b2 = (w >> 16) & 0xff, b3 = w >> 24;
+          lo = min(lo, min(min(b0, b1), min(b2, b3)));
+          hi = max(hi, max(max(b0, b1), max(b2, b3)));
+        }
+      }
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     lo = min(lo, (uint32_t)__shfl_xor((int)lo, off));
@@ -385,9 +428,9 @@ hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t 
   }
   const uint32_t mul = vsg_div_magic(size);
   const int s2 = size / 2;
-  const size_t lds = (size_t)(kBlurTH + 2 * s2) * (((kBlurTW + 2 * s2) + 3) & ~3) +
+  const size_t lds = (size_t)(kBlurTH + 2 * s2) * (((kBlurTW + 2 * s2) + 15) & ~15) +
                      (size_t)(kBlurTH + 2 * s2) * kBlurTW;
-  if (lds <= 64 * 1024) {
+  if (lds <= 64 * 1024 && W >= 16) {
     hipLaunchKernelGGL(k_blur, dim3((W + kBlurTW - 1) / kBlurTW, (H + kBlurTH - 1) / kBlurTH, n),
                        dim3(256), lds, st, d_src, dst, slotStride, dstPitch, W, H, size, mul);
     return hipGetLastError();
