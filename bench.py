@@ -166,6 +166,7 @@ def main():
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K plain vs learned-proposal measurement")
     ap.add_argument("--no-host", action="store_true", help="skip the host-frame (PCIe-inclusive) rate")
     ap.add_argument("--no-second-pass", action="store_true", help="skip the second-pass transform measurement")
+    ap.add_argument("--clip", type=int, default=0, help="--config 5: the held-out clip (0..99)")
     ap.add_argument("--net-layers", action="store_true",
                     help="only time the proposal network's layers one at a time (32 frames of 960x540)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -358,16 +359,17 @@ def run_multi(args):
     print(json.dumps(res), flush=True)
 
 
-def measure_forms_4k(nframes, steps, warmup, weights):
-    """4K motion-blur clip (config 5, clip 0, held out from training): the
-    plain form and the learned-proposal form on the same device-resident
-    frames; agreement with the full search per field and per frame transform"""
+def measure_forms_4k(nframes, steps, warmup, weights, clip_index=0):
+    """4K motion-blur clip (config 5, clip 0 by default; the trainer uses
+    clips 100 and up): the plain form and the learned-proposal form on the
+    same device-resident frames; agreement with the full search per field
+    and per frame transform"""
     import numpy as np
     import torch
     from vidstab_amd import api
     from vidstab_amd.synth import Clip
 
-    clip = Clip(5, 0)
+    clip = Clip(5, clip_index)
     s = clip.spec
     nfr = nframes or s.nframes
     log(f"generating {nfr} frames {s.width}x{s.height}")
@@ -410,8 +412,8 @@ def measure_forms_4k(nframes, steps, warmup, weights):
     cn, ce, an, ae = api.compare_forms(ref, got, S)
     errs = [api.transform_distance(s.width, s.height, api.simple_transform(s.width, s.height, p),
                                    api.simple_transform(s.width, s.height, q)) for p, q in zip(ref[1:], got[1:])]
-    out = {"workload": f"{s.width}x{s.height}, {nfr} frames of config 5 clip 0 (motion blur, not in the "
-                       "training set), device-resident", "frames": nfr,
+    out = {"workload": f"{s.width}x{s.height}, {nfr} frames of config 5 clip {clip_index} (motion blur, not in "
+                       "the training set), device-resident", "frames": nfr,
            "plain_fps": round(plain_fps, 2), "learned_fps": round(learned_fps, 2),
            "speedup_vs_plain": round(learned_fps / plain_fps, 3),
            "fields_equal_pct": round(100.0 * ae / max(an, 1), 3),
@@ -428,7 +430,7 @@ def measure_forms_4k(nframes, steps, warmup, weights):
 
 
 def run_learned(args):
-    f = measure_forms_4k(args.frames, args.steps, args.warmup, args.weights)
+    f = measure_forms_4k(args.frames, args.steps, args.warmup, args.weights, args.clip)
     res = {
         "metric": METRIC, "value": f["learned_fps"], "unit": "frames/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(f["frames"] * 1000.0 / f["learned_fps"], 3),

@@ -10,8 +10,9 @@ t-1 (channel 0) and t (channel 1) as (mean - 128) / 64; output channels 0, 1
 field's window to give that field's proposal.
 
 Weights file (little-endian): magic b"VSGP", int32 version, int32 layers,
-int32 dilations[layers], float32 residual threshold (the exact search's p95
-match), then per layer float32 W[9][32][32] and B[32].
+int32 dilations[layers], float32 residual threshold (the window search's
+acceptance bound in match units: twice the exact search's p95 match on the
+training clips), then per layer float32 W[9][32][32] and B[32].
 """
 import ctypes
 import struct
