@@ -21,6 +21,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "vsg_cnn.h"
 
@@ -396,6 +397,207 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
 }
 
+/* ---- band walker (inference, 32 input channels) ---------------------------------------
+   The tiled kernel above reads a tile's rows, computes and stores in turn, so
+   a workgroup has memory traffic in flight only between its phases.  Here a
+   persistent workgroup of 2 waves walks down one 64-px column band of one
+   (image, dilation phase): step t computes the R output rows y = ph + m*dil,
+   m = Rt .. Rt+R-1, from the R+2 input rows m = Rt-1 .. Rt+R, which sit in
+   an LDS ring of R+2 rows (row m in slot (m+1) mod (R+2)).  Each step needs
+   R new rows; every thread loads its share of the rows of step t+2 into
+   registers at the start of step t and writes those of step t+1 to the ring
+   after step t's MFMAs (to the slots of the R rows step t was the last to
+   read), so each load has two steps of compute to arrive in.  A band's steps
+   are split into segments (each with a 2-row warm-up) so that enough
+   workgroups run; block b takes unit (b mod 8) * ceil(units / 8) + b / 8, so
+   the segments of one band stay on one XCD.  The taps accumulate in the
+   tiled kernel's order: the results are identical. */
+constexpr int kWalkW = 64;
+constexpr int kWalkThreads = 128;
+constexpr int kWalkPad = kWalkW + 2 * kMaxTileDil;   /* staged row stride in pixels */
+constexpr int kWalkRowChunks = kWalkPad * 4;           /* 16-byte chunks of a staged row */
+
+struct ConvWalkArgs {
+  const uint16_t* in;
+  int N, H, W;
+  const uint16_t* pk;
+  const float* bias;
+  int dil, mode;
+  uint16_t* outb;
+  float* outf;
+  int steps;     /* 4-row steps of a phase's row sequence (the longest phase) */
+  int xTiles, segs, segLen, units, perXcd;
+};
+
+/* chunk u of a thread's share of rows m0 .. m0+NR-1 of phase ph: the address
+   to load (kZeroPad outside the image) and the ring position to store it at */
+template <int NR>
+__device__ __forceinline__ const uint16_t* walk_src(const ConvWalkArgs& a, int n, int ph, int x0, int m0, int u) {
+  const int i = (int)threadIdx.x + kWalkThreads * u;
+  const int r = i / kWalkRowChunks, rem = i - r * kWalkRowChunks;
+  const int p = rem >> 2, c = rem & 3;
+  const int y = ph + (m0 + r) * a.dil, x = x0 - a.dil + p;
+  const bool ok = i < NR * kWalkRowChunks && p < kWalkW + 2 * a.dil && y >= 0 && y < a.H && x >= 0 && x < a.W;
+  return ok ? a.in + (((size_t)n * a.H + y) * a.W + x) * kC + 8 * c
+            : reinterpret_cast<const uint16_t*>(kZeroPad);
+}
+
+template <int NR, int RING>
+__device__ __forceinline__ void walk_put(uint8_t* ring, int m0, int u, const uint4& v) {
+  const int i = (int)threadIdx.x + kWalkThreads * u;
+  if (i < NR * kWalkRowChunks) {
+    const int r = i / kWalkRowChunks, rem = i - r * kWalkRowChunks;
+    const int slot = (m0 + r + 1) % RING;
+    *reinterpret_cast<uint4*>(ring + (size_t)slot * kWalkPad * 64 + lds_chunk(rem >> 2, rem & 3)) = v;
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv16w(
+    const ConvWalkArgs a) {
+  constexpr int KS = 2, RING = R + 2;
+  extern __shared__ __align__(16) uint8_t lds[];
+  uint8_t* ring = lds;                              /* RING rows x kWalkPad px x 64 B */
+  uint8_t* outS = lds + RING * kWalkPad * 64;       /* R rows x kWalkW px x 64 B (BF16 output) */
+  const int unit = (int)(blockIdx.x & 7) * a.perXcd + (int)(blockIdx.x >> 3);
+  if (unit >= a.units) return;
+  int b = unit;
+  const int seg = b % a.segs;
+  b /= a.segs;
+  const int xt = b % a.xTiles;
+  b /= a.xTiles;
+  const int ph = b % a.dil;
+  const int n = b / a.dil;
+  const int t0 = seg * a.segLen, t1 = min(a.steps, t0 + a.segLen);
+  if (t0 >= t1) return;
+  const int dil = a.dil, H = a.H, W = a.W, x0 = xt * kWalkW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, co = lane & 31;
+
+  v8bf bq[9 * KS];
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      bq[tap * KS + ks] = *reinterpret_cast<const v8bf*>(a.pk + ((size_t)(tap * 2 + ks) * 64 + lane) * 8);
+  const float bco = a.bias ? a.bias[co] : 0.0f;
+
+  {
+    /* warm-up: the segment's first RING rows, in two halves of R + 2 rows'
+       worth of chunks at most 10 a thread */
+    static_assert((RING * kWalkRowChunks + kWalkThreads - 1) / kWalkThreads <= 10, "warm-up chunks");
+    const int m0 = R * t0 - 1;
+#define WALK_LD(u) const uint4 f##u = *reinterpret_cast<const uint4*>(walk_src<RING>(a, n, ph, x0, m0, u))
+#define WALK_ST(u) walk_put<RING, RING>(ring, m0, u, f##u)
+    WALK_LD(0); WALK_LD(1); WALK_LD(2); WALK_LD(3); WALK_LD(4);
+    WALK_LD(5); WALK_LD(6); WALK_LD(7); WALK_LD(8); WALK_LD(9);
+    WALK_ST(0); WALK_ST(1); WALK_ST(2); WALK_ST(3); WALK_ST(4);
+    WALK_ST(5); WALK_ST(6); WALK_ST(7); WALK_ST(8); WALK_ST(9);
+#undef WALK_LD
+#undef WALK_ST
+  }
+  __syncthreads();
+
+  const int pxBase = wave * 32 + (lane & 31);   /* staged pixel of tap dx = -dil */
+  /* the new rows of step t + 1 sit in registers (q*) while step t runs; the
+     loads for step t + 2 (p*) are issued at the start of step t, so each
+     row load has two steps of MFMAs to arrive in.  Loads past the segment
+     read rows that exist or the zero pad and are dropped. */
+  static_assert((R * kWalkRowChunks + kWalkThreads - 1) / kWalkThreads <= 5, "prefetch chunks");
+#define WALK_LD(v, u, m) const uint4 v##u = *reinterpret_cast<const uint4*>(walk_src<R>(a, n, ph, x0, m, u))
+  uint4 q0, q1, q2, q3, q4;
+  {
+    const int m1 = R * t0 + R + 1;
+    WALK_LD(l, 0, m1); WALK_LD(l, 1, m1); WALK_LD(l, 2, m1); WALK_LD(l, 3, m1); WALK_LD(l, 4, m1);
+    q0 = l0; q1 = l1; q2 = l2; q3 = l3; q4 = l4;
+  }
+  for (int t = t0; t < t1; ++t) {
+    const int m2 = R * t + 2 * R + 1;   /* first new row of step t + 2 */
+    WALK_LD(p, 0, m2); WALK_LD(p, 1, m2); WALK_LD(p, 2, m2); WALK_LD(p, 3, m2); WALK_LD(p, 4, m2);
+
+    v16f acc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k] = v16f{};
+#pragma unroll
+    for (int j = 0; j < R + 2; ++j) {
+      const uint8_t* row = ring + (size_t)((R * t + j) % RING) * kWalkPad * 64;   /* input row m = Rt - 1 + j */
+      v8bf af[3 * KS];
+#pragma unroll
+      for (int tx = 0; tx < 3; ++tx)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+          af[tx * KS + ks] = *reinterpret_cast<const v8bf*>(row + lds_chunk(pxBase + tx * dil, 2 * ks + half));
+#pragma unroll
+      for (int tx = 0; tx < 3; ++tx)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+          for (int ty = 0; ty < 3; ++ty) {
+            const int k = j - ty;   /* output row k reads input row k + ty */
+            if (k >= 0 && k < R)
+              acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[tx * KS + ks], bq[(ty * 3 + tx) * KS + ks], acc[k],
+                                                                0, 0, 0);
+          }
+    }
+
+    if (a.mode <= 1) {
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int pl = wave * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+          float val = acc[k][v] + bco;
+          if (a.mode == 0) val = val > 0.0f ? val : 0.0f;
+          *reinterpret_cast<uint16_t*>(outS + k * kWalkW * 64 + lds_chunk(pl, co >> 3) + 2 * (co & 7)) = f2bf(val);
+        }
+    }
+    __syncthreads();   /* every wave is done with the ring rows this step read last */
+    if (a.mode <= 1) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int y = ph + (R * t + k) * dil;
+        if (y < H) {
+          uint16_t* orow = a.outb + ((size_t)n * H + y) * W * kC;
+#pragma unroll
+          for (int u = 0; u < kWalkW * 4 / kWalkThreads; ++u) {
+            const int i = threadIdx.x + kWalkThreads * u;
+            const int pl = i >> 2, c = i & 3, x = x0 + pl;
+            if (x < W)
+              *reinterpret_cast<uint4*>(orow + (size_t)x * kC + 8 * c) =
+                  *reinterpret_cast<const uint4*>(outS + k * kWalkW * 64 + lds_chunk(pl, c));
+          }
+        }
+      }
+    } else {
+      const int oc = a.mode == 2 ? kC : 2;
+      if (co < oc) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const int y = ph + (R * t + k) * dil;
+          if (y < H) {
+            const size_t rowBase = ((size_t)n * H + y) * W;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+              const int px = x0 + wave * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+              if (px < W) a.outf[(rowBase + px) * oc + co] = acc[k][v] + bco;
+            }
+          }
+        }
+      }
+    }
+    if (t + 1 < t1) {
+      const int m1 = R * t + R + 1;
+      walk_put<R, RING>(ring, m1, 0, q0);
+      walk_put<R, RING>(ring, m1, 1, q1);
+      walk_put<R, RING>(ring, m1, 2, q2);
+      walk_put<R, RING>(ring, m1, 3, q3);
+      walk_put<R, RING>(ring, m1, 4, q4);
+    }
+    q0 = p0; q1 = p1; q2 = p2; q3 = p3; q4 = p4;
+    __syncthreads();   /* the new rows staged; the output rows read out */
+  }
+#undef WALK_LD
+}
+
 /* ---- weight gradient ----------------------------------------------------------------
    dW[tap][ci][co] += sum_p in[p + tap*dil][ci] * dz[p][co] for taps
    tap0 .. tap0+2: an MFMA GEMM with M = ci, N = co, K = pixels (8 per step). */
@@ -692,6 +894,38 @@ bool tiled_grid(ConvTileArgs& a, unsigned& blocks, size_t& lds) {
   return true;
 }
 
+/* the walker's grid: segments of a band's steps so that about 2048 workgroups
+   run (8 per CU at the walker's LDS use of 47 KB allows 3 resident) */
+constexpr int kWalkR = 2;   /* output rows per walker step */
+
+bool walk_grid(ConvWalkArgs& w, unsigned& blocks, size_t& lds) {
+  w.steps = (w.H + kWalkR * w.dil - 1) / (kWalkR * w.dil);
+  w.xTiles = (w.W + kWalkW - 1) / kWalkW;
+  const long long bands = (long long)w.N * w.dil * w.xTiles;
+  long long segs = (2048 + bands - 1) / bands;
+  if (segs < 1) segs = 1;
+  if (segs > w.steps) segs = w.steps;
+  w.segLen = (int)((w.steps + segs - 1) / segs);
+  w.segs = (w.steps + w.segLen - 1) / w.segLen;
+  const long long units = bands * w.segs;
+  if (units > 0x7FFFFFF0LL) return false;
+  w.units = (int)units;
+  w.perXcd = (w.units + 7) / 8;
+  blocks = 8u * (unsigned)w.perXcd;
+  lds = (size_t)(kWalkR + 2) * kWalkPad * 64 + (size_t)kWalkR * kWalkW * 64;
+  return true;
+}
+
+/* VIDSTAB_AMD_CONV_WALK=0 runs the 32-channel layers on the tiled kernel */
+bool conv_walker_off() {
+  static int off = -1;
+  if (off < 0) {
+    const char* e = getenv("VIDSTAB_AMD_CONV_WALK");
+    off = e && atoi(e) == 0 ? 1 : 0;
+  }
+  return off == 1;
+}
+
 }  // namespace
 
 extern "C" int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* packed16, const float* bias,
@@ -712,6 +946,22 @@ extern "C" int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16
     a.outf = outf;
     unsigned blocks;
     size_t lds;
+    if (inChannels > 16 && !conv_walker_off()) {
+      ConvWalkArgs wk = {};
+      wk.in = in;
+      wk.N = n;
+      wk.H = h;
+      wk.W = w;
+      wk.pk = packed16;
+      wk.bias = bias;
+      wk.dil = dil;
+      wk.mode = mode;
+      wk.outb = outb;
+      wk.outf = outf;
+      if (!walk_grid(wk, blocks, lds)) return (int)hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_conv16w<kWalkR>, dim3(blocks), dim3(kWalkThreads), lds, (hipStream_t)stream, wk);
+      CNN_RET;
+    }
     if (!tiled_grid(a, blocks, lds)) return (int)hipErrorInvalidValue;
     if (inChannels <= 16)
       hipLaunchKernelGGL((k_conv16t<1, false>), dim3(blocks), dim3(256), lds, (hipStream_t)stream, a);
