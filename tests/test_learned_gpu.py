@@ -96,3 +96,20 @@ def test_ablation_untrained_network(clip4k, tmp_path):
     assert f0 == f1 > 0
     assert b1 <= 0.5 * b0
     assert c1[3] >= c0[3] - 0.01 * c0[2]
+
+
+def test_fallback_list_reproduces_plain(clip4k, tmp_path):
+    """a residual threshold of ~0 rejects every window minimum above SAD 0,
+    so (nearly) every coarse and fine field is listed for the exact search
+    of the fallback kernel: the LocalMotions must be the plain form's, bit
+    for bit, field by field"""
+    layers, _ = learned.load_weights(WEIGHTS)
+    path = tmp_path / "thr0.bin"
+    learned.save_weights(str(path), layers, 1e-9)
+    got, (f, b), (ff, fb) = _run_learned(clip4k, str(path))
+    cn, ce, an, ae = api.compare_forms(clip4k["plain"], got, clip4k["S"])
+    _say(f"[gpu] threshold 0: {b}/{f} coarse and {fb}/{ff} fine fields to the fallback list; equal {ae}/{an}")
+    assert f == cn and b >= 0.99 * f
+    assert ff >= an - cn and fb >= 0.99 * ff
+    assert [len(x) for x in got] == [len(x) for x in clip4k["plain"]]
+    assert ae == an

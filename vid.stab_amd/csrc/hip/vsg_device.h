@@ -41,37 +41,48 @@ __device__ __forceinline__ int vsg_grid_rank(const VSGTile& t, int a, int b) {
    prev rows p, candidate k pairs prev row p with field row p - k*step.  A
    packed pixel is bpp consecutive bytes; the SAD runs over all of them
    (src/motiondetect.c:984-990).
-   Returns the lane's best (sad << 32 | rank), or ~0 for idle lanes. */
+   vsg_sad_lane returns the lane's best (sad << 32 | rank), or ~0 for idle
+   lanes; vsg_sad_acc sums a range of prev rows only (the fallback kernel
+   splits a lane's rows over several waves), vsg_lane_best ranks the sums. */
 template <int J, int K, int D, int CW, bool LDSP>
-__device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int lid) {
-  const int S = t.S, s = t.s, m = t.m, ahi = t.alo + t.G - 1;
+struct VSGLane {
+  int a0, b0;          /* the lane's first candidate */
+  bool ok;             /* the lane owns candidates */
+  uint32_t acc[K][J];  /* SAD sums */
+};
+
+/* the lane's partial sums over prev rows [p0, p1) of every column chunk */
+template <int J, int K, int D, int CW, bool LDSP>
+__device__ __forceinline__ void vsg_sad_acc(const VSGTile& t, int lid, int p0, int p1, VSGLane<J, K, D, CW, LDSP>& L) {
+  const int S = t.S, s = t.s, m = t.m;
   const int u = lid % t.U;
   int v = lid / t.U;
-  const bool laneOk = v < t.V;
-  if (!laneOk) v = t.V - 1;
+  L.ok = v < t.V;
+  if (!L.ok) v = t.V - 1;
   const int a0 = t.alo + (u % m) + m * J * (u / m);
   const int b0 = t.alo + K * v;
+  L.a0 = a0;
+  L.b0 = b0;
   const int prow0 = t.fyc - S / 2 + t.org + b0 * s + t.oy;
   const int pcol0 = (t.fxc - S / 2 + t.org + a0 * s + t.ox) * t.bpp;
   const int wordsPerRow = (S * t.bpp) >> 2;
   const int pitchWords = t.pitch >> 2;
 
-  uint32_t acc[K][J];
 #pragma unroll
   for (int k = 0; k < K; ++k)
 #pragma unroll
-    for (int j = 0; j < J; ++j) acc[k][j] = 0;
+    for (int j = 0; j < J; ++j) L.acc[k][j] = 0;
 
   constexpr int NP = CW + D * (J - 1);   /* shifted prev words per row segment */
   const int nchunks = wordsPerRow / CW;
-  const int prows = S + (K - 1) * s;
   for (int c = 0; c < nchunks; ++c) {
-    const uintptr_t start = reinterpret_cast<uintptr_t>(t.prev + (size_t)prow0 * t.pitch + pcol0 + 4 * CW * c);
+    const uintptr_t start =
+        reinterpret_cast<uintptr_t>(t.prev + (size_t)(prow0 + p0) * t.pitch + pcol0 + 4 * CW * c);
     vsg_gword_ptr wp = (vsg_gword_ptr)(start & ~(uintptr_t)3);
     const uint32_t sh = (uint32_t)(start & 3);
     vsg_cword_ptr cbase = t.patch + CW * c;
     const uint32_t* lbase = t.lpatch + CW * c;
-    for (int p = 0; p < prows; ++p) {
+    for (int p = p0; p < p1; ++p) {
       uint32_t raw[NP + 1];
 #pragma unroll
       for (int q = 0; q <= NP; ++q) raw[q] = wp[q];
@@ -88,26 +99,38 @@ __device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int
           for (int w = 0; w < CW; ++w) {
             const uint32_t cw = LDSP ? lrow[w] : crow[w];
 #pragma unroll
-            for (int j = 0; j < J; ++j) acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw, acc[k][j]);
+            for (int j = 0; j < J; ++j) L.acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw, L.acc[k][j]);
           }
         }
       }
       wp += pitchWords;
     }
   }
+}
 
+/* the lane's best (sad << 32 | rank) over its complete sums, ~0 for idle lanes */
+template <int J, int K, int D, int CW, bool LDSP>
+__device__ __forceinline__ unsigned long long vsg_lane_best(const VSGTile& t, const VSGLane<J, K, D, CW, LDSP>& L) {
+  const int ahi = t.alo + t.G - 1;
   unsigned long long best = ~0ull;
 #pragma unroll
   for (int k = 0; k < K; ++k)
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const int ga = a0 + m * j, gb = b0 + k;
-      if (laneOk && ga <= ahi && gb <= ahi) {
-        const unsigned long long key = ((unsigned long long)acc[k][j] << 32) | (uint32_t)vsg_grid_rank(t, ga, gb);
+      const int ga = L.a0 + t.m * j, gb = L.b0 + k;
+      if (L.ok && ga <= ahi && gb <= ahi) {
+        const unsigned long long key = ((unsigned long long)L.acc[k][j] << 32) | (uint32_t)vsg_grid_rank(t, ga, gb);
         best = key < best ? key : best;
       }
     }
   return best;
+}
+
+template <int J, int K, int D, int CW, bool LDSP>
+__device__ __forceinline__ unsigned long long vsg_sad_lane(const VSGTile& t, int lid) {
+  VSGLane<J, K, D, CW, LDSP> L;
+  vsg_sad_acc<J, K, D, CW, LDSP>(t, lid, 0, t.S + (K - 1) * t.s, L);
+  return vsg_lane_best<J, K, D, CW, LDSP>(t, L);
 }
 
 /* minimum over the 64 lanes of a wave */

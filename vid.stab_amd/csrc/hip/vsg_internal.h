@@ -106,12 +106,17 @@ hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st);
 
 /* the learned form's proposal window search over a job list (either stage):
    pred holds (dx, dy) per (frame of the batch, field), npred fields per
-   frame, frame = job slot - slot0; accepted jobs get their result and
-   field -1 in `fallback`, the others are copied to `fallback` for the exact
-   search; stats[0] += jobs, stats[1] += fallbacks, stats[2..4] += fallbacks
-   out of range / window minimum on the edge / above the residual threshold */
+   frame, frame = job slot - slot0; accepted jobs get their result, the
+   index of every other job goes to list[stats[1]++] for the exact search
+   (in no particular order); stats[0] += jobs, stats[2..4] += fallbacks out
+   of range / window minimum on the edge / above the residual threshold.
+   stats[1] must be 0 at the launch. */
 hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npred, int slot0, uint32_t maxSad,
-                             VSGJob* fallback, unsigned* stats, hipStream_t st);
+                             int* list, unsigned* stats, hipStream_t st);
+/* the exact search (grid + refinement) of the jobs a.jobs[list[i]],
+   i < *count (a device count, at most a.njobs), results to
+   a.results[list[i]] */
+hipError_t vsg_launch_fallback(const VSGSearchArgs& a, const int* list, const unsigned* count, hipStream_t st);
 
 /* one clip's stage inside a multi-clip search launch */
 struct VSGMultiSeg {

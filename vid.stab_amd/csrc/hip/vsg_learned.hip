@@ -14,8 +14,10 @@
  *   4. k_sad_window searches the 5x5 window of 1-px shifts around each
  *      proposal; a minimum inside the window whose SAD is within the typical
  *      residual of the exact search is the field's result;
- *   5. every other field goes through the exact full search (the plain
- *      coarse kernel) -- the fallback that keeps the form honest.
+ *   5. every other field is listed for the exact full search (the same
+ *      grid search and refinement as the plain form, k_sad_fallback spreading
+ *      each listed job over more waves) -- the fallback that keeps the form
+ *      honest.
  * The fine stage works the same way: the network output is also pooled over
  * every fine field's window when the network runs, and the fine search of
  * a field becomes a 5x5 window search around that displacement less the
@@ -92,8 +94,8 @@ struct VSGLearned {
   bool fine = true;                /* proposals for the fine stage too */
   int* d_winF = nullptr;           /* [B][nff][5] */
   float* d_predF[VSG_NSETS] = {};  /* [B][nff][2]: kept per set until its fine stage */
-  VSGJob* d_fjobs[VSG_NSETS] = {};
-  VSGJob* d_fjobsF[VSG_NSETS] = {};
+  int* d_flist[VSG_NSETS] = {};    /* coarse jobs left to the exact search */
+  int* d_flistF[VSG_NSETS] = {};   /* fine jobs left to the exact search */
   unsigned* d_stats[VSG_NSETS] = {};   /* 5 counters per stage: coarse, then fine */
   unsigned* h_stats[VSG_NSETS] = {};
   long long fields[2] = {0, 0}, fallbacks[2] = {0, 0}, why[2][3] = {{0, 0, 0}, {0, 0, 0}};
@@ -122,8 +124,8 @@ void vsg_learned_free(VSGContext* c) {
     if (b) (void)hipFree(b);
   if (L->netDone) (void)hipEventDestroy(L->netDone);
   for (int s = 0; s < VSG_NSETS; ++s) {
-    if (L->d_fjobs[s]) (void)hipFree(L->d_fjobs[s]);
-    if (L->d_fjobsF[s]) (void)hipFree(L->d_fjobsF[s]);
+    if (L->d_flist[s]) (void)hipFree(L->d_flist[s]);
+    if (L->d_flistF[s]) (void)hipFree(L->d_flistF[s]);
     if (L->d_predF[s]) (void)hipFree(L->d_predF[s]);
     if (L->d_stats[s]) (void)hipFree(L->d_stats[s]);
     if (L->h_stats[s]) (void)hipHostFree(L->h_stats[s]);
@@ -220,8 +222,8 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
       break;
     }
     for (int s = 0; s < VSG_NSETS; ++s) {
-      if (hipMalloc(&L->d_fjobs[s], sizeof(VSGJob) * (size_t)B * nc) != hipSuccess ||
-          hipMalloc(&L->d_fjobsF[s], sizeof(VSGJob) * (size_t)B * nff) != hipSuccess ||
+      if (hipMalloc(&L->d_flist[s], sizeof(int) * (size_t)B * nc) != hipSuccess ||
+          hipMalloc(&L->d_flistF[s], sizeof(int) * (size_t)B * nff) != hipSuccess ||
           hipMalloc(&L->d_predF[s], sizeof(float) * 2 * (size_t)B * nff) != hipSuccess ||
           hipMalloc(&L->d_stats[s], sizeof(unsigned) * 10) != hipSuccess ||
           hipHostMalloc(&L->h_stats[s], sizeof(unsigned) * 10, hipHostMallocDefault) != hipSuccess) {
@@ -295,13 +297,11 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
       (L->fine && vsgc_field_pool2(L->d_out, L->Hq, L->Wq, L->d_winF, n * c->st[1].nf, L->d_predF[set], st) != 0))
     return vsg_fail("proposal pooling", hipErrorLaunchFailure);
   VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
-  LCHECK(vsg_launch_window(ga, L->d_pred, nc, slot0, max_sad(L, c->st[0].S), L->d_fjobs[set], L->d_stats[set], st));
+  LCHECK(vsg_launch_window(ga, L->d_pred, nc, slot0, max_sad(L, c->st[0].S), L->d_flist[set], L->d_stats[set], st));
   /* the window search was the last reader of the shared network buffers */
   LCHECK(hipEventRecord(L->netDone, st));
   L->netLastSet = set;
-  VSGSearchArgs fb = ga;
-  fb.jobs = L->d_fjobs[set];
-  LCHECK(vsg_launch_search(fb, st));
+  LCHECK(vsg_launch_fallback(ga, L->d_flist[set], L->d_stats[set] + 1, st));
   LCHECK(hipMemcpyAsync(L->h_stats[set], L->d_stats[set], sizeof(unsigned) * 5, hipMemcpyDeviceToHost, st));
   return 0;
 }
@@ -315,10 +315,9 @@ int vsg_learned_fine(VSGContext* c, int set, int njobs, hipStream_t st) {
   VSGSet& s = c->set[set];
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set] + 5, 5);
   VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
-  LCHECK(vsg_launch_window(fa, L->d_predF[set], c->st[1].nf, set * c->B, max_sad(L, c->st[1].S), L->d_fjobsF[set],
+  LCHECK(vsg_launch_window(fa, L->d_predF[set], c->st[1].nf, set * c->B, max_sad(L, c->st[1].S), L->d_flistF[set],
                            L->d_stats[set] + 5, st));
-  fa.jobs = L->d_fjobsF[set];
-  LCHECK(vsg_launch_search(fa, st));
+  LCHECK(vsg_launch_fallback(fa, L->d_flistF[set], L->d_stats[set] + 6, st));
   LCHECK(hipMemcpyAsync(L->h_stats[set] + 5, L->d_stats[set] + 5, sizeof(unsigned) * 5, hipMemcpyDeviceToHost, st));
   return 0;
 }

@@ -207,18 +207,8 @@ __device__ void refine_job(const VSGSearchArgs& a, const VSGJob& jb, unsigned lo
   }
 }
 
-/* ---- grid search + refinement ---------------------------------------------------
-   Workgroup (job, tile) searches its share of the job's candidate grid.  With
-   one tile per job the workgroup's argmin is reduced in LDS; otherwise every
-   wave folds its argmin into best[job] with a u64 atomic min, and the
-   workgroup that finishes the job last (counted in done[job]) reads the
-   winner and runs the refinement, then resets best / done for the next
-   launch.  The refinement of early jobs overlaps the grid of later ones. */
-template <int J, int K, int D, int CW, bool LDSP>
-__device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int tile, RefineLds& L) {
-  extern __shared__ uint32_t lpatch[];   /* LDSP: the field, S*S*bpp/4 words */
-  const VSGJob jb = a.jobs[job];
-  if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
+/* job jb's search geometry (the LDS patch pointer is left null) */
+__device__ __forceinline__ VSGTile make_tile(const VSGSearchArgs& a, const VSGJob& jb) {
   VSGTile t;
   t.S = a.g.S;
   t.s = a.g.step;
@@ -235,10 +225,27 @@ __device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int t
   t.oy = jb.oy;
   t.pitch = a.pitch;
   t.prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
-  const size_t patchBytes = (size_t)t.S * t.S * t.bpp;
   t.patch = (vsg_cword_ptr)(a.patches + (size_t)jb.slot * a.patchSlotStride + a.stageOff +
-                            (size_t)jb.field * patchBytes);
+                            (size_t)jb.field * t.S * t.S * t.bpp);
+  t.lpatch = nullptr;
+  return t;
+}
+
+/* ---- grid search + refinement ---------------------------------------------------
+   Workgroup (job, tile) searches its share of the job's candidate grid.  With
+   one tile per job the workgroup's argmin is reduced in LDS; otherwise every
+   wave folds its argmin into best[job] with a u64 atomic min, and the
+   workgroup that finishes the job last (counted in done[job]) reads the
+   winner and runs the refinement, then resets best / done for the next
+   launch.  The refinement of early jobs overlaps the grid of later ones. */
+template <int J, int K, int D, int CW, bool LDSP>
+__device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int tile, RefineLds& L) {
+  extern __shared__ uint32_t lpatch[];   /* LDSP: the field, S*S*bpp/4 words */
+  const VSGJob jb = a.jobs[job];
+  if (jb.field < 0) return;   /* unselected slot of a fixed-size job list */
+  VSGTile t = make_tile(a, jb);
   t.lpatch = lpatch;
+  const size_t patchBytes = (size_t)t.S * t.S * t.bpp;
   if (LDSP) {
     /* the whole workgroup reads the same field words: stage them once, and
        every lane then reads them by LDS broadcast instead of scalar loads
@@ -317,6 +324,66 @@ __global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict
   search_wg<J, K, D, CW, LDSP>(sg->a, job, local - job * sg->a.tiles, L);
 }
 
+/* ---- exact search of a fallback list ----------------------------------------------
+   A fallback list holds few jobs, so its launch lasts as long as one job's
+   search.  The job's candidates are cut into 64-lane tiles with the plan's K
+   forced to 4 (the most lanes per job), and the 4 waves of a workgroup share
+   one tile, each summing a quarter of the prev rows; the partial sums meet
+   in LDS, so a tile's latency is a quarter of a grid-kernel lane's.  A fixed
+   grid walks the units (list entry, tile) of the count that the window
+   search left; the job's last workgroup refines it, as in the grid kernel. */
+constexpr int kFallbackBlocks = 2048;
+
+template <int J, int D, int CW>
+__global__ __launch_bounds__(256) void k_sad_fallback(VSGSearchArgs a, const int* __restrict__ list,
+                                                      const unsigned* __restrict__ count) {
+  constexpr int K = 4;
+  __shared__ RefineLds L;
+  __shared__ uint32_t part[3][K * J][64];
+  const unsigned n = min(*count, (unsigned)a.njobs);
+  const unsigned units = n * (unsigned)a.tiles;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (unsigned u = blockIdx.x; u < units; u += gridDim.x) {
+    const int job = list[u / (unsigned)a.tiles];
+    const int tile = (int)(u % (unsigned)a.tiles);
+    const VSGJob jb = a.jobs[job];
+    const VSGTile t = make_tile(a, jb);
+    const int prows = t.S + (K - 1) * t.s;
+    VSGLane<J, K, D, CW, false> ln;
+    vsg_sad_acc<J, K, D, CW, false>(t, tile * 64 + lane, wave * prows / 4, (wave + 1) * prows / 4, ln);
+    if (wave > 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int j = 0; j < J; ++j) part[wave - 1][k * J + j][lane] = ln.acc[k][j];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+          ln.acc[k][j] += part[0][k * J + j][lane] + part[1][k * J + j][lane] + part[2][k * J + j][lane];
+      const unsigned long long best = vsg_wave_min_u64(vsg_lane_best<J, K, D, CW, false>(t, ln));
+      if (lane == 0) {
+        if (best != ~0ull) atomicMin(&a.best[job], best);
+        __threadfence();   /* the min is visible before this workgroup counts as done */
+        const unsigned before = atomicAdd(&a.done[job], 1u);
+        L.last = before == (unsigned)a.tiles - 1;
+        if (L.last) {
+          __threadfence();
+          L.key = __hip_atomic_load(&a.best[job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          a.best[job] = ~0ull;   /* ready for the next launch */
+          a.done[job] = 0;
+        }
+      }
+    }
+    __syncthreads();
+    if (L.last) refine_job(a, jb, L.key, job, L);
+    __syncthreads();   /* part[] and L are rewritten by the next unit */
+  }
+}
+
 /* ---- proposal window (learned form) -----------------------------------------------
    Job j's field gets proposal p = round(4 * pred) - (ox, oy): the network's
    displacement (quarter-resolution units) of the field, less the job's
@@ -327,73 +394,183 @@ __global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict
    y inner, strict "<"); while its minimum lies on the window's edge the
    window moves to that minimum (a descent, kWindowSteps windows at most).
    The result is accepted when the minimum is inside the window and its SAD
-   is at most maxSad; then it is written to results[j] and the job is
-   removed from the fallback list, otherwise the job goes to the fallback
-   list for the exact full search. */
+   is at most maxSad; then it is written to results[j], otherwise j is
+   appended to the fallback list for the exact full search.
+
+   A job runs on WJ waves (1 for fields of at most 64 px, else 4; a 256-
+   thread workgroup holds 4 / WJ jobs).  Thread t of a job takes word
+   w = t % (S/4) of a run of consecutive field rows (run g = t / (S/4)), so
+   the 5 previous-frame rows a field row meets at the window's 5 vertical
+   shifts are shared with the next row: each field row costs one patch word,
+   two (unaligned) dwords of one new previous row, 4 alignbytes for the 5
+   horizontal shifts and the 25 v_sad_u8.  The 25 sums are reduced across
+   the wave by a transposing butterfly (a lane keeps half of its values and
+   sends the other half at each of 5 steps: 31 exchanges), after which lane l
+   holds candidate l / 2's wave total, and the scan's winner is the minimum
+   of (sad, scan rank) over the lanes. */
 constexpr int kWindowSteps = 4;
 
-__global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float* __restrict__ pred, int npred,
-                                                    int slot0, uint32_t maxSad, VSGJob* __restrict__ fallback,
-                                                    unsigned* __restrict__ stats) {
-  __shared__ RefineLds L;
-  const int job = blockIdx.x;
-  if (job >= a.njobs) return;
-  const VSGJob jb = a.jobs[job];
-  if (jb.field < 0) {
-    if (threadIdx.x == 0) fallback[job] = jb;
-    return;
-  }
-  const float* pp = pred + 2 * ((size_t)(jb.slot - slot0) * npred + jb.field);
-  const float fx = rintf(4.0f * pp[0]) - (float)jb.ox, fy = rintf(4.0f * pp[1]) - (float)jb.oy;
-  const float lim = (float)(a.g.n * a.g.step - 2);   /* the full search's reach, less the window */
-  bool inRange = fx >= -lim && fx <= lim && fy >= -lim && fy <= lim, interior = false;
-  int bx = 0, by = 0;
-  uint32_t minerr = 0;
-  if (inRange) {
-    /* a minimum on the window's edge moves the window there, up to
-       kWindowSteps windows in all; every thread sees the same sums, so the
-       walk is uniform across the workgroup */
-    int px = (int)fx, py = (int)fy;
-    for (int it = 0; it < kWindowSteps; ++it) {
-      level_sums(a, jb, px - 2, py - 2, 1, 5, 5, L);
-      bx = px;
-      by = py;
-      minerr = L.sums[2 * kMaxNi + 2];
-      for (int ti = 0; ti < 5; ++ti)
-        for (int tj = 0; tj < 5; ++tj) {
-          const uint32_t e = L.sums[ti * kMaxNi + tj];
-          if (!(ti == 2 && tj == 2) && e < minerr) {
-            minerr = e;
-            bx = px - 2 + ti;
-            by = py - 2 + tj;
+struct WindowLds {
+  uint32_t part[2][4][32];   /* [iteration parity][wave][candidate] (4-wave jobs) */
+  unsigned cnt[5];           /* the workgroup's outcome counts (cnt[1] unused) */
+};
+
+__device__ __forceinline__ uint32_t ld_any(const uint8_t* p) { return *reinterpret_cast<const u32_any*>(p); }
+
+/* rank of candidate c = ti*5 + tj in the reference's scan: the centre first,
+   then x outer / y inner */
+__device__ __forceinline__ uint32_t window_rank(int c) { return c == 12 ? 0u : (c < 12 ? (uint32_t)c + 1u : (uint32_t)c); }
+
+/* the job's 25 window SADs at shifts (x0 + ti, y0 + tj), left in lane l as
+   candidate (l >> 1)'s total over the wave (lanes of candidates >= 25: 0) */
+template <int WJ>
+__device__ uint32_t window_wave_sums(const VSGSearchArgs& a, const VSGJob& jb, int x0, int y0, int t) {
+  const int S = a.g.S, W4 = S >> 2;
+  const int groups = (64 * WJ) / W4;
+  const int run = (S + groups - 1) / groups;
+  const int w = t % W4, g = t / W4;
+  const int r0 = g * run;
+  const int len = g < groups ? min(run, S - r0) : 0;
+  uint32_t acc[32];
+#pragma unroll
+  for (int c = 0; c < 32; ++c) acc[c] = 0;
+  if (len > 0) {
+    const uint8_t* prev = jb.prev < 0 ? a.ref : a.blurred + (size_t)jb.prev * a.slotStride;
+    const uint32_t* patch = reinterpret_cast<const uint32_t*>(a.patches + (size_t)jb.slot * a.patchSlotStride +
+                                                              a.stageOff + (size_t)jb.field * S * S) +
+                            (size_t)r0 * W4 + w;
+    const uint8_t* row = prev + (size_t)(a.fy[jb.field] - S / 2 + y0 + jb.oy + r0) * a.pitch +
+                         (a.fx[jb.field] - S / 2 + x0 + jb.ox) + 4 * w;
+    const int pitch = a.pitch;
+    /* sw[q % 5][i]: previous row q (relative to the run) shifted by i bytes */
+    uint32_t sw[5][5];
+/* v_alignbyte_b32 takes the shift modulo 4: shift 4 is the high dword itself */
+#define WIN_SHIFTS(slot, lo, hi)                                               \
+  _Pragma("unroll") for (int i = 0; i < 4; ++i) sw[slot][i] = __builtin_amdgcn_alignbyte(hi, lo, i); \
+  sw[slot][4] = hi;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t lo = ld_any(row + (size_t)q * pitch), hi = ld_any(row + (size_t)q * pitch + 4);
+      WIN_SHIFTS(q, lo, hi)
+    }
+    uint32_t nlo = ld_any(row + (size_t)4 * pitch), nhi = ld_any(row + (size_t)4 * pitch + 4);
+    uint32_t nc = patch[0];
+    for (int yb = 0; yb < len; yb += 5) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int y = yb + k;
+        if (y < len) {
+          WIN_SHIFTS((k + 4) % 5, nlo, nhi)
+          const uint32_t c = nc;
+          if (y + 1 < len) {
+            const uint8_t* pr = row + (size_t)(y + 5) * pitch;
+            nlo = ld_any(pr);
+            nhi = ld_any(pr + 4);
+            nc = patch[(size_t)(y + 1) * W4];
           }
+#pragma unroll
+          for (int tj = 0; tj < 5; ++tj)
+#pragma unroll
+            for (int ti = 0; ti < 5; ++ti)
+              acc[ti * 5 + tj] = __builtin_amdgcn_sad_u8(sw[(k + tj) % 5][ti], c, acc[ti * 5 + tj]);
         }
-      interior = bx > px - 2 && bx < px + 2 && by > py - 2 && by < py + 2;
-      if (interior) break;
-      px = bx;
-      py = by;
-      inRange = px >= -lim && px <= lim && py >= -lim && py <= lim;
-      if (!inRange) break;
+      }
+    }
+#undef WIN_SHIFTS
+  }
+  /* transposing butterfly: after the step over lane bit m (32 .. 2), a lane
+     holds the half of the candidates whose index bit matches its lane bit */
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int h = 16; h >= 1; h >>= 1) {
+    const bool up = (lane & (2 * h)) != 0;
+#pragma unroll
+    for (int k = 0; k < h; ++k) {
+      const uint32_t keep = up ? acc[k + h] : acc[k];
+      const uint32_t send = up ? acc[k] : acc[k + h];
+      acc[k] = keep + (uint32_t)__shfl_xor((int)send, 2 * h);
     }
   }
-  const bool accept = inRange && interior && minerr <= maxSad;
-  if (threadIdx.x == 0) {
-    VSGJob f = jb;
-    if (accept) {
+  return acc[0] + (uint32_t)__shfl_xor((int)acc[0], 1);
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+  return v;
+}
+
+template <int WJ>
+__global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float* __restrict__ pred, int npred,
+                                                    int slot0, uint32_t maxSad, int* __restrict__ list,
+                                                    unsigned* __restrict__ stats) {
+  __shared__ WindowLds L;
+  constexpr int JB = 4 / WJ;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int jw = wave / WJ;                      /* the job's index in the workgroup */
+  const int t = threadIdx.x - jw * 64 * WJ;      /* thread index within the job */
+  const int job = blockIdx.x * JB + jw;
+  if (threadIdx.x < 5) L.cnt[threadIdx.x] = 0;
+  __syncthreads();
+  /* job and everything derived from it are uniform over the job's waves */
+  const bool live = job < a.njobs;
+  VSGJob jb;
+  jb.field = -1;
+  if (live) jb = a.jobs[job];
+  int outcome = -1;   /* -1: skipped, 0: accepted, 2: out of range, 3: edge, 4: residual */
+  if (live && jb.field >= 0) {
+    const float* pp = pred + 2 * ((size_t)(jb.slot - slot0) * npred + jb.field);
+    const float fx = rintf(4.0f * pp[0]) - (float)jb.ox, fy = rintf(4.0f * pp[1]) - (float)jb.oy;
+    const float lim = (float)(a.g.n * a.g.step - 2);   /* the full search's reach, less the window */
+    bool inRange = fx >= -lim && fx <= lim && fy >= -lim && fy <= lim, interior = false;
+    int bx = 0, by = 0;
+    uint32_t minerr = 0;
+    if (inRange) {
+      /* a minimum on the window's edge moves the window there, up to
+         kWindowSteps windows in all */
+      int px = (int)fx, py = (int)fy;
+      for (int it = 0; it < kWindowSteps; ++it) {
+        uint32_t tot = window_wave_sums<WJ>(a, jb, px - 2, py - 2, t);
+        const int cand = lane >> 1;
+        if (WJ > 1) {
+          uint32_t(*part)[32] = L.part[it & 1];
+          if ((lane & 1) == 0) part[wave][cand] = tot;
+          __syncthreads();
+          tot = 0;
+#pragma unroll
+          for (int q = 0; q < WJ; ++q) tot += part[q][cand];
+        }
+        const uint32_t key = wave_min(cand < 25 ? (tot << 5) | window_rank(cand) : 0xffffffffu);
+        const int c = (int)(key & 31u) == 0 ? 12 : ((int)(key & 31u) <= 12 ? (int)(key & 31u) - 1 : (int)(key & 31u));
+        minerr = key >> 5;
+        bx = px - 2 + c / 5;
+        by = py - 2 + c % 5;
+        interior = bx > px - 2 && bx < px + 2 && by > py - 2 && by < py + 2;
+        if (interior) break;
+        px = bx;
+        py = by;
+        inRange = px >= -lim && px <= lim && py >= -lim && py <= lim;
+        if (!inRange) break;
+      }
+    }
+    const bool accept = inRange && interior && minerr <= maxSad;
+    outcome = accept ? 0 : (!inRange ? 2 : (!interior ? 3 : 4));
+    if (t == 0 && accept) {
       VSGResult res;
       res.tx = (int16_t)bx;
       res.ty = (int16_t)by;
       res.minerror = minerr;
       a.results[job] = res;
-      f.field = -1;
     }
-    fallback[job] = f;
-    atomicAdd(&stats[0], 1u);
-    if (!accept) atomicAdd(&stats[1], 1u);
-    if (!inRange) atomicAdd(&stats[2], 1u);
-    else if (!interior) atomicAdd(&stats[3], 1u);
-    else if (!accept) atomicAdd(&stats[4], 1u);
   }
+  if (live && t == 0 && outcome >= 0) {
+    atomicAdd(&L.cnt[0], 1u);
+    if (outcome > 0) {
+      list[atomicAdd(&stats[1], 1u)] = job;
+      atomicAdd(&L.cnt[outcome], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 5 && threadIdx.x != 1 && L.cnt[threadIdx.x]) atomicAdd(&stats[threadIdx.x], L.cnt[threadIdx.x]);
 }
 
 template <int J, int K, int D, int CW>
@@ -459,6 +636,20 @@ hipError_t launch_multi_k(const VSGMultiSeg* segs, int nseg, int units, const VS
     case 6: return launch_multi_cw<J, 6, D>(segs, nseg, units, g, st);
     default: return launch_multi_cw<J, 4, D>(segs, nseg, units, g, st);
   }
+}
+
+template <int J, int D>
+hipError_t launch_fallback(const VSGSearchArgs& b, const int* list, const unsigned* count, hipStream_t st) {
+  const int nb = min(kFallbackBlocks, b.njobs * b.tiles);
+  switch (b.g.CW) {
+    case 28: hipLaunchKernelGGL((k_sad_fallback<J, D, 28>), dim3(nb), dim3(256), 0, st, b, list, count); break;
+    case 20: hipLaunchKernelGGL((k_sad_fallback<J, D, 20>), dim3(nb), dim3(256), 0, st, b, list, count); break;
+    case 16: hipLaunchKernelGGL((k_sad_fallback<J, D, 16>), dim3(nb), dim3(256), 0, st, b, list, count); break;
+    case 12: hipLaunchKernelGGL((k_sad_fallback<J, D, 12>), dim3(nb), dim3(256), 0, st, b, list, count); break;
+    case 8: hipLaunchKernelGGL((k_sad_fallback<J, D, 8>), dim3(nb), dim3(256), 0, st, b, list, count); break;
+    default: hipLaunchKernelGGL((k_sad_fallback<J, D, 4>), dim3(nb), dim3(256), 0, st, b, list, count); break;
+  }
+  return hipGetLastError();
 }
 
 int gcd_int(int x, int y) {
@@ -554,9 +745,27 @@ hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, 
 }
 
 hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npred, int slot0, uint32_t maxSad,
-                             VSGJob* fallback, unsigned* stats, hipStream_t st) {
+                             int* list, unsigned* stats, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
-  if (a.g.S > 256 || npred <= 0 || slot0 < 0 || a.g.bpp != 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_sad_window, dim3(a.njobs), dim3(256), 0, st, a, pred, npred, slot0, maxSad, fallback, stats);
+  if (a.g.S > 256 || a.g.S < 16 || (a.g.S & 3) || npred <= 0 || slot0 < 0 || a.g.bpp != 1)
+    return hipErrorInvalidValue;
+  if (a.g.S <= 64)
+    hipLaunchKernelGGL(k_sad_window<1>, dim3((a.njobs + 3) / 4), dim3(256), 0, st, a, pred, npred, slot0, maxSad,
+                       list, stats);
+  else
+    hipLaunchKernelGGL(k_sad_window<4>, dim3(a.njobs), dim3(256), 0, st, a, pred, npred, slot0, maxSad, list,
+                       stats);
   return hipGetLastError();
+}
+
+hipError_t vsg_launch_fallback(const VSGSearchArgs& a, const int* list, const unsigned* count, hipStream_t st) {
+  if (a.njobs <= 0) return hipSuccess;
+  if (!list || !count || a.g.S > 256 || a.g.S * a.g.bpp > 4 * 256) return hipErrorInvalidValue;
+  VSGSearchArgs b = a;
+  b.g.K = 4;
+  b.g.V = (b.g.G + 3) / 4;
+  b.tiles = (b.g.U * b.g.V + 63) / 64;
+  if (b.g.J == 4 && b.g.D == 1) return launch_fallback<4, 1>(b, list, count, st);
+  if (b.g.J == 4 && b.g.D == 3) return launch_fallback<4, 3>(b, list, count, st);
+  return launch_fallback<1, 1>(b, list, count, st);
 }
