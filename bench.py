@@ -114,6 +114,25 @@ def measure_network_layers(weights, n=32, h=540, w=960, reps=5):
         res.append({"layer": k, "dilation": d, "ms": round(ms, 4), "tflops": round(flop / ms / 1e9, 1)})
         log(f"network layer {k} (dilation {d}): {ms:.3f} ms per {n} frames of {w}x{h}, {flop / ms / 1e9:.1f} TFLOP/s")
     log(f"network: {sum(r['ms'] for r in res):.3f} ms per {n} frames")
+    if len(packed) >= 3:
+        # layers 0 and 1 in one pass (layer 0's map stays on chip), as the
+        # learned detect path runs them
+        (pk0, b0, d0), (pk1, b1, d1) = packed[0], packed[1]
+        run = lambda: L.vsgc_conv16_pair2(P(planes), h * w, P(planes), P(prev), 1, n, h, w, P(pk0), P(b0), d0,
+                                          P(pk1), P(b1), d1, 0, P(acts[1]), None, st)
+        if run() != 0:
+            raise RuntimeError("fused layers 0+1 failed")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res.append({"layer": "0+1 fused", "dilation": [d0, d1], "ms": round(ms, 4)})
+        net = ms + sum(r["ms"] for r in res[2:-1])
+        log(f"network layers 0+1 fused: {ms:.3f} ms per {n} frames; network with the fused pair: {net:.3f} ms")
     return res
 
 

@@ -39,9 +39,12 @@ VS_API int vsgc_conv_k(const uint16_t* in, int n, int h, int w, const uint16_t* 
    (v_mfma_f32_32x32x16_bf16), used by the learned detect path: weights
    packed by vsgc_pack_weights16 (9*2*64*8 BF16 values, forward only);
    inChannels <= 16 uses one K step per tap.  Dilations up to 8 run the
-   LDS-tiled kernel (4 output rows spaced by the dilation per workgroup),
-   larger ones the direct kernel; both accumulate the taps in the same order,
-   so their results are identical.  mode 3 (tiled dilations only) writes FP32
+   LDS-tiled kernel (4 output rows spaced by the dilation per workgroup) or,
+   for 32 input channels, the band walker (a persistent workgroup walking
+   down a 64-px band with its input rows in an LDS ring;
+   VIDSTAB_AMD_CONV_WALK=0 selects the tiled kernel), larger ones the direct
+   kernel; all accumulate the taps in the same order, so their results are
+   identical.  mode 3 (tiled dilations only) writes FP32
    channels 0 and 1 alone, as an NHWC-2 map [n][h][w][2]. */
 VS_API int vsgc_pack_weights16(const float* w, uint16_t* packed16, void* stream);
 VS_API int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* packed16, const float* bias,
@@ -55,6 +58,17 @@ VS_API int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16_t* 
 VS_API int vsgc_conv16_pair(const uint16_t* planes, size_t planeStride, const uint16_t* refPlane,
                             const int* prevIdx, int curIdx0, int n, int h, int w, const uint16_t* packed16,
                             const float* bias, int dil, int mode, uint16_t* outb, float* outf, void* stream);
+/* the first two layers in one pass: layer 0 (packed0, bias0, dilation
+   dil0, BF16 + ReLU) on the plane pair as vsgc_conv16_pair reads it, and
+   layer 1 (packed1, bias1, dil1 <= 8, 32 input channels, BF16 output:
+   mode 0 or 1, into outb) on its result, which never leaves the chip: the
+   band walker computes each staged row of layer 1's input with layer 0's
+   MFMAs.  dil0 <= 8.  The result equals vsgc_conv16_pair (mode 0) followed
+   by vsgc_conv16 (inChannels 32), bit for bit. */
+VS_API int vsgc_conv16_pair2(const uint16_t* planes, size_t planeStride, const uint16_t* refPlane,
+                             const int* prevIdx, int curIdx0, int n, int h, int w, const uint16_t* packed0,
+                             const float* bias0, int dil0, const uint16_t* packed1, const float* bias1, int dil1,
+                             int mode, uint16_t* outb, float* outf, void* stream);
 /* vsgc_conv16 on the direct kernel (one 32-px row tile per wave, every A
    fragment loaded from global memory) at any dilation */
 VS_API int vsgc_conv16_direct(const uint16_t* in, int n, int h, int w, const uint16_t* packed16,

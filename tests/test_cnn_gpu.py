@@ -222,6 +222,55 @@ def test_pair_input_layer_equals_conv_on_the_built_map(dil, mode):
         assert np.max(np.abs(outs[0].astype(np.float64) - ref[..., :oc])) <= 1e-3 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("dil0,dil1,mode", [(1, 2, 0), (1, 1, 0), (2, 8, 1), (8, 4, 0), (3, 5, 1)])
+def test_fused_first_two_layers_equal_two_launches(dil0, dil1, mode):
+    """vsgc_conv16_pair2 runs layer 0 on the plane pair inside layer 1's band
+    walker; its output equals vsgc_conv16_pair (BF16 + ReLU) followed by
+    vsgc_conv16 on that map, bit for bit, on a frame whose width is not a
+    multiple of the 64-px band (right-edge padding) and with a reference
+    plane for the first sample"""
+    from vidstab_amd import learned
+    torch = _torch()
+    L = learned.lib()
+    P = learned._p
+    rng = np.random.default_rng(90 + 10 * dil0 + dil1 + mode)
+    n, h, w, nplanes = 3, 37, 203, 6
+    planes = co.bf16_round(rng.standard_normal((nplanes, h, w)).astype(np.float32))
+    ref_plane = co.bf16_round(rng.standard_normal((h, w)).astype(np.float32))
+    prev = np.array([-1, 1, 4], dtype=np.int32)
+    cur0 = 2
+    st = learned._stream()
+    pks, dbs = [], []
+    for k in range(2):
+        wt = co.bf16_round(rng.standard_normal((9, 32, 32)).astype(np.float32) * 0.2)
+        b = rng.standard_normal(32).astype(np.float32) * 0.1
+        pk = torch.empty(9 * 2 * 64 * 8, dtype=torch.int16, device="cuda")
+        assert L.vsgc_pack_weights16(P(torch.from_numpy(wt).cuda()), P(pk), st) == 0
+        pks.append(pk)
+        dbs.append(torch.from_numpy(b).cuda())
+    dplanes = torch.from_numpy(learned.bf16_bits(planes)).cuda()
+    dref = torch.from_numpy(learned.bf16_bits(ref_plane)).cuda()
+    dprev = torch.from_numpy(prev).cuda()
+
+    def out_buf():
+        o = torch.full((n, h, w, 32), 7, dtype=torch.int16, device="cuda")
+        return o, P(o), None
+
+    o0, ob, of = out_buf()
+    assert L.vsgc_conv16_pair2(P(dplanes), h * w, P(dref), P(dprev), cur0, n, h, w, P(pks[0]), P(dbs[0]), dil0,
+                               P(pks[1]), P(dbs[1]), dil1, mode, ob, of, st) == 0
+    mid = torch.full((n, h, w, 32), 7, dtype=torch.int16, device="cuda")
+    assert L.vsgc_conv16_pair(P(dplanes), h * w, P(dref), P(dprev), cur0, n, h, w, P(pks[0]), P(dbs[0]), dil0, 0,
+                              P(mid), None, st) == 0
+    o1, ob, of = out_buf()
+    assert L.vsgc_conv16(P(mid), n, h, w, P(pks[1]), P(dbs[1]), dil1, mode, 32, ob, of, st) == 0
+    a0, a1 = o0.cpu().numpy(), o1.cpu().numpy()
+    assert a0.view(np.uint8).tobytes() == a1.view(np.uint8).tobytes()
+    # FP32 outputs are not offered (layer 1 is never the last layer there)
+    assert L.vsgc_conv16_pair2(P(dplanes), h * w, P(dref), P(dprev), cur0, n, h, w, P(pks[0]), P(dbs[0]), dil0,
+                               P(pks[1]), P(dbs[1]), dil1, 2, None, P(o0), st) != 0
+
+
 def test_two_channel_output_and_pooling():
     """mode 3 writes channels 0 and 1 of mode 2's FP32 output, and
     vsgc_field_pool2 over it equals vsgc_field_pool over the 32-channel map"""

@@ -425,9 +425,93 @@ struct ConvWalkArgs {
   int dil, mode;
   uint16_t* outb;
   float* outf;
-  int steps;     /* 4-row steps of a phase's row sequence (the longest phase) */
+  int steps;     /* R-row steps of a phase's row sequence (the longest phase) */
   int xTiles, segs, segLen, units, perXcd;
+  /* PAIR0: the ring rows are the ReLU output of a first layer over the pair
+     of quarter-resolution planes (as k_conv16t<1, true> reads them) */
+  const uint16_t* planes;
+  size_t planeStride;
+  const uint16_t* refPlane;
+  const int* prevIdx;
+  int curIdx0;
+  const uint16_t* pk0;
+  const float* bias0;
+  int dil0;
 };
+
+/* PAIR0: a wave stages the first layer's input for ring row m: the 3 plane
+   rows y - dil0, y, y + dil0 of its taps, wp = 64 + 2 dil + 2 dil0 pixels
+   from x0 - dil - dil0 on, as (prev | cur << 16) pairs at row stride kPairW,
+   zero outside the image (the first layer's padding).  pair_fetch issues
+   the loads (kPairLd per lane), pair_put writes them to LDS. */
+constexpr int kPairW = 112;   /* >= 64 + 2*8 + 2*8 staged pixels, + 16 read past the end by idle pixels */
+constexpr int kPairLd = (3 * (kWalkW + 4 * kMaxTileDil) + 63) / 64;
+
+struct PairFetch {
+  uint16_t lo[kPairLd], hi[kPairLd];
+};
+
+__device__ __forceinline__ void pair_fetch(const ConvWalkArgs& a, const uint16_t* pl0, const uint16_t* pl1, int ph,
+                                           int x0, int m, PairFetch& f) {
+  const int lane = threadIdx.x & 63;
+  const int y = ph + m * a.dil, xs = x0 - a.dil - a.dil0, wp = kWalkW + 2 * a.dil + 2 * a.dil0;
+#pragma unroll
+  for (int j = 0; j < kPairLd; ++j) {
+    const int idx = lane + 64 * j;
+    const int k = idx / wp, p = idx - k * wp;
+    const int yy = y + (k - 1) * a.dil0, xx = xs + p;
+    const bool ok = k < 3 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+    const size_t o = (size_t)yy * a.W + xx;
+    f.lo[j] = *(ok ? pl0 + o : reinterpret_cast<const uint16_t*>(kZeroPad));
+    f.hi[j] = *(ok ? pl1 + o : reinterpret_cast<const uint16_t*>(kZeroPad));
+  }
+}
+
+__device__ __forceinline__ void pair_put(const ConvWalkArgs& a, const PairFetch& f, uint32_t* pairS) {
+  const int lane = threadIdx.x & 63, wp = kWalkW + 2 * a.dil + 2 * a.dil0;
+#pragma unroll
+  for (int j = 0; j < kPairLd; ++j) {
+    const int idx = lane + 64 * j;
+    const int k = idx / wp, p = idx - k * wp;
+    if (k < 3) pairS[k * kPairW + p] = (uint32_t)f.lo[j] | ((uint32_t)f.hi[j] << 16);
+  }
+}
+
+/* the first layer's B fragments [tap][lane], then each wave's staged rows */
+constexpr int kPair0Lds = 9 * 64 * 16 + 2 * 3 * kPairW * 4;
+
+/* PAIR0: the first layer at tile i of ring row m (taps accumulated in order
+   0..8 from zero, then bias and ReLU, as k_conv16t<1, true> does), stored as
+   BF16 into the row's ring slot; pixels outside the image are the second
+   layer's zero padding */
+template <int RING>
+__device__ __forceinline__ void pair_store(const ConvWalkArgs& a, const uint32_t* pairS, const uint8_t* b0S,
+                                           float bco0, uint8_t* ring, int ph, int x0, int m, int i) {
+  const int lane = threadIdx.x & 63, half = lane >> 5, co = lane & 31;
+  const int pl0 = 32 * i + (lane & 31);   /* the lane's ring pixel (A row) */
+  v16f acc = {};
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    /* channels 0, 1 of the tap's pixel in lanes 0..31; the upper K half is zero */
+    const uint32_t pr = pairS[(tap / 3) * kPairW + pl0 + (tap % 3) * a.dil0];
+    const uint4 u = make_uint4(half ? 0u : pr, 0u, 0u, 0u);
+    const v8bf b = *reinterpret_cast<const v8bf*>(b0S + (tap * 64 + lane) * 16);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const v8bf*>(&u), b, acc, 0, 0, 0);
+  }
+  const int y = ph + m * a.dil;
+  const bool rowOk = y >= 0 && y < a.H;
+  uint8_t* row = ring + (size_t)((m + 1) % RING) * kWalkPad * 64;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int pl = 32 * i + 8 * (e >> 2) + 4 * half + (e & 3), x = x0 - a.dil + pl;
+    if (pl < kWalkW + 2 * a.dil) {
+      float val = acc[e] + bco0;
+      val = val > 0.0f ? val : 0.0f;
+      const uint16_t bv = rowOk && x >= 0 && x < a.W ? f2bf(val) : (uint16_t)0;
+      *reinterpret_cast<uint16_t*>(row + lds_chunk(pl, co >> 3) + 2 * (co & 7)) = bv;
+    }
+  }
+}
 
 /* chunk u of a thread's share of rows m0 .. m0+NR-1 of phase ph: the address
    to load (kZeroPad outside the image) and the ring position to store it at */
@@ -452,13 +536,16 @@ __device__ __forceinline__ void walk_put(uint8_t* ring, int m0, int u, const uin
   }
 }
 
-template <int R>
+template <int R, bool PAIR0>
 __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv16w(
     const ConvWalkArgs a) {
+  static_assert(!PAIR0 || R == 2, "PAIR0: wave w computes ring row m + w of each step");
   constexpr int KS = 2, RING = R + 2;
   extern __shared__ __align__(16) uint8_t lds[];
   uint8_t* ring = lds;                              /* RING rows x kWalkPad px x 64 B */
   uint8_t* outS = lds + RING * kWalkPad * 64;       /* R rows x kWalkW px x 64 B (BF16 output) */
+  uint8_t* b0S = outS + R * kWalkW * 64;            /* PAIR0: kPair0Lds bytes */
+  uint32_t* pairS = reinterpret_cast<uint32_t*>(b0S + 9 * 64 * 16) + (threadIdx.x >> 6) * 3 * kPairW;
   const int unit = (int)(blockIdx.x & 7) * a.perXcd + (int)(blockIdx.x >> 3);
   if (unit >= a.units) return;
   int b = unit;
@@ -480,8 +567,33 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(2,
     for (int ks = 0; ks < KS; ++ks)
       bq[tap * KS + ks] = *reinterpret_cast<const v8bf*>(a.pk + ((size_t)(tap * 2 + ks) * 64 + lane) * 8);
   const float bco = a.bias ? a.bias[co] : 0.0f;
-
-  {
+  float bco0 = 0.0f;
+  const uint16_t* pl0 = nullptr;
+  const uint16_t* pl1 = nullptr;
+  if constexpr (PAIR0) {
+    /* the first layer's K step 0 fragments (its input has 2 channels) */
+    for (int i = threadIdx.x; i < 9 * 64; i += kWalkThreads)
+      *reinterpret_cast<uint4*>(b0S + i * 16) =
+          *reinterpret_cast<const uint4*>(a.pk0 + ((size_t)((i >> 6) * 2) * 64 + (i & 63)) * 8);
+    __syncthreads();
+    bco0 = a.bias0 ? a.bias0[co] : 0.0f;
+    const int pv = a.prevIdx[n];
+    pl0 = pv < 0 ? a.refPlane : a.planes + (size_t)pv * a.planeStride;
+    pl1 = a.planes + (size_t)(a.curIdx0 + n) * a.planeStride;
+    /* warm-up: ring rows R*t0 - 1 .. R*t0 + R; wave w computes rows 2w, 2w + 1 */
+    const int m0 = R * t0 - 1;
+#pragma nounroll
+    for (int r = 0; r < 2; ++r) {
+      PairFetch f;
+      pair_fetch(a, pl0, pl1, ph, x0, m0 + 2 * wave + r, f);
+      pair_put(a, f, pairS);
+      __syncthreads();
+      pair_store<RING>(a, pairS, b0S, bco0, ring, ph, x0, m0 + 2 * wave + r, 0);
+      pair_store<RING>(a, pairS, b0S, bco0, ring, ph, x0, m0 + 2 * wave + r, 1);
+      pair_store<RING>(a, pairS, b0S, bco0, ring, ph, x0, m0 + 2 * wave + r, 2);
+      __syncthreads();   /* pairS is rewritten for the next row */
+    }
+  } else {
     /* warm-up: the segment's first RING rows, in two halves of R + 2 rows'
        worth of chunks at most 10 a thread */
     static_assert((RING * kWalkRowChunks + kWalkThreads - 1) / kWalkThreads <= 10, "warm-up chunks");
@@ -501,18 +613,27 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(2,
   /* the new rows of step t + 1 sit in registers (q*) while step t runs; the
      loads for step t + 2 (p*) are issued at the start of step t, so each
      row load has two steps of MFMAs to arrive in.  Loads past the segment
-     read rows that exist or the zero pad and are dropped. */
+     read rows that exist or the zero pad and are dropped.  PAIR0 instead
+     loads the plane pixels of step t + 1's new rows at the start of step t
+     and runs the first layer on them after step t's MFMAs. */
   static_assert((R * kWalkRowChunks + kWalkThreads - 1) / kWalkThreads <= 5, "prefetch chunks");
 #define WALK_LD(v, u, m) const uint4 v##u = *reinterpret_cast<const uint4*>(walk_src<R>(a, n, ph, x0, m, u))
   uint4 q0, q1, q2, q3, q4;
-  {
+  if constexpr (!PAIR0) {
     const int m1 = R * t0 + R + 1;
     WALK_LD(l, 0, m1); WALK_LD(l, 1, m1); WALK_LD(l, 2, m1); WALK_LD(l, 3, m1); WALK_LD(l, 4, m1);
     q0 = l0; q1 = l1; q2 = l2; q3 = l3; q4 = l4;
   }
   for (int t = t0; t < t1; ++t) {
-    const int m2 = R * t + 2 * R + 1;   /* first new row of step t + 2 */
-    WALK_LD(p, 0, m2); WALK_LD(p, 1, m2); WALK_LD(p, 2, m2); WALK_LD(p, 3, m2); WALK_LD(p, 4, m2);
+    uint4 p0, p1, p2, p3, p4;
+    PairFetch pf;
+    if constexpr (PAIR0) {
+      pair_fetch(a, pl0, pl1, ph, x0, R * t + R + 1 + wave, pf);   /* this wave's new row of step t + 1 */
+    } else {
+      const int m2 = R * t + 2 * R + 1;   /* first new row of step t + 2 */
+      WALK_LD(l, 0, m2); WALK_LD(l, 1, m2); WALK_LD(l, 2, m2); WALK_LD(l, 3, m2); WALK_LD(l, 4, m2);
+      p0 = l0; p1 = l1; p2 = l2; p3 = l3; p4 = l4;
+    }
 
     v16f acc[R];
 #pragma unroll
@@ -567,7 +688,7 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(2,
           }
         }
       }
-    } else {
+    } else if constexpr (!PAIR0) {
       const int oc = a.mode == 2 ? kC : 2;
       if (co < oc) {
 #pragma unroll
@@ -586,13 +707,24 @@ __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(2,
     }
     if (t + 1 < t1) {
       const int m1 = R * t + R + 1;
-      walk_put<R, RING>(ring, m1, 0, q0);
-      walk_put<R, RING>(ring, m1, 1, q1);
-      walk_put<R, RING>(ring, m1, 2, q2);
-      walk_put<R, RING>(ring, m1, 3, q3);
-      walk_put<R, RING>(ring, m1, 4, q4);
+      if constexpr (PAIR0) {
+        /* wave w's new row of step t + 1 */
+        pair_put(a, pf, pairS);
+        __syncthreads();
+        pair_store<RING>(a, pairS, b0S, bco0, ring, ph, x0, m1 + wave, 0);
+        pair_store<RING>(a, pairS, b0S, bco0, ring, ph, x0, m1 + wave, 1);
+        pair_store<RING>(a, pairS, b0S, bco0, ring, ph, x0, m1 + wave, 2);
+      } else {
+        walk_put<R, RING>(ring, m1, 0, q0);
+        walk_put<R, RING>(ring, m1, 1, q1);
+        walk_put<R, RING>(ring, m1, 2, q2);
+        walk_put<R, RING>(ring, m1, 3, q3);
+        walk_put<R, RING>(ring, m1, 4, q4);
+      }
     }
-    q0 = p0; q1 = p1; q2 = p2; q3 = p3; q4 = p4;
+    if constexpr (!PAIR0) {
+      q0 = p0; q1 = p1; q2 = p2; q3 = p3; q4 = p4;
+    }
     __syncthreads();   /* the new rows staged; the output rows read out */
   }
 #undef WALK_LD
@@ -959,7 +1091,7 @@ extern "C" int vsgc_conv16(const uint16_t* in, int n, int h, int w, const uint16
       wk.outb = outb;
       wk.outf = outf;
       if (!walk_grid(wk, blocks, lds)) return (int)hipErrorInvalidValue;
-      hipLaunchKernelGGL(k_conv16w<kWalkR>, dim3(blocks), dim3(kWalkThreads), lds, (hipStream_t)stream, wk);
+      hipLaunchKernelGGL((k_conv16w<kWalkR, false>), dim3(blocks), dim3(kWalkThreads), lds, (hipStream_t)stream, wk);
       CNN_RET;
     }
     if (!tiled_grid(a, blocks, lds)) return (int)hipErrorInvalidValue;
@@ -999,6 +1131,40 @@ extern "C" int vsgc_conv16_pair(const uint16_t* planes, size_t planeStride, cons
   size_t lds;
   if (!tiled_grid(a, blocks, lds)) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL((k_conv16t<1, true>), dim3(blocks), dim3(256), lds, (hipStream_t)stream, a);
+  CNN_RET;
+}
+
+extern "C" int vsgc_conv16_pair2(const uint16_t* planes, size_t planeStride, const uint16_t* refPlane,
+                                 const int* prevIdx, int curIdx0, int n, int h, int w, const uint16_t* packed0,
+                                 const float* bias0, int dil0, const uint16_t* packed1, const float* bias1, int dil1,
+                                 int mode, uint16_t* outb, float* outf, void* stream) {
+  if (!planes || !prevIdx || !packed0 || !packed1 || n <= 0 || h <= 0 || w <= 0 || dil0 < 1 || dil1 < 1 ||
+      dil0 > kMaxTileDil || dil1 > kMaxTileDil || mode < 0 || mode > 1 || !outb || curIdx0 < 0 ||
+      planeStride < (size_t)h * w)
+    return (int)hipErrorInvalidValue;
+  ConvWalkArgs wk = {};
+  wk.N = n;
+  wk.H = h;
+  wk.W = w;
+  wk.pk = packed1;
+  wk.bias = bias1;
+  wk.dil = dil1;
+  wk.mode = mode;
+  wk.outb = outb;
+  wk.outf = outf;
+  wk.planes = planes;
+  wk.planeStride = planeStride;
+  wk.refPlane = refPlane ? refPlane : planes;
+  wk.prevIdx = prevIdx;
+  wk.curIdx0 = curIdx0;
+  wk.pk0 = packed0;
+  wk.bias0 = bias0;
+  wk.dil0 = dil0;
+  unsigned blocks;
+  size_t lds;
+  if (!walk_grid(wk, blocks, lds)) return (int)hipErrorInvalidValue;
+  lds += kPair0Lds;
+  hipLaunchKernelGGL((k_conv16w<kWalkR, true>), dim3(blocks), dim3(kWalkThreads), lds, (hipStream_t)stream, wk);
   CNN_RET;
 }
 

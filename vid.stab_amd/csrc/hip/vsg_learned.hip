@@ -81,6 +81,7 @@ struct VSGLearned {
   int nl = 0;
   int dil[kMaxLayers] = {};
   float thr = 0;
+  bool fuse01 = true;   /* layers 0 and 1 in one pass (vsgc_conv16_pair2) */
   uint16_t* d_pk[kMaxLayers] = {};   /* K = 16 packings */
   float* d_bias[kMaxLayers] = {};
   int Wq = 0, Hq = 0;
@@ -152,6 +153,8 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
   const int B = c->B, nc = c->st[0].nf, nff = c->st[1].nf, slots = VSG_NSETS * B;
   const char* fineEnv = getenv("VIDSTAB_AMD_PROPOSAL_FINE");
   L->fine = !(fineEnv && atoi(fineEnv) == 0);
+  const char* fuseEnv = getenv("VIDSTAB_AMD_CONV_FUSE");
+  L->fuse01 = !(fuseEnv && atoi(fuseEnv) == 0);
   const size_t mapElems = (size_t)B * L->Hq * L->Wq * kC;
   float* d_w = nullptr;
   const size_t perLayer = 9 * kC * kC + kC;
@@ -282,14 +285,26 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 10);
   if (L->netLastSet >= 0 && L->netLastSet != set) LCHECK(hipStreamWaitEvent(st, L->netDone, 0));
   const uint16_t* x = nullptr;
+  /* the first two layers run fused (layer 0's map stays on chip) unless
+     VIDSTAB_AMD_CONV_FUSE=0 */
+  const bool fuse = L->nl >= 3 && L->fuse01;   /* layer 1 has a BF16 output */
   for (int k = 0; k < L->nl; ++k) {
     const bool last = k == L->nl - 1;
     uint16_t* y = last ? nullptr : L->d_act[k & 1];
     float* yf = last ? L->d_out : nullptr;
     const int mode = last ? 3 : 0;
-    const int rc = k == 0 ? vsgc_conv16_pair(L->d_down, L->planeStride, L->d_downRef, s.d_prev, slot0, n, L->Hq, L->Wq,
-                                             L->d_pk[k], L->d_bias[k], L->dil[k], mode, y, yf, st)
-                          : vsgc_conv16(x, n, L->Hq, L->Wq, L->d_pk[k], L->d_bias[k], L->dil[k], mode, 32, y, yf, st);
+    int rc;
+    if (k == 0 && fuse) {
+      continue;
+    } else if (k == 1 && fuse) {
+      rc = vsgc_conv16_pair2(L->d_down, L->planeStride, L->d_downRef, s.d_prev, slot0, n, L->Hq, L->Wq, L->d_pk[0],
+                             L->d_bias[0], L->dil[0], L->d_pk[1], L->d_bias[1], L->dil[1], mode, y, yf, st);
+    } else if (k == 0) {
+      rc = vsgc_conv16_pair(L->d_down, L->planeStride, L->d_downRef, s.d_prev, slot0, n, L->Hq, L->Wq, L->d_pk[k],
+                            L->d_bias[k], L->dil[k], mode, y, yf, st);
+    } else {
+      rc = vsgc_conv16(x, n, L->Hq, L->Wq, L->d_pk[k], L->d_bias[k], L->dil[k], mode, 32, y, yf, st);
+    }
     if (rc != 0) return vsg_fail("proposal network", hipErrorLaunchFailure);
     x = y;
   }

@@ -40,6 +40,8 @@ def lib():
         L.vsgc_conv16.argtypes = [vp, i32, i32, i32, vp, vp, i32, i32, i32, vp, vp, vp]
         L.vsgc_conv16_direct.argtypes = [vp, i32, i32, i32, vp, vp, i32, i32, i32, vp, vp, vp]
         L.vsgc_conv16_pair.argtypes = [vp, ctypes.c_size_t, vp, vp, i32, i32, i32, i32, vp, vp, i32, i32, vp, vp, vp]
+        L.vsgc_conv16_pair2.argtypes = [vp, ctypes.c_size_t, vp, vp, i32, i32, i32, i32, vp, vp, i32, vp, vp, i32,
+                                        i32, vp, vp, vp]
         L.vsgc_field_pool2.argtypes = [vp, i32, i32, vp, i32, vp, vp]
         L.vsgc_conv_wgrad.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, vp]
         L.vsgc_relu_mask.argtypes = [vp, vp, i64, vp]
