@@ -290,12 +290,15 @@ def main():
     forms = None
     if not args.no_4k:
         forms = measure_forms_4k(0, 1, 1, args.weights)
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         ns = min(args.cpu_frames, nfr - 1)
         log(f"cpu baseline: {ns} frames, {threads} threads")
         cpu = cpu_baseline(spec, host, ns, threads)
+        # the reference on one host thread beside it (a shorter sample)
+        cpu1 = cpu_baseline(spec, host, min(ns, 12), 1)
+        log(f"cpu baseline: {cpu['value']} frames/s on {threads} threads, {cpu1['value']} on 1")
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "frames/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
@@ -310,6 +313,7 @@ def main():
         "blur_roofline": {"bound": "hbm", "achieved": round(blur_gbs, 1), "peak": HBM_PEAK, "unit": "GB/s",
                           "frac": round(blur_gbs / HBM_PEAK, 4)},
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "host_frames_fps": round(host_fps, 2) if host_fps else None,
         "second_pass": second,
         "forms_4k": forms,
