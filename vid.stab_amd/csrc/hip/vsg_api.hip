@@ -338,13 +338,19 @@ VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, 
 /* the pinned per-frame descriptors of set `set` (h_src / h_prev / h_detect)
    must be filled for n frames; enqueues their upload, the blur, the field pass
    and the selection */
-int vsg_ctx_enqueue_front(VSGContext* c, int set, int n) {
+int vsg_ctx_upload_front(VSGContext* c, int set, int n) {
+  VSGSet& s = c->set[set];
+  VSG_CHECK(hipMemcpyAsync(s.d_src, s.h_src, sizeof(VSGBlurSrc) * n, hipMemcpyHostToDevice, s.stream));
+  VSG_CHECK(hipMemcpyAsync(s.d_prev, s.h_prev, sizeof(int) * n, hipMemcpyHostToDevice, s.stream));
+  VSG_CHECK(hipMemcpyAsync(s.d_detect, s.h_detect, n, hipMemcpyHostToDevice, s.stream));
+  return 0;
+}
+
+int vsg_ctx_enqueue_front(VSGContext* c, int set, int n, bool upload) {
   VSGSet& s = c->set[set];
   hipStream_t stream = s.stream;
   const int slot0 = set * c->B;
-  VSG_CHECK(hipMemcpyAsync(s.d_src, s.h_src, sizeof(VSGBlurSrc) * n, hipMemcpyHostToDevice, stream));
-  VSG_CHECK(hipMemcpyAsync(s.d_prev, s.h_prev, sizeof(int) * n, hipMemcpyHostToDevice, stream));
-  VSG_CHECK(hipMemcpyAsync(s.d_detect, s.h_detect, n, hipMemcpyHostToDevice, stream));
+  if (upload && vsg_ctx_upload_front(c, set, n) != 0) return -1;
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR0], stream));
   /* packed frames are not blurred (src/motiondetect.c:197-201): their blurSize
      is below 2, which copies Wb = W*bpp bytes a row */
@@ -453,8 +459,22 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
     s.h_detect[i] = detect[i] ? 1 : 0;
   }
   s.prof = g_profOn.load() != 0;
-  if (vsg_ctx_enqueue_front(c, set, n) != 0) return -1;
-  if (c->learned && vsg_learned_front(c, set, n, refSlot, stream) != 0) return -1;
+  /* learned form: the quarter-resolution planes first; then, with no
+     earlier network still running, the batch's network goes ahead of its
+     blur, field copies and selection (which need not wait for it); else the
+     front stage runs beside the earlier network and this one follows it.
+     Either way the network waits only for the previous network (whose
+     batch's planes hold this batch's first previous frame), not for other
+     sets' blurred planes. */
+  if (c->learned) {
+    if (vsg_ctx_upload_front(c, set, n) != 0 || vsg_learned_front(c, set, n, refSlot, stream) != 0) return -1;
+    const bool netFirst = !vsg_learned_net_busy(c);
+    if (netFirst && vsg_learned_net(c, set, n, stream) != 0) return -1;
+    if (vsg_ctx_enqueue_front(c, set, n, false) != 0) return -1;
+    if (!netFirst && vsg_learned_net(c, set, n, stream) != 0) return -1;
+  } else if (vsg_ctx_enqueue_front(c, set, n, true) != 0) {
+    return -1;
+  }
   if (refSlot >= 0) {
     /* the frozen tripod plane: after the blur, before any search reads it */
     VSG_CHECK(hipMemcpyAsync(c->d_ref, c->d_blur + (size_t)refSlot * c->slotStride, c->slotStride,

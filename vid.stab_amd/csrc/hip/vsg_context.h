@@ -107,6 +107,8 @@ int vsg_ctx_tune(VSGContext* c);
    front stage, the proposal coarse and fine stages, their statistics,
    teardown */
 int vsg_learned_front(VSGContext* c, int set, int n, int refSlot, hipStream_t st);
+bool vsg_learned_net_busy(VSGContext* c);
+int vsg_learned_net(VSGContext* c, int set, int n, hipStream_t st);
 int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st);
 bool vsg_learned_fine_enabled(const VSGContext* c);
 int vsg_learned_fine(VSGContext* c, int set, int njobs, hipStream_t st);
@@ -114,7 +116,9 @@ void vsg_learned_account(VSGContext* c, int set, int stage);
 void vsg_learned_free(VSGContext* c);
 void vsg_ctx_free(VSGContext* c);
 /* enqueue helpers (no host synchronisation) */
-int vsg_ctx_enqueue_front(VSGContext* c, int set, int n);      /* H2D descr., blur, fields, select */
+int vsg_ctx_upload_front(VSGContext* c, int set, int n);       /* H2D frame descriptors */
+/* (H2D descriptors unless upload is false), blur, fields, select */
+int vsg_ctx_enqueue_front(VSGContext* c, int set, int n, bool upload = true);
 int vsg_ctx_enqueue_coarse_readback(VSGContext* c, int set, int n);
 VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, int njobs,
                                   unsigned long long* best, unsigned* done, VSGResult* res);

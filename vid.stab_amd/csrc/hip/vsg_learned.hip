@@ -91,7 +91,7 @@ struct VSGLearned {
   uint16_t* d_act[2] = {};         /* [B][Hq][Wq][32] */
   float* d_out = nullptr;          /* [B][Hq][Wq][2] */
   int* d_win = nullptr;            /* [B][nc][5] */
-  float* d_pred = nullptr;         /* [B][nc][2] */
+  float* d_pred[VSG_NSETS] = {};   /* [B][nc][2]: kept per set until its coarse windows */
   bool fine = true;                /* proposals for the fine stage too */
   int* d_winF = nullptr;           /* [B][nff][5] */
   float* d_predF[VSG_NSETS] = {};  /* [B][nff][2]: kept per set until its fine stage */
@@ -100,10 +100,12 @@ struct VSGLearned {
   unsigned* d_stats[VSG_NSETS] = {};   /* 5 counters per stage: coarse, then fine */
   unsigned* h_stats[VSG_NSETS] = {};
   long long fields[2] = {0, 0}, fallbacks[2] = {0, 0}, why[2][3] = {{0, 0, 0}, {0, 0, 0}};
-  /* the network buffers above are shared by the slot sets, whose streams may
-     run concurrently: each set's network stage waits for the previous one */
+  /* the network buffers (d_act, d_out) are shared by the slot sets, whose
+     streams run concurrently: each batch's network waits for the previous
+     batch's (netDone, recorded after its pooling); the proposals it pools
+     are per set, so the next network need not wait for the windows */
   hipEvent_t netDone = nullptr;
-  int netLastSet = -1;
+  bool netRecorded = false;
 };
 
 /* the acceptance bound of a window minimum: the residual threshold (a match
@@ -120,11 +122,12 @@ void vsg_learned_free(VSGContext* c) {
     if (L->d_pk[k]) (void)hipFree(L->d_pk[k]);
     if (L->d_bias[k]) (void)hipFree(L->d_bias[k]);
   }
-  void* bufs[] = {L->d_down, L->d_downRef, L->d_act[0], L->d_act[1], L->d_out, L->d_win, L->d_pred, L->d_winF};
+  void* bufs[] = {L->d_down, L->d_downRef, L->d_act[0], L->d_act[1], L->d_out, L->d_win, L->d_winF};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (L->netDone) (void)hipEventDestroy(L->netDone);
   for (int s = 0; s < VSG_NSETS; ++s) {
+    if (L->d_pred[s]) (void)hipFree(L->d_pred[s]);
     if (L->d_flist[s]) (void)hipFree(L->d_flist[s]);
     if (L->d_flistF[s]) (void)hipFree(L->d_flistF[s]);
     if (L->d_predF[s]) (void)hipFree(L->d_predF[s]);
@@ -183,7 +186,6 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
         hipMalloc(&L->d_act[1], sizeof(uint16_t) * mapElems) != hipSuccess ||
         hipMalloc(&L->d_out, sizeof(float) * 2 * (size_t)B * L->Hq * L->Wq) != hipSuccess ||
         hipMalloc(&L->d_win, sizeof(int) * 5 * (size_t)B * nc) != hipSuccess ||
-        hipMalloc(&L->d_pred, sizeof(float) * 2 * (size_t)B * nc) != hipSuccess ||
         hipMalloc(&L->d_winF, sizeof(int) * 5 * (size_t)B * nff) != hipSuccess) {
       rc = vsg_fail("proposal buffers", hipErrorOutOfMemory);
       break;
@@ -225,7 +227,8 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
       break;
     }
     for (int s = 0; s < VSG_NSETS; ++s) {
-      if (hipMalloc(&L->d_flist[s], sizeof(int) * (size_t)B * nc) != hipSuccess ||
+      if (hipMalloc(&L->d_pred[s], sizeof(float) * 2 * (size_t)B * nc) != hipSuccess ||
+          hipMalloc(&L->d_flist[s], sizeof(int) * (size_t)B * nc) != hipSuccess ||
           hipMalloc(&L->d_flistF[s], sizeof(int) * (size_t)B * nff) != hipSuccess ||
           hipMalloc(&L->d_predF[s], sizeof(float) * 2 * (size_t)B * nff) != hipSuccess ||
           hipMalloc(&L->d_stats[s], sizeof(unsigned) * 10) != hipSuccess ||
@@ -277,13 +280,21 @@ int vsg_learned_front(VSGContext* c, int set, int n, int refSlot, hipStream_t st
   return 0;
 }
 
-/* the coarse stage: network, proposals, window search, exact fallback */
-int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
+/* whether a network launched earlier may still run (then the next batch's
+   blur, field copies and selection go ahead of its own network, so they
+   overlap the running one; otherwise the network goes first) */
+bool vsg_learned_net_busy(VSGContext* c) {
+  VSGLearned* L = c->learned;
+  return L && L->netRecorded && hipEventQuery(L->netDone) == hipErrorNotReady;
+}
+
+/* the network over the batch's planes and the pooling of its output into
+   the set's coarse (and fine) proposals */
+int vsg_learned_net(VSGContext* c, int set, int n, hipStream_t st) {
   VSGLearned* L = c->learned;
   VSGSet& s = c->set[set];
   const int slot0 = set * c->B, nc = c->st[0].nf;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 10);
-  if (L->netLastSet >= 0 && L->netLastSet != set) LCHECK(hipStreamWaitEvent(st, L->netDone, 0));
+  if (L->netRecorded) LCHECK(hipStreamWaitEvent(st, L->netDone, 0));
   const uint16_t* x = nullptr;
   /* the first two layers run fused (layer 0's map stays on chip) unless
      VIDSTAB_AMD_CONV_FUSE=0 */
@@ -308,14 +319,24 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
     if (rc != 0) return vsg_fail("proposal network", hipErrorLaunchFailure);
     x = y;
   }
-  if (vsgc_field_pool2(L->d_out, L->Hq, L->Wq, L->d_win, n * nc, L->d_pred, st) != 0 ||
+  if (vsgc_field_pool2(L->d_out, L->Hq, L->Wq, L->d_win, n * nc, L->d_pred[set], st) != 0 ||
       (L->fine && vsgc_field_pool2(L->d_out, L->Hq, L->Wq, L->d_winF, n * c->st[1].nf, L->d_predF[set], st) != 0))
     return vsg_fail("proposal pooling", hipErrorLaunchFailure);
-  VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
-  LCHECK(vsg_launch_window(ga, L->d_pred, nc, slot0, max_sad(L, c->st[0].S), L->d_flist[set], L->d_stats[set], st));
-  /* the window search was the last reader of the shared network buffers */
   LCHECK(hipEventRecord(L->netDone, st));
-  L->netLastSet = set;
+  L->netRecorded = true;
+  return 0;
+}
+
+/* the coarse stage: window search around the network's proposals, exact
+   fallback */
+int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
+  VSGLearned* L = c->learned;
+  VSGSet& s = c->set[set];
+  const int slot0 = set * c->B, nc = c->st[0].nf;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 10);
+  VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
+  LCHECK(vsg_launch_window(ga, L->d_pred[set], nc, slot0, max_sad(L, c->st[0].S), L->d_flist[set], L->d_stats[set],
+                           st));
   LCHECK(vsg_launch_fallback(ga, L->d_flist[set], L->d_stats[set] + 1, st));
   LCHECK(hipMemcpyAsync(L->h_stats[set], L->d_stats[set], sizeof(unsigned) * 5, hipMemcpyDeviceToHost, st));
   return 0;
