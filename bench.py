@@ -136,6 +136,22 @@ def measure_network_layers(weights, n=32, h=540, w=960, reps=5):
     return res
 
 
+def network_roofline(layers, n=32, h=540, w=960):
+    """the learned form's dominant kernel: the band walker on the 32-channel
+    layers, each reading and writing n*h*w*64 bytes of BF16 activations (the
+    algorithmic bytes; rocprofv3 FETCH_SIZE reads 1.06-1.09 GB for 32 frames),
+    against the HBM peak"""
+    single = [r for r in layers if isinstance(r["layer"], int)]
+    mid = single[1:-1]   # neither the first (2-channel input) nor the last (2-channel output) layer
+    byts = 2.0 * n * h * w * 64
+    ach = [byts / (r["ms"] * 1e-3) / 1e9 for r in mid]
+    avg_ms = sum(r["ms"] for r in mid) / len(mid)
+    return {"bound": "hbm", "kernel": "k_conv16w (32-channel layers)", "achieved": round(byts / (avg_ms * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK, "unit": "GB/s", "frac": round(byts / (avg_ms * 1e-3) / 1e9 / HBM_PEAK, 4),
+            "per_layer_gbs": [round(a, 1) for a in ach], "bytes_per_launch": byts, "avg_launch_ms": round(avg_ms, 4),
+            "layers": layers}
+
+
 def measure_second_pass(spec, ptrs, ls):
     """the second pass's per-frame transforms (vsMotionsToTransformBatch, one
     launch for the clip, host LocalMotions in and transforms out) beside the
@@ -290,6 +306,7 @@ def main():
     forms = None
     if not args.no_4k:
         forms = measure_forms_4k(0, 1, 1, args.weights)
+        forms["network_roofline"] = network_roofline(measure_network_layers(args.weights))
     cpu = cpu1 = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
