@@ -219,7 +219,8 @@ def main():
     from vidstab_amd.synth import Clip
 
     if args.net_layers:
-        print(json.dumps({"network_layers": measure_network_layers(args.weights)}), flush=True)
+        n = args.frames or 32
+        print(json.dumps({"network_layers": measure_network_layers(args.weights, n=n)}), flush=True)
         return
     if args.config == 4:
         return run_multi(args)
