@@ -26,6 +26,7 @@ for p in (os.path.join(REPO, "vid.stab_amd"), os.path.join(REPO, "oracle")):
 METRIC = "detect-pass frames/s at 1080p & 4K (plain vs learned form, same shift accuracy)"
 SAD_PEAK_T = 256 * 64 * 2.4e9 * 4 / 1e12      # T abs-diff/s
 HBM_PEAK = 8000.0                              # GB/s
+COARSE_TRAFFIC_1080P = 813.2e6 + 0.05e6        # bytes read + written per 64-frame coarse launch
 
 
 class VSGProfile(ctypes.Structure):
@@ -203,7 +204,7 @@ def main():
     ap.add_argument("--no-second-pass", action="store_true", help="skip the second-pass transform measurement")
     ap.add_argument("--clip", type=int, default=0, help="--config 5: the held-out clip (0..99)")
     ap.add_argument("--net-layers", action="store_true",
-                    help="only time the proposal network's layers one at a time (32 frames of 960x540)")
+                    help="only time the proposal network's layers one at a time (--frames of 960x540, default 32)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 256 MiB device copy (known bytes) to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -281,7 +282,11 @@ def main():
     achieved = work / avg_s / 1e12
     roofline = {"bound": "valu", "kernel": "k_sad_search (coarse)", "achieved": round(achieved, 3),
                 "peak": round(SAD_PEAK_T, 1), "unit": "T abs-diff/s", "frac": round(achieved / SAD_PEAK_T, 4),
-                "traffic": None, "avg_launch_ms": round(avg_s * 1000.0, 4),
+                # HBM bytes per 64-frame launch of this workload: rocprofv3 --pmc
+                # FETCH_SIZE (x 2.00) and WRITE_SIZE (x 1.00), each scaled by what the
+                # counter reads for a 256 MiB device copy (profiles/k_sad_search_1080p.md)
+                "traffic": COARSE_TRAFFIC_1080P if spec.width == 1920 and spec.height == 1080 else None,
+                "avg_launch_ms": round(avg_s * 1000.0, 4),
                 "peak_measured": round(peak_meas.value / 1e12, 1),
                 "frac_of_measured": round(achieved * 1e12 / peak_meas.value, 4) if peak_meas.value else None,
                 "work_per_launch": work}
