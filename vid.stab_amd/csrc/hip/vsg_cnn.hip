@@ -199,8 +199,8 @@ __global__ __launch_bounds__(256) void k_conv16(const uint16_t* __restrict__ in,
    8 channels of one pixel per lane) feeds the MFMAs of up to three output
    rows, (R+2)*3*KS reads for R*9*KS MFMAs.
    LDS pixel p (64 bytes, 4 chunks of 8 channels) keeps chunk c at position
-   c ^ ((p >> 1) & 3): the 8 lanes a ds_read_b128 serves together then read 8
-   distinct 16-byte bank groups.
+   c ^ ((p >> 2) & 3): the 16 consecutive pixels that 16 lanes of a
+   ds_read_b128 read together then cover all 64 banks (256 bytes) once.
    BF16 outputs go back through LDS (the staged rows are dead by then) and
    leave as 16-byte stores of 8 channels; FP32 outputs are stored directly.
    With PAIR the input is not an NHWC map: channel 0 of sample z is the
@@ -215,7 +215,8 @@ constexpr int kTileR = 4;
 constexpr int kMaxTileDil = 8;
 constexpr int kRowPad = kTileW + 2 * kMaxTileDil;   /* staged row stride in pixels, a multiple of 16 */
 
-__device__ __forceinline__ int lds_chunk(int p, int c) { return p * 64 + 16 * (c ^ ((p >> 1) & 3)); }
+__device__ __forceinline__ int lds_swz(int p) { return (p >> 2) & 3; }
+__device__ __forceinline__ int lds_chunk(int p, int c) { return p * 64 + 16 * (c ^ lds_swz(p)); }
 
 /* read by staging loads that fall outside the image (the zero padding): the
    loads need no select afterwards, so none of them waits before the last */
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       const int j = t / NG, g = t - j * NG;
       const int y = y0 + (j - 1) * dil;
       const int p = g * 16 + (lane >> 2);
-      const int c = (lane & 3) ^ ((p >> 1) & 3);
+      const int c = (lane & 3) ^ lds_swz(p);
       const int x = x0 - dil + p;
       const bool ok = y >= 0 && y < H && p < Wt && x >= 0 && x < W;
       const uint16_t* q =
