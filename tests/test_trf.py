@@ -98,3 +98,35 @@ def test_gpu_detect_then_writer_is_byte_identical(golden, tmp_path, name):
     for mode, key in ((BINARY, "trf_bin"), (ASCII, "trf_ascii")):
         data = _write(str(tmp_path / f"{name}.{mode}.trf"), case, mode, frames)
         assert len(data) == case[key + "_len"] and "%016x" % fnv1a(data) == case[key], f"{name} {key}"
+
+
+_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "vid.stab_amd", "lib", "vidstabdetect_like")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["c1_full", "c1_stride", "c1_tripod", "odd_502x298", "c2_1080p", "c3_4k"])
+def test_ffmpeg_call_sequence_in_c_is_byte_identical(golden, tmp_path, name):
+    """tests/c/vidstabdetect_like.c drives the library exactly as ffmpeg's
+    vidstabdetect filter does (vsFrameInfoInit, vsMotionDetectInit,
+    vsPrepareFile, then vsMotionDetection / vsWriteToFile / vs_vector_del per
+    frame, vsMotionDetectionCleanup): its .trf, binary and ASCII, must be the
+    reference's byte for byte"""
+    import subprocess
+    assert os.path.exists(_DRIVER), "the C driver is not built (make -C vid.stab_amd)"
+    case = golden[name]
+    clip, stride = case_clip(case)
+    n = len(case["frames"])
+    raw = tmp_path / "frames.raw"
+    with open(raw, "wb") as fh:
+        for t in range(n):
+            fh.write(np.ascontiguousarray(clip.render(t, stride=stride), dtype=np.uint8).tobytes())
+    p = case_params(case)
+    for mode, key in ((BINARY, "trf_bin"), (ASCII, "trf_ascii")):
+        out = tmp_path / f"{name}.{mode}.trf"
+        args = [_DRIVER, str(raw), str(case["width"]), str(case["height"]), str(n), str(stride),
+                str(p["shakiness"]), str(p["accuracy"]), str(p["stepsize"]), case["mincontrast"],
+                str(p["tripod"]), str(mode), str(out)]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, r.stderr
+        data = out.read_bytes()
+        assert len(data) == case[key + "_len"] and "%016x" % fnv1a(data) == case[key], f"{name} {key}"
