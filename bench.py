@@ -311,7 +311,7 @@ def main():
     del dev
     forms = None
     if not args.no_4k:
-        forms = measure_forms_4k(0, 1, 1, args.weights)
+        forms = measure_forms_4k(0, 3, 1, args.weights)   # 3 timed passes of each form
         forms["network_roofline"] = network_roofline(measure_network_layers(args.weights))
     cpu = cpu1 = None
     if not args.no_cpu:
