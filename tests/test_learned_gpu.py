@@ -113,3 +113,14 @@ def test_fallback_list_reproduces_plain(clip4k, tmp_path):
     assert ff >= an - cn and fb >= 0.99 * ff
     assert [len(x) for x in got] == [len(x) for x in clip4k["plain"]]
     assert ae == an
+
+
+def test_fused_first_layers_do_not_change_the_result(clip4k, monkeypatch):
+    """the network's first two layers fused in one pass (the default) or run
+    as two launches (VIDSTAB_AMD_CONV_FUSE=0, read when the proposal is
+    attached): the same proposals, so the same LocalMotions, bit for bit"""
+    got_fused, st_fused, _ = _run_learned(clip4k, WEIGHTS)
+    monkeypatch.setenv("VIDSTAB_AMD_CONV_FUSE", "0")
+    got_split, st_split, _ = _run_learned(clip4k, WEIGHTS)
+    assert st_fused == st_split
+    assert got_fused == got_split
