@@ -538,10 +538,11 @@ static int batch_fine(VSMotionDetect* md, MDState* s, Batch* b) {
   return VS_OK;
 }
 
-/* Runs n frames through batches of maxBatch, three in flight: while the host
-   waits for batch k's fine search and cleans it up, the device already runs
-   batch k+2's front stages and coarse search (queued right after batch k's
-   fine search), and batch k+1 waits with its coarse results. */
+/* Runs n frames through batches of maxBatch, three in flight: batch k+2's
+   front stages and coarse search are queued, then batch k+1's coarse results
+   become its fine jobs, and only then does the host wait for batch k's fine
+   search and clean it up -- so the device always holds the next batch's
+   fine search and the batch after next. */
 #define MD_INFLIGHT 3
 static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_t* const* luma,
                          const int* linesize, int onDevice, uint8_t* const* draw, int n) {
@@ -575,19 +576,25 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
     ok = batch_submit(md, s, &bs[k], luma + f0, linesize + f0, onDevice, draw ? draw + f0 : NULL, cnt,
                       motions + f0) == VS_OK;
   }
+  /* batch k's coarse results are turned into its fine jobs before the host
+     waits for batch k-1's fine results, so the device gets batch k's fine
+     stage as soon as its coarse stage is back (and the last batch's does not
+     wait behind the previous batch's cleanup) */
+  if (ok && nb > 0 && batch_coarse(md, s, &bs[0]) != VS_OK) ok = 0;
   for (int k = 0; k < nb && ok; k++) {
     Batch* b = &bs[k % MD_INFLIGHT];
-    if (batch_coarse(md, s, b) != VS_OK) {
-      ok = 0;
-      break;
-    }
     if (k + 2 < nb) {
+      /* the slot set of batch k-1, whose fine stage is done */
       const int f0 = (k + 2) * B, cnt = VS_MIN(B, n - f0);
       if (batch_submit(md, s, &bs[(k + 2) % MD_INFLIGHT], luma + f0, linesize + f0, onDevice,
                        draw ? draw + f0 : NULL, cnt, motions + f0) != VS_OK) {
         ok = 0;
         break;
       }
+    }
+    if (k + 1 < nb && batch_coarse(md, s, &bs[(k + 1) % MD_INFLIGHT]) != VS_OK) {
+      ok = 0;
+      break;
     }
     if (batch_fine(md, s, b) != VS_OK) {
       ok = 0;
