@@ -58,7 +58,10 @@ VS_API int vsMotionDetectProposalStageStats(const VSMotionDetect* md, int stage,
    lumaDev[c][0..nframes[c]-1] with linesize[c] bytes per row; motions[c][t]
    receives exactly what the t-th of nframes[c] vsMotionDetection calls on a
    fresh detector for that clip would return.  Up to VIDSTAB_AMD_MULTI_BATCH
-   (default 8) frames of every clip go through each device batch. */
+   (default 16) frames of every clip go through each device batch.  The
+   device state of a call is kept for the next call over clips of the same
+   shapes (frame sizes, settings), so repeated calls pay the set-up once;
+   concurrent calls are allowed (one of them uses the kept state). */
 VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameInfo* fis, int nclips,
                                const int* nframes, const uint8_t* const* const* lumaDev, const int* linesize,
                                LocalMotions* const* motions);

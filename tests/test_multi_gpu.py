@@ -63,6 +63,26 @@ def test_multi_clip_equals_per_clip(monkeypatch, batch):
     del keep
 
 
+def test_kept_state_between_calls(monkeypatch):
+    """a call over clips of the same shapes as the previous call reuses its
+    device state: the frames change (here: each clip played backwards), the
+    results follow them -- equal to per-clip runs of the new frames -- and
+    going back to the first frames gives the first results again"""
+    monkeypatch.setenv("VIDSTAB_AMD_MULTI_BATCH", "3")
+    clips, keep = _device_clips(CLIPS)
+    first = api.detect_multi(clips)
+    back = [dict(cl, ptrs=cl["ptrs"][::-1]) for cl in clips]
+    second = api.detect_multi(back)
+    for c, cl in enumerate(back):
+        det = api.Detector(cl["width"], cl["height"], shakiness=cl["shakiness"], accuracy=cl["accuracy"],
+                           stepsize=cl["stepsize"], mincontrast=cl["mincontrast"])
+        want = det.detect_device(cl["ptrs"], [cl["linesize"]] * len(cl["ptrs"]), keep=True)
+        det.close()
+        assert second[c] == want, f"clip {c}"
+    assert api.detect_multi(clips) == first
+    del keep
+
+
 def test_multi_clip_rejects_tripod():
     """virtualTripod has no multi-clip form: an error, while the same call
     without it (and no frames) succeeds"""

@@ -11,6 +11,7 @@
  * fine job lists, match cleanup) are the single-clip ones and run over all
  * (clip, frame) pairs of a batch on a thread pool.
  */
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 #include <unistd.h>
@@ -205,6 +206,129 @@ fail:
   return VS_ERROR;
 }
 
+/* ---- the previous call's device state ----------------------------------------
+   Setting up the contexts of many clips (buffers, graphs) and the host pool
+   costs more than detecting a few short clips, so the last call's state is
+   kept for a next call over clips of the same shapes (device, batch, frame
+   sizes, blur boxes, field grids): then only the frames change.  A call that
+   finds the kept state busy (another thread inside) sets up its own. */
+typedef struct {
+  int device, B, nclips;
+  int *w, *h, *blur;
+  VSGStageDesc *dc, *df;
+} MultiShape;
+
+static pthread_mutex_t g_keptMu = PTHREAD_MUTEX_INITIALIZER;
+static VSGMulti* g_keptGpu = NULL;
+static VSPool* g_keptPool = NULL;
+static MultiShape g_keptShape;
+
+static int same_desc(const VSGStageDesc* a, const VSGStageDesc* b) {
+  if (a->fieldSize != b->fieldSize || a->maxShift != b->maxShift || a->stepSize != b->stepSize ||
+      a->nfields != b->nfields || a->fieldRows != b->fieldRows || a->maxFields != b->maxFields ||
+      a->contrastThreshold != b->contrastThreshold)
+    return 0;
+  for (int f = 0; f < a->nfields; f++)
+    if (a->fields[f].x != b->fields[f].x || a->fields[f].y != b->fields[f].y || a->fields[f].size != b->fields[f].size)
+      return 0;
+  return 1;
+}
+
+static int same_shape(const MultiShape* a, const MultiShape* b) {
+  if (a->device != b->device || a->B != b->B || a->nclips != b->nclips) return 0;
+  for (int c = 0; c < a->nclips; c++)
+    if (a->w[c] != b->w[c] || a->h[c] != b->h[c] || a->blur[c] != b->blur[c] || !same_desc(&a->dc[c], &b->dc[c]) ||
+        !same_desc(&a->df[c], &b->df[c]))
+      return 0;
+  return 1;
+}
+
+static void shape_free(MultiShape* s) {
+  if (s->dc)
+    for (int c = 0; c < s->nclips; c++) {
+      free((void*)s->dc[c].fields);
+      free((void*)s->df[c].fields);
+    }
+  free(s->w);
+  free(s->h);
+  free(s->blur);
+  free(s->dc);
+  free(s->df);
+  memset(s, 0, sizeof(*s));
+}
+
+/* a deep copy of the shape (the field arrays belong to the call's detectors) */
+static int shape_copy(MultiShape* dst, const MultiShape* src) {
+  memset(dst, 0, sizeof(*dst));
+  dst->device = src->device;
+  dst->B = src->B;
+  dst->nclips = src->nclips;
+  const size_t n = (size_t)src->nclips;
+  dst->w = (int*)malloc(sizeof(int) * n);
+  dst->h = (int*)malloc(sizeof(int) * n);
+  dst->blur = (int*)malloc(sizeof(int) * n);
+  dst->dc = (VSGStageDesc*)calloc(n, sizeof(VSGStageDesc));
+  dst->df = (VSGStageDesc*)calloc(n, sizeof(VSGStageDesc));
+  if (!dst->w || !dst->h || !dst->blur || !dst->dc || !dst->df) {
+    shape_free(dst);
+    return 0;
+  }
+  memcpy(dst->w, src->w, sizeof(int) * n);
+  memcpy(dst->h, src->h, sizeof(int) * n);
+  memcpy(dst->blur, src->blur, sizeof(int) * n);
+  for (size_t c = 0; c < n; c++) {
+    const VSGStageDesc* sd[2] = {&src->dc[c], &src->df[c]};
+    VSGStageDesc* dd[2] = {&dst->dc[c], &dst->df[c]};
+    for (int k = 0; k < 2; k++) {
+      *dd[k] = *sd[k];
+      Field* f = (Field*)malloc(sizeof(Field) * (size_t)(sd[k]->nfields ? sd[k]->nfields : 1));
+      dd[k]->fields = f;
+      if (!f) {
+        shape_free(dst);
+        return 0;
+      }
+      memcpy(f, sd[k]->fields, sizeof(Field) * (size_t)sd[k]->nfields);
+    }
+  }
+  return 1;
+}
+
+/* takes the kept state if it fits `shape` and nobody else holds it */
+static void kept_take(const MultiShape* shape, VSGMulti** gpu, VSPool** pool) {
+  *gpu = NULL;
+  *pool = NULL;
+  if (pthread_mutex_trylock(&g_keptMu) != 0) return;
+  if (g_keptGpu && same_shape(&g_keptShape, shape)) {
+    *gpu = g_keptGpu;
+    g_keptGpu = NULL;
+  }
+  *pool = g_keptPool;
+  g_keptPool = NULL;
+  pthread_mutex_unlock(&g_keptMu);
+}
+
+/* keeps a call's state for the next call (replacing what was kept); false
+   when it could not be kept, and then the caller releases it */
+static int kept_put(const MultiShape* shape, VSGMulti* gpu, VSPool* pool) {
+  if (pthread_mutex_trylock(&g_keptMu) != 0) return 0;
+  MultiShape copy;
+  int ok = !gpu || shape_copy(&copy, shape);
+  if (ok) {
+    if (gpu) {
+      if (g_keptGpu) vsg_multi_destroy(g_keptGpu);
+      shape_free(&g_keptShape);
+      g_keptGpu = gpu;
+      g_keptShape = copy;
+    }
+    if (pool) {
+      if (g_keptPool) vs_pool_destroy(g_keptPool);
+      g_keptPool = pool;
+    }
+  }
+  pthread_mutex_unlock(&g_keptMu);
+  return ok;
+}
+
 static void multi_free(Multi* M, MBatch* bs) {
   if (bs) {
     for (int k = 0; k < MULTI_INFLIGHT; k++) {
@@ -223,7 +347,7 @@ static void multi_free(Multi* M, MBatch* bs) {
     free(bs);
   }
   if (M->gpu) vsg_multi_destroy(M->gpu);
-  vs_pool_destroy(M->pool);
+  if (M->pool) vs_pool_destroy(M->pool);
   if (M->clips) {
     for (int c = 0; c < M->nclips; c++) {
       VSMotionDetect* md = &M->clips[c].md;
@@ -246,7 +370,7 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
   Multi M;
   memset(&M, 0, sizeof(M));
   M.nclips = nclips;
-  M.B = VS_MIN(MULTI_MAX_B, VS_MAX(1, env_int("VIDSTAB_AMD_MULTI_BATCH", 8)));
+  M.B = VS_MIN(MULTI_MAX_B, VS_MAX(1, env_int("VIDSTAB_AMD_MULTI_BATCH", 16)));
   M.motions = motions;
   M.luma = lumaDev;
   M.nframes = nframes;
@@ -290,18 +414,17 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
     df[c] = vsd_stage_desc(&cl->md.fieldsfine);
     maxFrames = VS_MAX(maxFrames, nframes[c]);
   }
-  if (ok && vsg_multi_create(&M.gpu, env_int("VIDSTAB_AMD_DEVICE", 0), nclips, w, h, blur, dc, df, M.B) != 0) {
+  MultiShape shape = {env_int("VIDSTAB_AMD_DEVICE", 0), M.B, nclips, w, h, blur, dc, df};
+  if (ok) kept_take(&shape, &M.gpu, &M.pool);
+  if (ok && !M.gpu && vsg_multi_create(&M.gpu, shape.device, nclips, w, h, blur, dc, df, M.B) != 0) {
     vs_log_error("vidstab_amd", "GPU multi-clip initialisation failed: %s\n", vsg_last_error());
     ok = 0;
   }
-  free(w);
-  free(h);
-  free(blur);
-  free(dc);
-  free(df);
-  if (ok) {
+  if (ok && !M.pool) {
     const long cores = sysconf(_SC_NPROCESSORS_ONLN);
     M.pool = vs_pool_create(VS_MIN(16, VS_MAX(1, env_int("VIDSTAB_AMD_HOST_THREADS", (int)(cores * 0.8)))));
+  }
+  if (ok) {
     for (int k = 0; k < MULTI_INFLIGHT && ok; k++) {
       bs[k].cb = (MClipBatch*)calloc((size_t)nclips, sizeof(MClipBatch));
       bs[k].itemClip = (int*)malloc(sizeof(int) * (size_t)nclips * M.B);
@@ -341,6 +464,16 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
       for (int t = 0; t < upto; t++) vs_vector_del(&motions[c][t]);
     }
   }
+  /* a call that went through keeps its device state and pool for the next */
+  if (ok && kept_put(&shape, M.gpu, M.pool)) {
+    M.gpu = NULL;
+    M.pool = NULL;
+  }
+  free(w);
+  free(h);
+  free(blur);
+  free(dc);
+  free(df);
   multi_free(&M, bs);
   return ok ? VS_OK : VS_ERROR;
 }
