@@ -13,7 +13,7 @@ GOLDEN = os.path.join(REPO, "tests", "golden", "detect_golden.json")
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
     config.addinivalue_line("markers", "slow: larger CPU cases")
 
 

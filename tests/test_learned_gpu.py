@@ -1,7 +1,7 @@
 """The learned-proposal form on a held-out 4K motion-blur clip (config 5,
 clip 0; the trainer uses clips 100 and up).
 
-The exact full search stays the judge: the learned form's LocalMotions (its
+The exact full search stays the reference: the learned form's LocalMotions (its
 coarse and fine fields both come from window searches around the network's
 displacements) are compared field by field with the plain form's (which the fixtures pin to the
 reference), and the simple global transform of every frame is compared in
