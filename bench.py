@@ -338,6 +338,8 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_1thread": cpu1,
         "host_frames_fps": round(host_fps, 2) if host_fps else None,
+        # the luma bytes those frames carry over PCIe per second (W*H per frame)
+        "host_frames_h2d_gbs": round(host_fps * spec.width * spec.height / 1e9, 2) if host_fps else None,
         "second_pass": second,
         "forms_4k": forms,
     }
