@@ -495,7 +495,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
     if (vsg_learned_coarse(c, set, n, stream) != 0) return -1;
   } else {
     VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
-    VSG_CHECK(vsg_launch_search(ga, stream));
+    VSG_CHECK(vsg_launch_search_any(ga, stream));
   }
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID1], stream));
   VSG_CHECK(hipEventRecord(s.searchDone, stream));
@@ -558,7 +558,7 @@ extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* 
     if (vsg_learned_fine(c, set, njobs, s.stream) != 0) return -1;
   } else {
     VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
-    VSG_CHECK(vsg_launch_search(fa, s.stream));
+    VSG_CHECK(vsg_launch_search_any(fa, s.stream));
   }
   if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_GRID1], s.stream));
   VSG_CHECK(hipEventRecord(s.searchDone, s.stream));

@@ -115,8 +115,12 @@ hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npre
                              int* list, unsigned* stats, hipStream_t st);
 /* the exact search (grid + refinement) of the jobs a.jobs[list[i]],
    i < *count (a device count, at most a.njobs), results to
-   a.results[list[i]] */
+   a.results[list[i]]; list and count both null: every job of a.jobs.
+   Each job's rows are split over four waves (latency over throughput). */
 hipError_t vsg_launch_fallback(const VSGSearchArgs& a, const int* list, const unsigned* count, hipStream_t st);
+/* vsg_launch_search, or the split kernel for a batch too small to fill the
+   GPU (same results) */
+hipError_t vsg_launch_search_any(const VSGSearchArgs& a, hipStream_t st);
 
 /* one clip's stage inside a multi-clip search launch */
 struct VSGMultiSeg {

@@ -340,13 +340,16 @@ __global__ __launch_bounds__(256) void k_sad_fallback(VSGSearchArgs a, const int
   constexpr int K = 4;
   __shared__ RefineLds L;
   __shared__ uint32_t part[3][K * J][64];
-  const unsigned n = min(*count, (unsigned)a.njobs);
+  /* no list: every job of a.jobs (a whole small batch), skipping unselected
+     slots (field < 0; uniform over the workgroup, which holds one job) */
+  const unsigned n = list ? min(*count, (unsigned)a.njobs) : (unsigned)a.njobs;
   const unsigned units = n * (unsigned)a.tiles;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (unsigned u = blockIdx.x; u < units; u += gridDim.x) {
-    const int job = list[u / (unsigned)a.tiles];
+    const int job = list ? list[u / (unsigned)a.tiles] : (int)(u / (unsigned)a.tiles);
     const int tile = (int)(u % (unsigned)a.tiles);
     const VSGJob jb = a.jobs[job];
+    if (jb.field < 0) continue;
     const VSGTile t = make_tile(a, jb);
     const int prows = t.S + (K - 1) * t.s;
     VSGLane<J, K, D, CW, false> ln;
@@ -652,6 +655,8 @@ hipError_t launch_fallback(const VSGSearchArgs& b, const int* list, const unsign
   return hipGetLastError();
 }
 
+constexpr int kSplitCus = 256;   /* MI355X compute units */
+
 int gcd_int(int x, int y) {
   while (y) {
     int t = x % y;
@@ -726,6 +731,13 @@ double vsg_plan_cost(const VSGGrid& g, int tiles) {
   return perLane * (double)tiles * g.wg;
 }
 
+hipError_t vsg_launch_search_any(const VSGSearchArgs& a, hipStream_t st) {
+  /* a batch too small to give every CU two grid workgroups is latency-bound:
+     the split kernel spreads each job's rows over four times the waves */
+  if (a.njobs > 0 && (long long)a.njobs * a.tiles < 2LL * kSplitCus) return vsg_launch_fallback(a, nullptr, nullptr, st);
+  return vsg_launch_search(a, st);
+}
+
 hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
   if (a.chunk <= 0 || a.g.S > 256 || a.g.wg < 64 || a.g.wg > 256 || a.g.S * a.g.bpp > 4 * a.g.wg)
@@ -760,7 +772,7 @@ hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npre
 
 hipError_t vsg_launch_fallback(const VSGSearchArgs& a, const int* list, const unsigned* count, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
-  if (!list || !count || a.g.S > 256 || a.g.S * a.g.bpp > 4 * 256) return hipErrorInvalidValue;
+  if ((!list) != (!count) || a.g.S > 256 || a.g.S * a.g.bpp > 4 * 256) return hipErrorInvalidValue;
   VSGSearchArgs b = a;
   b.g.K = 4;
   b.g.V = (b.g.G + 3) / 4;
