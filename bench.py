@@ -153,6 +153,30 @@ def network_roofline(layers, n=32, h=540, w=960):
             "layers": layers}
 
 
+def measure_per_frame(nframes):
+    """ffmpeg's pattern: one vsMotionDetection call per pageable host frame,
+    each returning its frame's LocalMotions (the whole pass per call), on the
+    1080p clip after 5 untimed calls"""
+    from vidstab_amd import api
+    from vidstab_amd.synth import Clip
+    clip = Clip(2, 0)
+    s = clip.spec
+    frames = [clip.render(t) for t in range(nframes)]
+    det = api.Detector(s.width, s.height, shakiness=s.shakiness, accuracy=s.accuracy, stepsize=s.stepsize,
+                       mincontrast=s.mincontrast)
+    for t in range(5):
+        det.detect(frames[t])
+    t0 = time.perf_counter()
+    for t in range(5, nframes):
+        det.detect(frames[t])
+    dt = time.perf_counter() - t0
+    det.close()
+    fps = (nframes - 5) / dt
+    log(f"per-frame vsMotionDetection, {s.width}x{s.height} host frames: {fps:.1f} frames/s, "
+        f"{dt / (nframes - 5) * 1000:.3f} ms per call")
+    return {"frames": nframes - 5, "fps": round(fps, 2), "ms_per_call": round(dt / (nframes - 5) * 1000, 3)}
+
+
 def measure_second_pass(spec, ptrs, ls):
     """the second pass's per-frame transforms (vsMotionsToTransformBatch, one
     launch for the clip, host LocalMotions in and transforms out) beside the
@@ -205,6 +229,8 @@ def main():
     ap.add_argument("--clip", type=int, default=0, help="--config 5: the held-out clip (0..99)")
     ap.add_argument("--net-layers", action="store_true",
                     help="only time the proposal network's layers one at a time (--frames of 960x540, default 32)")
+    ap.add_argument("--per-frame", action="store_true",
+                    help="only time one vsMotionDetection call per host frame (ffmpeg's pattern) at 1080p")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 256 MiB device copy (known bytes) to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -219,6 +245,9 @@ def main():
     from vidstab_amd import api
     from vidstab_amd.synth import Clip
 
+    if args.per_frame:
+        print(json.dumps({"per_frame_api": measure_per_frame(args.frames or 120)}), flush=True)
+        return
     if args.net_layers:
         n = args.frames or 32
         print(json.dumps({"network_layers": measure_network_layers(args.weights, n=n)}), flush=True)
