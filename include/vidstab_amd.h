@@ -35,6 +35,10 @@ VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions,
    luma; for packed RGB24 / BGR24 / RGBA frames the unblurred plane 0,
    width * bytesPerPixel bytes a row) into dst; for parity checks */
 VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride);
+/* the same for the frame `back` frames before the last one (0: the last);
+   the planes of the detector's latest three device batches are kept, older
+   ones are overwritten (VS_ERROR) */
+VS_API int vsMotionDetectGetBlurredAt(const VSMotionDetect* md, int back, uint8_t* dst, int dstStride);
 
 /* The learned-proposal form: loads a proposal network (the weights file
    written by the project's trainer: magic "VSGP", int32 version 1, int32

@@ -161,6 +161,83 @@ static void run_case(FILE* out, const Case* cs, int first) {
   fprintf(stderr, "case %s done (%d frames)\n", cs->name, nfr);
 }
 
+/* FNV-1a over one frame's LocalMotions in output order: per motion the
+   little-endian int16 v.x, v.y, f.x, f.y, f.size, then the IEEE bits of
+   contrast and match (26 bytes; the struct's padding is left out) */
+static uint64_t hash_motions(const LocalMotions* lms) {
+  uint64_t h = 0xCBF29CE484222325ull;
+  for (int i = 0; i < vs_vector_size(lms); i++) {
+    const LocalMotion* m = (const LocalMotion*)vs_vector_get(lms, i);
+    int16_t v[5] = {m->v.x, m->v.y, m->f.x, m->f.y, m->f.size};
+    uint64_t c = dbits(m->contrast), q = dbits(m->match);
+    h = fnv(v, sizeof(v), h);
+    h = fnv(&c, 8, h);
+    h = fnv(&q, 8, h);
+  }
+  return h;
+}
+
+/* Whole clips, hashes only: per frame the input luma hash, the number of
+   LocalMotions, hash_motions and the blurred-plane hash.  For the full-length
+   workload clips that are too long to keep every motion in a fixture. */
+static void run_case_hashes(FILE* out, const Case* cs, int first, int threads) {
+  VSynSpec sp;
+  vsyn_spec(cs->config, cs->clip, &sp);
+  if (cs->stepsize) sp.stepsize = cs->stepsize;
+  int nfr = cs->nframes > 0 && cs->nframes < sp.nframes ? cs->nframes : sp.nframes;
+  VSynClip clip;
+  VSynMotion* path = (VSynMotion*)calloc((size_t)sp.nframes, sizeof(VSynMotion));
+  vsyn_open(&sp, &clip, path);
+  VSFrameInfo fi;
+  vsFrameInfoInit(&fi, sp.width, sp.height, PF_YUV420P);
+  VSMotionDetect md;
+  memset(&md, 0, sizeof(md));
+  VSMotionDetectConfig conf = vsMotionDetectGetDefaultConfig("golden");
+  conf.shakiness = sp.shakiness;
+  conf.accuracy = sp.accuracy;
+  conf.stepSize = sp.stepsize;
+  conf.contrastThreshold = sp.mincontrast;
+  conf.virtualTripod = cs->tripod;
+  conf.numThreads = threads;
+  if (vsMotionDetectInit(&md, &conf, &fi) != VS_OK) { fprintf(stderr, "init failed\n"); exit(1); }
+  const int ls = sp.width;
+  uint8_t* Y = (uint8_t*)malloc((size_t)ls * sp.height);
+  uint8_t* U = (uint8_t*)malloc((size_t)(sp.width / 2) * (sp.height / 2));
+  uint8_t* V = (uint8_t*)malloc((size_t)(sp.width / 2) * (sp.height / 2));
+  memset(U, 128, (size_t)(sp.width / 2) * (sp.height / 2));
+  memset(V, 128, (size_t)(sp.width / 2) * (sp.height / 2));
+  fprintf(out, "%s{\"name\":\"%s\",\"config\":%d,\"clip\":%d,\"width\":%d,\"height\":%d,\"stride\":%d,"
+               "\"shakiness\":%d,\"accuracy\":%d,\"stepsize\":%d,\"mincontrast\":\"%016" PRIx64 "\","
+               "\"tripod\":%d,\"frames\":[",
+          first ? "" : ",\n", cs->name, cs->config, cs->clip, sp.width, sp.height, ls, sp.shakiness, sp.accuracy,
+          sp.stepsize, dbits(sp.mincontrast), cs->tripod);
+  long total = 0;
+  for (int t = 0; t < nfr; t++) {
+    vsyn_render(&clip, path, t, Y, ls);
+    VSFrame fr;
+    memset(&fr, 0, sizeof(fr));
+    fr.data[0] = Y; fr.linesize[0] = ls;
+    fr.data[1] = U; fr.linesize[1] = sp.width / 2;
+    fr.data[2] = V; fr.linesize[2] = sp.width / 2;
+    LocalMotions lms;
+    const uint64_t hin = vsyn_hash_plane(Y, sp.width, sp.height, ls);
+    vsMotionDetection(&md, &lms, &fr);
+    const uint64_t hbl = vsyn_hash_plane(md.curr.data[0], sp.width, sp.height, md.curr.linesize[0]);
+    const int n = vs_vector_size(&lms);
+    total += n;
+    fprintf(out, "%s[\"%016" PRIx64 "\",%d,\"%016" PRIx64 "\",\"%016" PRIx64 "\"]", t ? "," : "", hin, n,
+            hash_motions(&lms), hbl);
+    vs_vector_del(&lms);
+  }
+  fprintf(out, "]}");
+  fflush(out);
+  vsMotionDetectionCleanup(&md);
+  vsyn_clip_free(&clip);
+  free(path);
+  free(Y); free(U); free(V);
+  fprintf(stderr, "hashes of %s done (%d frames, %ld motions)\n", cs->name, nfr, total);
+}
+
 /* the reference's vsMotionsToTransform of each frame of a planar case */
 static void run_case_transforms(FILE* out, const Case* cs, int first) {
   VSynSpec sp;
@@ -269,6 +346,45 @@ int main(int argc, char** argv) {
     { "s_bgr24_show1",   1, 65,   0,    0,   5,   0,   0,  0,   0,    12, -1, PF_BGR24, 1 },
     { "s_1080p_show2",   2, 60,   0,    0,   3,   0,   0,  0,   0,    0,  -1, 0,        2 },
   };
+  if (argc > 2 && strcmp(argv[2], "hashes") == 0) {
+    /* make_golden out.json hashes <threads> [config clip]...: whole clips of
+       the workload specs; default: the full 1080p (300 frames) and 4K step-4
+       (240 frames) clips and all 64 clips of the multi-clip workload */
+    const int threads = argc > 3 ? atoi(argv[3]) : 8;
+    FILE* out = fopen(path, "w");
+    if (!out) return 1;
+    fprintf(out, "{\"source\":\"reference vsMotionDetection (oracle/_ref/libvsref.so), whole clips\",\"cases\":[\n");
+    int first = 1;
+    char name[64];
+    if (argc > 5) {
+      for (int a = 4; a + 1 < argc; a += 2) {
+        Case c = {0};
+        c.config = atoi(argv[a]);
+        c.clip = atoi(argv[a + 1]);
+        snprintf(name, sizeof(name), "w%d_clip%d", c.config, c.clip);
+        c.name = name;
+        run_case_hashes(out, &c, first, threads);
+        first = 0;
+      }
+    } else {
+      Case c2 = {0}, c3 = {0};
+      c2.name = "w2_full"; c2.config = 2;
+      c3.name = "w3_full"; c3.config = 3;
+      run_case_hashes(out, &c2, 1, threads);
+      run_case_hashes(out, &c3, 0, threads);
+      for (int k = 0; k < 64; k++) {
+        Case c = {0};
+        c.config = 4;
+        c.clip = k;
+        snprintf(name, sizeof(name), "w4_clip%d", k);
+        c.name = name;
+        run_case_hashes(out, &c, 0, threads);
+      }
+    }
+    fprintf(out, "\n]}\n");
+    fclose(out);
+    return 0;
+  }
   const int useTransforms = argc > 2 && strcmp(argv[2], "transforms") == 0;
   if (useTransforms) {
     FILE* out = fopen(path, "w");

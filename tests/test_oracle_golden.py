@@ -42,3 +42,25 @@ def test_oracle_matches_reference(golden, name):
 @pytest.mark.parametrize("name", SLOW)
 def test_oracle_matches_reference_large(golden, name):
     _run(golden[name])
+
+
+def test_oracle_matches_whole_clip_hashes():
+    """the reference's whole-clip hashes (tests/golden/clip_hashes.json, written
+    by make_golden's hashes mode): every 640x360 clip of the multi-clip
+    workload in full, and the first frames of every other clip (1080p, 4K step
+    4, 720p, 1440p) -- the same hash of LocalMotions the GPU tests use"""
+    from golden_util import hash_motions, load_clip_hashes
+    from vidstab_amd.synth import Clip
+    hashes = load_clip_hashes()
+    assert {"w2_full", "w3_full"} <= set(hashes) and sum(k.startswith("w4_clip") for k in hashes) == 64
+    for name, case in hashes.items():
+        nfr = len(case["frames"]) if case["width"] == 640 else 3
+        clip = Clip(case["config"], case["clip"])
+        det = pyoracle.Detector(case["width"], case["height"], **case_params(case))
+        for t in range(nfr):
+            hin, cnt, hl, hb = case["frames"][t]
+            luma = clip.render(t)
+            assert "%016x" % hash_plane(luma) == hin, f"{name}: generator drift frame {t}"
+            got = det.detect(luma)
+            assert len(got) == cnt and "%016x" % hash_motions(got) == hl, f"{name} frame {t}"
+            assert "%016x" % hash_plane(det.blurred()) == hb, f"{name} blur frame {t}"

@@ -239,11 +239,9 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int bpp, int 
   VSG_CHECK(dalloc(&c->d_ref, c->slotStride));
   VSG_CHECK(hipMemset(c->d_blur, 0, (size_t)slots * c->slotStride));
   VSG_CHECK(hipMemset(c->d_ref, 0, c->slotStride));
-  if (blurSize >= 2) {
-    const int s2 = blurSize / 2;
-    const size_t lds = (size_t)(32 + 2 * s2) * (((256 + 2 * s2) + 3) & ~3) + (size_t)(32 + 2 * s2) * 256;
-    if (lds > 64 * 1024) VSG_CHECK(dalloc(&c->d_tmp, (size_t)slots * c->pitch * height));
-  }
+  /* the two-pass blur of wide boxes keeps one intermediate plane per slot:
+     the sets' streams blur concurrently, each into its own slots' planes */
+  if (vsg_blur_needs_tmp(c->Wb, blurSize)) VSG_CHECK(dalloc(&c->d_tmp, (size_t)slots * c->pitch * height));
   c->patchSlotStride = (patchBytes + 255) & ~(size_t)255;
   VSG_CHECK(dalloc(&c->d_patches, (size_t)slots * c->patchSlotStride));
   VSG_CHECK(dalloc(&c->d_minmax, (size_t)slots * c->nfTotal * c->nmm));
@@ -354,8 +352,9 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n, bool upload) {
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR0], stream));
   /* packed frames are not blurred (src/motiondetect.c:197-201): their blurSize
      is below 2, which copies Wb = W*bpp bytes a row */
+  uint8_t* tmp = c->d_tmp ? c->d_tmp + (size_t)slot0 * c->pitch * c->H : nullptr;
   VSG_CHECK(vsg_launch_blur(s.d_src, n, c->d_blur + (size_t)slot0 * c->slotStride, c->slotStride, c->pitch,
-                            c->Wb, c->H, c->blurSize, c->d_tmp, stream));
+                            c->Wb, c->H, c->blurSize, tmp, stream));
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR1], stream));
   VSGFieldArgs fa;
   fa.blurred = c->d_blur + (size_t)slot0 * c->slotStride;

@@ -19,10 +19,12 @@ struct VSGBlurSrc {
 };
 
 /* Two-pass box blur of n planes (src[i] -> dst + i*slotStride, pitch dstPitch).
-   size < 2 copies.  tmp (n*dstPitch*H bytes) is used only for boxes too wide
-   for the one-kernel LDS path. */
+   size < 2 copies.  tmp (n*dstPitch*H bytes, plane i at i*dstPitch*H) is used
+   only when vsg_blur_needs_tmp(W, size): boxes too wide for the one-kernel
+   path.  Launches that may overlap need disjoint tmp ranges. */
 hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t slotStride,
                            int dstPitch, int W, int H, int size, uint8_t* tmp, hipStream_t st);
+bool vsg_blur_needs_tmp(int W, int size);
 
 /* per-field min/max plus the aligned copy of the field ("patch").  Planar
    frames: (min, max) of the luma; packed frames: (min, max) of each of the

@@ -418,6 +418,13 @@ hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st
 
 /* ---- launchers ------------------------------------------------------------- */
 
+static size_t blur_lds_bytes(int size) {
+  const int s2 = size / 2;
+  return (size_t)(kBlurTH + 2 * s2) * (((kBlurTW + 2 * s2) + 15) & ~15) + (size_t)(kBlurTH + 2 * s2) * kBlurTW;
+}
+
+bool vsg_blur_needs_tmp(int W, int size) { return size >= 2 && (blur_lds_bytes(size) > 64 * 1024 || W < 16); }
+
 hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t slotStride,
                            int dstPitch, int W, int H, int size, uint8_t* tmp, hipStream_t st) {
   if (n <= 0) return hipSuccess;
@@ -427,10 +434,8 @@ hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t 
     return hipGetLastError();
   }
   const uint32_t mul = vsg_div_magic(size);
-  const int s2 = size / 2;
-  const size_t lds = (size_t)(kBlurTH + 2 * s2) * (((kBlurTW + 2 * s2) + 15) & ~15) +
-                     (size_t)(kBlurTH + 2 * s2) * kBlurTW;
-  if (lds <= 64 * 1024 && W >= 16) {
+  const size_t lds = blur_lds_bytes(size);
+  if (!vsg_blur_needs_tmp(W, size)) {
     hipLaunchKernelGGL(k_blur, dim3((W + kBlurTW - 1) / kBlurTW, (H + kBlurTH - 1) / kBlurTH, n),
                        dim3(256), lds, st, d_src, dst, slotStride, dstPitch, W, H, size, mul);
     return hipGetLastError();

@@ -47,6 +47,10 @@ typedef struct MDState {
   int maxBatch;
   int nextSet;     /* slot set of the next batch */
   int lastSlot;    /* global slot holding the previous frame's blurred plane */
+  /* the latest VSG_NSETS batches, newest at histHead: their planes stay on
+     the device until their set is submitted again */
+  int histSlot0[VSG_NSETS], histN[VSG_NSETS];
+  int histCount, histHead;
 } MDState;
 
 static MDState* g_states = NULL;
@@ -451,6 +455,10 @@ static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t*
   md->hasSeenOneFrame = (short)seen;
   md->frameNum += n;
   s->lastSlot = b->slot0 + n - 1;
+  s->histHead = (s->histHead + 1) % VSG_NSETS;
+  s->histSlot0[s->histHead] = b->slot0;
+  s->histN[s->histHead] = n;
+  if (s->histCount < VSG_NSETS) s->histCount++;
   b->active = 1;
   return VS_OK;
 }
@@ -738,7 +746,17 @@ VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n) {
 }
 
 VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride) {
+  return vsMotionDetectGetBlurredAt(md, 0, dst, dstStride);
+}
+
+VS_API int vsMotionDetectGetBlurredAt(const VSMotionDetect* md, int back, uint8_t* dst, int dstStride) {
   MDState* s = state_find(md);
-  if (!s || s->lastSlot < 0) return VS_ERROR;
-  return vsg_read_blurred(s->gpu, s->lastSlot, dst, dstStride) == 0 ? VS_OK : VS_ERROR;
+  if (!s || s->lastSlot < 0 || back < 0) return VS_ERROR;
+  for (int k = 0, h = s->histHead; k < s->histCount; k++, h = (h + VSG_NSETS - 1) % VSG_NSETS) {
+    if (back < s->histN[h])
+      return vsg_read_blurred(s->gpu, s->histSlot0[h] + s->histN[h] - 1 - back, dst, dstStride) == 0 ? VS_OK
+                                                                                                      : VS_ERROR;
+    back -= s->histN[h];
+  }
+  return VS_ERROR;   /* that frame's plane has been overwritten */
 }

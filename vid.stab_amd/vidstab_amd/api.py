@@ -113,6 +113,8 @@ def lib():
         L.vsMotionDetectionDevice.restype = c_int
         L.vsMotionDetectGetBlurred.argtypes = [ctypes.POINTER(VSMotionDetect), c_void_p, c_int]
         L.vsMotionDetectGetBlurred.restype = c_int
+        L.vsMotionDetectGetBlurredAt.argtypes = [ctypes.POINTER(VSMotionDetect), c_int, c_void_p, c_int]
+        L.vsMotionDetectGetBlurredAt.restype = c_int
         L.vsMotionDetectionCleanup.argtypes = [ctypes.POINTER(VSMotionDetect)]
         L.vsMotionDetectionCleanup.restype = None
         L.vs_vector_del.argtypes = [ctypes.POINTER(VSVector)]
@@ -246,12 +248,14 @@ class Detector:
             raise RuntimeError("no proposal network")
         return f.value, b.value
 
-    def blurred(self):
-        """the last working plane: blurred luma, or a packed frame's unblurred copy"""
+    def blurred(self, back=0):
+        """the working plane of the last frame (or of the frame `back` frames
+        before it, while its device batch is kept): blurred luma, or a packed
+        frame's unblurred copy"""
         wb = self.width * self.bpp
         out = np.zeros((self.height, wb), dtype=np.uint8)
-        if self.L.vsMotionDetectGetBlurred(ctypes.byref(self.md), out.ctypes.data, wb) != 0:
-            raise RuntimeError("vsMotionDetectGetBlurred failed")
+        if self.L.vsMotionDetectGetBlurredAt(ctypes.byref(self.md), back, out.ctypes.data, wb) != 0:
+            raise RuntimeError("vsMotionDetectGetBlurredAt failed")
         return out
 
     def close(self):
