@@ -44,20 +44,15 @@ VS_API VSVector vs_vector_filter(const VSVector* V, short (*pred)(void*, void*),
 /* moves the element pointers of both into a new vector; src/vsvector.c:195-203 */
 VS_API VSVector vs_vector_concat(const VSVector* V1, const VSVector* V2);
 
-/* fixed-size double array; src/vsvector.h:190-222 */
+/* the double array of match statistics (localmotionsGetMatch); of the
+   reference's VSArray helpers (src/vsvector.h:190-222) the detect path needs
+   only creation and release */
 typedef struct vsarray_ VSArray;
 struct vsarray_ {
   double* dat;
   int len;
 };
-VS_API VSArray vs_array(double vals[], int len);
 VS_API VSArray vs_array_new(int len);
-VS_API VSArray* vs_array_plus(VSArray* c, VSArray a, VSArray b);
-VS_API VSArray* vs_array_scale(VSArray* c, VSArray a, double f);
-VS_API VSArray vs_array_copy(VSArray a);
-VS_API void vs_array_zero(VSArray* a);
-VS_API void vs_array_swap(VSArray* a, VSArray* b);
 VS_API void vs_array_free(VSArray a);
-VS_API void vs_array_print(VSArray a, FILE* f);
 
 #endif /* VSVECTOR_H */

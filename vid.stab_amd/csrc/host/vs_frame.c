@@ -1,49 +1,57 @@
 /*
- * vs_frame.c -- frame description helpers of the vid.stab ABI
- * (reference: src/frameinfo.c:32-211).  Host memory only; the detect pass
- * keeps its working planes in device memory.
+ * vs_frame.c -- frame descriptions of the detect ABI (interface:
+ * include/frameinfo.h, which keeps the reference's src/frameinfo.h:34-133).
+ *
+ * A pixel format is a row of a table: how many planes it has, the chroma
+ * subsampling shifts of planes 1 and 2, and the bytes per pixel of a packed
+ * format (whose pixels all live in plane 0).  Every plane helper derives its
+ * row bytes and row count from that row.  These helpers only ever touch host
+ * memory: the detect pass keeps its working planes on the GPU.
  */
-#include <assert.h>
 #include <string.h>
 
 #include "frameinfo.h"
 #include "vidstabdefines.h"
 
-/* src/frameinfo.c:32-115: fills fi; 1 on success, 0 on a bad format/size */
+typedef struct {
+  VSPixelFormat format;
+  int planes;
+  int log2w, log2h;   /* chroma subsampling of planes 1 and 2 */
+  int bpp;            /* bytes per pixel (packed formats), else 1 */
+} FormatRow;
+
+static const FormatRow kFormats[] = {
+    {PF_GRAY8, 1, 0, 0, 1},    {PF_YUV420P, 3, 1, 1, 1}, {PF_YUV422P, 3, 1, 0, 1},
+    {PF_YUV444P, 3, 0, 0, 1},  {PF_YUV410P, 3, 2, 2, 1}, {PF_YUV411P, 3, 2, 0, 1},
+    {PF_YUV440P, 3, 0, 1, 1},  {PF_YUVA420P, 4, 1, 1, 1}, {PF_RGB24, 1, 0, 0, 3},
+    {PF_BGR24, 1, 0, 0, 3},    {PF_RGBA, 1, 0, 0, 4},
+};
+
+static const FormatRow* format_row(VSPixelFormat f) {
+  for (size_t i = 0; i < sizeof(kFormats) / sizeof(kFormats[0]); ++i)
+    if (kFormats[i].format == f) return &kFormats[i];
+  return NULL;
+}
+
+/* Returns 1, or 0 for an unknown format or a size the subsampling cannot
+   split; a failed call leaves pFormat at 0, as the reference's does. */
 VS_API int vsFrameInfoInit(VSFrameInfo* fi, int width, int height, VSPixelFormat pFormat) {
-  fi->pFormat = pFormat;
+  const FormatRow* r = format_row(pFormat);
   fi->width = width;
   fi->height = height;
-  fi->planes = 3;
-  fi->log2ChromaW = 0;
-  fi->log2ChromaH = 0;
-  fi->bytesPerPixel = 1;
-  switch (pFormat) {
-    case PF_GRAY8:    fi->planes = 1; break;
-    case PF_YUV420P:  fi->log2ChromaW = 1; fi->log2ChromaH = 1; break;
-    case PF_YUV422P:  fi->log2ChromaW = 1; break;
-    case PF_YUV444P:  break;
-    case PF_YUV410P:  fi->log2ChromaW = 2; fi->log2ChromaH = 2; break;
-    case PF_YUV411P:  fi->log2ChromaW = 2; break;
-    case PF_YUV440P:  fi->log2ChromaH = 1; break;
-    case PF_YUVA420P: fi->log2ChromaW = 1; fi->log2ChromaH = 1; fi->planes = 4; break;
-    case PF_RGB24:
-    case PF_BGR24:    fi->bytesPerPixel = 3; fi->planes = 1; break;
-    case PF_RGBA:     fi->bytesPerPixel = 4; fi->planes = 1; break;
-    default:
-      fi->pFormat = 0;
-      return 0;
-  }
-  if (width <= 0 || height <= 0) {
-    vs_log_error("vid.stab", "invalid frame dimensions %ix%i: width and height must be positive\n",
-                 width, height);
+  fi->pFormat = pFormat;
+  fi->planes = r ? r->planes : 3;
+  fi->log2ChromaW = r ? r->log2w : 0;
+  fi->log2ChromaH = r ? r->log2h : 0;
+  fi->bytesPerPixel = r ? r->bpp : 1;
+  if (!r) {
     fi->pFormat = 0;
     return 0;
   }
-  if (width % (1 << fi->log2ChromaW) != 0 || height % (1 << fi->log2ChromaH) != 0) {
-    vs_log_error("vid.stab", "invalid frame dimensions %ix%i for pixel format %i: "
-                 "width must be a multiple of %i and height a multiple of %i\n",
-                 width, height, (int)pFormat, 1 << fi->log2ChromaW, 1 << fi->log2ChromaH);
+  const int xdiv = 1 << r->log2w, ydiv = 1 << r->log2h;
+  if (width <= 0 || height <= 0 || width % xdiv || height % ydiv) {
+    vs_log_error("vid.stab", "frame size %ix%i does not fit pixel format %i (needs positive multiples of %i x %i)\n",
+                 width, height, (int)pFormat, xdiv, ydiv);
     fi->pFormat = 0;
     return 0;
   }
@@ -51,79 +59,75 @@ VS_API int vsFrameInfoInit(VSFrameInfo* fi, int width, int height, VSPixelFormat
 }
 
 VS_API int vsGetPlaneWidthSubS(const VSFrameInfo* fi, int plane) {
-  return (plane == 1 || plane == 2) ? fi->log2ChromaW : 0;
+  return plane == 1 || plane == 2 ? fi->log2ChromaW : 0;
 }
 
 VS_API int vsGetPlaneHeightSubS(const VSFrameInfo* fi, int plane) {
-  return (plane == 1 || plane == 2) ? fi->log2ChromaH : 0;
+  return plane == 1 || plane == 2 ? fi->log2ChromaH : 0;
 }
 
-VS_API int vsFrameIsNull(const VSFrame* frame) { return frame == 0 || frame->data[0] == 0; }
+/* bytes of image data in one row of `plane`, and its number of rows */
+static void plane_shape(const VSFrameInfo* fi, int plane, int* rowBytes, int* rows) {
+  *rowBytes = (fi->width >> vsGetPlaneWidthSubS(fi, plane)) * fi->bytesPerPixel;
+  *rows = fi->height >> vsGetPlaneHeightSubS(fi, plane);
+}
+
+static int plane_count(const VSFrameInfo* fi) { return fi->planes < 4 ? fi->planes : 4; }
+
+VS_API int vsFrameIsNull(const VSFrame* frame) { return !frame || !frame->data[0]; }
 
 VS_API int vsFramesEqual(const VSFrame* a, const VSFrame* b) {
-  return a && b && (a == b || a->data[0] == b->data[0]);
+  if (!a || !b) return 0;
+  return a == b || a->data[0] == b->data[0];
 }
 
-VS_API void vsFrameNull(VSFrame* frame) {
-  memset(frame->data, 0, sizeof(frame->data));
-  memset(frame->linesize, 0, sizeof(frame->linesize));
-}
+VS_API void vsFrameNull(VSFrame* frame) { memset(frame, 0, sizeof(*frame)); }
 
+/* zero-filled planes, rows packed (linesize = row bytes); packed formats get
+   their single plane 0 */
 VS_API void vsFrameAllocate(VSFrame* frame, const VSFrameInfo* fi) {
   vsFrameNull(frame);
-  if (fi->pFormat < PF_PACKED) {
-    for (int i = 0; i < fi->planes && i < 4; i++) {
-      int w = fi->width >> vsGetPlaneWidthSubS(fi, i);
-      int h = fi->height >> vsGetPlaneHeightSubS(fi, i);
-      frame->data[i] = (uint8_t*)vs_zalloc((size_t)w * h);
-      frame->linesize[i] = w;
-      if (!frame->data[i]) vs_log_error("vid.stab", "out of memory: cannot allocated buffer");
-    }
-  } else {
-    int w = fi->width, h = fi->height;
-    frame->data[0] = (uint8_t*)vs_zalloc((size_t)w * h * fi->bytesPerPixel);
-    frame->linesize[0] = w * fi->bytesPerPixel;
-    if (!frame->data[0]) vs_log_error("vid.stab", "out of memory: cannot allocated buffer");
+  for (int p = 0; p < plane_count(fi); ++p) {
+    int rowBytes, rows;
+    plane_shape(fi, p, &rowBytes, &rows);
+    frame->linesize[p] = rowBytes;
+    frame->data[p] = (uint8_t*)vs_zalloc((size_t)rowBytes * (size_t)rows);
+    if (!frame->data[p]) vs_log_error("vid.stab", "no memory for a %i x %i frame plane\n", rowBytes, rows);
   }
 }
 
 VS_API void vsFrameCopyPlane(VSFrame* dest, const VSFrame* src, const VSFrameInfo* fi, int plane) {
-  assert(src->data[plane]);
-  int h = fi->height >> vsGetPlaneHeightSubS(fi, plane);
-  if (src->linesize[plane] == dest->linesize[plane]) {
-    memcpy(dest->data[plane], src->data[plane], (size_t)src->linesize[plane] * h);
-  } else {
-    int w = (fi->width >> vsGetPlaneWidthSubS(fi, plane)) * fi->bytesPerPixel;
-    uint8_t* d = dest->data[plane];
-    const uint8_t* s = src->data[plane];
-    for (; h > 0; h--) {
-      memcpy(d, s, (size_t)w);
-      d += dest->linesize[plane];
-      s += src->linesize[plane];
-    }
+  int rowBytes, rows;
+  plane_shape(fi, plane, &rowBytes, &rows);
+  const int ds = dest->linesize[plane], ss = src->linesize[plane];
+  if (ds == ss) {
+    /* same layout: one block, padding included */
+    memcpy(dest->data[plane], src->data[plane], (size_t)ss * (size_t)rows);
+    return;
   }
+  for (int y = 0; y < rows; ++y)
+    memcpy(dest->data[plane] + (size_t)y * ds, src->data[plane] + (size_t)y * ss, (size_t)rowBytes);
 }
 
 VS_API void vsFrameCopy(VSFrame* dest, const VSFrame* src, const VSFrameInfo* fi) {
-  for (int plane = 0; plane < fi->planes && plane < 4; plane++) vsFrameCopyPlane(dest, src, fi, plane);
+  for (int p = 0; p < plane_count(fi); ++p) vsFrameCopyPlane(dest, src, fi, p);
 }
 
+/* planes back to back in one caller buffer, no copy */
 VS_API void vsFrameFillFromBuffer(VSFrame* frame, uint8_t* img, const VSFrameInfo* fi) {
   vsFrameNull(frame);
-  long offset = 0;
-  for (int i = 0; i < fi->planes && i < 4; i++) {
-    int w = fi->width >> vsGetPlaneWidthSubS(fi, i);
-    int h = fi->height >> vsGetPlaneHeightSubS(fi, i);
-    frame->data[i] = img + offset;
-    frame->linesize[i] = w * fi->bytesPerPixel;
-    offset += (long)h * w * fi->bytesPerPixel;
+  size_t at = 0;
+  for (int p = 0; p < plane_count(fi); ++p) {
+    int rowBytes, rows;
+    plane_shape(fi, p, &rowBytes, &rows);
+    frame->data[p] = img + at;
+    frame->linesize[p] = rowBytes;
+    at += (size_t)rowBytes * (size_t)rows;
   }
 }
 
 VS_API void vsFrameFree(VSFrame* frame) {
-  for (int plane = 0; plane < 4; plane++) {
-    if (frame->data[plane]) vs_free(frame->data[plane]);
-    frame->data[plane] = 0;
-    frame->linesize[plane] = 0;
-  }
+  for (int p = 0; p < 4; ++p)
+    if (frame->data[p]) vs_free(frame->data[p]);
+  vsFrameNull(frame);
 }
