@@ -168,6 +168,18 @@ VS_API int vsg_proposal_fallbacks(VSGContext* ctx, int stage, long long* outOfRa
 VS_API int vsg_motions_to_transform(int device, int width, int height, const int16_t* lm, const double* match,
                                     const int* count, int nframes, int smoothZoom, VSTransform* out,
                                     double* residual, int* rounds);
+/* vsg_motions_to_transform without any device allocation: no counterpart in
+   the reference; for callers that cannot allocate device memory (code next
+   to HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+   Same arguments and results.  The inputs and outputs are staged through a
+   fixed 4.8 MB device array of the library (one per GPU, held until the
+   process ends), so a call takes at most 8192 frames and 262144 motions in
+   all and fails with an error beyond that; it creates no stream (the null
+   stream) and calls no hipMalloc / hipFree.  Calls on one GPU share the
+   array and are serialized by a lock. */
+VS_API int vsg_motions_to_transform_static(int device, int width, int height, const int16_t* lm,
+                                           const double* match, const int* count, int nframes, int smoothZoom,
+                                           VSTransform* out, double* residual, int* rounds);
 
 /* ---- the batched multi-clip form ---------------------------------------------
    One context per clip, all on one stream; the searches of all clips that

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <vector>
 
 #include "transformtype.h"
@@ -31,7 +32,7 @@ namespace {
 struct M2TArgs {
   const short4* lm;       /* per motion: field x, y, motion x, y */
   const double* match;    /* per motion */
-  const int* first;       /* per frame: index of its first motion */
+  const int* first;       /* per frame: index of its first motion (null: summed from count) */
   const int* count;       /* per frame: motions */
   int nframes, cx, cy, smoothZoom, cap;
   VSTransform* out;
@@ -191,9 +192,18 @@ __global__ __launch_bounds__(64) void k_motions_to_transform(M2TArgs a) {
   if (F.n <= 0) {
     t.extra = 1;
   } else {
+    /* the frame's first motion: given, or the sum of the earlier frames' counts */
+    int first;
+    if (a.first) {
+      first = a.first[f];
+    } else {
+      first = 0;
+      for (int g = threadIdx.x; g < f; g += 64) first += a.count[g];
+      for (int off = 32; off > 0; off >>= 1) first += __shfl_xor(first, off);
+    }
     /* the frame's motions and match values into LDS; miss = 0 */
-    const short4* lmG = a.lm + a.first[f];
-    const double* match = a.match + a.first[f];
+    const short4* lmG = a.lm + first;
+    const double* match = a.match + first;
     int sx = 0, sy = 0;
     for (int i = threadIdx.x; i < F.n; i += 64) {
       const short4 m = lmG[i];
@@ -239,6 +249,105 @@ __global__ __launch_bounds__(64) void k_motions_to_transform(M2TArgs a) {
 size_t m2t_lds(int cap) { return (size_t)cap * (2 * sizeof(double) + sizeof(short4) + sizeof(int)); }
 
 }  // namespace
+
+/* ---- the allocation-free variant ---------------------------------------------
+   Staging for the inputs and outputs lives in one static device array per
+   GPU: the motions and match values of up to kStaticMotions motions, the
+   counts and the transform, residual and round count of up to kStaticFrames
+   frames (4.8 MB).  Its device address is looked up once per GPU; the host
+   name of the array is not a device address and is never passed to HIP.
+   Nothing in it needs clearing: the kernel writes every output slot of the
+   frames it is given and reads only what the call uploaded.  Calls on one GPU
+   share the array and would race if they overlapped, so a per-device lock
+   serializes them; the array keeps the last call's contents. */
+namespace {
+
+constexpr int kStaticMotions = 1 << 18;
+constexpr int kStaticFrames = 8192;
+constexpr size_t kOffMatch = sizeof(short4) * kStaticMotions;
+constexpr size_t kOffCount = kOffMatch + sizeof(double) * kStaticMotions;
+constexpr size_t kOffOut = kOffCount + sizeof(int) * kStaticFrames;
+constexpr size_t kOffRes = kOffOut + sizeof(VSTransform) * kStaticFrames;
+constexpr size_t kOffRounds = kOffRes + sizeof(double) * kStaticFrames;
+constexpr size_t kScratchBytes = kOffRounds + sizeof(int) * kStaticFrames;
+static_assert(kOffOut % 8 == 0 && kOffRes % 8 == 0, "aligned scratch sections");
+
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratchMu[kMaxDevices];
+uint8_t* g_scratchPtr[kMaxDevices];   /* device address of scratch on each GPU, once looked up */
+
+}  // namespace
+
+extern "C" int vsg_motions_to_transform_static(int device, int width, int height, const int16_t* lm,
+                                               const double* match, const int* count, int nframes, int smoothZoom,
+                                               VSTransform* out, double* residual, int* rounds) {
+  if (width <= 0 || height <= 0 || nframes < 0 || (nframes > 0 && (!count || !out)) || device < 0 ||
+      device >= kMaxDevices)
+    return vsg_fail("vsg_motions_to_transform_static: invalid arguments", hipSuccess);
+  if (nframes == 0) return 0;
+  if (nframes > kStaticFrames) return vsg_fail("vsg_motions_to_transform_static: more than 8192 frames", hipSuccess);
+  long long total = 0;
+  int cap = 1;
+  for (int f = 0; f < nframes; ++f) {
+    if (count[f] < 0) return vsg_fail("vsg_motions_to_transform_static: negative motion count", hipSuccess);
+    total += count[f];
+    if (count[f] > cap) cap = count[f];
+  }
+  if (total > kStaticMotions)
+    return vsg_fail("vsg_motions_to_transform_static: more than 262144 motions", hipSuccess);
+  if (total > 0 && (!lm || !match)) return vsg_fail("vsg_motions_to_transform_static: no motions", hipSuccess);
+  std::lock_guard<std::mutex> lock(g_scratchMu[device]);
+  int lmax = 0;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+    return vsg_fail("vsg_motions_to_transform_static: device", hipSuccess);
+  if (m2t_lds(cap) > (size_t)lmax)
+    return vsg_fail("vsg_motions_to_transform_static: too many motions in a frame", hipSuccess);
+  if (!g_scratchPtr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return vsg_fail("vsg_motions_to_transform_static: hipGetSymbolAddress", e);
+    g_scratchPtr[device] = static_cast<uint8_t*>(p);
+  }
+  uint8_t* base = g_scratchPtr[device];
+  /* the null stream: no stream is created, and its pageable-memory copies
+     return when the data has left or reached host memory */
+  hipStream_t st = nullptr;
+  if ((total > 0 && (hipMemcpyAsync(base, lm, sizeof(short4) * total, hipMemcpyHostToDevice, st) != hipSuccess ||
+                     hipMemcpyAsync(base + kOffMatch, match, sizeof(double) * total, hipMemcpyHostToDevice, st) !=
+                         hipSuccess)) ||
+      hipMemcpyAsync(base + kOffCount, count, sizeof(int) * nframes, hipMemcpyHostToDevice, st) != hipSuccess)
+    return vsg_fail("vsg_motions_to_transform_static: upload", hipSuccess);
+  M2TArgs a;
+  a.lm = reinterpret_cast<const short4*>(base);
+  a.match = reinterpret_cast<const double*>(base + kOffMatch);
+  a.first = nullptr;
+  a.count = reinterpret_cast<const int*>(base + kOffCount);
+  a.nframes = nframes;
+  a.cx = width / 2;
+  a.cy = height / 2;
+  a.smoothZoom = smoothZoom ? 1 : 0;
+  a.cap = cap;
+  a.out = reinterpret_cast<VSTransform*>(base + kOffOut);
+  a.residual = reinterpret_cast<double*>(base + kOffRes);
+  a.rounds = reinterpret_cast<int*>(base + kOffRounds);
+  hipLaunchKernelGGL(k_motions_to_transform, dim3(nframes), dim3(64), m2t_lds(cap), st, a);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(out, a.out, sizeof(VSTransform) * nframes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      (residual && hipMemcpyAsync(residual, a.residual, sizeof(double) * nframes, hipMemcpyDeviceToHost, st) !=
+                       hipSuccess) ||
+      (rounds && hipMemcpyAsync(rounds, a.rounds, sizeof(int) * nframes, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return vsg_fail("vsg_motions_to_transform_static: kernel", hipSuccess);
+  return 0;
+}
 
 extern "C" int vsg_motions_to_transform(int device, int width, int height, const int16_t* lm, const double* match,
                                         const int* count, int nframes, int smoothZoom, VSTransform* out,
