@@ -51,6 +51,24 @@ VSG_HD uint32_t vsg_div_magic(int size) {
   return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)size - 1) / (uint64_t)size);
 }
 
+/* XCD-aware block mapping.  The hardware hands workgroups to the 8 XCDs
+   round-robin (block b runs on XCD b % 8); units of work are grouped in
+   chunks and every chunk is placed on a single XCD (XCD x takes chunks x,
+   x+8, ...), so the data a chunk shares stays in that XCD's L2.
+   vsg_xcd_unit: the unit of block b, or -1 past nunits; vsg_xcd_blocks: the
+   grid size that covers nunits. */
+#define VSG_XCDS 8
+VSG_HD int vsg_xcd_unit(int b, int chunk, int nunits) {
+  const int x = b % VSG_XCDS, i = b / VSG_XCDS;
+  const int k = x + VSG_XCDS * (i / chunk);
+  const int u = k * chunk + (i % chunk);
+  return u < nunits ? u : -1;
+}
+VSG_HD int vsg_xcd_blocks(int nunits, int chunk) {
+  const int nchunks = (nunits + chunk - 1) / chunk;
+  return VSG_XCDS * ((nchunks + VSG_XCDS - 1) / VSG_XCDS) * chunk;
+}
+
 /* grid geometry of one search stage.  Candidate (a, b), a and b in
    [alo, alo + G), is the shift (org + a*step, org + b*step) px.
    Planar frames: the spiral's square, alo = -n, org = 0, ranked by vsg_rank.

@@ -2,8 +2,10 @@
  * vsg_kernels.hip -- gfx950 kernels of the detect pass that precede the
  * field searches (the searches are in vsg_search.hip).
  *
- *   k_blur        separable integer box blur, row pass and column pass through
- *                 LDS in one kernel (reference: src/boxblur.c:190-332)
+ *   k_blur_walk   separable integer box blur, row pass and column pass in
+ *                 one kernel, a wave walking down a strip of rows with both
+ *                 passes' running sums in registers (reference:
+ *                 src/boxblur.c:190-332; lane body in vsg_blur.h)
  *   k_fields      per-field 8-bit min/max (src/motiondetect.c:389-407) and an
  *                 aligned copy of the field for the SAD kernels
  *   k_select      selectfields (src/motiondetect.c:676-743) for every frame
@@ -11,122 +13,70 @@
  *
  * No MFMA: none of this is a dense contraction.  All arithmetic is integer.
  */
+#include "vsg_blur.h"
 #include "vsg_device.h"
 #include "vsg_internal.h"
 
 namespace {
 
-constexpr int kBlurTW = 256;   /* output tile width (px) */
-constexpr int kBlurTH = 32;    /* output tile height (rows) */
-
 /* ---- box blur ----------------------------------------------------------------
-   Output pixel = floor(sum over the clamped box / size), first along rows
-   (8-bit result) and then along columns, which is what the reference's two
-   running-sum passes compute. */
-__global__ __launch_bounds__(256) void k_blur(const VSGBlurSrc* __restrict__ srcs,
-                                              uint8_t* __restrict__ dst, size_t slotStride,
-                                              int dstPitch, int W, int H, int size, uint32_t mul) {
-  extern __shared__ __align__(16) uint8_t lds[];
-  const int s2 = size >> 1;
-  const int x0 = blockIdx.x * kBlurTW, y0 = blockIdx.y * kBlurTH;
-  const VSGBlurSrc src = srcs[blockIdx.z];
+   k_blur_walk: one wave per (plane, band, 1024-px strip), the lane body in
+   vsg_blur.h.  Loads and stores are plain global accesses: gfx950 takes
+   dword and 16-byte loads at any byte address, and the 16-byte stores land
+   on 16-byte-aligned columns of 64-byte-pitched rows (columns past the
+   width stay inside the row's pitch). */
+typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
+
+typedef const __attribute__((address_space(1))) uint8_t* g8_ptr;
+typedef const __attribute__((address_space(1))) u32_unaligned* g32_ptr;
+typedef uint32_t u32x4_vec __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4_vec* g128_out;
+
+/* global (not flat) loads and stores: the planes are device memory */
+struct BlurDev {
+  static __device__ __forceinline__ uint32_t ld8(const uint8_t* p) { return *(g8_ptr)p; }
+  static __device__ __forceinline__ uint32_t ld32(const uint8_t* p) { return *(g32_ptr)p; }
+  static __device__ __forceinline__ void ld128(const uint8_t* p, uint32_t (&m)[4]) {
+    const g32_ptr q = (g32_ptr)p;
+    m[0] = q[0];
+    m[1] = q[1];
+    m[2] = q[2];
+    m[3] = q[3];
+  }
+  static __device__ __forceinline__ void st128(uint8_t* p, const uint32_t (&o)[4]) {
+    u32x4_vec v;
+    v.x = o[0];
+    v.y = o[1];
+    v.z = o[2];
+    v.w = o[3];
+    *(g128_out)p = v;
+  }
+  static __device__ __forceinline__ uint32_t perm(uint32_t a, uint32_t b, uint32_t sel) {
+    return __builtin_amdgcn_perm(a, b, sel);
+  }
+  static __device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) { return __umul24(a, b); }
+};
+
+template <int S2>
+__global__ __launch_bounds__(256) void k_blur_walk(const VSGBlurSrc* __restrict__ srcs, uint8_t* __restrict__ dst,
+                                                   size_t slotStride, int dstPitch, int W, int H, uint32_t M,
+                                                   VSGBlurPlan pl) {
+  const int u = vsg_xcd_unit(blockIdx.x, pl.chunk, pl.units);
+  if (u < 0) return;
+  const int plane = u / pl.blocksPerPlane;
+  const int unit = (u - plane * pl.blocksPerPlane) * 4 + (int)(threadIdx.x >> 6);
+  if (unit >= pl.strips * pl.bands) return;
+  const VSGBlurSrc src = srcs[plane];
   if (!src.p) return;   /* idle frame slot of a fixed-shape (graph) launch */
-  uint8_t* out = dst + (size_t)blockIdx.z * slotStride;
-  const int rowsIn = kBlurTH + 2 * s2;
-  const int colsIn = kBlurTW + 2 * s2;
-  const int inPitch = (colsIn + 15) & ~15;
-  uint8_t* sIn = lds;
-  uint8_t* sHor = lds + rowsIn * inPitch;
-
-  /* stage the input tile in 16-byte chunks (any source alignment: gfx950
-     global loads take byte-aligned addresses).  Rows outside the plane are
-     clamped to the first / last row -- the reference's edge replication --
-     and every load of a batch is issued before the first LDS write; a chunk
-     that leaves the plane on the left or right is then rebuilt byte by byte
-     with clamped columns. */
-  typedef uint4 __attribute__((aligned(1))) u128_any;
-  const int cpr = inPitch >> 4, chunks = rowsIn * cpr;
-  for (int c0 = 0; c0 < chunks; c0 += 4 * 256) {
-    uint4 v[4];
-    int gxs[4];
-    const uint8_t* rows[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int c = c0 + u * 256 + (int)threadIdx.x;
-      const int cc = c < chunks ? c : 0;
-      const int r = cc / cpr, q = cc - r * cpr;
-      const int gy = min(max(y0 - s2 + r, 0), H - 1);
-      gxs[u] = x0 - s2 + 16 * q;
-      rows[u] = src.p + (size_t)gy * src.pitch;
-      v[u] = *reinterpret_cast<const u128_any*>(rows[u] + min(max(gxs[u], 0), W - 16));
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int c = c0 + u * 256 + (int)threadIdx.x;
-      if (c < chunks) {
-        if (gxs[u] < 0 || gxs[u] + 16 > W) {
-          uint32_t w[4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            uint32_t x = 0;
-#pragma unroll
-            for (int bb = 0; bb < 4; ++bb)
-              x |= (uint32_t)rows[u][min(max(gxs[u] + 4 * k + bb, 0), W - 1)] << (8 * bb);
-            w[k] = x;
-          }
-          v[u] = make_uint4(w[0], w[1], w[2], w[3]);
-        }
-        const int r = c / cpr, q = c - r * cpr;
-        *reinterpret_cast<uint4*>(sIn + r * inPitch + 16 * q) = v[u];
-      }
-    }
-  }
-  __syncthreads();
-
-  /* row pass: 4 adjacent outputs per item with a sliding sum */
-  for (int i = threadIdx.x; i < rowsIn * (kBlurTW / 4); i += 256) {
-    int r = i / (kBlurTW / 4), g = i - r * (kBlurTW / 4);
-    const uint8_t* row = sIn + r * inPitch + 4 * g;
-    uint32_t acc = 0;
-    for (int k = 0; k < size; ++k) acc += row[k];
-    uint32_t o = __umulhi(acc, mul);
-    acc += (uint32_t)row[size] - row[0];
-    o |= __umulhi(acc, mul) << 8;
-    acc += (uint32_t)row[size + 1] - row[1];
-    o |= __umulhi(acc, mul) << 16;
-    acc += (uint32_t)row[size + 2] - row[2];
-    o |= __umulhi(acc, mul) << 24;
-    *reinterpret_cast<uint32_t*>(sHor + r * kBlurTW + 4 * g) = o;
-  }
-  __syncthreads();
-
-  /* column pass: 4 columns x (kBlurTH/4) rows per thread */
-  const int g = threadIdx.x & 63, band = threadIdx.x >> 6;
-  const int ox = x0 + 4 * g;
-  if (ox >= W) return;
-  const int r0 = band * (kBlurTH / 4);
-  uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  for (int k = 0; k < size; ++k) {
-    uint32_t w = *reinterpret_cast<const uint32_t*>(sHor + (r0 + k) * kBlurTW + 4 * g);
-    a0 += w & 0xff; a1 += (w >> 8) & 0xff; a2 += (w >> 16) & 0xff; a3 += w >> 24;
-  }
-  for (int rr = 0; rr < kBlurTH / 4; ++rr) {
-    if (rr > 0) {
-      uint32_t wn = *reinterpret_cast<const uint32_t*>(sHor + (r0 + rr + size - 1) * kBlurTW + 4 * g);
-      uint32_t wo = *reinterpret_cast<const uint32_t*>(sHor + (r0 + rr - 1) * kBlurTW + 4 * g);
-      a0 += (wn & 0xff) - (wo & 0xff);
-      a1 += ((wn >> 8) & 0xff) - ((wo >> 8) & 0xff);
-      a2 += ((wn >> 16) & 0xff) - ((wo >> 16) & 0xff);
-      a3 += (wn >> 24) - (wo >> 24);
-    }
-    int oy = y0 + r0 + rr;
-    if (oy < H) {
-      uint32_t o = __umulhi(a0, mul) | (__umulhi(a1, mul) << 8) | (__umulhi(a2, mul) << 16) |
-                   (__umulhi(a3, mul) << 24);
-      /* the destination pitch is a multiple of 64, so the dword never leaves the row */
-      *reinterpret_cast<uint32_t*>(out + (size_t)oy * dstPitch + ox) = o;
-    }
-  }
+  const int band = unit / pl.strips, strip = unit - band * pl.strips;
+  const int x = strip * kBlurStrip + 16 * (int)(threadIdx.x & 63);
+  if (x >= W) return;
+  const int y0 = band * pl.R, y1 = min(H, y0 + pl.R);
+  uint8_t* out = dst + (size_t)plane * slotStride;
+  if ((W & 15) == 0)
+    VSGBlurLane<S2, BlurDev>::template walk<true>(src.p, src.pitch, out, dstPitch, W, H, x, y0, y1, M);
+  else
+    VSGBlurLane<S2, BlurDev>::template walk<false>(src.p, src.pitch, out, dstPitch, W, H, x, y0, y1, M);
 }
 
 /* fallback for boxes too wide for the LDS tile: one pass per kernel */
@@ -418,12 +368,15 @@ hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st
 
 /* ---- launchers ------------------------------------------------------------- */
 
-static size_t blur_lds_bytes(int size) {
-  const int s2 = size / 2;
-  return (size_t)(kBlurTH + 2 * s2) * (((kBlurTW + 2 * s2) + 15) & ~15) + (size_t)(kBlurTH + 2 * s2) * kBlurTW;
-}
+bool vsg_blur_needs_tmp(int W, int size) { return size >= 2 && (size / 2 > kBlurMaxS2 || W < 16); }
 
-bool vsg_blur_needs_tmp(int W, int size) { return size >= 2 && (blur_lds_bytes(size) > 64 * 1024 || W < 16); }
+template <int S2>
+static void launch_walk(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t slotStride, int dstPitch, int W, int H,
+                        int size, hipStream_t st) {
+  const VSGBlurPlan pl = vsg_blur_plan(W, H, n);
+  hipLaunchKernelGGL(k_blur_walk<S2>, dim3(vsg_xcd_blocks(pl.units, pl.chunk)), dim3(256), 0, st, d_src, dst,
+                     slotStride, dstPitch, W, H, vsg_blur_mul(size), pl);
+}
 
 hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t slotStride,
                            int dstPitch, int W, int H, int size, uint8_t* tmp, hipStream_t st) {
@@ -433,14 +386,21 @@ hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t 
                        slotStride, dstPitch, W, H);
     return hipGetLastError();
   }
-  const uint32_t mul = vsg_div_magic(size);
-  const size_t lds = blur_lds_bytes(size);
   if (!vsg_blur_needs_tmp(W, size)) {
-    hipLaunchKernelGGL(k_blur, dim3((W + kBlurTW - 1) / kBlurTW, (H + kBlurTH - 1) / kBlurTH, n),
-                       dim3(256), lds, st, d_src, dst, slotStride, dstPitch, W, H, size, mul);
+    switch (size / 2) {
+      case 1: launch_walk<1>(d_src, n, dst, slotStride, dstPitch, W, H, size, st); break;
+      case 2: launch_walk<2>(d_src, n, dst, slotStride, dstPitch, W, H, size, st); break;
+      case 3: launch_walk<3>(d_src, n, dst, slotStride, dstPitch, W, H, size, st); break;
+      case 4: launch_walk<4>(d_src, n, dst, slotStride, dstPitch, W, H, size, st); break;
+      case 5: launch_walk<5>(d_src, n, dst, slotStride, dstPitch, W, H, size, st); break;
+      case 6: launch_walk<6>(d_src, n, dst, slotStride, dstPitch, W, H, size, st); break;
+      case 7: launch_walk<7>(d_src, n, dst, slotStride, dstPitch, W, H, size, st); break;
+      default: launch_walk<8>(d_src, n, dst, slotStride, dstPitch, W, H, size, st); break;
+    }
     return hipGetLastError();
   }
   if (!tmp) return hipErrorInvalidValue;
+  const uint32_t mul = vsg_div_magic(size);
   hipLaunchKernelGGL(k_blur_rows, dim3((W + 255) / 256, H, n), dim3(256), 0, st, d_src, tmp, dstPitch,
                      W, H, size, mul);
   hipLaunchKernelGGL(k_blur_cols, dim3((W + 255) / 256, H, n), dim3(256), 0, st, d_src, tmp, dstPitch, dst,

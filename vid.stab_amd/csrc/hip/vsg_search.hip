@@ -24,21 +24,6 @@
 
 namespace {
 
-constexpr int kXcds = 8;
-
-/* unit handled by block b, or -1: XCD x = b % 8 takes chunks x, x+8, ... */
-__device__ __forceinline__ int xcd_unit(int b, int chunk, int nunits) {
-  const int x = b % kXcds, i = b / kXcds;
-  const int k = x + kXcds * (i / chunk);
-  const int u = k * chunk + (i % chunk);
-  return u < nunits ? u : -1;
-}
-
-static int xcd_blocks(int nunits, int chunk) {
-  const int nchunks = (nunits + chunk - 1) / chunk;
-  return kXcds * ((nchunks + kXcds - 1) / kXcds) * chunk;
-}
-
 /* ---- refinement --------------------------------------------------------------
    Run by one whole workgroup (1-4 waves) per job.  Thread t handles word
    w = t % (S/4) of field rows t / (S/4), + rows-per-pass, ...: its loads of
@@ -293,7 +278,7 @@ __device__ __forceinline__ void search_wg(const VSGSearchArgs& a, int job, int t
 template <int J, int K, int D, int CW, bool LDSP>
 __global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
   __shared__ RefineLds L;
-  const int unit = xcd_unit(blockIdx.x, a.chunk, a.njobs * a.tiles);
+  const int unit = vsg_xcd_unit(blockIdx.x, a.chunk, a.njobs * a.tiles);
   if (unit < 0) return;
   const int job = unit / a.tiles;
   search_wg<J, K, D, CW, LDSP>(a, job, unit - job * a.tiles, L);
@@ -308,7 +293,7 @@ template <int J, int K, int D, int CW, bool LDSP>
 __global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict__ segs, int nseg, int totalUnits,
                                                    int chunk) {
   __shared__ RefineLds L;
-  const int unit = xcd_unit(blockIdx.x, chunk, totalUnits);
+  const int unit = vsg_xcd_unit(blockIdx.x, chunk, totalUnits);
   if (unit < 0) return;
   int lo = 0, hi = nseg - 1;
   while (lo < hi) {
@@ -578,7 +563,7 @@ __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float
 
 template <int J, int K, int D, int CW>
 hipError_t launch_grid(const VSGSearchArgs& a, hipStream_t st) {
-  const int nb = xcd_blocks(a.njobs * a.tiles, a.chunk);
+  const int nb = vsg_xcd_blocks(a.njobs * a.tiles, a.chunk);
   if (a.g.lds)
     hipLaunchKernelGGL((k_sad_search<J, K, D, CW, true>), dim3(nb), dim3(a.g.wg), vsg_patch_bytes(a.g), st, a);
   else
@@ -611,7 +596,7 @@ template <int J, int K, int D, int CW>
 hipError_t launch_multi(const VSGMultiSeg* segs, int nseg, int totalUnits, int wg, int lds, size_t patchBytes,
                         hipStream_t st) {
   const int chunk = 32;
-  const int nb = xcd_blocks(totalUnits, chunk);
+  const int nb = vsg_xcd_blocks(totalUnits, chunk);
   if (lds)
     hipLaunchKernelGGL((k_sad_multi<J, K, D, CW, true>), dim3(nb), dim3(wg), patchBytes, st, segs, nseg,
                        totalUnits, chunk);
