@@ -177,6 +177,34 @@ def measure_per_frame(nframes):
     return {"frames": nframes - 5, "fps": round(fps, 2), "ms_per_call": round(dt / (nframes - 5) * 1000, 3)}
 
 
+def measure_per_frame_c(spec, host, nframes=120, warmup=10):
+    """ffmpeg's pattern through C, no Python in the loop: the ffmpeg-shaped
+    driver (tests/c/vidstabdetect_like.c: vsMotionDetection, vsWriteToFile and
+    vs_vector_del per pageable host frame) on the first frames of the clip;
+    the frames after `warmup` are timed by the driver itself"""
+    import subprocess
+    import tempfile
+    import numpy as np
+    exe = os.path.join(REPO, "vid.stab_amd", "lib", "vidstabdetect_like")
+    n = min(nframes, host.shape[0])
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, "frames.raw")
+        host[:n].tofile(raw)
+        bits = "%016x" % int(np.array([spec.mincontrast], dtype=np.float64).view(np.uint64)[0])
+        cmd = [exe, raw, str(spec.width), str(spec.height), str(n), str(spec.width), str(spec.shakiness),
+               str(spec.accuracy), str(spec.stepsize), bits, "0", "2", os.path.join(tmp, "out.trf"), str(warmup)]
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    if p.returncode != 0:
+        raise RuntimeError(f"vidstabdetect_like failed: {p.stderr[-500:]}")
+    line = [l for l in p.stdout.splitlines() if l.startswith("per-frame ")][-1].split()
+    res = {"frames": int(line[1]), "ms_per_call": float(line[2]), "fps": float(line[3]),
+           "driver": "tests/c/vidstabdetect_like.c: one vsMotionDetection + vsWriteToFile + vs_vector_del "
+                     "per pageable host frame (ffmpeg vidstabdetect's sequence), timed in C"}
+    log(f"per-frame C API, {spec.width}x{spec.height} host frames: {res['fps']:.1f} frames/s, "
+        f"{res['ms_per_call']:.3f} ms per call")
+    return res
+
+
 def measure_second_pass(spec, ptrs, ls):
     """the second pass's per-frame transforms (vsMotionsToTransformBatch, one
     launch for the clip, host LocalMotions in and transforms out) beside the
@@ -336,6 +364,7 @@ def main():
         host_fps = nfr / (time.perf_counter() - t0)
         det.close()
         log(f"host frames (PCIe-inclusive): {host_fps:.1f} frames/s")
+    per_frame = None if args.no_host else measure_per_frame_c(spec, host)
     second = None if args.no_second_pass else measure_second_pass(spec, ptrs, ls)
     del dev
     forms = None
@@ -369,6 +398,9 @@ def main():
         "host_frames_fps": round(host_fps, 2) if host_fps else None,
         # the luma bytes those frames carry over PCIe per second (W*H per frame)
         "host_frames_h2d_gbs": round(host_fps * spec.width * spec.height / 1e9, 2) if host_fps else None,
+        # the drop-in path: one vsMotionDetection call per host frame, from C
+        "per_frame_fps": per_frame["fps"] if per_frame else None,
+        "per_frame_c_api": per_frame,
         "second_pass": second,
         "forms_4k": forms,
     }

@@ -98,14 +98,27 @@ VS_API int vsg_create_fmt(VSGContext** ctx, int device, int width, int height, i
                           int blurSize, const VSGStageDesc* coarse, const VSGStageDesc* fine, int maxBatch);
 VS_API void vsg_destroy(VSGContext* ctx);
 
-/* Enqueues a batch of n <= maxBatch frames into `set`: upload (host luma) or
-   in-place read (onDevice), blur, field min/max, selection of both grids, and
+/* where vsg_submit finds a batch's frames */
+enum {
+  VSG_FRAMES_HOST = 0,     /* luma[i]: host memory, staged by vsg_submit */
+  VSG_FRAMES_DEVICE = 1,   /* luma[i]: device memory of the context's GPU, read in place */
+  VSG_FRAMES_STAGED = 2    /* already copied by the caller into vsg_stage_frames' buffers */
+};
+
+/* Enqueues a batch of n <= maxBatch frames into `set`: upload (host luma),
+   upload from the staging buffers, or in-place read (onDevice, one of the
+   VSG_FRAMES_* values), blur, field min/max, selection of both grids, and
    the coarse search of every frame with detect[i] != 0 against slot
    prevSlot[i] (or the stored reference when -1).  If refSlot >= 0, that slot's
    blurred plane becomes the stored reference right after the blur. */
 VS_API int vsg_submit(VSGContext* ctx, int set, int n, const uint8_t* const* luma,
                       const int* linesize, int onDevice, const int* prevSlot,
                       const uint8_t* detect, int refSlot);
+/* The pinned staging buffers of the next n frames of `set` (stage[i]: pitch
+   x height bytes, *pitch a multiple of 64), once the set's previous batch is
+   over; the caller copies each frame's luma rows (width bytes each) into
+   them, on as many threads as it likes, and submits with VSG_FRAMES_STAGED. */
+VS_API int vsg_stage_frames(VSGContext* ctx, int set, int n, uint8_t** stage, int* pitch);
 VS_API int vsg_wait_coarse(VSGContext* ctx, int set, VSGBatchView* view);
 
 /* Enqueues / waits for the fine searches of a set (jobs built by the host). */

@@ -12,22 +12,23 @@
  * (the same sequence as the reference's tests/test_store_restore.c:11-57)
  *
  * usage: vidstabdetect_like FRAMES.raw WIDTH HEIGHT NFRAMES STRIDE SHAKINESS
- *        ACCURACY STEPSIZE MINCONTRAST_BITS TRIPOD MODE OUT.trf
+ *        ACCURACY STEPSIZE MINCONTRAST_BITS TRIPOD MODE OUT.trf [WARMUP]
  *   FRAMES.raw      NFRAMES luma planes of HEIGHT rows of STRIDE bytes
  *   MINCONTRAST_BITS the IEEE-754 bits of the contrast threshold, in hex
  *   MODE            1 ASCII, 2 binary .trf (md->serializationMode)
+ *   WARMUP          if given: the frames after the first WARMUP are timed
+ *                   (each one's vsMotionDetection, vsWriteToFile and
+ *                   vs_vector_del, as the filter does per frame) and one line
+ *                   "per-frame <frames> <ms per frame> <frames/s>" is printed
  * Exits 0 on success; every failing call is reported on stderr.
  */
 #include <inttypes.h>
 #include <stdio.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
-#include "frameinfo.h"
-#include "motiondetect.h"
-#include "serialize.h"
-#include "vidstabdefines.h"
-#include "vsvector.h"
+#include "libvidstab.h"   /* what ffmpeg includes (as <vid.stab/libvidstab.h>) */
 
 static int fail(const char* what) {
   fprintf(stderr, "vidstabdetect_like: %s failed\n", what);
@@ -35,7 +36,7 @@ static int fail(const char* what) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 13) {
+  if (argc != 13 && argc != 14) {
     fprintf(stderr, "usage: %s FRAMES.raw W H N STRIDE SHAKINESS ACCURACY STEPSIZE MINCONTRAST_BITS TRIPOD MODE OUT\n",
             argv[0]);
     return 2;
@@ -73,7 +74,10 @@ int main(int argc, char** argv) {
   FILE* f = fopen(argv[12], "wb");
   if (!f) return fail("fopen");
   if (vsPrepareFile(&md, f) != VS_OK) return fail("vsPrepareFile");
+  const int warmup = argc == 14 ? atoi(argv[13]) : -1;
+  struct timespec t0, t1;
   for (int t = 0; t < n; ++t) {
+    if (t == warmup) clock_gettime(CLOCK_MONOTONIC, &t0);
     VSFrame frame;
     memset(&frame, 0, sizeof frame);
     frame.data[0] = frames + plane * (size_t)t;
@@ -84,6 +88,11 @@ int main(int argc, char** argv) {
     if (vsMotionDetection(&md, &localmotions, &frame) != VS_OK) return fail("vsMotionDetection");
     if (vsWriteToFile(&md, f, &localmotions) != VS_OK) return fail("vsWriteToFile");
     vs_vector_del(&localmotions);
+  }
+  if (warmup >= 0 && warmup < n) {
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    const double s = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    printf("per-frame %d %.4f %.2f\n", n - warmup, 1000.0 * s / (n - warmup), (n - warmup) / s);
   }
   vsMotionDetectionCleanup(&md);
   fclose(f);

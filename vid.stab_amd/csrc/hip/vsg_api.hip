@@ -102,9 +102,9 @@ void vsg_ctx_free(VSGContext* c) {
   dfree(c->d_minmax);
   for (VSGSet& s : c->set) {
     dfree(s.d_selIdx); dfree(s.d_selC); dfree(s.d_count); dfree(s.d_jobsC); dfree(s.d_jobsF);
-    dfree(s.d_best); dfree(s.d_done); dfree(s.d_resC); dfree(s.d_resF); dfree(s.d_src); dfree(s.d_prev); dfree(s.d_detect);
+    dfree(s.d_best); dfree(s.d_done); dfree(s.d_resC); dfree(s.d_resF); dfree(s.d_desc);
     hfree(s.h_selIdx); hfree(s.h_selC); hfree(s.h_count); hfree(s.h_resC); hfree(s.h_jobsF);
-    hfree(s.h_resF); hfree(s.h_src); hfree(s.h_prev); hfree(s.h_detect); hfree(s.h_stage);
+    hfree(s.h_resF); hfree(s.h_desc); hfree(s.h_stage);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.doneF) (void)hipEventDestroy(s.doneF);
     if (s.planesReady) (void)hipEventDestroy(s.planesReady);
@@ -131,6 +131,9 @@ static void free_ctx(VSGContext* c) {
 }
 
 extern "C" void vsg_destroy(VSGContext* ctx) { free_ctx(ctx); }
+
+/* bytes of a set's descriptor block for B frames */
+static size_t desc_bytes(int B) { return (sizeof(VSGBlurSrc) + sizeof(int) + 1) * (size_t)B; }
 
 template <class T>
 static hipError_t dalloc(T** p, size_t n) {
@@ -259,18 +262,22 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int bpp, int 
 
     VSG_CHECK(dalloc(&s.d_resC, (size_t)B * nc));
     VSG_CHECK(dalloc(&s.d_resF, (size_t)B * nf));
-    VSG_CHECK(dalloc(&s.d_src, B));
-    VSG_CHECK(dalloc(&s.d_prev, B));
-    VSG_CHECK(dalloc(&s.d_detect, B));
+    /* the per-frame descriptors (source plane, previous slot, detect flag)
+       are one block, uploaded with one copy per batch */
+    VSG_CHECK(dalloc(&s.d_desc, desc_bytes(B)));
+    s.d_src = reinterpret_cast<VSGBlurSrc*>(s.d_desc);
+    s.d_prev = reinterpret_cast<int*>(s.d_desc + sizeof(VSGBlurSrc) * B);
+    s.d_detect = s.d_desc + (sizeof(VSGBlurSrc) + sizeof(int)) * B;
     VSG_CHECK(halloc(&s.h_selIdx, (size_t)B * c->nfTotal));
     VSG_CHECK(halloc(&s.h_selC, (size_t)B * c->nfTotal));
     VSG_CHECK(halloc(&s.h_count, (size_t)B * 2));
     VSG_CHECK(halloc(&s.h_resC, (size_t)B * nc));
     VSG_CHECK(halloc(&s.h_jobsF, (size_t)B * nf));
     VSG_CHECK(halloc(&s.h_resF, (size_t)B * nf));
-    VSG_CHECK(halloc(&s.h_src, B));
-    VSG_CHECK(halloc(&s.h_prev, B));
-    VSG_CHECK(halloc(&s.h_detect, B));
+    VSG_CHECK(halloc(&s.h_desc, desc_bytes(B)));
+    s.h_src = reinterpret_cast<VSGBlurSrc*>(s.h_desc);
+    s.h_prev = reinterpret_cast<int*>(s.h_desc + sizeof(VSGBlurSrc) * B);
+    s.h_detect = s.h_desc + (sizeof(VSGBlurSrc) + sizeof(int)) * B;
     VSG_CHECK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     VSG_CHECK(hipEventCreateWithFlags(&s.doneF, hipEventDisableTiming));
     VSG_CHECK(hipEventCreateWithFlags(&s.planesReady, hipEventDisableTiming));
@@ -338,9 +345,8 @@ VSGSearchArgs vsg_ctx_search_args(VSGContext* c, int stage, const VSGJob* jobs, 
    and the selection */
 int vsg_ctx_upload_front(VSGContext* c, int set, int n) {
   VSGSet& s = c->set[set];
-  VSG_CHECK(hipMemcpyAsync(s.d_src, s.h_src, sizeof(VSGBlurSrc) * n, hipMemcpyHostToDevice, s.stream));
-  VSG_CHECK(hipMemcpyAsync(s.d_prev, s.h_prev, sizeof(int) * n, hipMemcpyHostToDevice, s.stream));
-  VSG_CHECK(hipMemcpyAsync(s.d_detect, s.h_detect, n, hipMemcpyHostToDevice, s.stream));
+  (void)n;   /* the whole block: one small copy instead of three */
+  VSG_CHECK(hipMemcpyAsync(s.d_desc, s.h_desc, desc_bytes(c->B), hipMemcpyHostToDevice, s.stream));
   return 0;
 }
 
@@ -411,6 +417,21 @@ int vsg_ctx_enqueue_coarse_readback(VSGContext* c, int set, int n) {
   return 0;
 }
 
+extern "C" int vsg_stage_frames(VSGContext* c, int set, int n, uint8_t** stage, int* pitch) {
+  if (!c || set < 0 || set >= VSG_NSETS || n <= 0 || n > c->B || !stage || !pitch)
+    return vsg_fail("vsg_stage_frames", hipSuccess);
+  VSGSet& s = c->set[set];
+  VSG_CHECK(hipSetDevice(c->device));
+  /* the set's previous batch (its uploads and read-backs) must be over */
+  VSG_CHECK(hipEventSynchronize(s.done));
+  VSG_CHECK(hipEventSynchronize(s.doneF));
+  const size_t plane = (size_t)c->pitch * c->H;
+  if (!s.h_stage) VSG_CHECK(halloc(&s.h_stage, plane * c->B));
+  for (int i = 0; i < n; ++i) stage[i] = s.h_stage + plane * i;
+  *pitch = c->pitch;
+  return 0;
+}
+
 extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* luma, const int* linesize,
                           int onDevice, const int* prevSlot, const uint8_t* detect, int refSlot) {
   if (!c || set < 0 || set >= VSG_NSETS || n <= 0 || n > c->B || !luma || !linesize || !prevSlot || !detect)
@@ -434,9 +455,17 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
      search must be over first */
   if (s.follower >= 0) VSG_CHECK(hipStreamWaitEvent(stream, c->set[s.follower].searchDone, 0));
   for (int i = 0; i < n; ++i) {
-    if (onDevice) {
+    if (onDevice == VSG_FRAMES_DEVICE) {
       s.h_src[i].p = luma[i];
       s.h_src[i].pitch = linesize[i];
+    } else if (onDevice == VSG_FRAMES_STAGED) {
+      /* the caller copied the frame into the set's staging area */
+      const size_t plane = (size_t)c->pitch * c->H;
+      if (!s.h_stage) return vsg_fail("vsg_submit: frames not staged", hipSuccess);
+      uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * plane;
+      VSG_CHECK(hipMemcpyAsync(dst, s.h_stage + plane * i, plane, hipMemcpyHostToDevice, stream));
+      s.h_src[i].p = dst;
+      s.h_src[i].pitch = c->pitch;
     } else {
       /* pageable host frame: rows into the set's pinned staging area (the
          set's previous batch is complete), then one DMA of the plane */

@@ -35,6 +35,7 @@ struct VSGSet {
   unsigned* d_done = nullptr;
   VSGResult* d_resC = nullptr;
   VSGResult* d_resF = nullptr;
+  uint8_t* d_desc = nullptr;     /* d_src, d_prev, d_detect in one block */
   VSGBlurSrc* d_src = nullptr;
   int* d_prev = nullptr;
   uint8_t* d_detect = nullptr;
@@ -45,6 +46,7 @@ struct VSGSet {
   VSGResult* h_resC = nullptr;
   VSGJob* h_jobsF = nullptr;
   VSGResult* h_resF = nullptr;
+  uint8_t* h_desc = nullptr;     /* h_src, h_prev, h_detect in one block */
   VSGBlurSrc* h_src = nullptr;
   int* h_prev = nullptr;
   uint8_t* h_detect = nullptr;

@@ -39,7 +39,7 @@ constexpr int kMaxNi = 5;   /* ni = 2r/ns + 1 <= 5 for every step */
 constexpr int kNc = kMaxNi * kMaxNi;
 
 struct RefineLds {
-  uint32_t part[4][kNc];
+  uint32_t part[16][kNc];   /* per wave of a workgroup of up to 1,024 threads */
   uint32_t sums[kNc];
   unsigned long long wmin[4];
   unsigned long long key;
@@ -312,19 +312,21 @@ __global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict
 /* ---- exact search of a fallback list ----------------------------------------------
    A fallback list holds few jobs, so its launch lasts as long as one job's
    search.  The job's candidates are cut into 64-lane tiles with the plan's K
-   forced to 4 (the most lanes per job), and the 4 waves of a workgroup share
-   one tile, each summing a quarter of the prev rows; the partial sums meet
-   in LDS, so a tile's latency is a quarter of a grid-kernel lane's.  A fixed
+   forced to 4 (the most lanes per job), and the WPT waves of a workgroup
+   share one tile, each summing 1/WPT of the prev rows; the partial sums meet
+   in LDS, so a tile's latency is 1/WPT of a grid-kernel lane's.  A fixed
    grid walks the units (list entry, tile) of the count that the window
-   search left; the job's last workgroup refines it, as in the grid kernel. */
+   search left; the job's last workgroup refines it with all its threads,
+   as in the grid kernel.  WPT = 4 for the learned form's fallback lists,
+   8 for plain batches too small to fill the GPU (a single frame). */
 constexpr int kFallbackBlocks = 2048;
 
-template <int J, int D, int CW>
-__global__ __launch_bounds__(256) void k_sad_fallback(VSGSearchArgs a, const int* __restrict__ list,
-                                                      const unsigned* __restrict__ count) {
+template <int J, int D, int CW, int WPT>
+__global__ __launch_bounds__(64 * WPT) void k_sad_fallback(VSGSearchArgs a, const int* __restrict__ list,
+                                                           const unsigned* __restrict__ count) {
   constexpr int K = 4;
   __shared__ RefineLds L;
-  __shared__ uint32_t part[3][K * J][64];
+  __shared__ uint32_t part[WPT - 1][K * J][64];
   /* no list: every job of a.jobs (a whole small batch), skipping unselected
      slots (field < 0; uniform over the workgroup, which holds one job) */
   const unsigned n = list ? min(*count, (unsigned)a.njobs) : (unsigned)a.njobs;
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(256) void k_sad_fallback(VSGSearchArgs a, const int
     const VSGTile t = make_tile(a, jb);
     const int prows = t.S + (K - 1) * t.s;
     VSGLane<J, K, D, CW, false> ln;
-    vsg_sad_acc<J, K, D, CW, false>(t, tile * 64 + lane, wave * prows / 4, (wave + 1) * prows / 4, ln);
+    vsg_sad_acc<J, K, D, CW, false>(t, tile * 64 + lane, wave * prows / WPT, (wave + 1) * prows / WPT, ln);
     if (wave > 0) {
 #pragma unroll
       for (int k = 0; k < K; ++k)
@@ -347,11 +349,11 @@ __global__ __launch_bounds__(256) void k_sad_fallback(VSGSearchArgs a, const int
     }
     __syncthreads();
     if (wave == 0) {
+      for (int w = 0; w < WPT - 1; ++w)
 #pragma unroll
-      for (int k = 0; k < K; ++k)
+        for (int k = 0; k < K; ++k)
 #pragma unroll
-        for (int j = 0; j < J; ++j)
-          ln.acc[k][j] += part[0][k * J + j][lane] + part[1][k * J + j][lane] + part[2][k * J + j][lane];
+          for (int j = 0; j < J; ++j) ln.acc[k][j] += part[w][k * J + j][lane];
       const unsigned long long best = vsg_wave_min_u64(vsg_lane_best<J, K, D, CW, false>(t, ln));
       if (lane == 0) {
         if (best != ~0ull) atomicMin(&a.best[job], best);
@@ -626,18 +628,33 @@ hipError_t launch_multi_k(const VSGMultiSeg* segs, int nseg, int units, const VS
   }
 }
 
-template <int J, int D>
+template <int J, int D, int WPT>
 hipError_t launch_fallback(const VSGSearchArgs& b, const int* list, const unsigned* count, hipStream_t st) {
   const int nb = min(kFallbackBlocks, b.njobs * b.tiles);
+  const dim3 wg(64 * WPT);
   switch (b.g.CW) {
-    case 28: hipLaunchKernelGGL((k_sad_fallback<J, D, 28>), dim3(nb), dim3(256), 0, st, b, list, count); break;
-    case 20: hipLaunchKernelGGL((k_sad_fallback<J, D, 20>), dim3(nb), dim3(256), 0, st, b, list, count); break;
-    case 16: hipLaunchKernelGGL((k_sad_fallback<J, D, 16>), dim3(nb), dim3(256), 0, st, b, list, count); break;
-    case 12: hipLaunchKernelGGL((k_sad_fallback<J, D, 12>), dim3(nb), dim3(256), 0, st, b, list, count); break;
-    case 8: hipLaunchKernelGGL((k_sad_fallback<J, D, 8>), dim3(nb), dim3(256), 0, st, b, list, count); break;
-    default: hipLaunchKernelGGL((k_sad_fallback<J, D, 4>), dim3(nb), dim3(256), 0, st, b, list, count); break;
+    case 28: hipLaunchKernelGGL((k_sad_fallback<J, D, 28, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
+    case 20: hipLaunchKernelGGL((k_sad_fallback<J, D, 20, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
+    case 16: hipLaunchKernelGGL((k_sad_fallback<J, D, 16, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
+    case 12: hipLaunchKernelGGL((k_sad_fallback<J, D, 12, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
+    case 8: hipLaunchKernelGGL((k_sad_fallback<J, D, 8, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
+    default: hipLaunchKernelGGL((k_sad_fallback<J, D, 4, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
   }
   return hipGetLastError();
+}
+
+/* the split kernel with WPT waves per tile */
+template <int WPT>
+hipError_t launch_split(const VSGSearchArgs& a, const int* list, const unsigned* count, hipStream_t st) {
+  if (a.njobs <= 0) return hipSuccess;
+  if ((!list) != (!count) || a.g.S > 256 || a.g.S * a.g.bpp > 4 * 256) return hipErrorInvalidValue;
+  VSGSearchArgs b = a;
+  b.g.K = 4;
+  b.g.V = (b.g.G + 3) / 4;
+  b.tiles = (b.g.U * b.g.V + 63) / 64;
+  if (b.g.J == 4 && b.g.D == 1) return launch_fallback<4, 1, WPT>(b, list, count, st);
+  if (b.g.J == 4 && b.g.D == 3) return launch_fallback<4, 3, WPT>(b, list, count, st);
+  return launch_fallback<1, 1, WPT>(b, list, count, st);
 }
 
 constexpr int kSplitCus = 256;   /* MI355X compute units */
@@ -718,8 +735,8 @@ double vsg_plan_cost(const VSGGrid& g, int tiles) {
 
 hipError_t vsg_launch_search_any(const VSGSearchArgs& a, hipStream_t st) {
   /* a batch too small to give every CU two grid workgroups is latency-bound:
-     the split kernel spreads each job's rows over four times the waves */
-  if (a.njobs > 0 && (long long)a.njobs * a.tiles < 2LL * kSplitCus) return vsg_launch_fallback(a, nullptr, nullptr, st);
+     the split kernel spreads each job's rows over eight times the waves */
+  if (a.njobs > 0 && (long long)a.njobs * a.tiles < 2LL * kSplitCus) return launch_split<8>(a, nullptr, nullptr, st);
   return vsg_launch_search(a, st);
 }
 
@@ -756,13 +773,5 @@ hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npre
 }
 
 hipError_t vsg_launch_fallback(const VSGSearchArgs& a, const int* list, const unsigned* count, hipStream_t st) {
-  if (a.njobs <= 0) return hipSuccess;
-  if ((!list) != (!count) || a.g.S > 256 || a.g.S * a.g.bpp > 4 * 256) return hipErrorInvalidValue;
-  VSGSearchArgs b = a;
-  b.g.K = 4;
-  b.g.V = (b.g.G + 3) / 4;
-  b.tiles = (b.g.U * b.g.V + 63) / 64;
-  if (b.g.J == 4 && b.g.D == 1) return launch_fallback<4, 1>(b, list, count, st);
-  if (b.g.J == 4 && b.g.D == 3) return launch_fallback<4, 3>(b, list, count, st);
-  return launch_fallback<1, 1>(b, list, count, st);
+  return launch_split<4>(a, list, count, st);
 }

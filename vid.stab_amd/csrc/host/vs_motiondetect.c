@@ -24,6 +24,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include <unistd.h>
 
 #include "localmotion2transform.h"
@@ -37,7 +38,32 @@
 
 #define MD_MAX_BATCH 64
 
-/* ---- side table --------------------------------------------------------- */
+/* ---- per-detector state ------------------------------------------------- */
+
+typedef struct Batch {
+  int set, n, slot0;
+  int fnum0;                   /* frame number of the batch's first frame */
+  uint8_t* draw[MD_MAX_BATCH]; /* show > 0: plane 0 of the caller's frames (null: no overlays) */
+  int drawLs[MD_MAX_BATCH];
+  int prevSlot[MD_MAX_BATCH];
+  uint8_t detect[MD_MAX_BATCH];
+  LocalMotions* out;
+  LocalMotions coarse[MD_MAX_BATCH];
+  int jobStart[MD_MAX_BATCH + 1];
+  int jobCount[MD_MAX_BATCH];
+  VSTransform offset[MD_MAX_BATCH];
+  int jobCap;                  /* fine jobs per frame (fine field count) */
+  VSGJob* jobs;                /* fine jobs of the batch (frame order) */
+  int* jobSel;
+  int njobs;
+  int active;
+} Batch;
+
+/* batches in flight: k (cleanup), k+1 (fine stage), k+2 (front and coarse stage) */
+#define MD_INFLIGHT 3
+
+/* host phases of a detect call, timed when VIDSTAB_AMD_CALL_TIMES is set */
+enum { PH_SUBMIT, PH_WAIT_COARSE, PH_COARSE_HOST, PH_SUBMIT_FINE, PH_WAIT_FINE, PH_FINE_HOST, PH_CALL, PH_COUNT };
 
 typedef struct MDState {
   const VSMotionDetect* key;
@@ -51,6 +77,15 @@ typedef struct MDState {
      the device until their set is submitted again */
   int histSlot0[VSG_NSETS], histN[VSG_NSETS];
   int histCount, histHead;
+  /* per-call host state, allocated once: the batches in flight and their
+     fine job lists (maxBatch x fine fields each) */
+  Batch* bs;
+  VSGJob* jobs;
+  int* jobSel;
+  /* VIDSTAB_AMD_CALL_TIMES: summed host time per phase, logged at cleanup */
+  int timed;
+  long long calls;
+  double phaseUs[PH_COUNT];
 } MDState;
 
 static MDState* g_states = NULL;
@@ -66,9 +101,25 @@ static MDState* state_find(const VSMotionDetect* md) {
 
 static void state_free(MDState* s) {
   if (!s) return;
+  if (s->timed && s->calls > 0) {
+    static const char* names[PH_COUNT] = {"submit (staging, enqueue)", "wait for coarse results", "coarse host step",
+                                          "submit fine", "wait for fine results", "fine host step", "whole call"};
+    for (int k = 0; k < PH_COUNT; k++)
+      vs_log_info("vidstab_amd", "call time: %-26s %9.1f us per call (%lld calls)\n", names[k],
+                  s->phaseUs[k] / (double)s->calls, s->calls);
+  }
   if (s->gpu) vsg_destroy(s->gpu);
   vs_pool_destroy(s->pool);
+  free(s->bs);
+  free(s->jobs);
+  free(s->jobSel);
   free(s);
+}
+
+static double now_us(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return 1e6 * (double)ts.tv_sec + 1e-3 * (double)ts.tv_nsec;
 }
 
 static void state_remove(const VSMotionDetect* md) {
@@ -277,6 +328,22 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
   }
   nth = VS_MIN(16, VS_MAX(1, env_int("VIDSTAB_AMD_HOST_THREADS", nth)));
   s->pool = vs_pool_create(nth);
+  const size_t nfj = (size_t)s->maxBatch * (size_t)md->fieldsfine.fieldNum;
+  s->bs = (Batch*)calloc(MD_INFLIGHT, sizeof(Batch));
+  s->jobs = (VSGJob*)malloc(sizeof(VSGJob) * nfj * MD_INFLIGHT);
+  s->jobSel = (int*)malloc(sizeof(int) * nfj * MD_INFLIGHT);
+  if (!s->pool || !s->bs || !s->jobs || !s->jobSel) {
+    vs_log_error(md->conf.modName, "out of host memory\n");
+    state_free(s);
+    free_fields(md);
+    return VS_ERROR;
+  }
+  for (int k = 0; k < MD_INFLIGHT; k++) {
+    s->bs[k].jobs = s->jobs + nfj * k;
+    s->bs[k].jobSel = s->jobSel + nfj * k;
+    s->bs[k].jobCap = md->fieldsfine.fieldNum;
+  }
+  s->timed = getenv("VIDSTAB_AMD_CALL_TIMES") != NULL;
   for (int k = 0; k < 2; k++) {
     int pl[7];
     float tm[6];
@@ -403,27 +470,47 @@ void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, Loca
 
 /* ---- batches ------------------------------------------------------------------ */
 
-typedef struct {
-  int set, n, slot0;
-  int fnum0;                   /* frame number of the batch's first frame */
-  uint8_t* draw[MD_MAX_BATCH]; /* show > 0: plane 0 of the caller's frames (null: no overlays) */
-  int drawLs[MD_MAX_BATCH];
-  int prevSlot[MD_MAX_BATCH];
-  uint8_t detect[MD_MAX_BATCH];
-  LocalMotions* out;
-  LocalMotions coarse[MD_MAX_BATCH];
-  int jobStart[MD_MAX_BATCH + 1];
-  int jobCount[MD_MAX_BATCH];
-  VSTransform offset[MD_MAX_BATCH];
-  int jobCap;                  /* fine jobs per frame (fine field count) */
-  VSGJob* jobs;                /* fine jobs of the batch (frame order) */
-  int* jobSel;
-  int njobs;
-  int active;
-} Batch;
 
 /* decides, for the next n frames, which are measured and against which plane
    (src/motiondetect.c:211-218, 294-303), and submits them */
+/* host frames into the pinned staging buffers, row bands of the frames
+   spread over the pool */
+typedef struct {
+  const uint8_t* const* luma;
+  const int* linesize;
+  uint8_t** stage;
+  int pitch, rowBytes, rows, bands;
+} StageWork;
+
+static void stage_fn(void* ctx, int t) {
+  const StageWork* w = (const StageWork*)ctx;
+  const int i = t / w->bands, band = t - i * w->bands;
+  const int r0 = (int)((long long)w->rows * band / w->bands), r1 = (int)((long long)w->rows * (band + 1) / w->bands);
+  const uint8_t* src = w->luma[i];
+  uint8_t* dst = w->stage[i];
+  if (w->linesize[i] == w->pitch) {
+    memcpy(dst + (size_t)r0 * w->pitch, src + (size_t)r0 * w->pitch,
+           (size_t)(r1 - r0 - 1) * w->pitch + (size_t)w->rowBytes);
+  } else {
+    for (int y = r0; y < r1; y++)
+      memcpy(dst + (size_t)y * w->pitch, src + (size_t)y * w->linesize[i], (size_t)w->rowBytes);
+  }
+}
+
+static int stage_host_frames(VSMotionDetect* md, MDState* s, int set, int n, const uint8_t* const* luma,
+                             const int* linesize) {
+  uint8_t* stage[MD_MAX_BATCH];
+  int pitch = 0;
+  if (vsg_stage_frames(s->gpu, set, n, stage, &pitch) != 0) return VS_ERROR;
+  const int bpp = md->fi.pFormat > PF_PACKED ? md->fi.bytesPerPixel : 1;
+  StageWork w = {luma, linesize, stage, pitch, md->fi.width * bpp, md->fi.height, 1};
+  const int threads = vs_pool_threads(s->pool);
+  if (n < threads) w.bands = VS_MIN(16, (threads + n - 1) / n);   /* a few frames: split their rows */
+  if (w.bands > w.rows) w.bands = w.rows;
+  vs_pool_run(s->pool, n * w.bands, stage_fn, &w);
+  return VS_OK;
+}
+
 static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t* const* luma,
                         const int* linesize, int onDevice, uint8_t* const* draw, int n, LocalMotions* out) {
   const int tripod = md->conf.virtualTripod;
@@ -448,7 +535,9 @@ static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t*
     b->drawLs[i] = linesize[i];
     if (tripod >= 1 && fnum == tripod - 1) refSlot = b->slot0 + i;
   }
-  if (vsg_submit(s->gpu, b->set, n, luma, linesize, onDevice, b->prevSlot, b->detect, refSlot) != 0) {
+  int mode = onDevice ? VSG_FRAMES_DEVICE : VSG_FRAMES_STAGED;
+  if (!onDevice && stage_host_frames(md, s, b->set, n, luma, linesize) != VS_OK) mode = -1;
+  if (mode < 0 || vsg_submit(s->gpu, b->set, n, luma, linesize, mode, b->prevSlot, b->detect, refSlot) != 0) {
     vs_log_error(md->conf.modName, "GPU detect failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
@@ -499,9 +588,15 @@ static void frame_fine_fn(void* ctx, int i) {
 /* coarse results -> coarse LocalMotions, transform, fine jobs; submits them */
 static int batch_coarse(VSMotionDetect* md, MDState* s, Batch* b) {
   VSGBatchView v;
+  double t = s->timed ? now_us() : 0;
   if (vsg_wait_coarse(s->gpu, b->set, &v) != 0) {
     vs_log_error(md->conf.modName, "GPU detect failed: %s\n", vsg_last_error());
     return VS_ERROR;
+  }
+  if (s->timed) {
+    const double t2 = now_us();
+    s->phaseUs[PH_WAIT_COARSE] += t2 - t;
+    t = t2;
   }
   BatchWork w = {md, b, &v, NULL};
   vs_pool_run(s->pool, b->n, frame_coarse_fn, &w);
@@ -525,10 +620,16 @@ static int batch_coarse(VSMotionDetect* md, MDState* s, Batch* b) {
     md->fieldsfine.offset = b->offset[last];
     md->fieldsfine.useOffset = 1;
   }
+  if (s->timed) {
+    const double t2 = now_us();
+    s->phaseUs[PH_COARSE_HOST] += t2 - t;
+    t = t2;
+  }
   if (vsg_submit_fine(s->gpu, b->set, nj, b->jobs) != 0) {
     vs_log_error(md->conf.modName, "GPU fine search failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
+  if (s->timed) s->phaseUs[PH_SUBMIT_FINE] += now_us() - t;
   return VS_OK;
 }
 
@@ -536,14 +637,30 @@ static int batch_coarse(VSMotionDetect* md, MDState* s, Batch* b) {
 static int batch_fine(VSMotionDetect* md, MDState* s, Batch* b) {
   const VSGResult* res = NULL;
   VSGBatchView v;
+  double t = s->timed ? now_us() : 0;
   if (vsg_wait_fine(s->gpu, b->set, &res) != 0 || vsg_wait_coarse(s->gpu, b->set, &v) != 0) {
     vs_log_error(md->conf.modName, "GPU fine search failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
+  if (s->timed) {
+    const double t2 = now_us();
+    s->phaseUs[PH_WAIT_FINE] += t2 - t;
+    t = t2;
+  }
   BatchWork w = {md, b, &v, res};
   vs_pool_run(s->pool, b->n, frame_fine_fn, &w);
   b->active = 0;
+  if (s->timed) s->phaseUs[PH_FINE_HOST] += now_us() - t;
   return VS_OK;
+}
+
+/* batch_submit, timed */
+static int batch_submit_t(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t* const* luma, const int* linesize,
+                          int onDevice, uint8_t* const* draw, int n, LocalMotions* out) {
+  const double t = s->timed ? now_us() : 0;
+  const int r = batch_submit(md, s, b, luma, linesize, onDevice, draw, n, out);
+  if (s->timed) s->phaseUs[PH_SUBMIT] += now_us() - t;
+  return r;
 }
 
 /* Runs n frames through batches of maxBatch, three in flight: batch k+2's
@@ -551,7 +668,6 @@ static int batch_fine(VSMotionDetect* md, MDState* s, Batch* b) {
    become its fine jobs, and only then does the host wait for batch k's fine
    search and clean it up -- so the device always holds the next batch's
    fine search and the batch after next. */
-#define MD_INFLIGHT 3
 static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_t* const* luma,
                          const int* linesize, int onDevice, uint8_t* const* draw, int n) {
   if (!md || md->initialized != 2 || !motions || n < 0) return VS_ERROR;
@@ -561,28 +677,16 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
     return VS_ERROR;
   }
   if (n == 0) return VS_OK;
+  const double tcall = s->timed ? now_us() : 0;
   const int B = s->maxBatch;
   const int nb = (n + B - 1) / B;
-  const size_t nfj = (size_t)B * md->fieldsfine.fieldNum;
-  Batch* bs = (Batch*)calloc(MD_INFLIGHT, sizeof(Batch));
-  VSGJob* jobs = (VSGJob*)malloc(sizeof(VSGJob) * nfj * MD_INFLIGHT);
-  int* sel = (int*)malloc(sizeof(int) * nfj * MD_INFLIGHT);
-  if (!bs || !jobs || !sel) {
-    free(bs);
-    free(jobs);
-    free(sel);
-    return VS_ERROR;
-  }
-  for (int k = 0; k < MD_INFLIGHT; k++) {
-    bs[k].jobs = jobs + nfj * k;
-    bs[k].jobSel = sel + nfj * k;
-    bs[k].jobCap = md->fieldsfine.fieldNum;
-  }
+  Batch* bs = s->bs;
+  for (int k = 0; k < MD_INFLIGHT; k++) bs[k].active = 0;
   int ok = 1, done = 0;
   for (int k = 0; k < nb && k < 2 && ok; k++) {
     const int f0 = k * B, cnt = VS_MIN(B, n - f0);
-    ok = batch_submit(md, s, &bs[k], luma + f0, linesize + f0, onDevice, draw ? draw + f0 : NULL, cnt,
-                      motions + f0) == VS_OK;
+    ok = batch_submit_t(md, s, &bs[k], luma + f0, linesize + f0, onDevice, draw ? draw + f0 : NULL, cnt,
+                        motions + f0) == VS_OK;
   }
   /* batch k's coarse results are turned into its fine jobs before the host
      waits for batch k-1's fine results, so the device gets batch k's fine
@@ -594,8 +698,8 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
     if (k + 2 < nb) {
       /* the slot set of batch k-1, whose fine stage is done */
       const int f0 = (k + 2) * B, cnt = VS_MIN(B, n - f0);
-      if (batch_submit(md, s, &bs[(k + 2) % MD_INFLIGHT], luma + f0, linesize + f0, onDevice,
-                       draw ? draw + f0 : NULL, cnt, motions + f0) != VS_OK) {
+      if (batch_submit_t(md, s, &bs[(k + 2) % MD_INFLIGHT], luma + f0, linesize + f0, onDevice,
+                         draw ? draw + f0 : NULL, cnt, motions + f0) != VS_OK) {
         ok = 0;
         break;
       }
@@ -616,9 +720,10 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
         for (int i = 0; i < bs[k].n; i++) vs_vector_del(&bs[k].coarse[i]);
     for (int i = 0; i < done; i++) vs_vector_del(&motions[i]);
   }
-  free(bs);
-  free(jobs);
-  free(sel);
+  if (s->timed) {
+    s->phaseUs[PH_CALL] += now_us() - tcall;
+    s->calls++;
+  }
   return ok ? VS_OK : VS_ERROR;
 }
 

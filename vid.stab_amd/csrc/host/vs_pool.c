@@ -67,6 +67,8 @@ VSPool* vs_pool_create(int nthreads) {
   return p;
 }
 
+int vs_pool_threads(const VSPool* p) { return p ? p->nthreads : 1; }
+
 void vs_pool_destroy(VSPool* p) {
   if (!p) return;
   pthread_mutex_lock(&p->mu);
