@@ -205,6 +205,56 @@ def measure_per_frame_c(spec, host, nframes=120, warmup=10):
     return res
 
 
+def measure_plain_4k_step4(steps=2, warmup=1):
+    """the 3840x2160 step-4 clip with moving rectangles (config 3, 240
+    frames), plain form, device-resident frames: frames/s and the coarse
+    search's abs-diff rate from HIP events on its stream"""
+    import numpy as np
+    import torch
+    from vidstab_amd import api
+    from vidstab_amd.synth import Clip
+    clip = Clip(3, 0)
+    s = clip.spec
+    n = s.nframes
+    host = np.empty((n, s.height, s.width), dtype=np.uint8)
+    for t in range(n):
+        clip.render(t, out=host[t])
+    dev = torch.from_numpy(host).to("cuda:0")
+    del host
+    torch.cuda.synchronize()
+    fb = s.width * s.height
+    ptrs = [dev.data_ptr() + i * fb for i in range(n)]
+    L = api.lib()
+    det = api.Detector(s.width, s.height, shakiness=s.shakiness, accuracy=s.accuracy, stepsize=s.stepsize,
+                       mincontrast=s.mincontrast)
+    for _ in range(warmup):
+        det.detect_device(ptrs, [s.width] * n)
+    L.vsg_profile_reset()
+    L.vsg_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        det.detect_device(ptrs, [s.width] * n)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    L.vsg_profile_enable(0)
+    prof = VSGProfile()
+    L.vsg_profile_get(ctypes.byref(prof))
+    det.close()
+    del dev
+    k = K_GRID_COARSE
+    avg_s = prof.ms[k] / prof.launches[k] / 1000.0
+    achieved = prof.work[k] / prof.launches[k] / avg_s / 1e12
+    res = {"workload": "3840x2160 YUV420p, 240-frame seeded clip with 3-6 moving textured rectangles (config 3), "
+                       "shakiness 10, accuracy 15, step 4, device-resident frames",
+           "frames": n, "fps": round(n / dt, 2), "ms_per_pass": round(dt * 1000.0, 2),
+           "coarse_roofline": {"bound": "valu", "kernel": "k_sad_search (coarse)", "achieved": round(achieved, 3),
+                               "peak": SAD_PEAK_T, "unit": "T abs-diff/s", "frac": round(achieved / SAD_PEAK_T, 4),
+                               "avg_launch_ms": round(avg_s * 1000.0, 3)}}
+    log(f"config 3 (4K, step 4) plain: {res['fps']:.1f} frames/s, coarse search {achieved:.1f} T abs-diff/s")
+    return res
+
+
 def measure_second_pass(spec, ptrs, ls):
     """the second pass's per-frame transforms (vsMotionsToTransformBatch, one
     launch for the clip, host LocalMotions in and transforms out) beside the
@@ -367,8 +417,9 @@ def main():
     per_frame = None if args.no_host else measure_per_frame_c(spec, host)
     second = None if args.no_second_pass else measure_second_pass(spec, ptrs, ls)
     del dev
-    forms = None
+    forms = plain4k = None
     if not args.no_4k:
+        plain4k = measure_plain_4k_step4()
         forms = measure_forms_4k(0, 3, 1, args.weights)   # 3 timed passes of each form
         forms["network_roofline"] = network_roofline(measure_network_layers(args.weights))
     cpu = cpu1 = None
@@ -403,6 +454,7 @@ def main():
         "per_frame_c_api": per_frame,
         "second_pass": second,
         "forms_4k": forms,
+        "plain_4k_step4": plain4k,
     }
     print(json.dumps(res), flush=True)
 

@@ -195,19 +195,20 @@ VS_API int vsg_motions_to_transform_static(int device, int width, int height, co
                                            VSTransform* out, double* residual, int* rounds);
 
 /* ---- the batched multi-clip form ---------------------------------------------
-   One context per clip, all on one stream; the searches of all clips that
-   share a plan run as one launch over their flattened job lists, and each
+   One context per clip, all on one stream; the searches of a stage run as one
+   persistent launch over a single queue of the units of every clip, and each
    slot set's stages are captured once as HIP graphs (fixed shapes: B slots
-   per clip, idle slots and padded fine lists skipped on the device).  Frames
-   must be in device memory.  Per-clip results are read with vsg_wait_coarse
-   / vsg_wait_fine on vsg_multi_clip(m, c) after the multi waits. */
+   per clip, idle slots and padded fine lists skipped on the device), with one
+   host<->device copy per crossing for all clips.  Frames must be in device
+   memory.  Per-clip results are read with vsg_wait_coarse / vsg_wait_fine on
+   vsg_multi_clip(m, c) after the multi waits. */
 typedef struct VSGMulti VSGMulti;
 VS_API int vsg_multi_create(VSGMulti** m, int device, int nclips, const int* width, const int* height,
                             const int* blurSize, const VSGStageDesc* coarse, const VSGStageDesc* fine,
                             int maxBatch);
 VS_API void vsg_multi_destroy(VSGMulti* m);
 VS_API VSGContext* vsg_multi_clip(VSGMulti* m, int clip);
-/* launches per batch of a stage (clips with one plan share a launch) */
+/* search launches per batch of a stage (1: the queue launch) */
 VS_API int vsg_multi_groups(VSGMulti* m, int stage);
 /* n[c] <= maxBatch frames of clip c (0: idle); lumaDev[c][i], linesize[c];
    prevSlot[c][i] a slot of the clip (>= 0), detect[c][i] as for vsg_submit */

@@ -87,6 +87,16 @@ static void hfree(T*& p) {
   p = nullptr;
 }
 
+void vsg_ctx_release_transfer(VSGContext* c, int set) {
+  VSGSet& s = c->set[set];
+  dfree(s.d_selIdx); dfree(s.d_selC); dfree(s.d_count); dfree(s.d_jobsF); dfree(s.d_resC); dfree(s.d_resF);
+  dfree(s.d_desc);
+  hfree(s.h_selIdx); hfree(s.h_selC); hfree(s.h_count); hfree(s.h_resC); hfree(s.h_jobsF); hfree(s.h_resF);
+  hfree(s.h_desc);
+  s.d_src = nullptr; s.d_prev = nullptr; s.d_detect = nullptr;
+  s.h_src = nullptr; s.h_prev = nullptr; s.h_detect = nullptr;
+}
+
 void vsg_ctx_free(VSGContext* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
@@ -100,11 +110,10 @@ void vsg_ctx_free(VSGContext* c) {
   dfree(c->d_allx); dfree(c->d_ally); dfree(c->d_allsz); dfree(c->d_patchOff); dfree(c->d_ctab);
   dfree(c->d_raw); dfree(c->d_blur); dfree(c->d_ref); dfree(c->d_tmp); dfree(c->d_patches);
   dfree(c->d_minmax);
-  for (VSGSet& s : c->set) {
-    dfree(s.d_selIdx); dfree(s.d_selC); dfree(s.d_count); dfree(s.d_jobsC); dfree(s.d_jobsF);
-    dfree(s.d_best); dfree(s.d_done); dfree(s.d_resC); dfree(s.d_resF); dfree(s.d_desc);
-    hfree(s.h_selIdx); hfree(s.h_selC); hfree(s.h_count); hfree(s.h_resC); hfree(s.h_jobsF);
-    hfree(s.h_resF); hfree(s.h_desc); hfree(s.h_stage);
+  for (int k = 0; k < VSG_NSETS; ++k) {
+    VSGSet& s = c->set[k];
+    if (!s.borrowed) vsg_ctx_release_transfer(c, k);
+    dfree(s.d_jobsC); dfree(s.d_best); dfree(s.d_done); hfree(s.h_stage);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.doneF) (void)hipEventDestroy(s.doneF);
     if (s.planesReady) (void)hipEventDestroy(s.planesReady);

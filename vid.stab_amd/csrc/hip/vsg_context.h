@@ -62,6 +62,7 @@ struct VSGSet {
   hipEvent_t ev[VSG_EV_COUNT] = {};
   hipEvent_t evf[VSG_EVF_COUNT] = {};
   bool prof = false, profF = false;
+  bool borrowed = false;   /* the transfer buffers point into a multi-clip arena (not freed here) */
 };
 
 struct VSGLearned;   /* vsg_learned.hip */
@@ -117,6 +118,9 @@ int vsg_learned_fine(VSGContext* c, int set, int njobs, hipStream_t st);
 void vsg_learned_account(VSGContext* c, int set, int stage);
 void vsg_learned_free(VSGContext* c);
 void vsg_ctx_free(VSGContext* c);
+/* frees a set's buffers that cross PCIe (descriptors, selection and coarse
+   results, fine jobs and results; device and pinned host) */
+void vsg_ctx_release_transfer(VSGContext* c, int set);
 /* enqueue helpers (no host synchronisation) */
 int vsg_ctx_upload_front(VSGContext* c, int set, int n);       /* H2D frame descriptors */
 /* (H2D descriptors unless upload is false), blur, fields, select */

@@ -124,14 +124,31 @@ hipError_t vsg_launch_fallback(const VSGSearchArgs& a, const int* list, const un
    GPU (same results) */
 hipError_t vsg_launch_search_any(const VSGSearchArgs& a, hipStream_t st);
 
-/* one clip's stage inside a multi-clip search launch */
+/* one clip's stage inside one XCD queue of a multi-clip queue launch */
 struct VSGMultiSeg {
-  VSGSearchArgs a;
-  int unitBase;     /* first unit (workgroup) of the segment */
+  VSGSearchArgs a;  /* vsg_queue_args of the clip's stage */
+  int unitBase;     /* first unit (workgroup-wide tile) of the segment in its queue */
+  int jobBase;      /* the segment's first job in this queue */
+  int variant;      /* vsg_queue_variant of a.g */
   int pad;
 };
-/* all segments share g's plan (J, K, D, CW, wg); dSegs is a device table
-   sorted by unitBase */
-hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, const VSGGrid& g, hipStream_t st);
+/* units per XCD queue, and the units a pull takes (set by the launcher) */
+struct VSGQueueTotals {
+  int units[VSG_XCDS];
+  int queues;   /* 1, or VSG_XCDS: one queue per XCD */
+  int chunk;
+};
+/* the queue kernel's plan variant of a grid (J, D, CW), or -1 if it has none */
+int vsg_queue_variant(const VSGGrid& g);
+/* a stage's search arguments as the queue kernel runs them: K = 4, the field
+   by scalar loads, K candidate rows per lane (4, 6, 8), tiles of wg lanes
+   (64, 128 or 256) */
+VSGSearchArgs vsg_queue_args(const VSGSearchArgs& a, int wg, int K);
+/* one persistent launch over all units of all segments: dSegs holds
+   totals.queues tables of nseg segments, each sorted by unitBase; queues:
+   two device counters per queue, zero before the launch and left zero
+   after it */
+hipError_t vsg_launch_queue(const VSGMultiSeg* dSegs, int nseg, const VSGQueueTotals& totals, int wg, int K,
+                            unsigned* queues, hipStream_t st);
 
 #endif /* VSG_INTERNAL_H */

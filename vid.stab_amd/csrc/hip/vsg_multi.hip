@@ -3,12 +3,10 @@
  *
  * Many independent clips (any mix of resolutions) advance together, up to B
  * frames of every clip per batch.  Each clip keeps its own VSGContext (its
- * planes, fields and selection lists); all contexts share one stream.  What
- * changes is the search: the coarse (and later the fine) jobs of all clips
- * whose search plans agree (same field size, step and grid, so the same
- * workgroup shape) form one flattened unit list and run as ONE launch of
- * k_sad_multi.  The plans come from the autotuner of each context, so clips
- * of one geometry always land in the same launch.
+ * planes, fields and selection lists); all contexts share one stream.  The
+ * searches of a stage run as ONE persistent launch of k_sad_queue
+ * (vsg_search.hip) over a single flattened queue of the units of every clip,
+ * whatever their plan shapes.
  *
  * Every batch has a fixed shape -- B frame slots per clip, idle slots marked
  * by a null source (skipped by the blur and field kernels, detect = 0 for the
@@ -16,15 +14,19 @@
  * capacity with field = -1 -- so each slot set's front + coarse stage and its
  * fine stage are captured once as HIP graphs and replayed for every batch
  * that uses the set.  Only pinned host descriptors change between replays.
- * No memset nodes are recorded: every buffer a graph reads is written by the
- * host or by an earlier node.
+ * What crosses PCIe for all clips of a batch sits in four per-set arenas, so
+ * each crossing is one copy node: the frame descriptors, the selection and
+ * coarse results, the fine jobs and the fine results.  The contexts' buffers
+ * for them point into the arenas.  No memset nodes are recorded: every
+ * buffer a graph reads is written by the host or by an earlier node, and the
+ * queue counters are left zero by the kernel itself.
  */
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
-#include <map>
+#include <algorithm>
 #include <new>
-#include <tuple>
 #include <vector>
 
 #include "vsg_context.h"
@@ -37,13 +39,14 @@ namespace {
     if (e_ != hipSuccess) return vsg_fail(#call, e_);     \
   } while (0)
 
-struct Group {
-  VSGGrid g{};
-  int tiles = 0;
-  std::vector<int> clips;
-  VSGMultiSeg* d_segs[VSG_NSETS] = {};
-  int units[VSG_NSETS] = {};
+/* host-visible regions of all clips' per-set buffers, one copy each */
+struct Arena {
+  uint8_t* d = nullptr;
+  uint8_t* h = nullptr;   /* pinned */
+  size_t bytes = 0;
 };
+
+enum { AR_DESC, AR_COARSE, AR_JOBS, AR_RESF, AR_COUNT };
 
 }  // namespace
 
@@ -52,7 +55,11 @@ struct VSGMulti {
   int B = 0;
   hipStream_t stream = nullptr;
   std::vector<VSGContext*> ctx;
-  std::vector<Group> grp[2];            /* per stage */
+  std::vector<int> order[2];                /* clips by descending field size, per stage */
+  VSGMultiSeg* d_segs[2][VSG_NSETS] = {};
+  VSGQueueTotals totals[2] = {};            /* units per XCD queue, per stage */
+  unsigned* d_queue = nullptr;              /* [stage][set][xcd][2] counters */
+  Arena ar[VSG_NSETS][AR_COUNT];
   std::vector<int> jobsFilled[VSG_NSETS];   /* per clip: fine entries that may hold jobs */
   hipGraphExec_t gA[VSG_NSETS] = {};
   hipGraphExec_t gF[VSG_NSETS] = {};
@@ -69,65 +76,180 @@ static void multi_free(VSGMulti* m) {
     if (m->gF[s]) (void)hipGraphExecDestroy(m->gF[s]);
     if (m->doneA[s]) (void)hipEventDestroy(m->doneA[s]);
     if (m->doneF[s]) (void)hipEventDestroy(m->doneF[s]);
+    for (int k = 0; k < 2; ++k)
+      if (m->d_segs[k][s]) (void)hipFree(m->d_segs[k][s]);
+    for (Arena& a : m->ar[s]) {
+      if (a.d) (void)hipFree(a.d);
+      if (a.h) (void)hipHostFree(a.h);
+    }
   }
-  for (auto& gs : m->grp)
-    for (Group& g : gs)
-      for (int s = 0; s < VSG_NSETS; ++s)
-        if (g.d_segs[s]) (void)hipFree(g.d_segs[s]);
+  if (m->d_queue) (void)hipFree(m->d_queue);
   for (VSGContext* c : m->ctx) {
     if (!c) continue;
-    vsg_ctx_free(c);
+    vsg_ctx_free(c);   /* skips the buffers that point into the arenas */
     delete c;
   }
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
 
-/* segment tables of one stage for one set: every member clip with all B
-   frame slots (fixed shape) */
-static int build_segments(VSGMulti* m, int stage, int set) {
-  for (Group& gr : m->grp[stage]) {
-    std::vector<VSGMultiSeg> segs;
-    int units = 0;
-    for (int c : gr.clips) {
-      VSGContext* x = m->ctx[c];
-      VSGSet& s = x->set[set];
-      const int njobs = m->B * x->st[stage].nf;
-      VSGMultiSeg sg;
-      sg.a = vsg_ctx_search_args(x, stage, stage == 0 ? s.d_jobsC : s.d_jobsF, njobs, s.d_best, s.d_done,
-                                 stage == 0 ? s.d_resC : s.d_resF);
-      sg.unitBase = units;
-      sg.pad = 0;
-      units += njobs * sg.a.tiles;
-      segs.push_back(sg);
+static size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+/* per clip and set: the bytes each arena holds for it */
+static void clip_arena_bytes(const VSGMulti* m, const VSGContext* x, size_t (&b)[AR_COUNT]) {
+  const size_t B = (size_t)m->B, nft = (size_t)x->nfTotal, nc = (size_t)x->st[0].nf, nf = (size_t)x->st[1].nf;
+  b[AR_DESC] = up256((sizeof(VSGBlurSrc) + sizeof(int) + 1) * B);
+  b[AR_COARSE] = up256(sizeof(double) * B * nft + sizeof(VSGResult) * B * nc + sizeof(int) * 2 * B +
+                       sizeof(int16_t) * B * nft);
+  b[AR_JOBS] = up256(sizeof(VSGJob) * B * nf);
+  b[AR_RESF] = up256(sizeof(VSGResult) * B * nf);
+}
+
+/* frees a context's own transfer buffers of every set and points them into
+   the arenas (the context's tuning, at creation, used its own) */
+static int bind_arenas(VSGMulti* m) {
+  const int B = m->B;
+  size_t total[AR_COUNT] = {0, 0, 0, 0};
+  std::vector<std::vector<size_t>> off(m->ctx.size(), std::vector<size_t>(AR_COUNT));
+  for (size_t c = 0; c < m->ctx.size(); ++c) {
+    size_t b[AR_COUNT];
+    clip_arena_bytes(m, m->ctx[c], b);
+    for (int k = 0; k < AR_COUNT; ++k) {
+      off[c][k] = total[k];
+      total[k] += b[k];
     }
-    if (!gr.d_segs[set]) MCHECK(hipMalloc(&gr.d_segs[set], sizeof(VSGMultiSeg) * segs.size()));
-    MCHECK(hipMemcpy(gr.d_segs[set], segs.data(), sizeof(VSGMultiSeg) * segs.size(), hipMemcpyHostToDevice));
-    gr.units[set] = units;
+  }
+  for (int set = 0; set < VSG_NSETS; ++set)
+    for (int k = 0; k < AR_COUNT; ++k) {
+      Arena& a = m->ar[set][k];
+      a.bytes = total[k];
+      MCHECK(hipMalloc(&a.d, a.bytes));
+      MCHECK(hipHostMalloc(reinterpret_cast<void**>(&a.h), a.bytes, hipHostMallocDefault));
+      memset(a.h, 0, a.bytes);
+      MCHECK(hipMemset(a.d, 0, a.bytes));
+    }
+  for (size_t c = 0; c < m->ctx.size(); ++c) {
+    VSGContext* x = m->ctx[c];
+    const size_t nft = (size_t)x->nfTotal, nc = (size_t)x->st[0].nf;
+    for (int set = 0; set < VSG_NSETS; ++set) {
+      VSGSet& s = x->set[set];
+      vsg_ctx_release_transfer(x, set);
+      s.borrowed = true;
+      /* descriptors: src[B], prev[B], detect[B] */
+      uint8_t* dd = m->ar[set][AR_DESC].d + off[c][AR_DESC];
+      uint8_t* hd = m->ar[set][AR_DESC].h + off[c][AR_DESC];
+      s.d_desc = dd;
+      s.h_desc = hd;
+      s.d_src = reinterpret_cast<VSGBlurSrc*>(dd);
+      s.h_src = reinterpret_cast<VSGBlurSrc*>(hd);
+      s.d_prev = reinterpret_cast<int*>(dd + sizeof(VSGBlurSrc) * B);
+      s.h_prev = reinterpret_cast<int*>(hd + sizeof(VSGBlurSrc) * B);
+      s.d_detect = dd + (sizeof(VSGBlurSrc) + sizeof(int)) * B;
+      s.h_detect = hd + (sizeof(VSGBlurSrc) + sizeof(int)) * B;
+      /* coarse out: selC, resC, count, selIdx (alignment order) */
+      uint8_t* dc = m->ar[set][AR_COARSE].d + off[c][AR_COARSE];
+      uint8_t* hc = m->ar[set][AR_COARSE].h + off[c][AR_COARSE];
+      size_t o = 0;
+      s.d_selC = reinterpret_cast<double*>(dc + o);
+      s.h_selC = reinterpret_cast<double*>(hc + o);
+      o += sizeof(double) * B * nft;
+      s.d_resC = reinterpret_cast<VSGResult*>(dc + o);
+      s.h_resC = reinterpret_cast<VSGResult*>(hc + o);
+      o += sizeof(VSGResult) * B * nc;
+      s.d_count = reinterpret_cast<int*>(dc + o);
+      s.h_count = reinterpret_cast<int*>(hc + o);
+      o += sizeof(int) * 2 * B;
+      s.d_selIdx = reinterpret_cast<int16_t*>(dc + o);
+      s.h_selIdx = reinterpret_cast<int16_t*>(hc + o);
+      s.d_jobsF = reinterpret_cast<VSGJob*>(m->ar[set][AR_JOBS].d + off[c][AR_JOBS]);
+      s.h_jobsF = reinterpret_cast<VSGJob*>(m->ar[set][AR_JOBS].h + off[c][AR_JOBS]);
+      s.d_resF = reinterpret_cast<VSGResult*>(m->ar[set][AR_RESF].d + off[c][AR_RESF]);
+      s.h_resF = reinterpret_cast<VSGResult*>(m->ar[set][AR_RESF].h + off[c][AR_RESF]);
+    }
   }
   return 0;
 }
 
-static int launch_groups(VSGMulti* m, int stage, int set) {
-  for (Group& gr : m->grp[stage])
-    MCHECK(vsg_launch_multi(gr.d_segs[set], (int)gr.clips.size(), gr.units[set], gr.g, m->stream));
+/* the queue launches' shape: workgroup lanes and candidate rows per lane of
+   each stage, and the queue count (1, or one per XCD); VIDSTAB_AMD_QUEUE_WG,
+   VIDSTAB_AMD_QUEUE_K (coarse,fine) and VIDSTAB_AMD_QUEUES override */
+static int env_pair(const char* name, int stage, int def) {
+  const char* v = getenv(name);
+  if (!v || !*v) return def;
+  int a = def, b = def;
+  if (sscanf(v, "%d,%d", &a, &b) < 2) b = a;
+  return stage == 0 ? a : b;
+}
+/* defaults measured on config 4's 64 clips (frames/s of the whole call):
+   one queue 6,013, one per XCD 4,961 (the eighths finish unevenly with no
+   stealing between them); 256 / 128 lanes 6,013, 256 / 64 5,651, 256 / 256
+   5,053; K 6 / 6 6,013, 4 / 6 5,651, 4 / 8 5,255 */
+static int queue_wg(int stage) { return env_pair("VIDSTAB_AMD_QUEUE_WG", stage, stage == 0 ? 256 : 128); }
+static int queue_k(int stage) { return env_pair("VIDSTAB_AMD_QUEUE_K", stage, 6); }
+static int queue_count(void) {
+  const char* v = getenv("VIDSTAB_AMD_QUEUES");
+  return v && atoi(v) == VSG_XCDS ? VSG_XCDS : 1;
+}
+
+/* the queues' segment tables of one stage for one set: every clip with all
+   B frame slots (fixed shape), larger fields first; XCD queue x takes the
+   x-th eighth of each clip's job list */
+static int build_segments(VSGMulti* m, int stage, int set) {
+  const int nseg = (int)m->ctx.size();
+  VSGQueueTotals& tot = m->totals[stage];
+  tot.queues = queue_count();
+  std::vector<VSGMultiSeg> segs((size_t)tot.queues * nseg);
+  for (int x = 0; x < tot.queues; ++x) {
+    int units = 0, k = 0;
+    for (int c : m->order[stage]) {
+      VSGContext* x_ = m->ctx[c];
+      VSGSet& s = x_->set[set];
+      const int njobs = m->B * x_->st[stage].nf;
+      VSGMultiSeg sg;
+      sg.a = vsg_queue_args(vsg_ctx_search_args(x_, stage, stage == 0 ? s.d_jobsC : s.d_jobsF, njobs, s.d_best,
+                                                s.d_done, stage == 0 ? s.d_resC : s.d_resF),
+                            queue_wg(stage), queue_k(stage));
+      sg.variant = vsg_queue_variant(sg.a.g);
+      if (sg.variant < 0) return vsg_fail("vsg_multi: search plan without a queue variant", hipSuccess);
+      const int j0 = (int)((long long)njobs * x / tot.queues), j1 = (int)((long long)njobs * (x + 1) / tot.queues);
+      sg.jobBase = j0;
+      sg.unitBase = units;
+      sg.pad = 0;
+      units += (j1 - j0) * sg.a.tiles;
+      segs[(size_t)x * nseg + k++] = sg;
+    }
+    tot.units[x] = units;
+  }
+  tot.chunk = 1;
+  if (!m->d_segs[stage][set]) MCHECK(hipMalloc(&m->d_segs[stage][set], sizeof(VSGMultiSeg) * segs.size()));
+  MCHECK(hipMemcpy(m->d_segs[stage][set], segs.data(), sizeof(VSGMultiSeg) * segs.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
-/* front + coarse search + coarse read-back of every clip, fixed shape */
+static int launch_queue(VSGMulti* m, int stage, int set) {
+  MCHECK(vsg_launch_queue(m->d_segs[stage][set], (int)m->ctx.size(), m->totals[stage], queue_wg(stage), queue_k(stage),
+                          m->d_queue + 2 * VSG_XCDS * (stage * VSG_NSETS + set), m->stream));
+  return 0;
+}
+
+/* descriptors upload + front of every clip + coarse queue + one read-back */
 static int capture_front(VSGMulti* m, int set) {
   if (build_segments(m, 0, set) != 0) return -1;
   hipGraph_t g = nullptr;
   MCHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
+  const Arena& ad = m->ar[set][AR_DESC];
+  if (hipMemcpyAsync(ad.d, ad.h, ad.bytes, hipMemcpyHostToDevice, m->stream) != hipSuccess)
+    rc = vsg_fail("descriptor upload capture", hipErrorUnknown);
   for (VSGContext* x : m->ctx) {
+    if (rc != 0) break;
     x->set[set].prof = false;
-    if ((rc = vsg_ctx_enqueue_front(x, set, m->B)) != 0) break;
+    rc = vsg_ctx_enqueue_front(x, set, m->B, false);
   }
-  if (rc == 0) rc = launch_groups(m, 0, set);
-  if (rc == 0)
-    for (VSGContext* x : m->ctx)
-      if ((rc = vsg_ctx_enqueue_coarse_readback(x, set, m->B)) != 0) break;
+  if (rc == 0) rc = launch_queue(m, 0, set);
+  const Arena& ac = m->ar[set][AR_COARSE];
+  if (rc == 0 && hipMemcpyAsync(ac.h, ac.d, ac.bytes, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
+    rc = vsg_fail("coarse read-back capture", hipErrorUnknown);
   hipError_t e = hipStreamEndCapture(m->stream, &g);
   if (rc != 0) {
     if (g) (void)hipGraphDestroy(g);
@@ -140,32 +262,19 @@ static int capture_front(VSGMulti* m, int set) {
   return 0;
 }
 
-/* fine jobs upload + fine search + results read-back, fixed shape */
+/* fine jobs upload + fine queue + results read-back, fixed shape */
 static int capture_fine(VSGMulti* m, int set) {
   if (build_segments(m, 1, set) != 0) return -1;
   hipGraph_t g = nullptr;
   MCHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (VSGContext* x : m->ctx) {
-    VSGSet& s = x->set[set];
-    const size_t cap = (size_t)m->B * x->st[1].nf;
-    if (hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * cap, hipMemcpyHostToDevice, m->stream) !=
-        hipSuccess) {
-      rc = vsg_fail("fine job upload capture", hipErrorUnknown);
-      break;
-    }
-  }
-  if (rc == 0) rc = launch_groups(m, 1, set);
-  if (rc == 0)
-    for (VSGContext* x : m->ctx) {
-      VSGSet& s = x->set[set];
-      const size_t cap = (size_t)m->B * x->st[1].nf;
-      if (hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * cap, hipMemcpyDeviceToHost, m->stream) !=
-          hipSuccess) {
-        rc = vsg_fail("fine read-back capture", hipErrorUnknown);
-        break;
-      }
-    }
+  const Arena& aj = m->ar[set][AR_JOBS];
+  const Arena& ar = m->ar[set][AR_RESF];
+  if (hipMemcpyAsync(aj.d, aj.h, aj.bytes, hipMemcpyHostToDevice, m->stream) != hipSuccess)
+    rc = vsg_fail("fine job upload capture", hipErrorUnknown);
+  if (rc == 0) rc = launch_queue(m, 1, set);
+  if (rc == 0 && hipMemcpyAsync(ar.h, ar.d, ar.bytes, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
+    rc = vsg_fail("fine read-back capture", hipErrorUnknown);
   hipError_t e = hipStreamEndCapture(m->stream, &g);
   if (rc != 0) {
     if (g) (void)hipGraphDestroy(g);
@@ -176,11 +285,6 @@ static int capture_fine(VSGMulti* m, int set) {
   (void)hipGraphDestroy(g);
   MCHECK(e);
   return 0;
-}
-
-static bool same_plan(const VSGStage& a, const VSGStage& b) {
-  return a.S == b.S && a.step == b.step && a.g.n == b.g.n && a.g.J == b.g.J && a.g.K == b.g.K &&
-         a.g.D == b.g.D && a.g.CW == b.g.CW && a.g.wg == b.g.wg && a.tiles == b.tiles;
 }
 
 extern "C" int vsg_multi_create(VSGMulti** out, int device, int nclips, const int* width, const int* height,
@@ -208,21 +312,17 @@ extern "C" int vsg_multi_create(VSGMulti** out, int device, int nclips, const in
     m->ctx.push_back(x);
     rc = vsg_ctx_init(x, device, width[c], height[c], 1, blurSize[c], &coarse[c], &fine[c], maxBatch, shared);
   }
-  /* group clips by plan, per stage */
-  for (int k = 0; k < 2 && rc == 0; ++k)
-    for (int c = 0; c < nclips; ++c) {
-      const VSGStage& S = m->ctx[c]->st[k];
-      Group* hit = nullptr;
-      for (Group& gr : m->grp[k])
-        if (same_plan(m->ctx[gr.clips[0]]->st[k], S)) hit = &gr;
-      if (!hit) {
-        m->grp[k].push_back(Group());
-        hit = &m->grp[k].back();
-        hit->g = S.g;
-        hit->tiles = S.tiles;
-      }
-      hit->clips.push_back(c);
-    }
+  /* queue order per stage: larger fields (longer units) first */
+  for (int k = 0; k < 2 && rc == 0; ++k) {
+    for (int c = 0; c < nclips; ++c) m->order[k].push_back(c);
+    std::stable_sort(m->order[k].begin(), m->order[k].end(),
+                     [&](int a, int b) { return m->ctx[a]->st[k].S > m->ctx[b]->st[k].S; });
+  }
+  if (rc == 0) rc = bind_arenas(m);
+  const size_t qbytes = sizeof(unsigned) * 2 * VSG_XCDS * 2 * VSG_NSETS;
+  if (rc == 0 && (hipMalloc(&m->d_queue, qbytes) != hipSuccess || hipMemset(m->d_queue, 0, qbytes) != hipSuccess ||
+                  hipDeviceSynchronize() != hipSuccess))
+    rc = vsg_fail("vsg_multi_create: queue counters", hipSuccess);
   for (int s = 0; s < VSG_NSETS && rc == 0; ++s) {
     if (hipEventCreateWithFlags(&m->doneA[s], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&m->doneF[s], hipEventDisableTiming) != hipSuccess)
@@ -250,7 +350,7 @@ extern "C" VSGContext* vsg_multi_clip(VSGMulti* m, int clip) {
 
 extern "C" int vsg_multi_groups(VSGMulti* m, int stage) {
   if (!m || stage < 0 || stage > 1) return -1;
-  return (int)m->grp[stage].size();
+  return 1;   /* one queue launch per stage for all clips */
 }
 
 extern "C" int vsg_multi_submit(VSGMulti* m, int set, const int* n, const uint8_t* const* const* lumaDev,

@@ -10,8 +10,9 @@
  *                 order (src/motiondetect.c:545-567), by the job's last
  *                 workgroup
  *
- * k_sad_multi runs the same body over the flattened job lists of several
- * clips at once (the batched multi-clip form).
+ * k_sad_queue runs the same body, one persistent launch per stage, over one
+ * flattened queue of the units of every clip of a batch (the batched
+ * multi-clip form).
  *
  * The launch is a 1-D grid whose block -> work mapping is XCD-aware: the
  * hardware hands workgroups to the 8 XCDs round-robin (block b runs on XCD
@@ -284,29 +285,95 @@ __global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
   search_wg<J, K, D, CW, LDSP>(a, job, unit - job * a.tiles, L);
 }
 
-/* The multi-clip form: one launch over the flattened units of several
-   segments (the same stage of clips that share a plan, so one workgroup
-   shape fits all).  Segment s owns units [unitBase, unitBase + njobs*tiles);
-   a workgroup finds its segment by binary search over the (wave-uniform)
-   table. */
-template <int J, int K, int D, int CW, bool LDSP>
-__global__ __launch_bounds__(256) void k_sad_multi(const VSGMultiSeg* __restrict__ segs, int nseg, int totalUnits,
-                                                   int chunk) {
+/* ---- the multi-clip queue kernel -----------------------------------------------
+   One persistent launch per stage for ALL clips of a batch: the units (clip
+   segment, job, 64-lane tile) of every clip and every plan shape form one
+   flattened queue; a unit is one workgroup-wide tile of a job's candidates
+   (256 lanes for the coarse stage, whose waves then share the field's
+   scalar loads, 64 for the small fine jobs).  A workgroup pulls the next unit
+   (a few consecutive units) with an atomic counter, runs the K = 4 grid body of the unit's plan
+   (selected by the segment's variant: J, D and the column chunk CW; K
+   candidate rows per lane is the launch's), folds
+   its argmin into best[job], and the job's last unit refines it -- the same
+   code as k_sad_search.  No workgroup ever waits for another: the only
+   barriers are inside the one-wave workgroup, reached by all its threads
+   for every unit, and a job's refinement is run by whichever unit finishes
+   it last.  There is one queue per XCD (workgroup b serves queue b % 8):
+   queue x holds the x-th eighth of every clip's job list, consecutive jobs
+   of consecutive frames, so a frame's planes are searched from one L2 (or a
+   single queue for the whole grid, balanced across the XCDs).
+   When its queue is empty a workgroup counts itself out; the last one out
+   of a queue resets that queue's counters for the next launch (every pull
+   of every member happened before its own count, so no pull can follow the
+   reset).  Segments are ordered by descending field size, so the largest
+   units are pulled first and the tail is made of small ones. */
+constexpr int kQueueWaves = 256 * 16;   /* resident waves: 256 CUs x 16 */
+
+template <int J, int K, int D, int CW>
+__device__ __forceinline__ void queue_unit(const VSGSearchArgs& a, int job, int tile, RefineLds& L) {
+  search_wg<J, K, D, CW, false>(a, job, tile, L);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_sad_queue(const VSGMultiSeg* __restrict__ segs, int nseg,
+                                                   VSGQueueTotals totals, unsigned* __restrict__ queues) {
   __shared__ RefineLds L;
-  const int unit = vsg_xcd_unit(blockIdx.x, chunk, totalUnits);
-  if (unit < 0) return;
-  int lo = 0, hi = nseg - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (segs[mid].unitBase <= unit)
-      lo = mid;
-    else
-      hi = mid - 1;
+  __shared__ int pulled;
+  /* workgroup b serves queue b % nq; with 8 queues, the queue of its XCD */
+  const int nq = totals.queues;
+  const int x = (int)(blockIdx.x % (unsigned)nq);
+  const VSGMultiSeg* qs = segs + (size_t)x * nseg;
+  const int totalUnits = totals.units[x], chunk = totals.chunk;
+  unsigned* queue = queues + 2 * x;
+  int seg = 0;   /* segment of the previous unit: consecutive units mostly share it */
+  for (;;) {
+    /* a pull takes `chunk` consecutive units: fewer atomics on one address */
+    if (threadIdx.x == 0) pulled = (int)atomicAdd(&queue[0], (unsigned)chunk);
+    __syncthreads();
+    const int first = pulled;
+    __syncthreads();   /* `pulled` is rewritten by the next pull */
+    if (first >= totalUnits) break;
+    const int last = min(first + chunk, totalUnits);
+    for (int unit = first; unit < last; ++unit) {
+      const int end = seg + 1 < nseg ? qs[seg + 1].unitBase : totalUnits;
+      if (unit < qs[seg].unitBase || unit >= end) {
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (qs[mid].unitBase <= unit)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        seg = lo;
+      }
+      const VSGMultiSeg* sg = qs + seg;
+      const int local = unit - sg->unitBase;
+      const int job = sg->jobBase + local / sg->a.tiles, tile = local % sg->a.tiles;
+      switch (sg->variant) {
+#define VSG_QUEUE_CASE(jd, cwi, J, D, CW) \
+  case (jd) * 6 + (cwi): queue_unit<J, K, D, CW>(sg->a, job, tile, L); break;
+        VSG_QUEUE_CASE(0, 0, 4, 1, 28) VSG_QUEUE_CASE(0, 1, 4, 1, 20) VSG_QUEUE_CASE(0, 2, 4, 1, 16)
+        VSG_QUEUE_CASE(0, 3, 4, 1, 12) VSG_QUEUE_CASE(0, 4, 4, 1, 8) VSG_QUEUE_CASE(0, 5, 4, 1, 4)
+        VSG_QUEUE_CASE(1, 0, 4, 3, 28) VSG_QUEUE_CASE(1, 1, 4, 3, 20) VSG_QUEUE_CASE(1, 2, 4, 3, 16)
+        VSG_QUEUE_CASE(1, 3, 4, 3, 12) VSG_QUEUE_CASE(1, 4, 4, 3, 8) VSG_QUEUE_CASE(1, 5, 4, 3, 4)
+        VSG_QUEUE_CASE(2, 0, 1, 1, 28) VSG_QUEUE_CASE(2, 1, 1, 1, 20) VSG_QUEUE_CASE(2, 2, 1, 1, 16)
+        VSG_QUEUE_CASE(2, 3, 1, 1, 12) VSG_QUEUE_CASE(2, 4, 1, 1, 8) VSG_QUEUE_CASE(2, 5, 1, 1, 4)
+#undef VSG_QUEUE_CASE
+        default: break;
+      }
+      __syncthreads();   /* L is rewritten by the next unit */
+    }
   }
-  const VSGMultiSeg* sg = segs + lo;
-  const int local = unit - sg->unitBase;
-  const int job = local / sg->a.tiles;
-  search_wg<J, K, D, CW, LDSP>(sg->a, job, local - job * sg->a.tiles, L);
+  if (threadIdx.x == 0) {
+    /* the workgroups serving queue x: b = x, x + nq, ... < gridDim.x */
+    const unsigned members = (gridDim.x - (unsigned)x + (unsigned)nq - 1) / (unsigned)nq;
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == members - 1) {
+      queue[0] = 0;
+      queue[1] = 0;
+    }
+  }
 }
 
 /* ---- exact search of a fallback list ----------------------------------------------
@@ -594,40 +661,6 @@ hipError_t launch_grid_k(const VSGSearchArgs& a, hipStream_t st) {
   }
 }
 
-template <int J, int K, int D, int CW>
-hipError_t launch_multi(const VSGMultiSeg* segs, int nseg, int totalUnits, int wg, int lds, size_t patchBytes,
-                        hipStream_t st) {
-  const int chunk = 32;
-  const int nb = vsg_xcd_blocks(totalUnits, chunk);
-  if (lds)
-    hipLaunchKernelGGL((k_sad_multi<J, K, D, CW, true>), dim3(nb), dim3(wg), patchBytes, st, segs, nseg,
-                       totalUnits, chunk);
-  else
-    hipLaunchKernelGGL((k_sad_multi<J, K, D, CW, false>), dim3(nb), dim3(wg), 0, st, segs, nseg, totalUnits, chunk);
-  return hipGetLastError();
-}
-
-template <int J, int K, int D>
-hipError_t launch_multi_cw(const VSGMultiSeg* segs, int nseg, int units, const VSGGrid& g, hipStream_t st) {
-  switch (g.CW) {
-    case 28: return launch_multi<J, K, D, 28>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
-    case 20: return launch_multi<J, K, D, 20>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
-    case 16: return launch_multi<J, K, D, 16>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
-    case 12: return launch_multi<J, K, D, 12>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
-    case 8: return launch_multi<J, K, D, 8>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
-    default: return launch_multi<J, K, D, 4>(segs, nseg, units, g.wg, g.lds, vsg_patch_bytes(g), st);
-  }
-}
-
-template <int J, int D>
-hipError_t launch_multi_k(const VSGMultiSeg* segs, int nseg, int units, const VSGGrid& g, hipStream_t st) {
-  switch (g.K) {
-    case 8: return launch_multi_cw<J, 8, D>(segs, nseg, units, g, st);
-    case 6: return launch_multi_cw<J, 6, D>(segs, nseg, units, g, st);
-    default: return launch_multi_cw<J, 4, D>(segs, nseg, units, g, st);
-  }
-}
-
 template <int J, int D, int WPT>
 hipError_t launch_fallback(const VSGSearchArgs& b, const int* list, const unsigned* count, hipStream_t st) {
   const int nb = min(kFallbackBlocks, b.njobs * b.tiles);
@@ -750,12 +783,52 @@ hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st) {
   return launch_grid_k<1, 1>(a, st);
 }
 
-hipError_t vsg_launch_multi(const VSGMultiSeg* dSegs, int nseg, int totalUnits, const VSGGrid& g, hipStream_t st) {
-  if (totalUnits <= 0 || nseg <= 0) return hipSuccess;
-  if (!dSegs || g.S > 256 || g.wg < 64 || g.wg > 256 || g.S * g.bpp > 4 * g.wg) return hipErrorInvalidValue;
-  if (g.J == 4 && g.D == 1) return launch_multi_k<4, 1>(dSegs, nseg, totalUnits, g, st);
-  if (g.J == 4 && g.D == 3) return launch_multi_k<4, 3>(dSegs, nseg, totalUnits, g, st);
-  return launch_multi_k<1, 1>(dSegs, nseg, totalUnits, g, st);
+int vsg_queue_variant(const VSGGrid& g) {
+  static const int kCWs[6] = {28, 20, 16, 12, 8, 4};
+  const int jd = g.J == 4 && g.D == 1 ? 0 : (g.J == 4 && g.D == 3 ? 1 : (g.J == 1 && g.D == 1 ? 2 : -1));
+  if (jd < 0 || g.bpp != 1) return -1;
+  for (int i = 0; i < 6; ++i)
+    if (g.CW == kCWs[i]) return jd * 6 + i;
+  return -1;
+}
+
+VSGSearchArgs vsg_queue_args(const VSGSearchArgs& a, int wg, int K) {
+  VSGSearchArgs b = a;
+  b.g.K = K;
+  b.g.V = (b.g.G + K - 1) / K;
+  b.g.wg = wg;
+  b.g.lds = 0;
+  b.tiles = (b.g.U * b.g.V + wg - 1) / wg;
+  return b;
+}
+
+hipError_t vsg_launch_queue(const VSGMultiSeg* dSegs, int nseg, const VSGQueueTotals& totals, int wg, int K,
+                            unsigned* queues, hipStream_t st) {
+  int sum = 0, most = 0;
+  if (totals.queues != 1 && totals.queues != VSG_XCDS) return hipErrorInvalidValue;
+  for (int x = 0; x < totals.queues; ++x) {
+    sum += totals.units[x];
+    most = totals.units[x] > most ? totals.units[x] : most;
+  }
+  if (sum <= 0 || nseg <= 0) return hipSuccess;
+  if (!dSegs || !queues || (wg != 64 && wg != 128 && wg != 256) || (K != 4 && K != 6 && K != 8))
+    return hipErrorInvalidValue;
+  /* every queue gets workgroups: at least 8, at most the resident count */
+  const int resident = kQueueWaves / (wg / 64);
+  int nb = sum < resident ? sum : resident;
+  nb = nb < totals.queues ? totals.queues : nb;
+  VSGQueueTotals t = totals;
+  /* about 16 pulls per workgroup, at most 16 units a pull */
+  const int perQueueWg = nb / totals.queues;
+  int chunk = most / (perQueueWg * 16);
+  t.chunk = chunk < 1 ? 1 : (chunk > 16 ? 16 : chunk);
+  if (K == 4)
+    hipLaunchKernelGGL(k_sad_queue<4>, dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
+  else if (K == 6)
+    hipLaunchKernelGGL(k_sad_queue<6>, dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
+  else
+    hipLaunchKernelGGL(k_sad_queue<8>, dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
+  return hipGetLastError();
 }
 
 hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npred, int slot0, uint32_t maxSad,
