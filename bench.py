@@ -621,19 +621,37 @@ def measure_forms_4k(nframes, steps, warmup, weights, clip_index=0):
     ref = plain.detect_device(ptrs, ls, keep=True)
     S = plain.md.fieldscoarse.fieldSize
     plain.close()
-    lrn = api.Detector(s.width, s.height, **kw)
-    lrn.set_proposal(weights)
-    learned_fps = timed(lrn)
-    f0, b0 = lrn.proposal_stats(0)
-    g0, c0 = lrn.proposal_stats(1)
-    got = lrn.detect_device(ptrs, ls, keep=True)
-    f1, b1 = lrn.proposal_stats(0)
-    g1, c1 = lrn.proposal_stats(1)
-    lrn.close()
+    def learned(walk):
+        """the learned form with windows that may walk `walk` times before
+        the full search (1: the first edge minimum falls back)"""
+        old = os.environ.get("VIDSTAB_AMD_PROPOSAL_WALK")
+        os.environ["VIDSTAB_AMD_PROPOSAL_WALK"] = str(walk)
+        try:
+            lrn = api.Detector(s.width, s.height, **kw)
+            lrn.set_proposal(weights)
+        finally:
+            if old is None:
+                del os.environ["VIDSTAB_AMD_PROPOSAL_WALK"]
+            else:
+                os.environ["VIDSTAB_AMD_PROPOSAL_WALK"] = old
+        fps = timed(lrn)
+        f0, b0 = lrn.proposal_stats(0)
+        g0, c0 = lrn.proposal_stats(1)
+        got = lrn.detect_device(ptrs, ls, keep=True)
+        f1, b1 = lrn.proposal_stats(0)
+        g1, c1 = lrn.proposal_stats(1)
+        lrn.close()
+        return fps, got, (f1 - f0, b1 - b0, g1 - g0, c1 - c0)
+
+    learned_fps, got, (nf0, nb0, nf1, nb1) = learned(4)
+    edge_fps, got1, (ef0, eb0, ef1, eb1) = learned(1)
     del dev
     cn, ce, an, ae = api.compare_forms(ref, got, S)
     errs = [api.transform_distance(s.width, s.height, api.simple_transform(s.width, s.height, p),
                                    api.simple_transform(s.width, s.height, q)) for p, q in zip(ref[1:], got[1:])]
+    cn1, ce1, an1, ae1 = api.compare_forms(ref, got1, S)
+    errs1 = [api.transform_distance(s.width, s.height, api.simple_transform(s.width, s.height, p),
+                                    api.simple_transform(s.width, s.height, q)) for p, q in zip(ref[1:], got1[1:])]
     out = {"workload": f"{s.width}x{s.height}, {nfr} frames of config 5 clip {clip_index} (motion blur, not in "
                        "the training set), device-resident", "frames": nfr,
            "plain_fps": round(plain_fps, 2), "learned_fps": round(learned_fps, 2),
@@ -641,13 +659,24 @@ def measure_forms_4k(nframes, steps, warmup, weights, clip_index=0):
            "fields_equal_pct": round(100.0 * ae / max(an, 1), 3),
            "coarse_fields_equal_pct": round(100.0 * ce / max(cn, 1), 3),
            "fine_fields_equal_pct": round(100.0 * (ae - ce) / max(an - cn, 1), 3),
-           "fallback_pct": round(100.0 * (b1 - b0) / max(f1 - f0, 1), 3),
-           "fine_fallback_pct": round(100.0 * (c1 - c0) / max(g1 - g0, 1), 3),
-           "global_transform_err_px": round(float(np.mean(errs)), 5)}
+           "fallback_pct": round(100.0 * nb0 / max(nf0, 1), 3),
+           "fine_fallback_pct": round(100.0 * nb1 / max(nf1, 1), 3),
+           "global_transform_err_px": round(float(np.mean(errs)), 5),
+           "window_walk": 4,
+           # the rule as first specified: the full search at the first edge minimum
+           "first_edge_fallback": {"learned_fps": round(edge_fps, 2), "speedup_vs_plain": round(edge_fps / plain_fps, 3),
+                                   "fields_equal_pct": round(100.0 * ae1 / max(an1, 1), 3),
+                                   "fallback_pct": round(100.0 * eb0 / max(ef0, 1), 3),
+                                   "fine_fallback_pct": round(100.0 * eb1 / max(ef1, 1), 3),
+                                   "global_transform_err_px": round(float(np.mean(errs1)), 5)}}
     log(f"4K: plain {plain_fps:.1f} fps, learned {learned_fps:.1f} fps ({learned_fps / plain_fps:.2f}x); fields "
         f"equal {out['fields_equal_pct']:.2f}% (fine {out['fine_fields_equal_pct']:.2f}%), fallbacks coarse "
         f"{out['fallback_pct']:.2f}% / fine {out['fine_fallback_pct']:.2f}%, mean transform error "
         f"{out['global_transform_err_px']:.4f} px")
+    e = out["first_edge_fallback"]
+    log(f"4K learned, full search at the first edge minimum: {e['learned_fps']:.1f} fps ({e['speedup_vs_plain']:.2f}x), "
+        f"fields equal {e['fields_equal_pct']:.2f}%, fallbacks coarse {e['fallback_pct']:.2f}% / fine "
+        f"{e['fine_fallback_pct']:.2f}%")
     return out
 
 

@@ -112,9 +112,11 @@ hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st);
    index of every other job goes to list[stats[1]++] for the exact search
    (in no particular order); stats[0] += jobs, stats[2..4] += fallbacks out
    of range / window minimum on the edge / above the residual threshold.
-   stats[1] must be 0 at the launch. */
+   A minimum on the window's edge moves the window there, at most `steps`
+   windows in all (1: the first edge minimum falls back).  stats[1] must be
+   0 at the launch. */
 hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npred, int slot0, uint32_t maxSad,
-                             int* list, unsigned* stats, hipStream_t st);
+                             int* list, unsigned* stats, int steps, hipStream_t st);
 /* the exact search (grid + refinement) of the jobs a.jobs[list[i]],
    i < *count (a device count, at most a.njobs), results to
    a.results[list[i]]; list and count both null: every job of a.jobs.

@@ -93,6 +93,7 @@ struct VSGLearned {
   int* d_win = nullptr;            /* [B][nc][5] */
   float* d_pred[VSG_NSETS] = {};   /* [B][nc][2]: kept per set until its coarse windows */
   bool fine = true;                /* proposals for the fine stage too */
+  int walk = 4;                    /* windows a search may walk before the full search (1: none) */
   int* d_winF = nullptr;           /* [B][nff][5] */
   float* d_predF[VSG_NSETS] = {};  /* [B][nff][2]: kept per set until its fine stage */
   int* d_flist[VSG_NSETS] = {};    /* coarse jobs left to the exact search */
@@ -156,6 +157,8 @@ extern "C" int vsg_set_proposal(VSGContext* c, int nlayers, const int* dil, cons
   const int B = c->B, nc = c->st[0].nf, nff = c->st[1].nf, slots = VSG_NSETS * B;
   const char* fineEnv = getenv("VIDSTAB_AMD_PROPOSAL_FINE");
   L->fine = !(fineEnv && atoi(fineEnv) == 0);
+  const char* walkEnv = getenv("VIDSTAB_AMD_PROPOSAL_WALK");
+  L->walk = walkEnv && *walkEnv ? (atoi(walkEnv) < 1 ? 1 : (atoi(walkEnv) > 8 ? 8 : atoi(walkEnv))) : 4;
   const char* fuseEnv = getenv("VIDSTAB_AMD_CONV_FUSE");
   L->fuse01 = !(fuseEnv && atoi(fuseEnv) == 0);
   const size_t mapElems = (size_t)B * L->Hq * L->Wq * kC;
@@ -336,7 +339,7 @@ int vsg_learned_coarse(VSGContext* c, int set, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set], 10);
   VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
   LCHECK(vsg_launch_window(ga, L->d_pred[set], nc, slot0, max_sad(L, c->st[0].S), L->d_flist[set], L->d_stats[set],
-                           st));
+                           L->walk, st));
   LCHECK(vsg_launch_fallback(ga, L->d_flist[set], L->d_stats[set] + 1, st));
   LCHECK(hipMemcpyAsync(L->h_stats[set], L->d_stats[set], sizeof(unsigned) * 5, hipMemcpyDeviceToHost, st));
   return 0;
@@ -352,7 +355,7 @@ int vsg_learned_fine(VSGContext* c, int set, int njobs, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L->d_stats[set] + 5, 5);
   VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
   LCHECK(vsg_launch_window(fa, L->d_predF[set], c->st[1].nf, set * c->B, max_sad(L, c->st[1].S), L->d_flistF[set],
-                           L->d_stats[set] + 5, st));
+                           L->d_stats[set] + 5, L->walk, st));
   LCHECK(vsg_launch_fallback(fa, L->d_flistF[set], L->d_stats[set] + 6, st));
   LCHECK(hipMemcpyAsync(L->h_stats[set] + 5, L->d_stats[set] + 5, sizeof(unsigned) * 5, hipMemcpyDeviceToHost, st));
   return 0;

@@ -449,7 +449,8 @@ __global__ __launch_bounds__(64 * WPT) void k_sad_fallback(VSGSearchArgs a, cons
    frame.  The 5x5 window of 1-px shifts around p is searched exhaustively,
    keeping the reference refinement's rule (the centre first, then x outer /
    y inner, strict "<"); while its minimum lies on the window's edge the
-   window moves to that minimum (a descent, kWindowSteps windows at most).
+   window moves to that minimum (a descent of at most `steps` windows; 1
+   falls back at the first edge minimum).
    The result is accepted when the minimum is inside the window and its SAD
    is at most maxSad; then it is written to results[j], otherwise j is
    appended to the fallback list for the exact full search.
@@ -465,7 +466,6 @@ __global__ __launch_bounds__(64 * WPT) void k_sad_fallback(VSGSearchArgs a, cons
    sends the other half at each of 5 steps: 31 exchanges), after which lane l
    holds candidate l / 2's wave total, and the scan's winner is the minimum
    of (sad, scan rank) over the lanes. */
-constexpr int kWindowSteps = 4;
 
 struct WindowLds {
   uint32_t part[2][4][32];   /* [iteration parity][wave][candidate] (4-wave jobs) */
@@ -559,7 +559,7 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 template <int WJ>
 __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float* __restrict__ pred, int npred,
                                                     int slot0, uint32_t maxSad, int* __restrict__ list,
-                                                    unsigned* __restrict__ stats) {
+                                                    unsigned* __restrict__ stats, int steps) {
   __shared__ WindowLds L;
   constexpr int JB = 4 / WJ;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -583,9 +583,9 @@ __global__ __launch_bounds__(256) void k_sad_window(VSGSearchArgs a, const float
     uint32_t minerr = 0;
     if (inRange) {
       /* a minimum on the window's edge moves the window there, up to
-         kWindowSteps windows in all */
+         `steps` windows in all */
       int px = (int)fx, py = (int)fy;
-      for (int it = 0; it < kWindowSteps; ++it) {
+      for (int it = 0; it < steps; ++it) {
         uint32_t tot = window_wave_sums<WJ>(a, jb, px - 2, py - 2, t);
         const int cand = lane >> 1;
         if (WJ > 1) {
@@ -832,16 +832,17 @@ hipError_t vsg_launch_queue(const VSGMultiSeg* dSegs, int nseg, const VSGQueueTo
 }
 
 hipError_t vsg_launch_window(const VSGSearchArgs& a, const float* pred, int npred, int slot0, uint32_t maxSad,
-                             int* list, unsigned* stats, hipStream_t st) {
+                             int* list, unsigned* stats, int steps, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
+  if (steps < 1) return hipErrorInvalidValue;
   if (a.g.S > 256 || a.g.S < 16 || (a.g.S & 3) || npred <= 0 || slot0 < 0 || a.g.bpp != 1)
     return hipErrorInvalidValue;
   if (a.g.S <= 64)
     hipLaunchKernelGGL(k_sad_window<1>, dim3((a.njobs + 3) / 4), dim3(256), 0, st, a, pred, npred, slot0, maxSad,
-                       list, stats);
+                       list, stats, steps);
   else
     hipLaunchKernelGGL(k_sad_window<4>, dim3(a.njobs), dim3(256), 0, st, a, pred, npred, slot0, maxSad, list,
-                       stats);
+                       stats, steps);
   return hipGetLastError();
 }
 
