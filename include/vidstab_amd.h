@@ -58,7 +58,7 @@ VS_API int vsMotionDetectProposalStageStats(const VSMotionDetect* md, int stage,
 
 /* The batched multi-clip form: the detect pass of nclips independent clips
    at once, frames already in device memory.  Clip c (config confs[c], frame
-   info fis[c]; virtualTripod must be 0) has nframes[c] luma planes
+   info fis[c], planar; virtualTripod as for vsMotionDetection) has nframes[c] luma planes
    lumaDev[c][0..nframes[c]-1] with linesize[c] bytes per row; motions[c][t]
    receives exactly what the t-th of nframes[c] vsMotionDetection calls on a
    fresh detector for that clip would return.  Up to VIDSTAB_AMD_MULTI_BATCH

@@ -211,9 +211,13 @@ VS_API VSGContext* vsg_multi_clip(VSGMulti* m, int clip);
 /* search launches per batch of a stage (1: the queue launch) */
 VS_API int vsg_multi_groups(VSGMulti* m, int stage);
 /* n[c] <= maxBatch frames of clip c (0: idle); lumaDev[c][i], linesize[c];
-   prevSlot[c][i] a slot of the clip (>= 0), detect[c][i] as for vsg_submit */
+   prevSlot[c][i] a slot of the clip or -1 (the clip's stored reference
+   plane), detect[c][i] as for vsg_submit; refSlot (NULL: none) names per
+   clip a slot of this batch whose blurred plane becomes the clip's stored
+   reference before any search of the batch, or -1 */
 VS_API int vsg_multi_submit(VSGMulti* m, int set, const int* n, const uint8_t* const* const* lumaDev,
-                            const int* linesize, const int* const* prevSlot, const uint8_t* const* detect);
+                            const int* linesize, const int* const* prevSlot, const uint8_t* const* detect,
+                            const int* refSlot);
 VS_API int vsg_multi_wait_coarse(VSGMulti* m, int set);
 VS_API int vsg_multi_submit_fine(VSGMulti* m, int set, const int* njobs, const VSGJob* const* jobs);
 VS_API int vsg_multi_wait_fine(VSGMulti* m, int set);
@@ -228,6 +232,13 @@ VS_API int vsg_get_plan(VSGContext* ctx, int stage, int* out, float* tuneMs);
    v_sad_u8 chains with a wave-uniform operand and no memory traffic;
    *absdiffPerSec = 4 abs-diffs per lane per instruction.  Host-synchronous. */
 VS_API int vsg_sad_peak(int device, double* absdiffPerSec, double* ms);
+
+/* Mean duration of the batch box blur (the detector's blur kernel, blur box
+   `blurSize`) over `reps` back-to-back launches on `n` device-resident planes
+   of width x height bytes, on an otherwise idle device, after one untimed
+   launch.  Host-synchronous; allocates its own output slots. */
+VS_API int vsg_blur_time(int device, const uint8_t* const* lumaDev, const int* linesize, int n, int width,
+                         int height, int blurSize, int reps, double* msPerLaunch);
 
 /* host versions of the device spiral helpers, for tests */
 VS_API int vsg_spiral_rank(int a, int b);

@@ -83,21 +83,40 @@ def test_kept_state_between_calls(monkeypatch):
     del keep
 
 
-def test_multi_clip_rejects_tripod():
-    """virtualTripod has no multi-clip form: an error, while the same call
-    without it (and no frames) succeeds"""
-    import ctypes
-    L = api.lib()
-    fi = (api.VSFrameInfo * 1)()
-    assert L.vsFrameInfoInit(fi, 640, 360, api.PF_YUV420P)
-    nf = (ctypes.c_int * 1)(0)
-    ls = (ctypes.c_int * 1)(640)
-    rows = (ctypes.c_void_p * 1)()
-    vecs = (api.VSVector * 1)()
-    outs = (ctypes.c_void_p * 1)(ctypes.cast(vecs, ctypes.c_void_p))
-    for tripod, want_ok in ((2, False), (0, True)):
-        conf = (api.VSMotionDetectConfig * 1)()
-        conf[0] = L.vsMotionDetectGetDefaultConfig(b"t")
-        conf[0].virtualTripod = tripod
-        rc = L.vsMotionDetectMulti(conf, fi, 1, nf, rows, ls, outs)
-        assert (rc == 0) == want_ok, f"tripod {tripod}: rc {rc}"
+@pytest.mark.parametrize("batch", [16, 3])
+def test_multi_clip_tripod_equals_per_clip(monkeypatch, batch):
+    """virtualTripod in the multi-clip form: per clip, the lead-in frames
+    emit the empty vector, the reference frame's plane is frozen in the batch
+    that holds it, and every later frame is measured against it -- equal to a
+    per-clip tripod stream.  Reference frames at the first frame, inside a
+    batch, on a batch boundary (batch 3: frame 2, frame 5), past the clip's
+    end, and a clip without tripod in the same call"""
+    monkeypatch.setenv("VIDSTAB_AMD_MULTI_BATCH", str(batch))
+    tripods = [3, 1, 6, 0, 2, 9, 4]
+    _say(f"[gpu] multi-clip tripod {tripods}, batch {batch}")
+    clips, keep = _device_clips(CLIPS)
+    for cl, tp in zip(clips, tripods):
+        cl["tripod"] = tp
+    got = api.detect_multi(clips)
+    for c, cl in enumerate(clips):
+        det = api.Detector(cl["width"], cl["height"], shakiness=cl["shakiness"], accuracy=cl["accuracy"],
+                           stepsize=cl["stepsize"], mincontrast=cl["mincontrast"], tripod=cl["tripod"])
+        want = det.detect_device(cl["ptrs"], [cl["linesize"]] * len(cl["ptrs"]), keep=True)
+        det.close()
+        assert len(got[c]) == len(want)
+        for t in range(len(want)):
+            assert got[c][t] == want[t], f"clip {c} (tripod {cl['tripod']}) frame {t}"
+            if t < max(cl["tripod"], 1):
+                assert got[c][t] == [], f"clip {c} lead-in frame {t}"
+    # clip 0 (11 frames, tripod 3) measures frames 3.. against frame 2
+    assert any(len(m) > 0 for m in got[0][3:])
+    # the same clips without tripod next (kept device state): the chains again
+    for cl in clips:
+        cl["tripod"] = 0
+    plain = api.detect_multi(clips)
+    for c, cl in enumerate(clips):
+        det = api.Detector(cl["width"], cl["height"], shakiness=cl["shakiness"], accuracy=cl["accuracy"],
+                           stepsize=cl["stepsize"], mincontrast=cl["mincontrast"])
+        assert plain[c] == det.detect_device(cl["ptrs"], [cl["linesize"]] * len(cl["ptrs"]), keep=True)
+        det.close()
+    del keep

@@ -14,6 +14,10 @@
  * capacity with field = -1 -- so each slot set's front + coarse stage and its
  * fine stage are captured once as HIP graphs and replayed for every batch
  * that uses the set.  Only pinned host descriptors change between replays.
+ * Virtual-tripod clips keep their reference frame in the context's stored
+ * reference plane: each clip's descriptor block carries the slot to freeze
+ * this batch (-1: none), and one k_copy_refs launch between the fronts and
+ * the coarse queue copies the planes of the clips that name one.
  * What crosses PCIe for all clips of a batch sits in four per-set arenas, so
  * each crossing is one copy node: the frame descriptors, the selection and
  * coarse results, the fine jobs and the fine results.  The contexts' buffers
@@ -48,6 +52,32 @@ struct Arena {
 
 enum { AR_DESC, AR_COARSE, AR_JOBS, AR_RESF, AR_COUNT };
 
+/* one clip's tripod copy: blurred slot *refSlot -> ref, when >= 0 */
+struct RefCopy {
+  const uint8_t* blurred;
+  uint8_t* ref;
+  const int* refSlot;   /* in the set's descriptor arena */
+  unsigned long long slotStride;
+};
+
+constexpr int kRefBlocks = 32;   /* workgroups per clip */
+
+/* grid (kRefBlocks, clips) x 256: 16-byte copies of whole slots (slot
+   strides are multiples of 64 bytes); clips with no slot to freeze exit */
+__global__ __launch_bounds__(256) void k_copy_refs(const RefCopy* __restrict__ tab) {
+  const RefCopy t = tab[blockIdx.y];
+  const int slot = *t.refSlot;
+  if (slot < 0) return;
+  const uint4* src = reinterpret_cast<const uint4*>(t.blurred + (size_t)slot * t.slotStride);
+  uint4* dst = reinterpret_cast<uint4*>(t.ref);
+  const size_t n = t.slotStride / 16;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kRefBlocks * 256) dst[i] = src[i];
+}
+
+/* descriptor block of a clip: src[B], prev[B], detect[B], then the slot to
+   freeze as the tripod reference at the next 4-byte boundary */
+size_t desc_ref_offset(int B) { return ((sizeof(VSGBlurSrc) + sizeof(int) + 1) * (size_t)B + 3) & ~(size_t)3; }
+
 }  // namespace
 
 struct VSGMulti {
@@ -61,6 +91,8 @@ struct VSGMulti {
   unsigned* d_queue = nullptr;              /* [stage][set][xcd][2] counters */
   Arena ar[VSG_NSETS][AR_COUNT];
   std::vector<int> jobsFilled[VSG_NSETS];   /* per clip: fine entries that may hold jobs */
+  std::vector<int*> h_refSlot[VSG_NSETS];   /* per clip: pinned slot to freeze as the tripod plane */
+  RefCopy* d_refTab[VSG_NSETS] = {};
   hipGraphExec_t gA[VSG_NSETS] = {};
   hipGraphExec_t gF[VSG_NSETS] = {};
   hipEvent_t doneA[VSG_NSETS] = {};
@@ -78,6 +110,7 @@ static void multi_free(VSGMulti* m) {
     if (m->doneF[s]) (void)hipEventDestroy(m->doneF[s]);
     for (int k = 0; k < 2; ++k)
       if (m->d_segs[k][s]) (void)hipFree(m->d_segs[k][s]);
+    if (m->d_refTab[s]) (void)hipFree(m->d_refTab[s]);
     for (Arena& a : m->ar[s]) {
       if (a.d) (void)hipFree(a.d);
       if (a.h) (void)hipHostFree(a.h);
@@ -98,7 +131,7 @@ static size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 /* per clip and set: the bytes each arena holds for it */
 static void clip_arena_bytes(const VSGMulti* m, const VSGContext* x, size_t (&b)[AR_COUNT]) {
   const size_t B = (size_t)m->B, nft = (size_t)x->nfTotal, nc = (size_t)x->st[0].nf, nf = (size_t)x->st[1].nf;
-  b[AR_DESC] = up256((sizeof(VSGBlurSrc) + sizeof(int) + 1) * B);
+  b[AR_DESC] = up256(desc_ref_offset(m->B) + sizeof(int));
   b[AR_COARSE] = up256(sizeof(double) * B * nft + sizeof(VSGResult) * B * nc + sizeof(int) * 2 * B +
                        sizeof(int16_t) * B * nft);
   b[AR_JOBS] = up256(sizeof(VSGJob) * B * nf);
@@ -128,6 +161,7 @@ static int bind_arenas(VSGMulti* m) {
       memset(a.h, 0, a.bytes);
       MCHECK(hipMemset(a.d, 0, a.bytes));
     }
+  std::vector<RefCopy> tab[VSG_NSETS];
   for (size_t c = 0; c < m->ctx.size(); ++c) {
     VSGContext* x = m->ctx[c];
     const size_t nft = (size_t)x->nfTotal, nc = (size_t)x->st[0].nf;
@@ -146,6 +180,11 @@ static int bind_arenas(VSGMulti* m) {
       s.h_prev = reinterpret_cast<int*>(hd + sizeof(VSGBlurSrc) * B);
       s.d_detect = dd + (sizeof(VSGBlurSrc) + sizeof(int)) * B;
       s.h_detect = hd + (sizeof(VSGBlurSrc) + sizeof(int)) * B;
+      int* hr = reinterpret_cast<int*>(hd + desc_ref_offset(B));
+      *hr = -1;
+      m->h_refSlot[set].push_back(hr);
+      tab[set].push_back(RefCopy{x->d_blur, x->d_ref, reinterpret_cast<const int*>(dd + desc_ref_offset(B)),
+                                 (unsigned long long)x->slotStride});
       /* coarse out: selC, resC, count, selIdx (alignment order) */
       uint8_t* dc = m->ar[set][AR_COARSE].d + off[c][AR_COARSE];
       uint8_t* hc = m->ar[set][AR_COARSE].h + off[c][AR_COARSE];
@@ -166,6 +205,10 @@ static int bind_arenas(VSGMulti* m) {
       s.d_resF = reinterpret_cast<VSGResult*>(m->ar[set][AR_RESF].d + off[c][AR_RESF]);
       s.h_resF = reinterpret_cast<VSGResult*>(m->ar[set][AR_RESF].h + off[c][AR_RESF]);
     }
+  }
+  for (int set = 0; set < VSG_NSETS; ++set) {
+    MCHECK(hipMalloc(&m->d_refTab[set], sizeof(RefCopy) * tab[set].size()));
+    MCHECK(hipMemcpy(m->d_refTab[set], tab[set].data(), sizeof(RefCopy) * tab[set].size(), hipMemcpyHostToDevice));
   }
   return 0;
 }
@@ -232,7 +275,8 @@ static int launch_queue(VSGMulti* m, int stage, int set) {
   return 0;
 }
 
-/* descriptors upload + front of every clip + coarse queue + one read-back */
+/* descriptors upload + front of every clip + tripod planes + coarse queue +
+   one read-back */
 static int capture_front(VSGMulti* m, int set) {
   if (build_segments(m, 0, set) != 0) return -1;
   hipGraph_t g = nullptr;
@@ -245,6 +289,12 @@ static int capture_front(VSGMulti* m, int set) {
     if (rc != 0) break;
     x->set[set].prof = false;
     rc = vsg_ctx_enqueue_front(x, set, m->B, false);
+  }
+  if (rc == 0) {
+    /* after every clip's blur, before any search reads a tripod plane */
+    hipLaunchKernelGGL(k_copy_refs, dim3(kRefBlocks, (unsigned)m->ctx.size()), dim3(256), 0, m->stream,
+                       m->d_refTab[set]);
+    if (hipGetLastError() != hipSuccess) rc = vsg_fail("tripod copy capture", hipErrorUnknown);
   }
   if (rc == 0) rc = launch_queue(m, 0, set);
   const Arena& ac = m->ar[set][AR_COARSE];
@@ -354,7 +404,8 @@ extern "C" int vsg_multi_groups(VSGMulti* m, int stage) {
 }
 
 extern "C" int vsg_multi_submit(VSGMulti* m, int set, const int* n, const uint8_t* const* const* lumaDev,
-                                const int* linesize, const int* const* prevSlot, const uint8_t* const* detect) {
+                                const int* linesize, const int* const* prevSlot, const uint8_t* const* detect,
+                                const int* refSlot) {
   if (!m || set < 0 || set >= VSG_NSETS || !n || !lumaDev || !linesize || !prevSlot || !detect)
     return vsg_fail("vsg_multi_submit", hipSuccess);
   MCHECK(hipSetDevice(m->device));
@@ -368,9 +419,13 @@ extern "C" int vsg_multi_submit(VSGMulti* m, int set, const int* n, const uint8_
     VSGSet& s = x->set[set];
     if (n[c] < 0 || n[c] > m->B || (n[c] > 0 && linesize[c] < x->W))
       return vsg_fail("vsg_multi_submit: frame count or linesize", hipSuccess);
+    const int ref = refSlot ? refSlot[c] : -1;
+    if (ref >= 0 && (ref < set * m->B || ref >= set * m->B + n[c]))
+      return vsg_fail("vsg_multi_submit: refSlot outside the clip's batch", hipSuccess);
+    *m->h_refSlot[set][c] = ref < 0 ? -1 : ref;
     for (int i = 0; i < m->B; ++i) {
       if (i < n[c]) {
-        if (prevSlot[c][i] < 0 || prevSlot[c][i] >= VSG_NSETS * m->B)
+        if (prevSlot[c][i] < -1 || prevSlot[c][i] >= VSG_NSETS * m->B)
           return vsg_fail("vsg_multi_submit: prevSlot", hipSuccess);
         s.h_src[i].p = lumaDev[c][i];
         s.h_src[i].pitch = linesize[c];
@@ -410,7 +465,7 @@ extern "C" int vsg_multi_submit_fine(VSGMulti* m, int set, const int* njobs, con
     if (njobs[c] < 0 || njobs[c] > cap) return vsg_fail("vsg_multi_submit_fine: too many jobs", hipSuccess);
     for (int i = 0; i < njobs[c]; ++i) {
       const VSGJob& j = jobs[c][i];
-      if (j.slot < 0 || j.slot >= VSG_NSETS * m->B || j.prev < 0 || j.prev >= VSG_NSETS * m->B ||
+      if (j.slot < 0 || j.slot >= VSG_NSETS * m->B || j.prev < -1 || j.prev >= VSG_NSETS * m->B ||
           j.field >= x->st[1].nf)
         return vsg_fail("vsg_multi_submit_fine: job out of range", hipSuccess);
     }

@@ -4,7 +4,8 @@
  *
  * Each clip is an independent detect stream with its own VSMotionDetect
  * state (configured exactly as vsMotionDetectInit does, src/motiondetect.c:
- * 83-165) and its own previous-frame chain.  A batch takes up to B frames of
+ * 83-165) and its own previous-frame chain, or, with virtualTripod >= 1, its
+ * own frozen reference frame.  A batch takes up to B frames of
  * every clip; batches rotate through the VSG_NSETS slot sets in the same
  * three-in-flight order as the single-clip driver (vs_motiondetect.c).  The
  * per-frame host steps (coarse LocalMotions, vsSimpleMotionsToTransform,
@@ -71,6 +72,7 @@ typedef struct {
   const uint8_t* const** rows;
   const int** prevs;
   const uint8_t** dets;
+  int* refs;
   const VSGJob** jobs;
 } Multi;
 
@@ -127,12 +129,20 @@ static int batch_submit(Multi* M, MBatch* b, int set) {
     cb->n = n;
     cb->fnum0 = cl->next;
     cb->slot0 = set * M->B;
+    const int tripod = cl->md.conf.virtualTripod;
+    M->refs[c] = -1;
     for (int i = 0; i < n; i++) {
       const int fnum = cl->next + i;
-      /* every frame after a clip's first is measured against the frame before
+      /* every frame after a clip's first is measured against the frame
+         before; with virtualTripod >= 1 the frames before the reference
+         frame (index virtualTripod-1) are a lead-in that emits no motion,
+         the reference frame's blurred plane is frozen as the clip's stored
+         reference, and every later frame is measured against it
          (src/motiondetect.c:211-218, 294-303) */
-      cb->detect[i] = (uint8_t)(fnum > 0);
-      cb->prevSlot[i] = i == 0 ? (cl->lastSlot >= 0 ? cl->lastSlot : cb->slot0) : cb->slot0 + i - 1;
+      cb->detect[i] = (uint8_t)(tripod >= 1 ? fnum >= tripod : fnum > 0);
+      cb->prevSlot[i] =
+          tripod >= 1 ? -1 : (i == 0 ? (cl->lastSlot >= 0 ? cl->lastSlot : cb->slot0) : cb->slot0 + i - 1);
+      if (tripod >= 1 && fnum == tripod - 1) M->refs[c] = cb->slot0 + i;
       cb->luma[i] = M->luma[c][fnum];
       b->itemClip[b->nitems] = c;
       b->itemFrame[b->nitems] = i;
@@ -147,7 +157,7 @@ static int batch_submit(Multi* M, MBatch* b, int set) {
     M->prevs[c] = cb->prevSlot;
     M->dets[c] = cb->detect;
   }
-  if (vsg_multi_submit(M->gpu, set, M->n, M->rows, M->linesize, M->prevs, M->dets) != 0) {
+  if (vsg_multi_submit(M->gpu, set, M->n, M->rows, M->linesize, M->prevs, M->dets, M->refs) != 0) {
     vs_log_error("vidstab_amd", "GPU multi-clip detect failed: %s\n", vsg_last_error());
     return VS_ERROR;
   }
@@ -360,6 +370,7 @@ static void multi_free(Multi* M, MBatch* bs) {
   free((void*)M->rows);
   free((void*)M->prevs);
   free((void*)M->dets);
+  free(M->refs);
   free((void*)M->jobs);
 }
 
@@ -380,6 +391,7 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
   M.rows = (const uint8_t* const**)calloc((size_t)nclips, sizeof(void*));
   M.prevs = (const int**)calloc((size_t)nclips, sizeof(void*));
   M.dets = (const uint8_t**)calloc((size_t)nclips, sizeof(void*));
+  M.refs = (int*)calloc((size_t)nclips, sizeof(int));
   M.jobs = (const VSGJob**)calloc((size_t)nclips, sizeof(void*));
   MBatch* bs = (MBatch*)calloc(MULTI_INFLIGHT, sizeof(MBatch));
   int* w = (int*)calloc((size_t)nclips, sizeof(int));
@@ -387,15 +399,10 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
   int* blur = (int*)calloc((size_t)nclips, sizeof(int));
   VSGStageDesc* dc = (VSGStageDesc*)calloc((size_t)nclips, sizeof(VSGStageDesc));
   VSGStageDesc* df = (VSGStageDesc*)calloc((size_t)nclips, sizeof(VSGStageDesc));
-  int ok = M.clips && M.n && M.rows && M.prevs && M.dets && M.jobs && bs && w && h && blur && dc && df;
+  int ok = M.clips && M.n && M.rows && M.prevs && M.dets && M.refs && M.jobs && bs && w && h && blur && dc && df;
   int maxFrames = 0;
   for (int c = 0; c < nclips && ok; c++) {
     MClip* cl = &M.clips[c];
-    if (confs[c].virtualTripod > 0) {
-      vs_log_error(confs[c].modName, "the multi-clip form does not take virtualTripod\n");
-      ok = 0;
-      break;
-    }
     if (nframes[c] < 0 || vsd_configure(&cl->md, &confs[c], &fis[c]) != VS_OK) {
       ok = 0;
       break;

@@ -274,7 +274,8 @@ def detect_multi(clips, keep=True, mod=b"vidstab_amd"):
 
     clips: sequence of dicts with width, height, shakiness, accuracy,
     stepsize, mincontrast, ptrs (device addresses of the luma planes, one per
-    frame) and linesize.  Returns per clip the per-frame LocalMotions
+    frame), linesize and optionally tripod (virtualTripod, default 0).
+    Returns per clip the per-frame LocalMotions
     (tuples, as motions_of) or, with keep=False, the per-frame counts."""
     L = lib()
     n = len(clips)
@@ -291,6 +292,7 @@ def detect_multi(clips, keep=True, mod=b"vidstab_amd"):
         conf = L.vsMotionDetectGetDefaultConfig(mod)
         conf.shakiness, conf.accuracy, conf.stepSize = cl["shakiness"], cl["accuracy"], cl["stepsize"]
         conf.contrastThreshold = float(cl["mincontrast"])
+        conf.virtualTripod = int(cl.get("tripod", 0))
         confs[c] = conf
         nf = len(cl["ptrs"])
         nframes[c] = nf
