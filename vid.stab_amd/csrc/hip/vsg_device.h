@@ -1,6 +1,8 @@
 /*
- * vsg_device.h -- device-side building blocks shared by the per-job kernels
- * (vsg_kernels.hip) and the persistent multi-clip kernels (vsg_multi.hip).
+ * vsg_device.h -- device-side building blocks shared by the blur, field and
+ * selection kernels (vsg_kernels.hip) and the SAD searches (vsg_search.hip:
+ * the per-job searches and the persistent queue kernel of the multi-clip
+ * form).
  */
 #ifndef VSG_DEVICE_H
 #define VSG_DEVICE_H
