@@ -52,7 +52,9 @@ struct VSGSet {
   uint8_t* h_detect = nullptr;
   uint8_t* h_stage = nullptr;   /* pinned staging of host frames (B x pitch x H), allocated on first use */
   /* state */
-  hipStream_t stream = nullptr;   /* all work of the set runs in order here */
+  hipStream_t stream = nullptr;   /* all device work of the set runs in order here (by default one
+                                     stream shared by the sets: one batch stage at a time) */
+  hipEvent_t uploaded = nullptr;  /* the batch's host frames are on the device (upload stream) */
   int n = 0;              /* frames in flight */
   int njobsF = 0;
   int follower = -1;      /* set submitted right after this set's latest batch */
@@ -70,6 +72,7 @@ struct VSGLearned;   /* vsg_learned.hip */
 struct VSGContext {
   int device = 0;
   bool ownStreams = true;
+  hipStream_t upStream = nullptr;   /* host-frame uploads of large batches, beside the compute stream */
   int lastSet = -1;       /* set of the latest batch */
   int W = 0, H = 0, pitch = 0, rowsAlloc = 0;
   int bpp = 1;            /* bytes per pixel: 1 planar luma, 3 or 4 packed */
