@@ -95,6 +95,9 @@ void vsg_ctx_release_transfer(VSGContext* c, int set) {
   hfree(s.h_desc);
   s.d_src = nullptr; s.d_prev = nullptr; s.d_detect = nullptr;
   s.h_src = nullptr; s.h_prev = nullptr; s.h_detect = nullptr;
+  s.m_selIdx = nullptr; s.m_selC = nullptr; s.m_count = nullptr; s.m_resC = nullptr;
+  s.m_jobsF = nullptr; s.m_resF = nullptr; s.m_desc = nullptr;
+  s.zc = false;
 }
 
 void vsg_ctx_free(VSGContext* c) {
@@ -167,6 +170,24 @@ static hipError_t dalloc(T** p, size_t n) {
 template <class T>
 static hipError_t halloc(T** p, size_t n) {
   return hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1), hipHostMallocDefault);
+}
+/* the device address of pinned host memory */
+template <class T>
+static hipError_t hmap(T** dev, T* host) {
+  return hipHostGetDevicePointer(reinterpret_cast<void**>(dev), host, 0);
+}
+
+/* small batches (fewer than kUploadAsideMin frames: a single frame per
+   vsMotionDetection call) skip the copies across PCIe: their host frames go
+   up by a kernel, their descriptors and fine jobs are read and their
+   selection and search results written by the kernels in pinned memory.
+   VIDSTAB_AMD_ZERO_COPY=0 keeps the copies. */
+static bool zero_copy_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("VIDSTAB_AMD_ZERO_COPY");
+    return !(e && atoi(e) == 0);
+  }();
+  return on;
 }
 
 int vsg_ctx_init(VSGContext* c, int device, int width, int height, int bpp, int blurSize, const VSGStageDesc* coarse,
@@ -312,6 +333,13 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int bpp, int 
     s.h_src = reinterpret_cast<VSGBlurSrc*>(s.h_desc);
     s.h_prev = reinterpret_cast<int*>(s.h_desc + sizeof(VSGBlurSrc) * B);
     s.h_detect = s.h_desc + (sizeof(VSGBlurSrc) + sizeof(int)) * B;
+    VSG_CHECK(hmap(&s.m_selIdx, s.h_selIdx));
+    VSG_CHECK(hmap(&s.m_selC, s.h_selC));
+    VSG_CHECK(hmap(&s.m_count, s.h_count));
+    VSG_CHECK(hmap(&s.m_resC, s.h_resC));
+    VSG_CHECK(hmap(&s.m_jobsF, s.h_jobsF));
+    VSG_CHECK(hmap(&s.m_resF, s.h_resF));
+    VSG_CHECK(hmap(&s.m_desc, s.h_desc));
     VSG_CHECK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     VSG_CHECK(hipEventCreateWithFlags(&s.doneF, hipEventDisableTiming));
     VSG_CHECK(hipEventCreateWithFlags(&s.planesReady, hipEventDisableTiming));
@@ -389,12 +417,16 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n, bool upload) {
   VSGSet& s = c->set[set];
   hipStream_t stream = s.stream;
   const int slot0 = set * c->B;
-  if (upload && vsg_ctx_upload_front(c, set, n) != 0) return -1;
+  /* zero-copy batch: the kernels read the descriptors in pinned memory */
+  const VSGBlurSrc* src = s.zc ? reinterpret_cast<const VSGBlurSrc*>(s.m_desc) : s.d_src;
+  const int* prevSlot = s.zc ? reinterpret_cast<const int*>(s.m_desc + sizeof(VSGBlurSrc) * c->B) : s.d_prev;
+  const uint8_t* detect = s.zc ? s.m_desc + (sizeof(VSGBlurSrc) + sizeof(int)) * c->B : s.d_detect;
+  if (upload && !s.zc && vsg_ctx_upload_front(c, set, n) != 0) return -1;
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR0], stream));
   /* packed frames are not blurred (src/motiondetect.c:197-201): their blurSize
      is below 2, which copies Wb = W*bpp bytes a row */
   uint8_t* tmp = c->d_tmp ? c->d_tmp + (size_t)slot0 * c->pitch * c->H : nullptr;
-  VSG_CHECK(vsg_launch_blur(s.d_src, n, c->d_blur + (size_t)slot0 * c->slotStride, c->slotStride, c->pitch,
+  VSG_CHECK(vsg_launch_blur(src, n, c->d_blur + (size_t)slot0 * c->slotStride, c->slotStride, c->pitch,
                             c->Wb, c->H, c->blurSize, tmp, stream));
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_BLUR1], stream));
   VSGFieldArgs fa;
@@ -409,7 +441,7 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n, bool upload) {
   fa.patches = c->d_patches + (size_t)slot0 * c->patchSlotStride;
   fa.patchSlotStride = c->patchSlotStride;
   fa.minmax = c->d_minmax + (size_t)slot0 * c->nfTotal * c->nmm;
-  fa.src = s.d_src;
+  fa.src = src;
   fa.bpp = c->bpp;
   fa.nmm = c->nmm;
   VSG_CHECK(vsg_launch_fields(fa, n, stream));
@@ -427,11 +459,11 @@ int vsg_ctx_enqueue_front(VSGContext* c, int set, int n, bool upload) {
     sa.st[k].ct = c->st[k].ct;
   }
   sa.slot0 = slot0;
-  sa.prevSlot = s.d_prev;
-  sa.detect = s.d_detect;
-  sa.selIdx = s.d_selIdx;
-  sa.selC = s.d_selC;
-  sa.count = s.d_count;
+  sa.prevSlot = prevSlot;
+  sa.detect = detect;
+  sa.selIdx = s.zc ? s.m_selIdx : s.d_selIdx;
+  sa.selC = s.zc ? s.m_selC : s.d_selC;
+  sa.count = s.zc ? s.m_count : s.d_count;
   sa.jobs = s.d_jobsC;
   VSG_CHECK(vsg_launch_select(sa, n, stream));
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_SELECT1], stream));
@@ -461,7 +493,10 @@ extern "C" int vsg_stage_frames(VSGContext* c, int set, int n, uint8_t** stage, 
   VSG_CHECK(hipEventSynchronize(s.done));
   VSG_CHECK(hipEventSynchronize(s.doneF));
   const size_t plane = (size_t)c->pitch * c->H;
-  if (!s.h_stage) VSG_CHECK(halloc(&s.h_stage, plane * c->B));
+  if (!s.h_stage) {
+    VSG_CHECK(halloc(&s.h_stage, plane * c->B));
+    VSG_CHECK(hmap(&s.m_stage, s.h_stage));
+  }
   for (int i = 0; i < n; ++i) stage[i] = s.h_stage + plane * i;
   *pitch = c->pitch;
   return 0;
@@ -492,6 +527,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   /* host frames of a large batch go up on a stream of their own, beside the
      compute stream's earlier batches (the set's raw planes are free: its
      previous batch completed above); a few frames go in order */
+  s.zc = zero_copy_enabled() && n < kUploadAsideMin && !c->learned && !s.borrowed;
   hipStream_t up = stream;
   if (onDevice != VSG_FRAMES_DEVICE && n >= kUploadAsideMin) {
     if (!c->upStream) VSG_CHECK(hipStreamCreateWithFlags(&c->upStream, hipStreamNonBlocking));
@@ -506,14 +542,17 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
       const size_t plane = (size_t)c->pitch * c->H;
       if (!s.h_stage) return vsg_fail("vsg_submit: frames not staged", hipSuccess);
       uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * plane;
-      VSG_CHECK(hipMemcpyAsync(dst, s.h_stage + plane * i, plane, hipMemcpyHostToDevice, up));
+      if (!s.zc) VSG_CHECK(hipMemcpyAsync(dst, s.h_stage + plane * i, plane, hipMemcpyHostToDevice, up));
       s.h_src[i].p = dst;
       s.h_src[i].pitch = c->pitch;
     } else {
       /* pageable host frame: rows into the set's pinned staging area (the
          set's previous batch is complete), then one DMA of the plane */
       const size_t plane = (size_t)c->pitch * c->H;
-      if (!s.h_stage) VSG_CHECK(halloc(&s.h_stage, plane * c->B));
+      if (!s.h_stage) {
+        VSG_CHECK(halloc(&s.h_stage, plane * c->B));
+        VSG_CHECK(hmap(&s.m_stage, s.h_stage));
+      }
       uint8_t* stg = s.h_stage + plane * i;
       if (linesize[i] == c->pitch) {
         memcpy(stg, luma[i], plane - (c->pitch - c->Wb));
@@ -521,7 +560,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
         for (int y = 0; y < c->H; ++y) memcpy(stg + (size_t)y * c->pitch, luma[i] + (size_t)y * linesize[i], c->Wb);
       }
       uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * plane;
-      VSG_CHECK(hipMemcpyAsync(dst, stg, plane, hipMemcpyHostToDevice, up));
+      if (!s.zc) VSG_CHECK(hipMemcpyAsync(dst, stg, plane, hipMemcpyHostToDevice, up));
       s.h_src[i].p = dst;
       s.h_src[i].pitch = c->pitch;
     }
@@ -532,6 +571,11 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   if (up != stream) {
     VSG_CHECK(hipEventRecord(s.uploaded, up));
     VSG_CHECK(hipStreamWaitEvent(stream, s.uploaded, 0));
+  }
+  if (s.zc && onDevice != VSG_FRAMES_DEVICE) {
+    /* the staged planes are consecutive, as are the batch's raw slots */
+    const size_t plane = (size_t)c->pitch * c->H;
+    VSG_CHECK(vsg_launch_upload(s.m_stage, c->d_raw + (size_t)slot0 * plane, plane * n, stream));
   }
   s.prof = g_profOn.load() != 0;
   /* learned form: the quarter-resolution planes first; then, with no
@@ -569,12 +613,12 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   if (c->learned) {
     if (vsg_learned_coarse(c, set, n, stream) != 0) return -1;
   } else {
-    VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.d_resC);
+    VSGSearchArgs ga = vsg_ctx_search_args(c, 0, s.d_jobsC, n * nc, s.d_best, s.d_done, s.zc ? s.m_resC : s.d_resC);
     VSG_CHECK(vsg_launch_search_any(ga, stream));
   }
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID1], stream));
   VSG_CHECK(hipEventRecord(s.searchDone, stream));
-  if (vsg_ctx_enqueue_coarse_readback(c, set, n) != 0) return -1;
+  if (!s.zc && vsg_ctx_enqueue_coarse_readback(c, set, n) != 0) return -1;
   VSG_CHECK(hipEventRecord(s.done, stream));
   s.n = n;
   return 0;
@@ -626,18 +670,20 @@ extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* 
     return 0;
   }
   if (jobs != s.h_jobsF) memcpy(s.h_jobsF, jobs, sizeof(VSGJob) * njobs);
-  VSG_CHECK(hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, s.stream));
+  if (!s.zc) VSG_CHECK(hipMemcpyAsync(s.d_jobsF, s.h_jobsF, sizeof(VSGJob) * njobs, hipMemcpyHostToDevice, s.stream));
   s.profF = g_profOn.load() != 0;
   if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_0], s.stream));
   if (vsg_learned_fine_enabled(c)) {
     if (vsg_learned_fine(c, set, njobs, s.stream) != 0) return -1;
   } else {
-    VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.d_jobsF, njobs, s.d_best, s.d_done, s.d_resF);
+    VSGSearchArgs fa = vsg_ctx_search_args(c, 1, s.zc ? s.m_jobsF : s.d_jobsF, njobs, s.d_best, s.d_done,
+                                           s.zc ? s.m_resF : s.d_resF);
     VSG_CHECK(vsg_launch_search_any(fa, s.stream));
   }
   if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_GRID1], s.stream));
   VSG_CHECK(hipEventRecord(s.searchDone, s.stream));
-  VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, s.stream));
+  if (!s.zc)
+    VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, s.stream));
   VSG_CHECK(hipEventRecord(s.doneF, s.stream));
   return 0;
 }

@@ -51,6 +51,18 @@ struct VSGSet {
   int* h_prev = nullptr;
   uint8_t* h_detect = nullptr;
   uint8_t* h_stage = nullptr;   /* pinned staging of host frames (B x pitch x H), allocated on first use */
+  /* device addresses of the pinned buffers above: a small batch's kernels
+     read its descriptors and fine jobs and write its selection and results
+     there directly, with no copies (zc) */
+  int16_t* m_selIdx = nullptr;
+  double* m_selC = nullptr;
+  int* m_count = nullptr;
+  VSGResult* m_resC = nullptr;
+  VSGJob* m_jobsF = nullptr;
+  VSGResult* m_resF = nullptr;
+  uint8_t* m_desc = nullptr;
+  uint8_t* m_stage = nullptr;
+  bool zc = false;         /* the batch in flight crosses PCIe through the mapped buffers */
   /* state */
   hipStream_t stream = nullptr;   /* all device work of the set runs in order here (by default one
                                      stream shared by the sets: one batch stage at a time) */

@@ -47,6 +47,9 @@ struct VSGFieldArgs {
   int nmm;                    /* min/max bytes per field: 2 (planar) or 6 (packed) */
 };
 hipError_t vsg_launch_fields(const VSGFieldArgs& a, int nslots, hipStream_t st);
+/* bytes (a multiple of 16, 16-byte aligned ends) from src to dst by a kernel;
+   src may be pinned host memory (its device address) */
+hipError_t vsg_launch_upload(const uint8_t* src, uint8_t* dst, size_t bytes, hipStream_t st);
 
 /* one search stage over a job list */
 struct VSGSearchArgs {

@@ -174,6 +174,24 @@ __device__ void fields_packed(const VSGFieldArgs& a, int f, int slot, int lane) 
   }
 }
 
+/* host frames of a small batch: pinned staging -> raw planes, by the
+   compute queue itself.  A copy-engine transfer would hand the stream from
+   the DMA engine to the compute queue before the blur (tens of microseconds
+   per batch: most of a single-frame call's upload).  Each thread keeps four
+   16-byte loads in flight; n16 16-byte words, both ends 16-byte aligned. */
+__global__ __launch_bounds__(256) void k_upload(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  const size_t step = (size_t)gridDim.x * 1024;
+  for (size_t i = (size_t)blockIdx.x * 1024 + threadIdx.x; i < n16; i += step) {
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + 256 * k < n16) v[k] = src[i + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i + 256 * k < n16) dst[i + 256 * k] = v[k];
+  }
+}
+
 /* one wave per (field, slot) */
 __global__ __launch_bounds__(256) void k_fields(VSGFieldArgs a) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -405,6 +423,15 @@ hipError_t vsg_launch_blur(const VSGBlurSrc* d_src, int n, uint8_t* dst, size_t 
                      W, H, size, mul);
   hipLaunchKernelGGL(k_blur_cols, dim3((W + 255) / 256, H, n), dim3(256), 0, st, d_src, tmp, dstPitch, dst,
                      slotStride, dstPitch, W, H, size, mul);
+  return hipGetLastError();
+}
+
+hipError_t vsg_launch_upload(const uint8_t* src, uint8_t* dst, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return hipSuccess;
+  if ((bytes & 15) || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15)) return hipErrorInvalidValue;
+  const size_t n16 = bytes / 16, blocks = (n16 + 1023) / 1024;
+  hipLaunchKernelGGL(k_upload, dim3(blocks < 2048 ? (unsigned)blocks : 2048u), dim3(256), 0, st,
+                     reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), n16);
   return hipGetLastError();
 }
 

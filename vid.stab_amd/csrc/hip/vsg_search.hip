@@ -768,8 +768,21 @@ double vsg_plan_cost(const VSGGrid& g, int tiles) {
 
 hipError_t vsg_launch_search_any(const VSGSearchArgs& a, hipStream_t st) {
   /* a batch too small to give every CU two grid workgroups is latency-bound:
-     the split kernel spreads each job's rows over eight times the waves */
-  if (a.njobs > 0 && (long long)a.njobs * a.tiles < 2LL * kSplitCus) return launch_split<8>(a, nullptr, nullptr, st);
+     the split kernel spreads each job's rows over four or eight waves.
+     Eight halve each wave's rows, but (88 VGPRs, 30 KB of LDS) only two
+     such workgroups fit a CU: they are used while every unit finds a
+     slot at once (a single frame's coarse jobs), four otherwise (its fine
+     jobs).  VIDSTAB_AMD_SPLIT_WAVES=4 or 8 forces one. */
+  static const int forced = [] {
+    const char* e = getenv("VIDSTAB_AMD_SPLIT_WAVES");
+    const int v = e ? atoi(e) : 0;
+    return v == 4 || v == 8 ? v : 0;
+  }();
+  if (a.njobs > 0 && (long long)a.njobs * a.tiles < 2LL * kSplitCus) {
+    const long long units = (long long)a.njobs * ((a.g.U * ((a.g.G + 3) / 4) + 63) / 64);
+    const int wpt = forced ? forced : (units <= 2LL * kSplitCus ? 8 : 4);
+    return wpt == 4 ? launch_split<4>(a, nullptr, nullptr, st) : launch_split<8>(a, nullptr, nullptr, st);
+  }
   return vsg_launch_search(a, st);
 }
 

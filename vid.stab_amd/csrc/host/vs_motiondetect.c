@@ -63,13 +63,14 @@ typedef struct Batch {
 #define MD_INFLIGHT 3
 
 /* host phases of a detect call, timed when VIDSTAB_AMD_CALL_TIMES is set */
-enum { PH_SUBMIT, PH_WAIT_COARSE, PH_COARSE_HOST, PH_SUBMIT_FINE, PH_WAIT_FINE, PH_FINE_HOST, PH_CALL, PH_COUNT };
+enum { PH_SUBMIT, PH_STAGE, PH_WAIT_COARSE, PH_COARSE_HOST, PH_SUBMIT_FINE, PH_WAIT_FINE, PH_FINE_HOST, PH_CALL, PH_COUNT };
 
 typedef struct MDState {
   const VSMotionDetect* key;
   struct MDState* next;
   VSGContext* gpu;
   VSPool* pool;    /* per-frame host steps of a batch */
+  VSPool* stagePool;   /* copies of a few host frames into pinned staging (at most 4 threads) */
   int maxBatch;
   int nextSet;     /* slot set of the next batch */
   int lastSlot;    /* global slot holding the previous frame's blurred plane */
@@ -102,14 +103,16 @@ static MDState* state_find(const VSMotionDetect* md) {
 static void state_free(MDState* s) {
   if (!s) return;
   if (s->timed && s->calls > 0) {
-    static const char* names[PH_COUNT] = {"submit (staging, enqueue)", "wait for coarse results", "coarse host step",
-                                          "submit fine", "wait for fine results", "fine host step", "whole call"};
+    static const char* names[PH_COUNT] = {"submit (staging, enqueue)", "  of which host-frame staging",
+                                          "wait for coarse results", "coarse host step", "submit fine",
+                                          "wait for fine results", "fine host step", "whole call"};
     for (int k = 0; k < PH_COUNT; k++)
       vs_log_info("vidstab_amd", "call time: %-26s %9.1f us per call (%lld calls)\n", names[k],
                   s->phaseUs[k] / (double)s->calls, s->calls);
   }
   if (s->gpu) vsg_destroy(s->gpu);
   vs_pool_destroy(s->pool);
+  vs_pool_destroy(s->stagePool);
   free(s->bs);
   free(s->jobs);
   free(s->jobSel);
@@ -328,11 +331,12 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
   }
   nth = VS_MIN(16, VS_MAX(1, env_int("VIDSTAB_AMD_HOST_THREADS", nth)));
   s->pool = vs_pool_create(nth);
+  s->stagePool = vs_pool_create(VS_MIN(4, nth));
   const size_t nfj = (size_t)s->maxBatch * (size_t)md->fieldsfine.fieldNum;
   s->bs = (Batch*)calloc(MD_INFLIGHT, sizeof(Batch));
   s->jobs = (VSGJob*)malloc(sizeof(VSGJob) * nfj * MD_INFLIGHT);
   s->jobSel = (int*)malloc(sizeof(int) * nfj * MD_INFLIGHT);
-  if (!s->pool || !s->bs || !s->jobs || !s->jobSel) {
+  if (!s->pool || !s->stagePool || !s->bs || !s->jobs || !s->jobSel) {
     vs_log_error(md->conf.modName, "out of host memory\n");
     state_free(s);
     free_fields(md);
@@ -504,10 +508,17 @@ static int stage_host_frames(VSMotionDetect* md, MDState* s, int set, int n, con
   if (vsg_stage_frames(s->gpu, set, n, stage, &pitch) != 0) return VS_ERROR;
   const int bpp = md->fi.pFormat > PF_PACKED ? md->fi.bytesPerPixel : 1;
   StageWork w = {luma, linesize, stage, pitch, md->fi.width * bpp, md->fi.height, 1};
-  const int threads = vs_pool_threads(s->pool);
-  if (n < threads) w.bands = VS_MIN(16, (threads + n - 1) / n);   /* a few frames: split their rows */
+  /* a few frames: their rows split over a pool of its own, of at most four
+     threads.  The copy is bound by host memory bandwidth, not by threads:
+     one 1080p plane took 51 us on such a pool, 76-85 us on the 12 threads of
+     the batch pool (threads that only wait slow down the ones that copy) */
+  VSPool* pool = s->pool;
+  if (n < 4) {
+    pool = s->stagePool;
+    w.bands = VS_MAX(1, vs_pool_threads(pool) / n);
+  }
   if (w.bands > w.rows) w.bands = w.rows;
-  vs_pool_run(s->pool, n * w.bands, stage_fn, &w);
+  vs_pool_run(pool, n * w.bands, stage_fn, &w);
   return VS_OK;
 }
 
@@ -536,7 +547,9 @@ static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t*
     if (tripod >= 1 && fnum == tripod - 1) refSlot = b->slot0 + i;
   }
   int mode = onDevice ? VSG_FRAMES_DEVICE : VSG_FRAMES_STAGED;
+  const double t0 = s->timed ? now_us() : 0;
   if (!onDevice && stage_host_frames(md, s, b->set, n, luma, linesize) != VS_OK) mode = -1;
+  if (s->timed) s->phaseUs[PH_STAGE] += now_us() - t0;
   if (mode < 0 || vsg_submit(s->gpu, b->set, n, luma, linesize, mode, b->prevSlot, b->detect, refSlot) != 0) {
     vs_log_error(md->conf.modName, "GPU detect failed: %s\n", vsg_last_error());
     return VS_ERROR;
