@@ -152,8 +152,9 @@ VSGSearchArgs vsg_queue_args(const VSGSearchArgs& a, int wg, int K);
 /* one persistent launch over all units of all segments: dSegs holds
    totals.queues tables of nseg segments, each sorted by unitBase; queues:
    two device counters per queue, zero before the launch and left zero
-   after it */
+   after it.  split: the segments' units are 64-lane tiles whose prev rows
+   the workgroup's waves share (K must be 4; a.tiles counts 64-lane tiles) */
 hipError_t vsg_launch_queue(const VSGMultiSeg* dSegs, int nseg, const VSGQueueTotals& totals, int wg, int K,
-                            unsigned* queues, hipStream_t st);
+                            bool split, unsigned* queues, hipStream_t st);
 
 #endif /* VSG_INTERNAL_H */

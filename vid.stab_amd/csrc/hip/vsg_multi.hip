@@ -6,7 +6,9 @@
  * planes, fields and selection lists); all contexts share one stream.  The
  * searches of a stage run as ONE persistent launch of k_sad_queue
  * (vsg_search.hip) over a single flattened queue of the units of every clip,
- * whatever their plan shapes.
+ * whatever their plan shapes -- plus, when some clips' jobs are too small to
+ * fill half a workgroup, a second persistent launch over the split units of
+ * those clips.
  *
  * Every batch has a fixed shape -- B frame slots per clip, idle slots marked
  * by a null source (skipped by the blur and field kernels, detect = 0 for the
@@ -86,9 +88,11 @@ struct VSGMulti {
   hipStream_t stream = nullptr;
   std::vector<VSGContext*> ctx;
   std::vector<int> order[2];                /* clips by descending field size, per stage */
-  VSGMultiSeg* d_segs[2][VSG_NSETS] = {};
-  VSGQueueTotals totals[2] = {};            /* units per XCD queue, per stage */
-  unsigned* d_queue = nullptr;              /* [stage][set][xcd][2] counters */
+  /* per kind of unit (0: workgroup-wide tiles, 1: split 64-lane tiles) */
+  VSGMultiSeg* d_segs[2][2][VSG_NSETS] = {};   /* [kind][stage][set] */
+  int nseg[2][2] = {};                          /* [kind][stage] */
+  VSGQueueTotals totals[2][2] = {};             /* [kind][stage]: units per XCD queue */
+  unsigned* d_queue = nullptr;                  /* [kind][stage][set][xcd][2] counters */
   Arena ar[VSG_NSETS][AR_COUNT];
   std::vector<int> jobsFilled[VSG_NSETS];   /* per clip: fine entries that may hold jobs */
   std::vector<int*> h_refSlot[VSG_NSETS];   /* per clip: pinned slot to freeze as the tripod plane */
@@ -108,8 +112,9 @@ static void multi_free(VSGMulti* m) {
     if (m->gF[s]) (void)hipGraphExecDestroy(m->gF[s]);
     if (m->doneA[s]) (void)hipEventDestroy(m->doneA[s]);
     if (m->doneF[s]) (void)hipEventDestroy(m->doneF[s]);
-    for (int k = 0; k < 2; ++k)
-      if (m->d_segs[k][s]) (void)hipFree(m->d_segs[k][s]);
+    for (int kind = 0; kind < 2; ++kind)
+      for (int k = 0; k < 2; ++k)
+        if (m->d_segs[kind][k][s]) (void)hipFree(m->d_segs[kind][k][s]);
     if (m->d_refTab[s]) (void)hipFree(m->d_refTab[s]);
     for (Arena& a : m->ar[s]) {
       if (a.d) (void)hipFree(a.d);
@@ -229,6 +234,11 @@ static int env_pair(const char* name, int stage, int def) {
    5,053; K 6 / 6 6,013, 4 / 6 5,651, 4 / 8 5,255 */
 static int queue_wg(int stage) { return env_pair("VIDSTAB_AMD_QUEUE_WG", stage, stage == 0 ? 256 : 128); }
 static int queue_k(int stage) { return env_pair("VIDSTAB_AMD_QUEUE_K", stage, 6); }
+/* VIDSTAB_AMD_QUEUE_SPLIT=0 keeps workgroup-wide units for every job */
+static bool queue_split_on(void) {
+  const char* v = getenv("VIDSTAB_AMD_QUEUE_SPLIT");
+  return !(v && atoi(v) == 0);
+}
 static int queue_count(void) {
   const char* v = getenv("VIDSTAB_AMD_QUEUES");
   return v && atoi(v) == VSG_XCDS ? VSG_XCDS : 1;
@@ -236,42 +246,74 @@ static int queue_count(void) {
 
 /* the queues' segment tables of one stage for one set: every clip with all
    B frame slots (fixed shape), larger fields first; XCD queue x takes the
-   x-th eighth of each clip's job list */
+   x-th eighth of each clip's job list.  A clip whose jobs would leave more
+   than half of a workgroup-wide unit idle (the small fields of 360p-1080p
+   clips) goes to the split queue instead, in 64-lane tiles whose rows the
+   workgroup's waves share (K = 4 there) */
 static int build_segments(VSGMulti* m, int stage, int set) {
-  const int nseg = (int)m->ctx.size();
-  VSGQueueTotals& tot = m->totals[stage];
-  tot.queues = queue_count();
-  std::vector<VSGMultiSeg> segs((size_t)tot.queues * nseg);
-  for (int x = 0; x < tot.queues; ++x) {
-    int units = 0, k = 0;
+  std::vector<VSGMultiSeg> segs[2];
+  const int queues = queue_count();
+  const int wg = queue_wg(stage);
+  for (int kind = 0; kind < 2; ++kind) {
+    VSGQueueTotals& tot = m->totals[kind][stage];
+    tot = VSGQueueTotals{};
+    tot.queues = queues;
+    tot.chunk = 1;
+  }
+  std::vector<int> kindOf(m->ctx.size());
+  for (int c : m->order[stage]) {
+    VSGSearchArgs a = vsg_queue_args(vsg_ctx_search_args(m->ctx[c], stage, nullptr, 0, nullptr, nullptr, nullptr),
+                                     wg, queue_k(stage));
+    kindOf[c] = queue_split_on() && 2 * a.g.U * a.g.V <= wg ? 1 : 0;
+  }
+  for (int kind = 0; kind < 2; ++kind) {
+    int n = 0;
+    for (int c : m->order[stage]) n += kindOf[c] == kind;
+    m->nseg[kind][stage] = n;
+    segs[kind].resize((size_t)queues * n);
+  }
+  for (int x = 0; x < queues; ++x) {
+    int units[2] = {0, 0}, k[2] = {0, 0};
     for (int c : m->order[stage]) {
+      const int kind = kindOf[c];
       VSGContext* x_ = m->ctx[c];
       VSGSet& s = x_->set[set];
       const int njobs = m->B * x_->st[stage].nf;
       VSGMultiSeg sg;
       sg.a = vsg_queue_args(vsg_ctx_search_args(x_, stage, stage == 0 ? s.d_jobsC : s.d_jobsF, njobs, s.d_best,
                                                 s.d_done, stage == 0 ? s.d_resC : s.d_resF),
-                            queue_wg(stage), queue_k(stage));
+                            wg, kind ? 4 : queue_k(stage));
+      if (kind) sg.a.tiles = (sg.a.g.U * sg.a.g.V + 63) / 64;
       sg.variant = vsg_queue_variant(sg.a.g);
       if (sg.variant < 0) return vsg_fail("vsg_multi: search plan without a queue variant", hipSuccess);
-      const int j0 = (int)((long long)njobs * x / tot.queues), j1 = (int)((long long)njobs * (x + 1) / tot.queues);
+      const int j0 = (int)((long long)njobs * x / queues), j1 = (int)((long long)njobs * (x + 1) / queues);
       sg.jobBase = j0;
-      sg.unitBase = units;
+      sg.unitBase = units[kind];
       sg.pad = 0;
-      units += (j1 - j0) * sg.a.tiles;
-      segs[(size_t)x * nseg + k++] = sg;
+      units[kind] += (j1 - j0) * sg.a.tiles;
+      segs[kind][(size_t)x * m->nseg[kind][stage] + k[kind]++] = sg;
     }
-    tot.units[x] = units;
+    for (int kind = 0; kind < 2; ++kind) m->totals[kind][stage].units[x] = units[kind];
   }
-  tot.chunk = 1;
-  if (!m->d_segs[stage][set]) MCHECK(hipMalloc(&m->d_segs[stage][set], sizeof(VSGMultiSeg) * segs.size()));
-  MCHECK(hipMemcpy(m->d_segs[stage][set], segs.data(), sizeof(VSGMultiSeg) * segs.size(), hipMemcpyHostToDevice));
+  for (int kind = 0; kind < 2; ++kind) {
+    if (segs[kind].empty()) continue;
+    VSGMultiSeg*& d = m->d_segs[kind][stage][set];
+    if (!d) MCHECK(hipMalloc(&d, sizeof(VSGMultiSeg) * segs[kind].size()));
+    MCHECK(hipMemcpy(d, segs[kind].data(), sizeof(VSGMultiSeg) * segs[kind].size(), hipMemcpyHostToDevice));
+  }
   return 0;
 }
 
+/* the stage's wide-unit queue, then its split-unit queue (run as two
+   branches of the graph, side by side, they measured the same: config 4
+   5,894 against 5,843 frames/s, 64 small clips 95,251 against 94,705) */
 static int launch_queue(VSGMulti* m, int stage, int set) {
-  MCHECK(vsg_launch_queue(m->d_segs[stage][set], (int)m->ctx.size(), m->totals[stage], queue_wg(stage), queue_k(stage),
-                          m->d_queue + 2 * VSG_XCDS * (stage * VSG_NSETS + set), m->stream));
+  for (int kind = 0; kind < 2; ++kind) {
+    if (m->nseg[kind][stage] == 0) continue;
+    unsigned* q = m->d_queue + 2 * VSG_XCDS * ((kind * 2 + stage) * VSG_NSETS + set);
+    MCHECK(vsg_launch_queue(m->d_segs[kind][stage][set], m->nseg[kind][stage], m->totals[kind][stage],
+                            queue_wg(stage), kind ? 4 : queue_k(stage), kind == 1, q, m->stream));
+  }
   return 0;
 }
 
@@ -369,7 +411,7 @@ extern "C" int vsg_multi_create(VSGMulti** out, int device, int nclips, const in
                      [&](int a, int b) { return m->ctx[a]->st[k].S > m->ctx[b]->st[k].S; });
   }
   if (rc == 0) rc = bind_arenas(m);
-  const size_t qbytes = sizeof(unsigned) * 2 * VSG_XCDS * 2 * VSG_NSETS;
+  const size_t qbytes = sizeof(unsigned) * 2 * VSG_XCDS * 2 * 2 * VSG_NSETS;
   if (rc == 0 && (hipMalloc(&m->d_queue, qbytes) != hipSuccess || hipMemset(m->d_queue, 0, qbytes) != hipSuccess ||
                   hipDeviceSynchronize() != hipSuccess))
     rc = vsg_fail("vsg_multi_create: queue counters", hipSuccess);

@@ -286,39 +286,102 @@ __global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
 }
 
 /* ---- the multi-clip queue kernel -----------------------------------------------
-   One persistent launch per stage for ALL clips of a batch: the units (clip
-   segment, job, 64-lane tile) of every clip and every plan shape form one
-   flattened queue; a unit is one workgroup-wide tile of a job's candidates
-   (256 lanes for the coarse stage, whose waves then share the field's
-   scalar loads, 64 for the small fine jobs).  A workgroup pulls the next unit
-   (a few consecutive units) with an atomic counter, runs the K = 4 grid body of the unit's plan
-   (selected by the segment's variant: J, D and the column chunk CW; K
-   candidate rows per lane is the launch's), folds
-   its argmin into best[job], and the job's last unit refines it -- the same
-   code as k_sad_search.  No workgroup ever waits for another: the only
-   barriers are inside the one-wave workgroup, reached by all its threads
-   for every unit, and a job's refinement is run by whichever unit finishes
-   it last.  There is one queue per XCD (workgroup b serves queue b % 8):
-   queue x holds the x-th eighth of every clip's job list, consecutive jobs
-   of consecutive frames, so a frame's planes are searched from one L2 (or a
-   single queue for the whole grid, balanced across the XCDs).
-   When its queue is empty a workgroup counts itself out; the last one out
-   of a queue resets that queue's counters for the next launch (every pull
-   of every member happened before its own count, so no pull can follow the
-   reset).  Segments are ordered by descending field size, so the largest
-   units are pulled first and the tail is made of small ones. */
+   One persistent launch per stage for ALL clips of a batch: the units of
+   every clip and every plan shape form one flattened queue.  A unit is one
+   workgroup-wide tile of a job's candidates (256 lanes for the coarse stage,
+   whose waves then share the field's scalar loads, 128 for the fine one),
+   or, for a job too small to fill half a workgroup (a split segment), one
+   64-lane tile whose prev rows the workgroup's waves share out.  A
+   workgroup pulls the next units (a few consecutive ones) with an atomic
+   counter, runs the grid body of the unit's plan (selected by the
+   segment's variant: J, D and the column chunk CW; K candidate rows per lane
+   is the launch's), folds its argmin into best[job], and the job's last
+   unit refines it -- the same code as k_sad_search.  No workgroup ever waits
+   for another: the only barriers are inside a workgroup, reached by all its
+   threads for every unit, and a job's refinement is run by whichever unit
+   finishes it last.  There is one queue per XCD (workgroup b serves queue
+   b % 8): queue x holds the x-th eighth of every clip's job list,
+   consecutive jobs of consecutive frames, so a frame's planes are searched
+   from one L2 (or a single queue for the whole grid, balanced across the
+   XCDs).  When its queue is empty a workgroup counts itself out; the last
+   one out of a queue resets that queue's counters for the next launch
+   (every pull of every member happened before its own count, so no pull can
+   follow the reset).  Segments are ordered by descending field size, so the
+   largest units are pulled first and the tail is made of small ones. */
 constexpr int kQueueWaves = 256 * 16;   /* resident waves: 256 CUs x 16 */
 
+/* a split unit: one 64-lane tile of a job whose candidates fill at most half
+   of the workgroup, its prev rows shared out over all W waves of the
+   workgroup (each sums 1/W of them; the partial sums meet in LDS), so no
+   wave idles and the tile's latency is 1/W of one lane's walk.  With one
+   tile per job the wave minimum is the job's grid winner; otherwise it is
+   folded into best[job] and the job's last unit refines, as in search_wg. */
+constexpr int kQueueMaxWaves = 4;   /* 256-lane workgroups */
+
 template <int J, int K, int D, int CW>
-__device__ __forceinline__ void queue_unit(const VSGSearchArgs& a, int job, int tile, RefineLds& L) {
-  search_wg<J, K, D, CW, false>(a, job, tile, L);
+__device__ __forceinline__ void split_unit(const VSGSearchArgs& a, int job, int tile, RefineLds& L,
+                                           uint32_t (*part)[K * 4][64]) {
+  const VSGJob jb = a.jobs[job];
+  if (jb.field < 0) return;   /* unselected slot: uniform over the workgroup */
+  const VSGTile t = make_tile(a, jb);
+  const int W = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int prows = t.S + (K - 1) * t.s;
+  VSGLane<J, K, D, CW, false> ln;
+  vsg_sad_acc<J, K, D, CW, false>(t, tile * 64 + lane, wave * prows / W, (wave + 1) * prows / W, ln);
+  if (wave > 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int j = 0; j < J; ++j) part[wave - 1][k * J + j][lane] = ln.acc[k][j];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    for (int w = 0; w < W - 1; ++w)
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int j = 0; j < J; ++j) ln.acc[k][j] += part[w][k * J + j][lane];
+    const unsigned long long best = vsg_wave_min_u64(vsg_lane_best<J, K, D, CW, false>(t, ln));
+    if (lane == 0) {
+      if (a.tiles == 1) {
+        L.key = best;
+        L.last = 1;
+      } else {
+        if (best != ~0ull) atomicMin(&a.best[job], best);
+        __threadfence();   /* the min is visible before this unit counts as done */
+        const unsigned before = atomicAdd(&a.done[job], 1u);
+        L.last = before == (unsigned)a.tiles - 1;
+        if (L.last) {
+          __threadfence();
+          L.key = __hip_atomic_load(&a.best[job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          a.best[job] = ~0ull;   /* ready for the next launch */
+          a.done[job] = 0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (L.last) refine_job(a, jb, L.key, job, L);
 }
 
-template <int K>
+template <int J, int K, int D, int CW, bool SPLIT>
+__device__ __forceinline__ void queue_unit(const VSGSearchArgs& a, int job, int tile, RefineLds& L,
+                                           uint32_t (*part)[K * 4][64]) {
+  if constexpr (SPLIT)
+    split_unit<J, K, D, CW>(a, job, tile, L, part);
+  else
+    search_wg<J, K, D, CW, false>(a, job, tile, L);
+}
+
+/* SPLIT: the queue of split units (K = 4), whose partial sums need LDS the
+   wide units' kernel does without (it keeps its occupancy) */
+template <int K, bool SPLIT>
 __global__ __launch_bounds__(256) void k_sad_queue(const VSGMultiSeg* __restrict__ segs, int nseg,
                                                    VSGQueueTotals totals, unsigned* __restrict__ queues) {
   __shared__ RefineLds L;
   __shared__ int pulled;
+  __shared__ uint32_t partS[SPLIT ? kQueueMaxWaves - 1 : 1][SPLIT ? K * 4 : 1][SPLIT ? 64 : 1];
+  uint32_t(*part)[K * 4][64] = SPLIT ? reinterpret_cast<uint32_t(*)[K * 4][64]>(&partS[0][0][0]) : nullptr;
   /* workgroup b serves queue b % nq; with 8 queues, the queue of its XCD */
   const int nq = totals.queues;
   const int x = (int)(blockIdx.x % (unsigned)nq);
@@ -352,7 +415,7 @@ __global__ __launch_bounds__(256) void k_sad_queue(const VSGMultiSeg* __restrict
       const int job = sg->jobBase + local / sg->a.tiles, tile = local % sg->a.tiles;
       switch (sg->variant) {
 #define VSG_QUEUE_CASE(jd, cwi, J, D, CW) \
-  case (jd) * 6 + (cwi): queue_unit<J, K, D, CW>(sg->a, job, tile, L); break;
+  case (jd) * 6 + (cwi): queue_unit<J, K, D, CW, SPLIT>(sg->a, job, tile, L, part); break;
         VSG_QUEUE_CASE(0, 0, 4, 1, 28) VSG_QUEUE_CASE(0, 1, 4, 1, 20) VSG_QUEUE_CASE(0, 2, 4, 1, 16)
         VSG_QUEUE_CASE(0, 3, 4, 1, 12) VSG_QUEUE_CASE(0, 4, 4, 1, 8) VSG_QUEUE_CASE(0, 5, 4, 1, 4)
         VSG_QUEUE_CASE(1, 0, 4, 3, 28) VSG_QUEUE_CASE(1, 1, 4, 3, 20) VSG_QUEUE_CASE(1, 2, 4, 3, 16)
@@ -816,7 +879,7 @@ VSGSearchArgs vsg_queue_args(const VSGSearchArgs& a, int wg, int K) {
 }
 
 hipError_t vsg_launch_queue(const VSGMultiSeg* dSegs, int nseg, const VSGQueueTotals& totals, int wg, int K,
-                            unsigned* queues, hipStream_t st) {
+                            bool split, unsigned* queues, hipStream_t st) {
   int sum = 0, most = 0;
   if (totals.queues != 1 && totals.queues != VSG_XCDS) return hipErrorInvalidValue;
   for (int x = 0; x < totals.queues; ++x) {
@@ -824,7 +887,8 @@ hipError_t vsg_launch_queue(const VSGMultiSeg* dSegs, int nseg, const VSGQueueTo
     most = totals.units[x] > most ? totals.units[x] : most;
   }
   if (sum <= 0 || nseg <= 0) return hipSuccess;
-  if (!dSegs || !queues || (wg != 64 && wg != 128 && wg != 256) || (K != 4 && K != 6 && K != 8))
+  if (!dSegs || !queues || (wg != 64 && wg != 128 && wg != 64 * kQueueMaxWaves) || (K != 4 && K != 6 && K != 8) ||
+      (split && K != 4))
     return hipErrorInvalidValue;
   /* every queue gets workgroups: at least 8, at most the resident count */
   const int resident = kQueueWaves / (wg / 64);
@@ -835,12 +899,14 @@ hipError_t vsg_launch_queue(const VSGMultiSeg* dSegs, int nseg, const VSGQueueTo
   const int perQueueWg = nb / totals.queues;
   int chunk = most / (perQueueWg * 16);
   t.chunk = chunk < 1 ? 1 : (chunk > 16 ? 16 : chunk);
-  if (K == 4)
-    hipLaunchKernelGGL(k_sad_queue<4>, dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
+  if (split)
+    hipLaunchKernelGGL((k_sad_queue<4, true>), dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
+  else if (K == 4)
+    hipLaunchKernelGGL((k_sad_queue<4, false>), dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
   else if (K == 6)
-    hipLaunchKernelGGL(k_sad_queue<6>, dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
+    hipLaunchKernelGGL((k_sad_queue<6, false>), dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
   else
-    hipLaunchKernelGGL(k_sad_queue<8>, dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
+    hipLaunchKernelGGL((k_sad_queue<8, false>), dim3(nb), dim3(wg), 0, st, dSegs, nseg, t, queues);
   return hipGetLastError();
 }
 
