@@ -26,7 +26,7 @@ for p in (os.path.join(REPO, "vid.stab_amd"), os.path.join(REPO, "oracle")):
 METRIC = "detect-pass frames/s at 1080p & 4K (plain vs learned form, same shift accuracy)"
 SAD_PEAK_T = 256 * 64 * 2.4e9 * 4 / 1e12      # T abs-diff/s
 HBM_PEAK = 8000.0                              # GB/s
-COARSE_TRAFFIC_1080P = 813.2e6 + 0.05e6        # bytes read + written per 64-frame coarse launch
+COARSE_TRAFFIC_1080P = 826.0e6 + 0.05e6        # bytes read + written per 64-frame coarse launch
 
 
 class VSGProfile(ctypes.Structure):
@@ -309,6 +309,9 @@ def main():
                     help="only time the proposal network's layers one at a time (--frames of 960x540, default 32)")
     ap.add_argument("--per-frame", action="store_true",
                     help="only time one vsMotionDetection call per host frame (ffmpeg's pattern) at 1080p")
+    ap.add_argument("--per-frame-c", action="store_true",
+                    help="only run the ffmpeg-shaped C driver (one vsMotionDetection call per host frame, "
+                         "no Python in the loop) on the 1080p clip")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one 256 MiB device copy (known bytes) to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -325,6 +328,14 @@ def main():
 
     if args.per_frame:
         print(json.dumps({"per_frame_api": measure_per_frame(args.frames or 120)}), flush=True)
+        return
+    if args.per_frame_c:
+        clip = Clip(2, 0)
+        n = args.frames or 130
+        host = np.empty((n, clip.spec.height, clip.spec.width), dtype=np.uint8)
+        for t in range(n):
+            clip.render(t, out=host[t])
+        print(json.dumps({"per_frame_c_api": measure_per_frame_c(clip.spec, host, nframes=n)}), flush=True)
         return
     if args.net_layers:
         n = args.frames or 32
@@ -479,8 +490,9 @@ def main():
 
 def run_multi(args):
     """64 clips of uneven resolution and length through vsMotionDetectMulti
-    (one search launch per plan group, HIP graphs per slot set), against the
-    same clips run one after another through the single-clip entry point."""
+    (persistent queue launches over the units of every clip, HIP graphs per
+    slot set), against the same clips run one after another through the
+    single-clip entry point."""
     import numpy as np
     import torch
     from vidstab_amd import api

@@ -6,8 +6,8 @@
  * planes, fields and selection lists); all contexts share one stream.  The
  * searches of a stage run as ONE persistent launch of k_sad_queue
  * (vsg_search.hip) over a single flattened queue of the units of every clip,
- * whatever their plan shapes -- plus, when some clips' jobs are too small to
- * fill half a workgroup, a second persistent launch over the split units of
+ * whatever their plan shapes -- plus, when some clips' jobs are very small
+ * (360p fields), a second persistent launch over the split units of
  * those clips.
  *
  * Every batch has a fixed shape -- B frame slots per clip, idle slots marked
@@ -239,6 +239,19 @@ static bool queue_split_on(void) {
   const char* v = getenv("VIDSTAB_AMD_QUEUE_SPLIT");
   return !(v && atoi(v) == 0);
 }
+/* jobs of at most this many lanes (at the wide queue's K) go to the split
+   queue (VIDSTAB_AMD_QUEUE_SPLIT_LANES overrides).  Measured on config 4's
+   64 clips (frames/s of the whole call) and on 64 clips of 640x360:
+   32 lanes (only the 360p clips' jobs, 18 lanes) 6,038 / 96,714; 64 (also
+   720p coarse and the 1080p-1440p fine jobs) 5,799 / 96,194; 128 (also
+   1080p coarse) 5,298 / 92,557: in the wide queue a small job's idle waves
+   leave their SIMDs to other workgroups' waves, while the split queue runs
+   after it and adds its whole length */
+static int queue_split_lanes(int wg) {
+  (void)wg;
+  const char* v = getenv("VIDSTAB_AMD_QUEUE_SPLIT_LANES");
+  return v && atoi(v) > 0 ? atoi(v) : 32;
+}
 static int queue_count(void) {
   const char* v = getenv("VIDSTAB_AMD_QUEUES");
   return v && atoi(v) == VSG_XCDS ? VSG_XCDS : 1;
@@ -246,10 +259,9 @@ static int queue_count(void) {
 
 /* the queues' segment tables of one stage for one set: every clip with all
    B frame slots (fixed shape), larger fields first; XCD queue x takes the
-   x-th eighth of each clip's job list.  A clip whose jobs would leave more
-   than half of a workgroup-wide unit idle (the small fields of 360p-1080p
-   clips) goes to the split queue instead, in 64-lane tiles whose rows the
-   workgroup's waves share (K = 4 there) */
+   x-th eighth of each clip's job list.  A clip whose jobs fill at most
+   queue_split_lanes lanes (360p clips) goes to the split queue instead, in
+   64-lane tiles whose rows the workgroup's waves share (K = 4 there) */
 static int build_segments(VSGMulti* m, int stage, int set) {
   std::vector<VSGMultiSeg> segs[2];
   const int queues = queue_count();
@@ -264,7 +276,7 @@ static int build_segments(VSGMulti* m, int stage, int set) {
   for (int c : m->order[stage]) {
     VSGSearchArgs a = vsg_queue_args(vsg_ctx_search_args(m->ctx[c], stage, nullptr, 0, nullptr, nullptr, nullptr),
                                      wg, queue_k(stage));
-    kindOf[c] = queue_split_on() && 2 * a.g.U * a.g.V <= wg ? 1 : 0;
+    kindOf[c] = queue_split_on() && a.g.U * a.g.V <= queue_split_lanes(wg) ? 1 : 0;
   }
   for (int kind = 0; kind < 2; ++kind) {
     int n = 0;

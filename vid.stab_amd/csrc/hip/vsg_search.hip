@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
    every clip and every plan shape form one flattened queue.  A unit is one
    workgroup-wide tile of a job's candidates (256 lanes for the coarse stage,
    whose waves then share the field's scalar loads, 128 for the fine one),
-   or, for a job too small to fill half a workgroup (a split segment), one
+   or, for a job that fills a small part of a workgroup (a split segment), one
    64-lane tile whose prev rows the workgroup's waves share out.  A
    workgroup pulls the next units (a few consecutive ones) with an atomic
    counter, runs the grid body of the unit's plan (selected by the
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) void k_sad_search(VSGSearchArgs a) {
    largest units are pulled first and the tail is made of small ones. */
 constexpr int kQueueWaves = 256 * 16;   /* resident waves: 256 CUs x 16 */
 
-/* a split unit: one 64-lane tile of a job whose candidates fill at most half
+/* a split unit: one 64-lane tile of a job whose candidates fill a small part
    of the workgroup, its prev rows shared out over all W waves of the
    workgroup (each sums 1/W of them; the partial sums meet in LDS), so no
    wave idles and the tile's latency is 1/W of one lane's walk.  With one
