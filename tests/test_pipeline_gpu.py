@@ -1,7 +1,8 @@
 """GPU parity of the multi-batch pipeline and of whole workload clips.
 
 One detect call on more frames than a device batch runs the pipelined path:
-three slot sets in flight on their own streams, batch k+1's coarse results
+three slot sets in flight (on one shared compute stream, or on one stream
+each with VIDSTAB_AMD_STREAMS=3), batch k+1's coarse results
 turned into fine jobs before the host waits for batch k, cross-set event
 ordering, and slot sets reused after three batches.  These tests drive that
 path with small batches on the fixtures, and at the default batch on the
