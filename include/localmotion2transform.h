@@ -1,19 +1,34 @@
 /*
  * localmotion2transform.h -- the LocalMotions->transform steps: the one inside
  * the detect pass (coarse result -> offset of the fine search) and the
- * second pass's per-frame fit, batched over a clip on the GPU.
- *
- * vsLocalmotions2Transforms itself (it takes the transform pass's
- * VSTransformData, with the lens estimate and the fov model) stays outside
- * this library; vsMotionsToTransformBatch covers its per-frame loop for the
- * similarity model.
+ * second pass's, on the GPU: vsLocalmotions2Transforms (the reference's
+ * entry point, with the lens estimate and the rotational camera model) and
+ * vsMotionsToTransformBatch (its per-frame fit for callers without a
+ * VSTransformData).
  */
 #ifndef __LOCALMOTION2TRANSFORM_H
 #define __LOCALMOTION2TRANSFORM_H
 
 #include "transformtype.h"
 #include "frameinfo.h"
+#include "serialize.h"
+#include "transform.h"
 #include "vidstab_api.h"
+
+/* every frame's global transform of a clip's LocalMotions into trans (empty
+   on entry; trans->ts is vs_malloc'd): with the transform config's
+   simpleMotionCalculation, vsSimpleMotionsToTransform per frame; otherwise,
+   with estimateLensDistortion, first the clip's lens strength k (the GPU
+   estimate, vsEstimateLensDistortion), written to td->lensK unless
+   conf.lensK is set; then, when k is determined and |k| > 0.01, every
+   frame's fit through that lens (vsLensMotionsToTransform), else
+   vsMotionsToTransform with conf.fov's camera model.  With storeTransforms
+   the transforms also go to global_motions.trf.  Transforms through the lens
+   agree with the reference within the estimate's tolerance; the others match
+   it to the last bits the device's sin and cos leave open.
+   src/localmotion2transform.h:38-40, src/localmotion2transform.c:40-118 */
+VS_API int vsLocalmotions2Transforms(VSTransformData* td, const VSManyLocalMotions* motions,
+                                     VSTransformations* trans);
 
 /* rotation of one field about (center_x, center_y) implied by its motion;
    0 for fields within 2*size (L1) of the centre.

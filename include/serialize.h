@@ -20,6 +20,12 @@
 /* src/serialize.h:30 */
 #define LIBVIDSTAB_FILE_FORMAT_VERSION 1
 
+/* the LocalMotions of every frame of a clip (elements are LocalMotions);
+   src/serialize.h:59-62 */
+typedef VSVector VSManyLocalMotions;
+#define VSMLMGet(manylocalmotions,index) \
+    ((LocalMotions*)vs_vector_get(manylocalmotions,index))
+
 /* one frame's LocalMotions in the file's mode; src/serialize.c:252-280 */
 VS_API int vsStoreLocalmotions(FILE* f, const LocalMotions* lms, const int serializationMode);
 /* reads one LocalMotions list back; src/serialize.c:282-350 */

@@ -24,7 +24,8 @@ VS_API VSTransform new_transform(double x, double y, double alpha,
                                  double zoom, double barrel, double rshutter, int extra);
 
 /* Same layout as src/transformtype_operations.h:56-68 (the rotational-model
-   members are carried so the struct matches, and stay unused: f == 0). */
+   members are carried so the struct matches; the host steps of the detect
+   pass use the similarity model, f == 0). */
 typedef struct _preparedtransform {
   const VSTransform* t;
   double zcos_a;
@@ -38,6 +39,9 @@ typedef struct _preparedtransform {
 
 /* src/transformtype.c:158-160 (similarity model only) */
 VS_API PreparedTransform prepare_transform(const VSTransform* t, const VSFrameInfo* fi);
+/* focal length in px of a horizontal field of view in degrees, 0 for
+   fov <= 0 or >= 180; src/transformtype.c:123-127 */
+VS_API double focal_from_fov(double fovDeg, int width);
 /* truncating integer version; src/transformtype.c:186-191 */
 VS_API Vec transform_vec(const PreparedTransform* t, const Vec* v);
 /* src/transformtype.c:193-209 */

@@ -181,6 +181,41 @@ VS_API int vsg_proposal_fallbacks(VSGContext* ctx, int stage, long long* outOfRa
 VS_API int vsg_motions_to_transform(int device, int width, int height, const int16_t* lm, const double* match,
                                     const int* count, int nframes, int smoothZoom, VSTransform* out,
                                     double* residual, int* rounds);
+/* vsg_motions_to_transform with the rotational model: focal > 0 is the focal
+   length in px (focal_from_fov of the transform config's fov; 0 is exactly
+   vsg_motions_to_transform) -- calcTransformQuality's prepare_transform_fov
+   branch, src/localmotion2transform.c:153-155, src/transformtype.c:133-209 */
+VS_API int vsg_motions_to_transform_fov(int device, int width, int height, const int16_t* lm, const double* match,
+                                        const int* count, int nframes, int smoothZoom, double focal,
+                                        VSTransform* out, double* residual, int* rounds);
+
+/* ---- second pass: the lens model (src/lensdistortion.c) -----------------------
+   One clip's LocalMotions on the device (same host layout as above; at most
+   4096 motions a frame) and a per-motion mask for the estimator.  Each call
+   runs every frame at once, one workgroup per frame, and returns per frame:
+     vsg_lens_fit         the fit at strength k (focal f, gnSteps Gauss-Newton
+                          steps; with useMask the masked points left out):
+                          r[f] = RMS residual, status[f] 0 fitted, 1 failed
+                          (vsLensFitSimilarityFov returned VS_ERROR), 2 fewer
+                          than 3 motions (:202-365, :455-478)
+     vsg_lens_mask        the estimator's masking pass at k (:562-581):
+                          similarity fit with the mask, residuals, median /
+                          MAD rejection (stddevs); cut[f] newly masked points
+     vsg_lens_transforms  vsLensMotionsToTransform of every frame (:725-786):
+                          out[f], residual[f], status[f] as above
+   vsg_lens_reset_mask marks every motion active; vsg_lens_rejected counts
+   the masked ones. */
+typedef struct VSGLens VSGLens;
+VS_API int vsg_lens_create(VSGLens** out, int device, int width, int height, const int16_t* lm, const double* match,
+                           const int* count, int nframes);
+VS_API void vsg_lens_destroy(VSGLens* lens);
+VS_API int vsg_lens_reset_mask(VSGLens* lens);
+VS_API int vsg_lens_rejected(VSGLens* lens, long long* rejected);
+VS_API int vsg_lens_fit(VSGLens* lens, double k, double f, int gnSteps, int useMask, double* r, int* status);
+VS_API int vsg_lens_mask(VSGLens* lens, double k, int gnSteps, double stddevs, int* cut);
+VS_API int vsg_lens_transforms(VSGLens* lens, double k, double f, int gnSteps, int reject, double stddevs,
+                               VSTransform* out, double* residual, int* status);
+
 /* vsg_motions_to_transform without any device allocation: no counterpart in
    the reference; for callers that cannot allocate device memory (code next
    to HIP graph capture, serving loops, harnesses that forbid hipMalloc).

@@ -84,6 +84,10 @@ int main(int argc, char** argv) {
     const char* base = strrchr(where.dli_fname, '/');
     printf("vsMotionDetection from %s\n", base ? base + 1 : where.dli_fname);
   }
+  if (dladdr((void*)vsLocalmotions2Transforms, &where) && where.dli_fname) {
+    const char* base = strrchr(where.dli_fname, '/');
+    printf("vsLocalmotions2Transforms from %s\n", base ? base + 1 : where.dli_fname);
+  }
 
   VSynSpec sp;
   VSynClip clip;

@@ -13,7 +13,10 @@ gcc -O2 -std=gnu11 -ffp-contract=off -fno-fast-math -fopenmp \
     -o "$REPO/oracle/_ref/make_golden" "$HERE/make_golden.c" \
     "$REPO/vid.stab_amd/csrc/synth/vsynth.c" \
     -L"$REPO/oracle/_ref" -Wl,-rpath,"$REPO/oracle/_ref" -lvsref -lm
-# make_golden.sh [planar|packed|show [case index]|transforms|hashes]: one fixture set, or all
+gcc -O2 -std=gnu11 -ffp-contract=off -fno-fast-math \
+    -I"$REF/src" -o "$REPO/oracle/_ref/make_golden_l2t" "$HERE/make_golden_l2t.c" \
+    -L"$REPO/oracle/_ref" -Wl,-rpath,"$REPO/oracle/_ref" -lvsref -lm
+# make_golden.sh [planar|packed|show [case index]|transforms|l2t|hashes]: one fixture set, or all
 # (hashes, the whole workload clips, is slow: run it on its own)
 case "$1" in
   hashes) "$REPO/oracle/_ref/make_golden" "$HERE/clip_hashes.json" hashes ${THREADS:-8} ;;
@@ -21,8 +24,10 @@ case "$1" in
   packed) "$REPO/oracle/_ref/make_golden" "$HERE/packed_golden.json" packed $2 ;;
   show) "$REPO/oracle/_ref/make_golden" "$HERE/show_golden.json" show $2 ;;
   transforms) "$REPO/oracle/_ref/make_golden" "$HERE/transforms_golden.json" transforms ;;
+  l2t) "$REPO/oracle/_ref/make_golden_l2t" "$HERE/l2t_golden.json" ;;
   *) "$REPO/oracle/_ref/make_golden" "$HERE/detect_golden.json" planar
      "$REPO/oracle/_ref/make_golden" "$HERE/packed_golden.json" packed
      "$REPO/oracle/_ref/make_golden" "$HERE/show_golden.json" show
-     "$REPO/oracle/_ref/make_golden" "$HERE/transforms_golden.json" transforms ;;
+     "$REPO/oracle/_ref/make_golden" "$HERE/transforms_golden.json" transforms
+     "$REPO/oracle/_ref/make_golden_l2t" "$HERE/l2t_golden.json" ;;
 esac

@@ -55,6 +55,45 @@ def test_abi_struct_sizes_match_reference_layout():
     assert api.LocalMotion.contrast.offset == 16 and api.LocalMotion.match.offset == 24
 
 
+def test_second_pass_struct_layouts_match_reference():
+    """the transform pass's structs that vsLocalmotions2Transforms takes, and
+    the lens estimate's, as the reference lays them out on x86-64 (measured
+    with offsetof on src/transform.h and src/lensdistortion.h in a release
+    build): in include/ and in the Python mirrors"""
+    import subprocess
+    import tempfile
+    want = {"VSTransformData": 720, "VSTransformData.conf": 216, "VSTransformData.lensMode": 328,
+            "VSTransformData.lensK": 336, "VSTransformData.lensMaps": 352, "VSTransformData.initialized": 712,
+            "VSTransformConfig": 112, "VSTransformConfig.modName": 56, "VSTransformConfig.estimateLensDistortion": 84,
+            "VSTransformConfig.lensK": 96, "VSTransformConfig.fov": 104, "VSTransformations": 24,
+            "VSTransformations.len": 12, "VSLensPlaneMap": 120, "VSLensEstimateConfig": 72,
+            "VSLensEstimateConfig.f": 64, "VSLensEstimate": 48, "VSLensEstimate.determined": 44,
+            "VSLensDistortion": 32}
+    probe = "#include <stddef.h>\n#include <stdio.h>\n#include \"transform.h\"\n#include \"lensdistortion.h\"\n"
+    probe += "int main(void) {\n"
+    for key in want:
+        if "." in key:
+            t, m = key.split(".")
+            probe += f'  printf("{key} %zu\\n", offsetof({t}, {m}));\n'
+        else:
+            probe += f'  printf("{key} %zu\\n", sizeof({key}));\n'
+    probe += "  return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as tmp:
+        src = os.path.join(tmp, "probe.c")
+        open(src, "w").write(probe)
+        exe = os.path.join(tmp, "probe")
+        subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), "-o", exe, src], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    got = {l.split()[0]: int(l.split()[1]) for l in out.splitlines()}
+    assert got == want
+    for key, v in want.items():
+        if "." in key:
+            t, m = key.split(".")
+            assert getattr(getattr(api, t), m).offset == v, key
+        elif hasattr(api, key):
+            assert ctypes.sizeof(getattr(api, key)) == v, key
+
+
 @pytest.mark.parametrize("n", [0, 1, 2, 8, 25, 77, 154])
 def test_spiral_rank_closed_form_matches_walk(n):
     """the kernels' closed-form spiral rank (and its inverse) against the

@@ -4,11 +4,13 @@ vidstabdetect and vidstabtransform filters would be.
 tests/c/two_libraries.c (built by oracle/Makefile against the reference's
 headers) replaces the allocation / log hooks and remaps VS_OK / VS_ERROR as
 ffmpeg's vidstabutils does, runs the detect filter's call sequence, writes
-the .trf stream, then reads it back and fits and smooths the transforms with
-the reference's transform-side calls.  It is linked twice: with this
+the .trf stream, then reads it back, fits the transforms and smooths them
+through the transform-side calls.  It is linked twice: with this
 library ahead of the reference library (detection binds here, the transform
-side to the reference, the shared helpers and hook variables to one
-definition), and with the reference library alone.  Both runs must print the
+side -- vsLocalmotions2Transforms, the lens estimate and the per-frame fits --
+here as well, the rest of the transform side (reading the .trf, smoothing)
+to the reference, the shared helpers and hook variables to one definition),
+and with the reference library alone.  Both runs must print the
 same .trf bytes and the same transforms bit for bit, with every hook
 allocation freed.
 """
@@ -40,6 +42,10 @@ def test_mixed_process_matches_the_reference_alone(tmp_path):
     alone = _run(ref, str(tmp_path / "ref.trf"), frames)
     assert mixed[0] == "vsMotionDetection from libvidstab_amd.so"
     assert alone[0] == "vsMotionDetection from libvidstab.so"
+    # the second pass's entry binds here too: the lens estimate and the
+    # per-frame fits run on the GPU
+    assert mixed[1] == "vsLocalmotions2Transforms from libvidstab_amd.so"
+    assert alone[1] == "vsLocalmotions2Transforms from libvidstab.so"
     assert mixed[-1] == alone[-1] == "hook allocations all freed"
     assert f"frames read {frames}, transforms {frames}" in alone
-    assert mixed[1:] == alone[1:]
+    assert mixed[2:] == alone[2:]

@@ -1,7 +1,8 @@
 /*
  * vsg_transform.hip -- the second pass's per-frame global transform on the
- * GPU: vsMotionsToTransform (src/localmotion2transform.c:213-283) with the
- * similarity model (fov 0, no lens) for every frame of a clip in one launch.
+ * GPU: vsMotionsToTransform (src/localmotion2transform.c:213-283) for every
+ * frame of a clip in one launch, with the similarity model (fov 0) or the
+ * rotational one (fov > 0, src/transformtype.c:123-209), no lens.
  *
  * One wave per frame.  The frame's LocalMotions sit in LDS with the
  * "missmatch" array of the reference (the last error of every enabled field,
@@ -35,6 +36,7 @@ struct M2TArgs {
   const int* first;       /* per frame: index of its first motion (null: summed from count) */
   const int* count;       /* per frame: motions */
   int nframes, cx, cy, smoothZoom, cap;
+  double focal;           /* 0: similarity model; > 0: rotation about the optical centre */
   VSTransform* out;
   double* residual;
   int* rounds;
@@ -47,6 +49,7 @@ struct Frame {
   int* eidx;        /* [n]: the enabled fields, ascending */
   int n, ne;
   double cx, cy;
+  double f;         /* focal length in px (VSTransformConfig.fov), 0 for the similarity model */
 };
 
 /* ascending list of the fields with miss >= 0 (a wave-wide ordered compaction) */
@@ -72,12 +75,66 @@ __device__ __forceinline__ double lane_value(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+/* the error sum of calcTransformQuality under the rotational model
+   (VSTransformConfig.fov > 0): prepare_transform_fov and the f > 0 branch of
+   transform_vec_double (src/transformtype.c:133-209), every expression in
+   the order the reference's -ffast-math x86 build evaluates it.  That build
+   computes rotation_matrix_backward's two 3x3 products with the constant
+   zeros and ones of Ry, Rx and Rz folded into pairwise sums (the products
+   by 0.0 stay: they are not foldable in IEEE arithmetic), takes cos(yaw)
+   for cos(-yaw), and sin(-yaw) as such.  Lane l computes enabled field
+   j0 + l of each 64-field chunk and the wave adds the errors in field
+   order, as in quality(). */
+__device__ double quality_fov_sum(Frame& F, const double p[4]) {
+  const double f = F.f;
+  const double z = p[3] * 0.01 + 1.0;
+  const double yaw = p[0] / f, pitch = p[1] / f;
+  double sr, cr, sp, cp;
+  sincos(p[2], &sr, &cr);
+  sincos(pitch, &sp, &cp);
+  const double cy = cos(yaw), sy = sin(-yaw), nsy = -sy;
+  const double A = cp * 0.0 + sy * sp, B = cp + 0.0 * sp, spcy = sp * cy;
+  /* b = Ry(-yaw) Rx(pitch) Rz(roll), row major */
+  const double b0 = sr * A + cr * cy, b1 = cr * A - sr * cy, b2 = sy * cp - sp * 0.0;
+  const double b3 = sr * B + cr * 0.0, b4 = cr * B - sr * 0.0, b5 = 0.0 * cp - sp;
+  const double b6 = nsy * cr + sr * spcy, b7 = cr * spcy - nsy * sr, b8 = cy * cp;
+  /* the forward rotation is its transpose */
+  const double r0 = b0, r1 = b3, r2 = b6, r3 = b1, r4 = b4, r5 = b7, r6 = b2, r7 = b5, r8 = b8;
+  const double fz = f * z;
+  double err = 0.0;
+  for (int j0 = 0; j0 < F.ne; j0 += 64) {
+    const int j = j0 + (int)threadIdx.x;
+    double e = 0.0;
+    if (j < F.ne) {
+      const int i = F.eidx[j];
+      const short4 m = F.lm[i];
+      const double vx = (double)m.x, vy = (double)m.y;
+      const double rx = vx - F.cx, ry = vy - F.cy;
+      const double X = (r0 * rx + r1 * ry) + r2 * f;
+      const double Y = (r3 * rx + r4 * ry) + r5 * f;
+      const double Z = (r6 * rx + r7 * ry) + r8 * f;
+      const double x = (X * fz) / Z + F.cx, y = (fz * Y) / Z + F.cy;
+      const double dx = (x - vx) - (double)m.z, dy = (y - vy) - (double)m.w;
+      e = dy * dy + dx * dx;
+      F.miss[i] = e;
+    }
+#pragma unroll
+    for (int l = 0; l < 64; ++l) err += lane_value(e, l);
+  }
+  return err;
+}
+
 /* calcTransformQuality for params p: updates miss[] of the enabled fields.
    Lane l computes the error of enabled field j0 + l of each 64-field chunk;
    the wave then adds the chunk's 64 errors in field order from registers
    (lanes past the last field hold +0.0, which leaves a sum of non-negative
    terms unchanged), so the total is the reference's sequential sum. */
 __device__ double quality(Frame& F, const double p[4]) {
+  const double reg = fabs(p[2]) * 0.2 + fabs(p[3]) * 0.002;
+  if (F.f > 0.0) {
+    const double err = quality_fov_sum(F, p);
+    return reg + (F.n > 0 ? err / (double)(F.ne + 1) : 0.0);
+  }
   double sn, cs;
   sincos(p[2], &sn, &cs);
   const double z = p[3] * 0.01 + 1.0;
@@ -100,7 +157,6 @@ __device__ double quality(Frame& F, const double p[4]) {
 #pragma unroll
     for (int l = 0; l < 64; ++l) err += lane_value(e, l);
   }
-  const double reg = fabs(p[2]) * 0.2 + fabs(p[3]) * 0.002;
   return reg + (F.n > 0 ? err / (double)(F.ne + 1) : 0.0);
 }
 
@@ -186,6 +242,7 @@ __global__ __launch_bounds__(64) void k_motions_to_transform(M2TArgs a) {
   F.eidx = reinterpret_cast<int*>(lmS + a.cap);
   F.cx = (double)a.cx;
   F.cy = (double)a.cy;
+  F.f = a.focal;
   VSTransform t = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
   double res = 0.0;
   int rounds = 0;
@@ -335,6 +392,7 @@ extern "C" int vsg_motions_to_transform_static(int device, int width, int height
   a.cy = height / 2;
   a.smoothZoom = smoothZoom ? 1 : 0;
   a.cap = cap;
+  a.focal = 0.0;
   a.out = reinterpret_cast<VSTransform*>(base + kOffOut);
   a.residual = reinterpret_cast<double*>(base + kOffRes);
   a.rounds = reinterpret_cast<int*>(base + kOffRounds);
@@ -352,7 +410,14 @@ extern "C" int vsg_motions_to_transform_static(int device, int width, int height
 extern "C" int vsg_motions_to_transform(int device, int width, int height, const int16_t* lm, const double* match,
                                         const int* count, int nframes, int smoothZoom, VSTransform* out,
                                         double* residual, int* rounds) {
-  if (width <= 0 || height <= 0 || nframes < 0 || (nframes > 0 && (!count || !out)))
+  return vsg_motions_to_transform_fov(device, width, height, lm, match, count, nframes, smoothZoom, 0.0, out,
+                                      residual, rounds);
+}
+
+extern "C" int vsg_motions_to_transform_fov(int device, int width, int height, const int16_t* lm,
+                                            const double* match, const int* count, int nframes, int smoothZoom,
+                                            double focal, VSTransform* out, double* residual, int* rounds) {
+  if (width <= 0 || height <= 0 || nframes < 0 || (nframes > 0 && (!count || !out)) || !(focal >= 0.0))
     return vsg_fail("vsg_motions_to_transform: invalid arguments", hipSuccess);
   if (nframes == 0) return 0;
   long long total = 0;
@@ -409,6 +474,7 @@ extern "C" int vsg_motions_to_transform(int device, int width, int height, const
     a.cy = height / 2;
     a.smoothZoom = smoothZoom ? 1 : 0;
     a.cap = cap;
+    a.focal = focal;
     a.out = (VSTransform*)d_out;
     a.residual = (double*)d_res;
     a.rounds = (int*)d_rounds;

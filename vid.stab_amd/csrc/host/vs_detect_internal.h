@@ -6,7 +6,9 @@
 #ifndef VS_DETECT_INTERNAL_H
 #define VS_DETECT_INTERNAL_H
 
+#include "lensdistortion.h"
 #include "motiondetect.h"
+#include "serialize.h"
 #include "vsg.h"
 
 /* vsMotionDetectInit without the GPU context: validation, config clamps, the
@@ -40,5 +42,20 @@ void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, Loca
    whose line the reference draws from its end (vs_draw.c) */
 void vsd_draw_overlays(const VSMotionDetect* md, uint8_t* img, int linesize, LocalMotions* coarse,
                        LocalMotions* fine);
+
+/* ---- second pass (vs_lensdistortion.c) ---- */
+
+/* the HIP device of this process's detectors (VIDSTAB_AMD_DEVICE, default 0) */
+int vsd_device(void);
+
+/* a clip's LocalMotions in the GPU's host layout: 4 int16 per motion (field
+   x, y, motion x, y), the match values, the motion count of each frame
+   (malloc'd; the caller frees them).  VS_OK or VS_ERROR. */
+int vsd_flatten_motions(const VSManyLocalMotions* motions, int16_t** lm, double** match, int** count);
+
+/* the clip-wide lens estimate over the clip held by `gpu`
+   (src/lensdistortion.c:528-639); *failed is set when a GPU call failed */
+VSLensEstimate vsd_lens_estimate(VSGLens* gpu, int nframes, const int* count, const VSLensEstimateConfig* cfg,
+                                 int* failed);
 
 #endif /* VS_DETECT_INTERNAL_H */

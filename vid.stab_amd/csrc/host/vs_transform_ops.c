@@ -45,6 +45,14 @@ VS_API VSTransform new_transform(double x, double y, double alpha, double zoom, 
 
 VS_API VSTransform null_transform(void) { return new_transform(0, 0, 0, 0, 0, 0, 0); }
 
+/* src/transformtype.c:123-127, in its compiled order: (width * 0.5) /
+   tan(fov * (pi / 360)) */
+VS_API double focal_from_fov(double fovDeg, int width) {
+  if (fovDeg <= 0.0 || fovDeg >= 180.0 || width <= 0) return 0.0;
+  const double halfDegree = M_PI / 360.0;
+  return ((double)width * 0.5) / tan(fovDeg * halfDegree);
+}
+
 VS_API PreparedTransform prepare_transform(const VSTransform* t, const VSFrameInfo* fi) {
   PreparedTransform pt;
   memset(&pt, 0, sizeof(pt));

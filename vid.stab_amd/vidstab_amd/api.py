@@ -71,6 +71,46 @@ class VSVector(ctypes.Structure):
     _fields_ = [("data", ctypes.POINTER(c_void_p)), ("buffersize", c_int), ("nelems", c_int)]
 
 
+class VSTransformations(ctypes.Structure):
+    _fields_ = [("ts", ctypes.POINTER(VSTransform)), ("current", c_int), ("len", c_int),
+                ("warned_end", c_short)]
+
+
+class VSTransformConfig(ctypes.Structure):
+    _fields_ = [("relative", c_int), ("smoothing", c_int), ("crop", c_int), ("invert", c_int),
+                ("zoom", c_double), ("optZoom", c_int), ("zoomSpeed", c_double), ("interpolType", c_int),
+                ("maxShift", c_int), ("maxAngle", c_double), ("modName", ctypes.c_char_p), ("verbose", c_int),
+                ("simpleMotionCalculation", c_int), ("storeTransforms", c_int), ("smoothZoom", c_int),
+                ("camPathAlgo", c_int), ("estimateLensDistortion", c_int), ("lensCorrection", c_int),
+                ("lensK", c_double), ("fov", c_double)]
+
+
+class VSLensPlaneMap(ctypes.Structure):
+    _fields_ = [("active", c_int), ("mode", c_int), ("k", c_double), ("cdx", c_double), ("cdy", c_double),
+                ("csx", c_double), ("csy", c_double), ("tMaxU", c_double), ("tMaxD", c_double),
+                ("tDomD", c_double), ("invRho2", c_double), ("f", c_double), ("sxShift", c_int),
+                ("syShift", c_int), ("idxScaleU", ctypes.c_int32), ("idxScaleD", ctypes.c_int32),
+                ("gU", c_void_p), ("gD", c_void_p)]
+
+
+class VSTransformData(ctypes.Structure):
+    _fields_ = [("fiSrc", VSFrameInfo), ("fiDest", VSFrameInfo), ("src", VSFrame), ("destbuf", VSFrame),
+                ("dest", VSFrame), ("srcMalloced", c_short), ("interpolate", c_void_p),
+                ("conf", VSTransformConfig), ("lensMode", c_int), ("lensActive", c_int), ("lensK", c_double),
+                ("lensMapK", c_double), ("lensMaps", VSLensPlaneMap * 3), ("initialized", c_int)]
+
+
+class VSLensEstimateConfig(ctypes.Structure):
+    _fields_ = [("kMin", c_double), ("kMax", c_double), ("tolerance", c_double), ("maxIterations", c_int),
+                ("gaussNewtonSteps", c_int), ("maxUncertainty", c_double), ("rejectOutliers", c_int),
+                ("outlierStddevs", c_double), ("outlierPasses", c_int), ("f", c_double)]
+
+
+class VSLensEstimate(ctypes.Structure):
+    _fields_ = [("k", c_double), ("residual", c_double), ("curvature", c_double), ("uncertainty", c_double),
+                ("iterations", c_int), ("rejected", c_int), ("used", c_int), ("determined", c_int)]
+
+
 class VSGStageDesc(ctypes.Structure):
     _fields_ = [("fieldSize", c_int), ("maxShift", c_int), ("stepSize", c_int),
                 ("nfields", c_int), ("fieldRows", c_int), ("maxFields", c_int),
@@ -143,6 +183,18 @@ def lib():
         L.vsg_last_error.restype = ctypes.c_char_p
         L.vsPrepareFile.argtypes = [ctypes.POINTER(VSMotionDetect), c_void_p]
         L.vsWriteToFile.argtypes = [ctypes.POINTER(VSMotionDetect), c_void_p, ctypes.POINTER(VSVector)]
+        L.vsLocalmotions2Transforms.argtypes = [ctypes.POINTER(VSTransformData), ctypes.POINTER(VSVector),
+                                                ctypes.POINTER(VSTransformations)]
+        L.vsLocalmotions2Transforms.restype = c_int
+        L.vsLensEstimateGetDefaultConfig.argtypes = []
+        L.vsLensEstimateGetDefaultConfig.restype = VSLensEstimateConfig
+        L.vsEstimateLensDistortion.argtypes = [ctypes.POINTER(VSFrameInfo), ctypes.POINTER(VSVector),
+                                               ctypes.POINTER(VSLensEstimateConfig)]
+        L.vsEstimateLensDistortion.restype = VSLensEstimate
+        L.focal_from_fov.argtypes = [c_double, c_int]
+        L.focal_from_fov.restype = c_double
+        # the library's release hook (a function pointer variable, as in the reference)
+        L.vs_free_fn = ctypes.CFUNCTYPE(None, c_void_p).in_dll(L, "vs_free")
         _lib = L
     return _lib
 
@@ -384,6 +436,74 @@ def motions_to_transforms(width, height, frames, smooth_zoom=0):
     if L.vsMotionsToTransformBatch(ctypes.byref(fi), clip.vecs, clip.n, smooth_zoom, out) != 0:
         raise RuntimeError("vsMotionsToTransformBatch failed: " + L.vsg_last_error().decode())
     return [(t.x, t.y, t.alpha, t.zoom, t.extra) for t in out[:clip.n]]
+
+
+def _many(clip):
+    """a VSManyLocalMotions (VSVector of LocalMotions pointers) over a ClipMotions"""
+    ptrs = (c_void_p * max(clip.n, 1))()
+    for f in range(clip.n):
+        ptrs[f] = ctypes.cast(ctypes.byref(clip.vecs[f]), c_void_p)
+    many = VSVector()
+    many.data = ctypes.cast(ptrs, ctypes.POINTER(c_void_p))
+    many.buffersize = max(clip.n, 1)
+    many.nelems = clip.n
+    clip._keep.append(ptrs)
+    return many
+
+
+def localmotions_to_transforms(width, height, frames, fov=0.0, estimate=1, simple=0, smooth_zoom=0,
+                               lens_k=0.0, store=False, mod=b"vidstab_amd"):
+    """vsLocalmotions2Transforms on a clip's LocalMotions (motion tuples per
+    frame, or a ClipMotions) with a transform config holding the given fov,
+    estimateLensDistortion, simpleMotionCalculation, smoothZoom, lensK and
+    storeTransforms -> (list of (x, y, alpha, zoom, barrel, extra), the lens
+    strength the call left in the transform data)"""
+    L = lib()
+    fi = VSFrameInfo()
+    if not L.vsFrameInfoInit(ctypes.byref(fi), width, height, PF_YUV420P):
+        raise ValueError("vsFrameInfoInit failed")
+    clip = frames if isinstance(frames, ClipMotions) else ClipMotions(frames)
+    many = _many(clip)
+    td = VSTransformData()
+    td.fiSrc = fi
+    td.fiDest = fi
+    td.conf.modName = mod
+    td.conf.relative = 1
+    td.conf.smoothing = 15
+    td.conf.optZoom = 1
+    td.conf.zoomSpeed = 0.25
+    td.conf.interpolType = 2
+    td.conf.maxShift = -1
+    td.conf.maxAngle = -1.0
+    td.conf.simpleMotionCalculation = simple
+    td.conf.storeTransforms = 1 if store else 0
+    td.conf.smoothZoom = smooth_zoom
+    td.conf.estimateLensDistortion = estimate
+    td.conf.lensCorrection = 1
+    td.conf.lensK = lens_k
+    td.conf.fov = fov
+    trans = VSTransformations()
+    if L.vsLocalmotions2Transforms(ctypes.byref(td), ctypes.byref(many), ctypes.byref(trans)) != 0:
+        raise RuntimeError("vsLocalmotions2Transforms failed: " + L.vsg_last_error().decode())
+    out = [(t.x, t.y, t.alpha, t.zoom, t.barrel, t.extra) for t in trans.ts[:trans.len]]
+    if trans.ts:
+        L.vs_free_fn(ctypes.cast(trans.ts, c_void_p))
+    return out, td.lensK
+
+
+def estimate_lens(width, height, frames, focal=0.0):
+    """vsEstimateLensDistortion of a clip's LocalMotions with the default
+    search settings and the given focal length -> dict of the estimate"""
+    L = lib()
+    fi = VSFrameInfo()
+    if not L.vsFrameInfoInit(ctypes.byref(fi), width, height, PF_YUV420P):
+        raise ValueError("vsFrameInfoInit failed")
+    clip = frames if isinstance(frames, ClipMotions) else ClipMotions(frames)
+    many = _many(clip)
+    cfg = L.vsLensEstimateGetDefaultConfig()
+    cfg.f = focal
+    e = L.vsEstimateLensDistortion(ctypes.byref(fi), ctypes.byref(many), ctypes.byref(cfg))
+    return {n: getattr(e, n) for n, _ in VSLensEstimate._fields_}
 
 
 def transform_distance(width, height, a, b):
