@@ -286,6 +286,39 @@ def measure_second_pass(spec, ptrs, ls):
            "cpu_port_ms_1core": round(cpu_ms, 3), "frames_bit_identical": same}
     log(f"second pass: {len(frames)} frames on the GPU in {gpu_ms:.2f} ms (host-inclusive), oracle on one core "
         f"{cpu_ms:.1f} ms; {same}/{len(frames)} transforms bit-identical")
+    out["default_config"] = measure_second_pass_default(spec, frames, clip)
+    return out
+
+
+def measure_second_pass_default(spec, frames, clip):
+    """vsLocalmotions2Transforms with the reference's default transform
+    configuration (lens estimate on, fov 0) on the clip's LocalMotions: the
+    GPU library against the reference's own build on the host, and the lens
+    estimate alone (vsEstimateLensDistortion: Brent's search, one k_lens_frame
+    launch over every frame per objective evaluation)"""
+    import pyoracle
+    from vidstab_amd import api
+    api.localmotions_to_transforms(spec.width, spec.height, api.ClipMotions(frames[:8]))   # warm-up
+    t0 = time.perf_counter()
+    got, k = api.localmotions_to_transforms(spec.width, spec.height, clip)
+    gpu_ms = (time.perf_counter() - t0) * 1000.0
+    t0 = time.perf_counter()
+    est = api.estimate_lens(spec.width, spec.height, clip)
+    est_ms = (time.perf_counter() - t0) * 1000.0
+    out = {"gpu_ms": round(gpu_ms, 3), "lens_k": k, "estimate_gpu_ms": round(est_ms, 3),
+           "estimate_evaluations": est["iterations"], "estimate_determined": est["determined"],
+           "estimate_ms_per_evaluation": round(est_ms / max(1, est["iterations"]), 4)}
+    if os.path.exists(pyoracle.REF_PATH):
+        t0 = time.perf_counter()
+        want, rk = pyoracle.ref_localmotions_to_transforms(spec.width, spec.height, clip)
+        ref_ms = (time.perf_counter() - t0) * 1000.0
+        worst = max((abs(g[i] - w[i]) for g, w in zip(got, want) for i in range(4)), default=0.0)
+        same = sum(g == w for g, w in zip(got, want))
+        out.update({"reference_ms_1core": round(ref_ms, 3), "reference_lens_k": rk,
+                    "frames_identical": same, "largest_difference": worst})
+        log(f"second pass, default configuration (lens estimate on): GPU {gpu_ms:.2f} ms, reference on one core "
+            f"{ref_ms:.1f} ms; lens k {k:.6g} vs {rk:.6g}; {same}/{len(got)} transforms identical, largest "
+            f"difference {worst:.3g}; estimate alone {est_ms:.2f} ms over {est['iterations']} evaluations")
     return out
 
 

@@ -253,3 +253,38 @@ class RefDetector:
     def close(self):
         if self.md.initialized:
             self.L.vsMotionDetectionCleanup(ctypes.byref(self.md))
+
+
+def ref_localmotions_to_transforms(width, height, frames, fov=0.0, estimate=1):
+    """The reference's own vsLocalmotions2Transforms (oracle/_ref/libvsref.so)
+    on a clip's LocalMotions with the reference's default transform
+    configuration apart from fov and estimateLensDistortion -> (list of (x, y,
+    alpha, zoom, barrel, extra), the lens strength left in the transform data).
+    The CPU baseline of the second pass."""
+    from vidstab_amd import api  # struct layouts only (identical ABI)
+    if not os.path.exists(REF_PATH):
+        raise RuntimeError(f"reference build missing: {REF_PATH}")
+    L = ctypes.CDLL(REF_PATH)
+    ctypes.c_int.in_dll(L, "vs_log_level").value = 0
+    L.vsFrameInfoInit.argtypes = [ctypes.POINTER(api.VSFrameInfo)] + [ctypes.c_int] * 3
+    L.vsTransformGetDefaultConfig.restype = api.VSTransformConfig
+    L.vsTransformGetDefaultConfig.argtypes = [ctypes.c_char_p]
+    L.vsLocalmotions2Transforms.argtypes = [ctypes.POINTER(api.VSTransformData), ctypes.POINTER(api.VSVector),
+                                            ctypes.POINTER(api.VSTransformations)]
+    fi = api.VSFrameInfo()
+    L.vsFrameInfoInit(ctypes.byref(fi), width, height, api.PF_YUV420P)
+    clip = frames if isinstance(frames, api.ClipMotions) else api.ClipMotions(frames)
+    many = api._many(clip)
+    td = api.VSTransformData()
+    td.fiSrc = fi
+    td.fiDest = fi
+    td.conf = L.vsTransformGetDefaultConfig(b"ref")
+    td.conf.fov = fov
+    td.conf.estimateLensDistortion = estimate
+    trans = api.VSTransformations()
+    if L.vsLocalmotions2Transforms(ctypes.byref(td), ctypes.byref(many), ctypes.byref(trans)) != 0:
+        raise RuntimeError("reference vsLocalmotions2Transforms failed")
+    out = [(t.x, t.y, t.alpha, t.zoom, t.barrel, t.extra) for t in trans.ts[:trans.len]]
+    free = ctypes.CFUNCTYPE(None, ctypes.c_void_p).in_dll(L, "vs_free")
+    free(ctypes.cast(trans.ts, ctypes.c_void_p))
+    return out, td.lensK

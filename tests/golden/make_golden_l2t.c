@@ -219,6 +219,8 @@ int main(int argc, char** argv) {
     { "lens_none",      1280, 720, 24,    64,     0.0,   0.0,  0.08, 1,  0,     1,  106 },
     { "lens_pincushion",960,  540, 20,    48,     0.08,  0.0,  0.05, 1,  0,     0,  107 },
     { "simple",         1280, 720, 16,    64,     0.0,   0.0,  0.08, 1,  1,     1,  108 },
+    /* 1,947 fields a frame, near the 2,048 a frame may hold */
+    { "lens_dense_4k",  3840, 2160, 6,    64,     -0.2,  0.0,  0.06, 1,  0,     1,  109 },
   };
   FILE* out = fopen(path, "w");
   if (!out) return 1;

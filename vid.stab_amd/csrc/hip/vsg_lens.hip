@@ -4,18 +4,26 @@
  * distortion estimate minimises over, its outlier masks, and the per-frame
  * lens transforms (reference: src/lensdistortion.c).
  *
- * One 512-thread workgroup per frame, one kernel per mode.  The frame's
- * points (at most 2,048 LocalMotions) sit in LDS: the field position p and
- * the observed destination q = p + v (integers), the active flag, the
- * undistorted, centred source a and a residual; thread t handles points
- * j = t, t + 512, ....
+ * One 256-thread workgroup per frame, one kernel per mode and camera model
+ * (similarity, or rotational when a focal length is given).  The frame's
+ * points sit in LDS, in arrays as long as the clip's largest frame (at most
+ * 2,048 LocalMotions): the field position p and the observed destination
+ * q = p + v (integers), the active flag, the undistorted, centred source a,
+ * and in the masking modes a residual per point and the sort buffer; thread
+ * t handles points j = t, t + 256, ....  A 1080p clip's frames (about 300
+ * points) take 8 KB in the objective's mode, so many frames share a CU and a
+ * clip of a few hundred frames runs in one round.
  * Sums over a frame's points (means, the closed-form start, the 4x4
  * Gauss-Newton normal equations, the residual) are reduced across the wave
- * by shuffles and across the 8 waves in LDS, in a fixed order, and every
- * thread then carries the frame's scalar state (parameters, solve) alike.
+ * by shuffles and across the 4 waves in LDS, in a fixed order, and every
+ * thread then carries the frame's scalar state (parameters, solve) alike,
+ * in registers.
  * Outlier masks take the median and the median absolute deviation of the
  * active points' residuals: the residuals are sorted in LDS by a bitonic
  * network (+inf pads to a power of two).
+ * The per-frame outputs go straight into pinned host memory (mapped into
+ * the device's address space): an evaluation of the estimator's objective
+ * is one launch and one stream synchronisation.
  *
  * The reference sums sequentially in point order, so frame residuals agree
  * with it to rounding, not bit for bit; the estimate's Brent search (host,
@@ -45,7 +53,7 @@
 
 namespace {
 
-constexpr int kLensThreads = 512;
+constexpr int kLensThreads = 256;
 constexpr int kLensWaves = kLensThreads / 64;
 constexpr int kLensCap = 2048;                            /* points per frame */
 
@@ -56,16 +64,18 @@ struct LensArgs {
   const short4* lm;        /* per motion: field x, y, motion x, y */
   const double* match;     /* per motion */
   const int* first;        /* per frame: first motion */
-  const int* count;        /* per frame: motions */
+  const int* count;        /* per frame: motions, at most cap */
   int nframes;
+  int cap;                 /* the clip's largest frame: the LDS arrays' length */
+  int np;                  /* cap rounded up to a power of two: the sort buffer's length */
   uint8_t* flags;          /* per motion: active (estimator mask); null: all active */
   double k, f, cx, cy, rho;
-  int gn, mode, reject;
+  int gn, reject;
   double stddevs;
-  double* r;               /* OBJ / FINAL: per frame residual */
-  int* status;             /* per frame */
-  int* cut;                /* MASK: per frame, points newly masked */
-  VSTransform* out;        /* FINAL */
+  double* r;               /* OBJ / FINAL: per frame residual (host memory) */
+  int* status;             /* per frame (host memory) */
+  int* cut;                /* MASK: per frame, points newly masked (host memory) */
+  VSTransform* out;        /* FINAL (host memory) */
 };
 
 struct Lens {
@@ -148,26 +158,40 @@ __device__ void fwd_init(Fwd& w, double f, double tx, double ty, double c, doubl
 
 struct LensLds {
   double red[kLensWaves][16];
-  double sbuf[kLensCap];
-  Fwd fw[5];   /* the forward map of the current parameters and its four perturbations */
+  Fwd fw[5];   /* rotational model: the current parameters' forward map and its four perturbations */
   int cnt;
 };
 
+/* the camera's forward map on a centred point; ROT: the rotational model
+   (f > 0), else the similarity */
+template <bool ROT>
 __device__ void fwd_apply(const Fwd& w, double ax, double ay, double* wx, double* wy) {
-  if (w.f > 0.0) {
+  if constexpr (ROT) {
     const double X = w.rf[0] * ax + w.rf[1] * ay + w.rf[2] * w.f;
     const double Y = w.rf[3] * ax + w.rf[4] * ay + w.rf[5] * w.f;
     const double Z = w.rf[6] * ax + w.rf[7] * ay + w.rf[8] * w.f;
     *wx = w.z * w.f * X / Z;
     *wy = w.z * w.f * Y / Z;
-    return;
+  } else {
+    *wx = w.c * ax + w.s * ay + w.tx;
+    *wy = -w.s * ax + w.c * ay + w.ty;
   }
-  *wx = w.c * ax + w.s * ay + w.tx;
-  *wy = -w.s * ax + w.c * ay + w.ty;
+}
+
+/* the similarity's forward map, held in registers by every thread */
+__device__ Fwd fwd_similarity(double tx, double ty, double c, double s) {
+  Fwd w;
+  w.f = 0.0;
+  w.c = c;
+  w.s = s;
+  w.tx = tx;
+  w.ty = ty;
+  w.z = 1.0;
+  return w;
 }
 
 /* the sum over the workgroup of K values per thread, the same in every
-   thread: wave shuffles, then the 8 wave totals in wave order */
+   thread: wave shuffles, then the wave totals in wave order */
 template <int K>
 __device__ void block_sum(double (&v)[K], LensLds& L) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -181,6 +205,7 @@ __device__ void block_sum(double (&v)[K], LensLds& L) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     double t = 0.0;
+#pragma unroll
     for (int w = 0; w < kLensWaves; ++w) t += L.red[w][k];
     v[k] = t;
   }
@@ -193,47 +218,76 @@ __device__ int block_count(int c, LensLds& L) {
   return (int)v[0];
 }
 
-/* lensSolve4 (:111-152): Gaussian elimination with partial pivoting */
+/* lensSolve4 (:111-152): Gaussian elimination with partial pivoting (the
+   first largest pivot candidate wins).  Every index is a compile-time
+   constant once the loops unroll and rows are exchanged under uniform
+   branches, so the system stays in registers. */
 __device__ bool solve4(double A[16], double b[4], double x[4]) {
+#pragma unroll
   for (int i = 0; i < 4; ++i) {
     int p = i;
-    for (int j = i + 1; j < 4; ++j)
-      if (fabs(A[j * 4 + i]) > fabs(A[p * 4 + i])) p = j;
-    if (fabs(A[p * 4 + i]) < 1e-12) return false;
-    if (p != i) {
-      for (int q = 0; q < 4; ++q) {
-        const double t = A[i * 4 + q];
-        A[i * 4 + q] = A[p * 4 + q];
-        A[p * 4 + q] = t;
+    double best = fabs(A[i * 4 + i]);
+#pragma unroll
+    for (int j = i + 1; j < 4; ++j) {
+      const double v = fabs(A[j * 4 + i]);
+      if (v > best) {
+        best = v;
+        p = j;
       }
-      const double t = b[i];
-      b[i] = b[p];
-      b[p] = t;
     }
+    if (best < 1e-12) return false;
+#pragma unroll
+    for (int j = i + 1; j < 4; ++j) {
+      if (p == j) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double t = A[i * 4 + q];
+          A[i * 4 + q] = A[j * 4 + q];
+          A[j * 4 + q] = t;
+        }
+        const double t = b[i];
+        b[i] = b[j];
+        b[j] = t;
+      }
+    }
+#pragma unroll
     for (int j = i + 1; j < 4; ++j) {
       const double f = A[j * 4 + i] / A[i * 4 + i];
       if (f == 0) continue;
+#pragma unroll
       for (int q = i; q < 4; ++q) A[j * 4 + q] -= f * A[i * 4 + q];
       b[j] -= f * b[i];
     }
   }
+#pragma unroll
   for (int i = 3; i >= 0; --i) {
     double sum = b[i];
+#pragma unroll
     for (int q = i + 1; q < 4; ++q) sum -= A[i * 4 + q] * x[q];
     x[i] = sum / A[i * 4 + i];
   }
   return true;
 }
 
-/* one frame's points, held in LDS by the frame's workgroup: the field
-   position p and the observed destination q = p + v (integer pixels), the
-   active flag, the undistorted centred source a, and a residual per point */
+/* one frame's points in the workgroup's LDS: the field position p and the
+   observed destination q = p + v (integer pixels), the undistorted centred
+   source a, the active flag; the masking modes add a residual per point and
+   the sort buffer */
 struct FrameLds {
-  short4 pq[kLensCap];      /* p.x, p.y, q.x, q.y */
-  uint8_t act[kLensCap];
-  double ax[kLensCap], ay[kLensCap];
-  double res[kLensCap];
+  short4* pq;      /* p.x, p.y, q.x, q.y */
+  double* ax;
+  double* ay;
+  double* res;
+  double* sbuf;
+  uint8_t* act;
 };
+
+/* bytes of dynamic LDS a mode's workgroup takes for frames of at most cap
+   points (np: cap rounded up to a power of two) */
+__host__ __device__ constexpr size_t lens_lds_bytes(int mode, int cap, int np) {
+  return (size_t)cap * (sizeof(short4) + 2 * sizeof(double) + 1) +
+         (mode == LENS_OBJ ? 0 : (size_t)cap * sizeof(double) + (size_t)np * sizeof(double));
+}
 
 struct FitOut {
   bool ok;
@@ -248,8 +302,10 @@ __device__ int count_active(int n, const FrameLds& F, LensLds& L) {
   return block_count(a, L);
 }
 
-/* vsLensFitSimilarityFov (:202-365) over the frame's active points */
-__device__ FitOut lens_fit(int n, FrameLds& F, const Lens& ld, int gn, double f, LensLds& L) {
+/* vsLensFitSimilarityFov (:202-365) over the frame's active points; ROT:
+   the rotational model at focal length f */
+template <bool ROT>
+__device__ FitOut lens_fit(int n, const FrameLds& F, const Lens& ld, int gn, double f, LensLds& L) {
   FitOut o;
   o.ok = false;
   o.res2 = 0.0;
@@ -295,17 +351,21 @@ __device__ FitOut lens_fit(int n, FrameLds& F, const Lens& ld, int gn, double f,
   double ty = mBy - (-s * mAx + c * mAy);
   const double kHT = 1e-3, kHCS = 1e-6;   /* LENS_H_T, LENS_H_CS (:196-199) */
   for (int it = 0; it < gn; ++it) {
-    /* built once per iteration, by one thread, into LDS */
-    if (threadIdx.x == 0) {
-      fwd_init(L.fw[0], f, tx, ty, c, s);
-      if (f > 0.0) {
+    Fwd w0;
+    if constexpr (ROT) {
+      /* built once per iteration, by one thread, into LDS; the previous
+         iteration's readers are past block_sum's barriers */
+      if (threadIdx.x == 0) {
+        fwd_init(L.fw[0], f, tx, ty, c, s);
         fwd_init(L.fw[1], f, tx + kHT, ty, c, s);
         fwd_init(L.fw[2], f, tx, ty + kHT, c, s);
         fwd_init(L.fw[3], f, tx, ty, c + kHCS, s);
         fwd_init(L.fw[4], f, tx, ty, c, s + kHCS);
       }
+      __syncthreads();
+    } else {
+      w0 = fwd_similarity(tx, ty, c, s);
     }
-    __syncthreads();
     double acc[14];   /* g[4], then H's upper triangle row by row */
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
@@ -314,19 +374,22 @@ __device__ FitOut lens_fit(int n, FrameLds& F, const Lens& ld, int gn, double f,
       const short4 pq = F.pq[j];
       const double ax = F.ax[j], ay = F.ay[j];
       double wx, wy, dpx, dpy, J[4];
-      fwd_apply(L.fw[0], ax, ay, &wx, &wy);
+      if constexpr (ROT)
+        fwd_apply<true>(L.fw[0], ax, ay, &wx, &wy);
+      else
+        fwd_apply<false>(w0, ax, ay, &wx, &wy);
       if (!distort(ld, wx + ld.cx, wy + ld.cy, &dpx, &dpy) || !distort_jacobian(ld, wx, wy, J)) {
         bad = true;
         continue;
       }
       const double ex = (pq.z - ld.cx) - (dpx - ld.cx), ey = (pq.w - ld.cy) - (dpy - ld.cy);
       double Jp[4][2];
-      if (f > 0.0) {
+      if constexpr (ROT) {
         const double H[4] = {kHT, kHT, kHCS, kHCS};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           double pwx, pwy;
-          fwd_apply(L.fw[1 + i], ax, ay, &pwx, &pwy);
+          fwd_apply<true>(L.fw[1 + i], ax, ay, &pwx, &pwy);
           const double dwx = (pwx - wx) / H[i], dwy = (pwy - wy) / H[i];
           Jp[i][0] = J[0] * dwx + J[1] * dwy;
           Jp[i][1] = J[2] * dwx + J[3] * dwy;
@@ -369,14 +432,22 @@ __device__ FitOut lens_fit(int n, FrameLds& F, const Lens& ld, int gn, double f,
     c += delta[2];
     s += delta[3];
   }
-  if (threadIdx.x == 0) fwd_init(L.fw[0], f, tx, ty, c, s);
-  __syncthreads();
+  Fwd w0;
+  if constexpr (ROT) {
+    if (threadIdx.x == 0) fwd_init(L.fw[0], f, tx, ty, c, s);
+    __syncthreads();
+  } else {
+    w0 = fwd_similarity(tx, ty, c, s);
+  }
   double r2[1] = {0.0};
   for (int j = threadIdx.x; j < n; j += kLensThreads) {
     if (!F.act[j]) continue;
     const short4 pq = F.pq[j];
     double wx, wy, dpx, dpy;
-    fwd_apply(L.fw[0], F.ax[j], F.ay[j], &wx, &wy);
+    if constexpr (ROT)
+      fwd_apply<true>(L.fw[0], F.ax[j], F.ay[j], &wx, &wy);
+    else
+      fwd_apply<false>(w0, F.ax[j], F.ay[j], &wx, &wy);
     if (!distort(ld, wx + ld.cx, wy + ld.cy, &dpx, &dpy)) {
       bad = true;
       continue;
@@ -398,11 +469,18 @@ __device__ FitOut lens_fit(int n, FrameLds& F, const Lens& ld, int gn, double f,
 /* vsLensMatchResidualsFov (:700-723): |q - D(fwd(U(p)))| of every point
    (masked ones too) into F.res under transform t; false if a point leaves
    the model's domain */
-__device__ bool match_residuals(int n, FrameLds& F, const Lens& ld, double f, const VSTransform& t, LensLds& L) {
+template <bool ROT>
+__device__ bool match_residuals(int n, const FrameLds& F, const Lens& ld, double f, const VSTransform& t,
+                                LensLds& L) {
   const double z = 1.0 + t.zoom / 100.0;
-  __syncthreads();   /* every thread is done with the forward maps in LDS */
-  if (threadIdx.x == 0) fwd_init(L.fw[0], f, t.x, t.y, z * cos(t.alpha), z * sin(t.alpha));
-  __syncthreads();
+  Fwd w0;
+  if constexpr (ROT) {
+    __syncthreads();   /* every thread is done with the forward maps in LDS */
+    if (threadIdx.x == 0) fwd_init(L.fw[0], f, t.x, t.y, z * cos(t.alpha), z * sin(t.alpha));
+    __syncthreads();
+  } else {
+    w0 = fwd_similarity(t.x, t.y, z * cos(t.alpha), z * sin(t.alpha));
+  }
   bool bad = false;
   for (int j = threadIdx.x; j < n; j += kLensThreads) {
     const short4 pq = F.pq[j];
@@ -412,7 +490,10 @@ __device__ bool match_residuals(int n, FrameLds& F, const Lens& ld, double f, co
       bad = true;
       continue;
     }
-    fwd_apply(L.fw[0], ux - ld.cx, uy - ld.cy, &wx, &wy);
+    if constexpr (ROT)
+      fwd_apply<true>(L.fw[0], ux - ld.cx, uy - ld.cy, &wx, &wy);
+    else
+      fwd_apply<false>(w0, ux - ld.cx, uy - ld.cy, &wx, &wy);
     if (!distort(ld, wx + ld.cx, wy + ld.cy, &dpx, &dpy)) {
       bad = true;
       continue;
@@ -423,18 +504,18 @@ __device__ bool match_residuals(int n, FrameLds& F, const Lens& ld, double f, co
   return !__syncthreads_or(bad);
 }
 
-/* bitonic sort of L.sbuf[0, np), np a power of two <= kLensCap */
-__device__ void sort_lds(LensLds& L, int np) {
+/* bitonic sort of sbuf[0, np), np a power of two */
+__device__ void sort_lds(double* sbuf, int np) {
   for (int size = 2; size <= np; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       for (int i = threadIdx.x; i < np; i += kLensThreads) {
         const int j = i ^ stride;
         if (j > i) {
           const bool up = (i & size) == 0;
-          const double a = L.sbuf[i], b = L.sbuf[j];
+          const double a = sbuf[i], b = sbuf[j];
           if ((a > b) == up) {
-            L.sbuf[i] = b;
-            L.sbuf[j] = a;
+            sbuf[i] = b;
+            sbuf[j] = a;
           }
         }
       }
@@ -445,25 +526,25 @@ __device__ void sort_lds(LensLds& L, int np) {
 /* lensMaskOutliers (:412-453) on F.res: masks active points whose residual
    exceeds median + stddevs * 1.4826 * MAD; returns how many were newly
    masked */
-__device__ int mask_outliers(int n, FrameLds& F, double stddevs, LensLds& L) {
+__device__ int mask_outliers(int n, const FrameLds& F, double stddevs, LensLds& L) {
   const int m = count_active(n, F, L);
   if (m < 8) return 0;
   if (threadIdx.x == 0) L.cnt = 0;
   __syncthreads();
   for (int j = threadIdx.x; j < n; j += kLensThreads)
-    if (F.act[j]) L.sbuf[atomicAdd(&L.cnt, 1)] = F.res[j];
+    if (F.act[j]) F.sbuf[atomicAdd(&L.cnt, 1)] = F.res[j];
   int np = 1;
   while (np < m) np <<= 1;
   __syncthreads();
-  for (int i = m + (int)threadIdx.x; i < np; i += kLensThreads) L.sbuf[i] = __builtin_inf();
+  for (int i = m + (int)threadIdx.x; i < np; i += kLensThreads) F.sbuf[i] = __builtin_inf();
   __syncthreads();
-  sort_lds(L, np);
-  const double median = L.sbuf[m / 2];
+  sort_lds(F.sbuf, np);
+  const double median = F.sbuf[m / 2];
   __syncthreads();
-  for (int i = threadIdx.x; i < m; i += kLensThreads) L.sbuf[i] = fabs(L.sbuf[i] - median);
+  for (int i = threadIdx.x; i < m; i += kLensThreads) F.sbuf[i] = fabs(F.sbuf[i] - median);
   __syncthreads();
-  sort_lds(L, np);
-  const double mad = L.sbuf[m / 2];
+  sort_lds(F.sbuf, np);
+  const double mad = F.sbuf[m / 2];
   __syncthreads();
   if (mad <= 1e-9) return 0;
   const double thresh = median + stddevs * 1.4826 * mad;
@@ -489,13 +570,27 @@ __device__ VSTransform to_transform(const FitOut& o, double k) {
   return t;
 }
 
-template <int MODE>
+/* MODE: LENS_OBJ, LENS_MASK or LENS_FINAL; ROT: the rotational model at
+   focal length a.f (the masking pass always fits the similarity) */
+template <int MODE, bool ROT>
 __global__ __launch_bounds__(kLensThreads) void k_lens_frame(LensArgs a) {
   __shared__ LensLds L;
-  __shared__ FrameLds F;
+  extern __shared__ __align__(16) uint8_t lds[];
   const int fr = blockIdx.x;
   if (fr >= a.nframes) return;
-  const int n = a.count[fr];
+  FrameLds F;
+  F.pq = reinterpret_cast<short4*>(lds);
+  F.ax = reinterpret_cast<double*>(lds + sizeof(short4) * (size_t)a.cap);
+  F.ay = F.ax + a.cap;
+  if constexpr (MODE == LENS_OBJ) {
+    F.res = F.sbuf = nullptr;
+    F.act = reinterpret_cast<uint8_t*>(F.ay + a.cap);
+  } else {
+    F.res = F.ay + a.cap;
+    F.sbuf = F.res + a.cap;
+    F.act = reinterpret_cast<uint8_t*>(F.sbuf + a.np);
+  }
+  const int n = a.count[fr];   /* at most a.cap (the host takes the largest) */
   const int first = a.first[fr];
   for (int j = threadIdx.x; j < n; j += kLensThreads) {
     const short4 m = a.lm[first + j];
@@ -512,7 +607,7 @@ __global__ __launch_bounds__(kLensThreads) void k_lens_frame(LensArgs a) {
     int st = LENS_SKIP;
     double r = 0.0;
     if (n >= 3) {
-      const FitOut o = lens_fit(n, F, ld, a.gn, a.f, L);
+      const FitOut o = lens_fit<ROT>(n, F, ld, a.gn, a.f, L);
       st = o.ok ? LENS_OK : LENS_FAIL;
       if (o.ok) r = sqrt(o.res2 / o.nAct);
     }
@@ -524,8 +619,8 @@ __global__ __launch_bounds__(kLensThreads) void k_lens_frame(LensArgs a) {
   if constexpr (MODE == LENS_MASK) {
     /* the estimator's masking pass fits the similarity model (f = 0) */
     int cut = 0;
-    const FitOut o = lens_fit(n, F, ld, a.gn, 0.0, L);
-    if (o.ok && match_residuals(n, F, ld, 0.0, to_transform(o, ld.k), L)) {
+    const FitOut o = lens_fit<false>(n, F, ld, a.gn, 0.0, L);
+    if (o.ok && match_residuals<false>(n, F, ld, 0.0, to_transform(o, ld.k), L)) {
       cut = mask_outliers(n, F, a.stddevs, L);
       __syncthreads();
       for (int j = threadIdx.x; j < n; j += kLensThreads) a.flags[first + j] = F.act[j];
@@ -548,7 +643,7 @@ __global__ __launch_bounds__(kLensThreads) void k_lens_frame(LensArgs a) {
         mask_outliers(n, F, a.stddevs, L);
         __syncthreads();
       }
-      const FitOut o = lens_fit(n, F, ld, a.gn, a.f, L);
+      const FitOut o = lens_fit<ROT>(n, F, ld, a.gn, a.f, L);
       if (!o.ok) {
         t.extra = 1;
         st = LENS_FAIL;
@@ -556,9 +651,9 @@ __global__ __launch_bounds__(kLensThreads) void k_lens_frame(LensArgs a) {
         t = to_transform(o, ld.k);
         residual = sqrt(o.res2 / o.nAct);
         /* stage two: the residual at the known k */
-        if (a.reject && match_residuals(n, F, ld, a.f, t, L) && mask_outliers(n, F, a.stddevs, L) > 0) {
+        if (a.reject && match_residuals<ROT>(n, F, ld, a.f, t, L) && mask_outliers(n, F, a.stddevs, L) > 0) {
           __syncthreads();
-          const FitOut o2 = lens_fit(n, F, ld, a.gn, a.f, L);
+          const FitOut o2 = lens_fit<ROT>(n, F, ld, a.gn, a.f, L);
           if (o2.ok) {
             t = to_transform(o2, ld.k);
             residual = sqrt(o2.res2 / o2.nAct);
@@ -589,6 +684,7 @@ struct VSGLens {
   int device = 0;
   int nframes = 0;
   long long total = 0;
+  int cap = 1, np = 1;   /* the largest frame's points, and that rounded up to a power of two */
   double cx = 0, cy = 0, rho = 0;
   hipStream_t st = nullptr;
   short4* d_lm = nullptr;
@@ -596,28 +692,37 @@ struct VSGLens {
   int* d_first = nullptr;
   int* d_count = nullptr;
   uint8_t* d_flags = nullptr;
-  double* d_r = nullptr;
-  int* d_status = nullptr;
-  int* d_cut = nullptr;
-  VSTransform* d_out = nullptr;
-  /* pinned read-back of the per-frame outputs */
+  /* the per-frame outputs: pinned host memory the kernels write through its
+     device mapping (m_*) */
   double* h_r = nullptr;
   int* h_status = nullptr;
   int* h_cut = nullptr;
+  VSTransform* h_out = nullptr;
+  double* m_r = nullptr;
+  int* m_status = nullptr;
+  int* m_cut = nullptr;
+  VSTransform* m_out = nullptr;
 };
 
 extern "C" void vsg_lens_destroy(VSGLens* L) {
   if (!L) return;
   (void)hipSetDevice(L->device);
   if (L->st) (void)hipStreamSynchronize(L->st);
-  void* d[] = {L->d_lm, L->d_match, L->d_first, L->d_count, L->d_flags, L->d_r, L->d_status, L->d_cut, L->d_out};
+  void* d[] = {L->d_lm, L->d_match, L->d_first, L->d_count, L->d_flags};
   for (void* p : d)
     if (p) (void)hipFree(p);
-  void* h[] = {L->h_r, L->h_status, L->h_cut};
+  void* h[] = {L->h_r, L->h_status, L->h_cut, L->h_out};
   for (void* p : h)
     if (p) (void)hipHostFree(p);
   if (L->st) (void)hipStreamDestroy(L->st);
   delete L;
+}
+
+template <typename T>
+static hipError_t pinned_mapped(T** host, T** dev, size_t n) {
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(host), sizeof(T) * n, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(dev), *host, 0);
+  return e;
 }
 
 extern "C" int vsg_lens_create(VSGLens** out, int device, int width, int height, const int16_t* lm,
@@ -626,36 +731,43 @@ extern "C" int vsg_lens_create(VSGLens** out, int device, int width, int height,
   *out = nullptr;
   std::vector<int> first(nframes);
   long long total = 0;
+  int cap = 1;
   for (int f = 0; f < nframes; ++f) {
     if (count[f] < 0 || count[f] > kLensCap)
       return vsg_fail("vsg_lens_create: a frame holds more than 2048 LocalMotions", hipSuccess);
     first[f] = (int)total;
     total += count[f];
+    if (count[f] > cap) cap = count[f];
   }
   if (total > 0x7FFFFFFFLL || (total > 0 && (!lm || !match))) return vsg_fail("vsg_lens_create: motions", hipSuccess);
+  int np = 1;
+  while (np < cap) np <<= 1;
+  int lmax = 0;
+  LCHECK(hipSetDevice(device));
+  LCHECK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  if (lens_lds_bytes(LENS_FINAL, cap, np) + sizeof(LensLds) > (size_t)lmax)
+    return vsg_fail("vsg_lens_create: a frame's points do not fit the workgroup's LDS", hipSuccess);
   VSGLens* L = new (std::nothrow) VSGLens();
   if (!L) return vsg_fail("vsg_lens_create: out of host memory", hipSuccess);
   L->device = device;
   L->nframes = nframes;
   L->total = total;
+  L->cap = cap;
+  L->np = np;
   L->cx = width / 2.0;
   L->cy = height / 2.0;
   L->rho = 0.5 * sqrt((double)width * width + (double)height * height);
   const size_t nm = total > 0 ? (size_t)total : 1;
-  hipError_t e = hipSetDevice(device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking);
+  hipError_t e = hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&L->d_lm, sizeof(short4) * nm);
   if (e == hipSuccess) e = hipMalloc(&L->d_match, sizeof(double) * nm);
   if (e == hipSuccess) e = hipMalloc(&L->d_flags, nm);
   if (e == hipSuccess) e = hipMalloc(&L->d_first, sizeof(int) * nframes);
   if (e == hipSuccess) e = hipMalloc(&L->d_count, sizeof(int) * nframes);
-  if (e == hipSuccess) e = hipMalloc(&L->d_r, sizeof(double) * nframes);
-  if (e == hipSuccess) e = hipMalloc(&L->d_status, sizeof(int) * nframes);
-  if (e == hipSuccess) e = hipMalloc(&L->d_cut, sizeof(int) * nframes);
-  if (e == hipSuccess) e = hipMalloc(&L->d_out, sizeof(VSTransform) * nframes);
-  if (e == hipSuccess) e = hipHostMalloc(&L->h_r, sizeof(double) * nframes, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipHostMalloc(&L->h_status, sizeof(int) * nframes, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipHostMalloc(&L->h_cut, sizeof(int) * nframes, hipHostMallocDefault);
+  if (e == hipSuccess) e = pinned_mapped(&L->h_r, &L->m_r, nframes);
+  if (e == hipSuccess) e = pinned_mapped(&L->h_status, &L->m_status, nframes);
+  if (e == hipSuccess) e = pinned_mapped(&L->h_cut, &L->m_cut, nframes);
+  if (e == hipSuccess) e = pinned_mapped(&L->h_out, &L->m_out, nframes);
   if (e == hipSuccess && total > 0)
     e = hipMemcpyAsync(L->d_lm, lm, sizeof(short4) * total, hipMemcpyHostToDevice, L->st);
   if (e == hipSuccess && total > 0)
@@ -672,13 +784,15 @@ extern "C" int vsg_lens_create(VSGLens** out, int device, int width, int height,
   return 0;
 }
 
-static LensArgs lens_args(VSGLens* L, double k, double f, int gn, int mode) {
+static LensArgs lens_args(VSGLens* L, double k, double f, int gn) {
   LensArgs a;
   a.lm = L->d_lm;
   a.match = L->d_match;
   a.first = L->d_first;
   a.count = L->d_count;
   a.nframes = L->nframes;
+  a.cap = L->cap;
+  a.np = L->np;
   a.flags = nullptr;
   a.k = k;
   a.f = f;
@@ -686,14 +800,31 @@ static LensArgs lens_args(VSGLens* L, double k, double f, int gn, int mode) {
   a.cy = L->cy;
   a.rho = L->rho;
   a.gn = gn;
-  a.mode = mode;
   a.reject = 0;
   a.stddevs = 0.0;
-  a.r = L->d_r;
-  a.status = L->d_status;
-  a.cut = L->d_cut;
-  a.out = L->d_out;
+  a.r = L->m_r;
+  a.status = L->m_status;
+  a.cut = L->m_cut;
+  a.out = L->m_out;
   return a;
+}
+
+/* one launch over the clip's frames, then the stream's synchronisation: the
+   outputs are in the host buffers when it returns 0 */
+template <int MODE>
+static int lens_run(VSGLens* L, const LensArgs& a, const char* what) {
+  const size_t lds = lens_lds_bytes(MODE, a.cap, a.np);
+  if constexpr (MODE == LENS_MASK) {
+    hipLaunchKernelGGL((k_lens_frame<MODE, false>), dim3(a.nframes), dim3(kLensThreads), lds, L->st, a);
+  } else {
+    if (a.f > 0.0)
+      hipLaunchKernelGGL((k_lens_frame<MODE, true>), dim3(a.nframes), dim3(kLensThreads), lds, L->st, a);
+    else
+      hipLaunchKernelGGL((k_lens_frame<MODE, false>), dim3(a.nframes), dim3(kLensThreads), lds, L->st, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(L->st);
+  return e == hipSuccess ? 0 : vsg_fail(what, e);
 }
 
 extern "C" int vsg_lens_reset_mask(VSGLens* L) {
@@ -707,13 +838,9 @@ extern "C" int vsg_lens_reset_mask(VSGLens* L) {
 extern "C" int vsg_lens_fit(VSGLens* L, double k, double f, int gnSteps, int useMask, double* r, int* status) {
   if (!L || !r || !status || gnSteps < 0) return vsg_fail("vsg_lens_fit", hipSuccess);
   LCHECK(hipSetDevice(L->device));
-  LensArgs a = lens_args(L, k, f, gnSteps, LENS_OBJ);
+  LensArgs a = lens_args(L, k, f, gnSteps);
   if (useMask) a.flags = L->d_flags;
-  hipLaunchKernelGGL(k_lens_frame<LENS_OBJ>, dim3(L->nframes), dim3(kLensThreads), 0, L->st, a);
-  LCHECK(hipGetLastError());
-  LCHECK(hipMemcpyAsync(L->h_r, L->d_r, sizeof(double) * L->nframes, hipMemcpyDeviceToHost, L->st));
-  LCHECK(hipMemcpyAsync(L->h_status, L->d_status, sizeof(int) * L->nframes, hipMemcpyDeviceToHost, L->st));
-  LCHECK(hipStreamSynchronize(L->st));
+  if (lens_run<LENS_OBJ>(L, a, "vsg_lens_fit")) return -1;
   memcpy(r, L->h_r, sizeof(double) * L->nframes);
   memcpy(status, L->h_status, sizeof(int) * L->nframes);
   return 0;
@@ -722,13 +849,10 @@ extern "C" int vsg_lens_fit(VSGLens* L, double k, double f, int gnSteps, int use
 extern "C" int vsg_lens_mask(VSGLens* L, double k, int gnSteps, double stddevs, int* cut) {
   if (!L || !cut || gnSteps < 0) return vsg_fail("vsg_lens_mask", hipSuccess);
   LCHECK(hipSetDevice(L->device));
-  LensArgs a = lens_args(L, k, 0.0, gnSteps, LENS_MASK);
+  LensArgs a = lens_args(L, k, 0.0, gnSteps);
   a.flags = L->d_flags;
   a.stddevs = stddevs;
-  hipLaunchKernelGGL(k_lens_frame<LENS_MASK>, dim3(L->nframes), dim3(kLensThreads), 0, L->st, a);
-  LCHECK(hipGetLastError());
-  LCHECK(hipMemcpyAsync(L->h_cut, L->d_cut, sizeof(int) * L->nframes, hipMemcpyDeviceToHost, L->st));
-  LCHECK(hipStreamSynchronize(L->st));
+  if (lens_run<LENS_MASK>(L, a, "vsg_lens_mask")) return -1;
   memcpy(cut, L->h_cut, sizeof(int) * L->nframes);
   return 0;
 }
@@ -749,15 +873,11 @@ extern "C" int vsg_lens_transforms(VSGLens* L, double k, double f, int gnSteps, 
                                    VSTransform* out, double* residual, int* status) {
   if (!L || !out || gnSteps < 0) return vsg_fail("vsg_lens_transforms", hipSuccess);
   LCHECK(hipSetDevice(L->device));
-  LensArgs a = lens_args(L, k, f, gnSteps, LENS_FINAL);
+  LensArgs a = lens_args(L, k, f, gnSteps);
   a.reject = reject ? 1 : 0;
   a.stddevs = stddevs;
-  hipLaunchKernelGGL(k_lens_frame<LENS_FINAL>, dim3(L->nframes), dim3(kLensThreads), 0, L->st, a);
-  LCHECK(hipGetLastError());
-  LCHECK(hipMemcpyAsync(out, L->d_out, sizeof(VSTransform) * L->nframes, hipMemcpyDeviceToHost, L->st));
-  LCHECK(hipMemcpyAsync(L->h_r, L->d_r, sizeof(double) * L->nframes, hipMemcpyDeviceToHost, L->st));
-  LCHECK(hipMemcpyAsync(L->h_status, L->d_status, sizeof(int) * L->nframes, hipMemcpyDeviceToHost, L->st));
-  LCHECK(hipStreamSynchronize(L->st));
+  if (lens_run<LENS_FINAL>(L, a, "vsg_lens_transforms")) return -1;
+  memcpy(out, L->h_out, sizeof(VSTransform) * L->nframes);
   if (residual) memcpy(residual, L->h_r, sizeof(double) * L->nframes);
   if (status) memcpy(status, L->h_status, sizeof(int) * L->nframes);
   return 0;
