@@ -8,9 +8,10 @@
  * "missmatch" array of the reference (the last error of every enabled field,
  * -1 for disabled ones) and a compacted, ascending list of the enabled
  * fields.  A quality evaluation (calcTransformQuality, :139-171) computes
- * the fields' errors across the lanes, then the wave adds them in field
- * order from registers -- the reference's loop calls transform_vec_double in
- * another translation unit, so its sum is strictly sequential.  The
+ * the fields' errors across the lanes, then every lane adds them in field
+ * order, reading each 64-field chunk back from LDS -- the reference's loop
+ * calls transform_vec_double in another translation unit, so its sum is
+ * strictly sequential.  The
  * gradient descent (vsGradientDescent, :288-328) is scalar code that every
  * lane runs identically.  Every double expression is evaluated in the order
  * the reference's -ffast-math x86 build uses (the host restatement in the
@@ -44,6 +45,7 @@ struct M2TArgs {
 
 struct Frame {
   const short4* lm;
+  double* wsum;     /* [64]: one chunk of values on their way to an ordered sum */
   double* miss;     /* [n] */
   double* ecomp;    /* [n]: the match values, for the first disable pass */
   int* eidx;        /* [n]: the enabled fields, ascending */
@@ -67,12 +69,35 @@ __device__ void compact(Frame& F) {
   __syncthreads();
 }
 
-/* lane l's double as a wave-uniform value */
-__device__ __forceinline__ double lane_value(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+/* acc plus the wave's 64 values, one after another in lane order (lane 0
+   first), the same in every lane: the values pass through LDS and every lane
+   reads all of them back (broadcast reads) before the dependent adds */
+__device__ __forceinline__ double ordered_sum(double acc, double v, double* buf) {
+  buf[threadIdx.x] = v;
+  __syncthreads();
+  const double2* b = reinterpret_cast<const double2*>(buf);
+#pragma unroll
+  for (int l = 0; l < 32; ++l) {
+    const double2 p = b[l];
+    acc += p.x;
+    acc += p.y;
+  }
+  __syncthreads();   /* every lane has read buf before it is written again */
+  return acc;
+}
+
+/* the same for two interleaved sums: even lanes into even, odd lanes into odd */
+__device__ __forceinline__ void ordered_pair_sums(double& even, double& odd, double v, double* buf) {
+  buf[threadIdx.x] = v;
+  __syncthreads();
+  const double2* b = reinterpret_cast<const double2*>(buf);
+#pragma unroll
+  for (int l = 0; l < 32; ++l) {
+    const double2 p = b[l];
+    even += p.x;
+    odd += p.y;
+  }
+  __syncthreads();
 }
 
 /* the error sum of calcTransformQuality under the rotational model
@@ -118,16 +143,15 @@ __device__ double quality_fov_sum(Frame& F, const double p[4]) {
       e = dy * dy + dx * dx;
       F.miss[i] = e;
     }
-#pragma unroll
-    for (int l = 0; l < 64; ++l) err += lane_value(e, l);
+    err = ordered_sum(err, e, F.wsum);
   }
   return err;
 }
 
 /* calcTransformQuality for params p: updates miss[] of the enabled fields.
    Lane l computes the error of enabled field j0 + l of each 64-field chunk;
-   the wave then adds the chunk's 64 errors in field order from registers
-   (lanes past the last field hold +0.0, which leaves a sum of non-negative
+   every lane then adds the chunk's 64 errors in field order (ordered_sum;
+   lanes past the last field hold +0.0, which leaves a sum of non-negative
    terms unchanged), so the total is the reference's sequential sum. */
 __device__ double quality(Frame& F, const double p[4]) {
   const double reg = fabs(p[2]) * 0.2 + fabs(p[3]) * 0.002;
@@ -154,8 +178,7 @@ __device__ double quality(Frame& F, const double p[4]) {
       e = dy * dy + dx * dx;
       F.miss[i] = e;
     }
-#pragma unroll
-    for (int l = 0; l < 64; ++l) err += lane_value(e, l);
+    err = ordered_sum(err, e, F.wsum);
   }
   return reg + (F.n > 0 ? err / (double)(F.ne + 1) : 0.0);
 }
@@ -163,7 +186,7 @@ __device__ double quality(Frame& F, const double p[4]) {
 /* disableFields (:180-211) of q against mask = miss; q may be miss itself.
    mean: two interleaved partial sums over the pairs (even and odd indices),
    added, then the odd tail; stddev: one sequential sum -- both taken in
-   index order from registers as in quality(), with +0.0 past the end */
+   index order as in quality(), with +0.0 past the end */
 __device__ void disable_fields(Frame& F, const double* q, double stddevs) {
   __syncthreads();   /* every lane's miss[] writes visible to every lane */
   const int len = F.n, lane = threadIdx.x, paired = 2 * (len / 2);
@@ -171,11 +194,7 @@ __device__ void disable_fields(Frame& F, const double* q, double stddevs) {
   for (int i0 = 0; i0 < paired; i0 += 64) {
     const int i = i0 + lane;
     const double v = i < paired ? q[i] : 0.0;
-#pragma unroll
-    for (int l = 0; l < 64; l += 2) {
-      even += lane_value(v, l);
-      odd += lane_value(v, l + 1);
-    }
+    ordered_pair_sums(even, odd, v, F.wsum);
   }
   double sum = odd + even;
   if (len & 1) sum += q[len - 1];
@@ -185,8 +204,7 @@ __device__ void disable_fields(Frame& F, const double* q, double stddevs) {
     const int i = i0 + lane;
     const double d = i < len ? q[i] - mu : 0.0;
     const double d2 = d * d;
-#pragma unroll
-    for (int l = 0; l < 64; ++l) ss += lane_value(d2, l);
+    ss = ordered_sum(ss, d2, F.wsum);
   }
   const double thresh = sqrt(ss / (double)len) * stddevs + mu;
   __syncthreads();
@@ -235,7 +253,8 @@ __global__ __launch_bounds__(64) void k_motions_to_transform(M2TArgs a) {
   if (f >= a.nframes) return;
   Frame F;
   F.n = a.count[f];
-  F.miss = reinterpret_cast<double*>(lds);
+  F.wsum = reinterpret_cast<double*>(lds);
+  F.miss = F.wsum + 64;
   F.ecomp = F.miss + a.cap;
   short4* lmS = reinterpret_cast<short4*>(F.ecomp + a.cap);
   F.lm = lmS;
@@ -303,7 +322,7 @@ __global__ __launch_bounds__(64) void k_motions_to_transform(M2TArgs a) {
 }
 
 /* LDS bytes for frames of up to cap motions */
-size_t m2t_lds(int cap) { return (size_t)cap * (2 * sizeof(double) + sizeof(short4) + sizeof(int)); }
+size_t m2t_lds(int cap) { return 64 * sizeof(double) + (size_t)cap * (2 * sizeof(double) + sizeof(short4) + sizeof(int)); }
 
 }  // namespace
 
