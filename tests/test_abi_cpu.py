@@ -36,6 +36,18 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_library_exports_no_cpp_symbols():
+    """the dynamic symbol table holds C names only: the C++ template
+    instantiations of the host code stay local to the library"""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", api.lib_path("libvidstab_amd.so")], capture_output=True, text=True,
+                         check=True).stdout
+    names = [l.split()[-1] for l in out.splitlines() if l.strip()]
+    assert len(names) > 100
+    cxx = [n for n in names if n.startswith("_Z")]
+    assert not cxx, cxx[:5]
+
+
 def test_abi_struct_sizes_match_reference_layout():
     # sizes and offsets of the reference ABI on x86-64
     assert ctypes.sizeof(api.VSFrameInfo) == 28
