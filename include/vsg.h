@@ -260,9 +260,7 @@ VS_API int vsg_multi_wait_fine(VSGMulti* m, int set);
 /* The search plan of a stage (0 coarse, 1 fine) chosen by the autotuner:
    out[7] = J, K, D, CW, workgroup size, workgroups per job, LDS-staged field
    (0/1); tuneMs[6] (may be NULL) = the timed batch search with K = 4, 6, 8
-   reading the field by scalar loads, then staged in LDS, each the faster of
-   its column chunks (CW: the widest, or half of it for 28-word rows; 0 when
-   not timed). */
+   reading the field by scalar loads, then staged in LDS (0 when not timed). */
 VS_API int vsg_get_plan(VSGContext* ctx, int stage, int* out, float* tuneMs);
 
 /* Measured packed-u8 SAD ceiling of `device`: a kernel of independent

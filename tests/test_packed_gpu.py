@@ -71,20 +71,18 @@ def test_packed_batch_matches_reference_fixture(packed_golden, name):
     det.close()
 
 
-@pytest.mark.parametrize("narrow", [0, 1])
 @pytest.mark.parametrize("lds", [0, 1])
 @pytest.mark.parametrize("K", [4, 6, 8])
 @pytest.mark.parametrize("name", ["p_rgba_step8acc6", "p_rgb24_1080p"])
-def test_every_packed_search_plan_is_bit_identical(packed_golden, monkeypatch, name, K, lds, narrow):
+def test_every_packed_search_plan_is_bit_identical(packed_golden, monkeypatch, name, K, lds):
     monkeypatch.setenv("VIDSTAB_AMD_SEARCH_K", str(K))
     monkeypatch.setenv("VIDSTAB_AMD_SEARCH_LDS", str(lds))
-    monkeypatch.setenv("VIDSTAB_AMD_SEARCH_NARROW", str(narrow))
     case = packed_golden[name]
-    _say(f"[gpu] packed plan K={K} lds={lds} narrow={narrow} fixture {name}")
+    _say(f"[gpu] packed plan K={K} lds={lds} fixture {name}")
     det = _detector(case)
     got = det.detect_batch(_frames(case))
     for t, fr in enumerate(case["frames"]):
-        assert got[t] == golden_motions(fr), f"{name} K={K} lds={lds} narrow={narrow} frame {t}"
+        assert got[t] == golden_motions(fr), f"{name} K={K} lds={lds} frame {t}"
     det.close()
 
 

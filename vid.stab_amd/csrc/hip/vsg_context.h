@@ -19,8 +19,7 @@ struct VSGStage {
   VSGGrid g{};
   int tiles = 0;
   size_t patchOff = 0;
-  float tuneMs[6] = {0, 0, 0, 0, 0, 0};   /* timed plans K = 4, 6, 8 with scalar, then LDS patch, the faster
-                                              column chunk of each (0: not timed) */
+  float tuneMs[6] = {0, 0, 0, 0, 0, 0};   /* timed plans K = 4, 6, 8 with scalar, then LDS patch (0: not timed) */
   double candsPerJob = 0;     /* (2n+1)^2 */
   double refinePerJob = 0;    /* refinement candidates the reference visits */
 };

@@ -101,10 +101,6 @@ hipError_t vsg_launch_select(const VSGSelectArgs& a, int nframes, hipStream_t st
    per lane (4, 6 or 8) on frames of bpp bytes per pixel (1: planar luma);
    returns the workgroups per job */
 int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K, int bpp);
-/* the narrow column chunk of a plan: half the widest (28 -> 14 words), which
-   halves the prev row segment a lane holds (fewer VGPRs, more waves per SIMD)
-   at the cost of loading the segment's overlap twice; 0 where there is none */
-int vsg_narrow_cw(const VSGGrid& g);
 /* bytes of one field's patch: S rows of S*bpp bytes */
 static inline size_t vsg_patch_bytes(const VSGGrid& g) { return (size_t)g.S * g.S * g.bpp; }
 /* modelled instruction count of one job under a plan (for ranking plans) */

@@ -709,7 +709,6 @@ hipError_t launch_grid_cw(const VSGSearchArgs& a, hipStream_t st) {
     case 28: return launch_grid<J, K, D, 28>(a, st);
     case 20: return launch_grid<J, K, D, 20>(a, st);
     case 16: return launch_grid<J, K, D, 16>(a, st);
-    case 14: return launch_grid<J, K, D, 14>(a, st);
     case 12: return launch_grid<J, K, D, 12>(a, st);
     case 8: return launch_grid<J, K, D, 8>(a, st);
     default: return launch_grid<J, K, D, 4>(a, st);
@@ -733,7 +732,6 @@ hipError_t launch_fallback(const VSGSearchArgs& b, const int* list, const unsign
     case 28: hipLaunchKernelGGL((k_sad_fallback<J, D, 28, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
     case 20: hipLaunchKernelGGL((k_sad_fallback<J, D, 20, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
     case 16: hipLaunchKernelGGL((k_sad_fallback<J, D, 16, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
-    case 14: hipLaunchKernelGGL((k_sad_fallback<J, D, 14, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
     case 12: hipLaunchKernelGGL((k_sad_fallback<J, D, 12, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
     case 8: hipLaunchKernelGGL((k_sad_fallback<J, D, 8, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
     default: hipLaunchKernelGGL((k_sad_fallback<J, D, 4, WPT>), dim3(nb), wg, 0, st, b, list, count); break;
@@ -768,20 +766,6 @@ int gcd_int(int x, int y) {
 
 }  // namespace
 
-/* the widest column chunk (<= 28 words) that divides a field row: one prev
-   row segment load then feeds J*K*CW v_sad_u8 */
-static int widest_cw(int rowWords) {
-  static const int kCW[] = {28, 20, 16, 12, 8, 4};
-  for (int cw : kCW)
-    if (rowWords % cw == 0) return cw;
-  return 4;
-}
-
-int vsg_narrow_cw(const VSGGrid& g) {
-  const int rowWords = g.S * g.bpp / 4;
-  return widest_cw(rowWords) == 28 && rowWords % 14 == 0 ? 14 : 0;
-}
-
 int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K, int bpp) {
   g->S = S;
   g->step = step;
@@ -802,8 +786,16 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K, int bpp) {
   const int D = lcm4 / 4;
   g->K = (K == 6 || K == 8) ? K : 4;
   g->lds = 0;
+  /* the widest column chunk (<= 28 words) that divides a field row: one prev
+     row segment load then feeds J*K*CW v_sad_u8 */
   const int rowWords = S * bpp / 4;
-  g->CW = widest_cw(rowWords);
+  static const int kCW[] = {28, 20, 16, 12, 8, 4};
+  g->CW = 4;
+  for (int cw : kCW)
+    if (rowWords % cw == 0) {
+      g->CW = cw;
+      break;
+    }
   if (D == 1 || D == 3) {
     g->J = 4;
     g->D = D;
@@ -882,7 +874,6 @@ VSGSearchArgs vsg_queue_args(const VSGSearchArgs& a, int wg, int K) {
   b.g.V = (b.g.G + K - 1) / K;
   b.g.wg = wg;
   b.g.lds = 0;
-  b.g.CW = widest_cw(b.g.S * b.g.bpp / 4);   /* the queue's variants take the widest chunk */
   b.tiles = (b.g.U * b.g.V + wg - 1) / wg;
   return b;
 }

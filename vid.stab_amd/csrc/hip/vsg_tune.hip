@@ -4,10 +4,7 @@
  * A plan fixes how a job's (2n+1)^2 candidate grid is spread over lanes:
  * K candidate rows per lane (4, 6 or 8; J columns follow from the step)
  * and with it the workgroup size and workgroups per job; the column chunk CW
- * follows from the field size: the widest that divides a field row, or, for
- * rows of 28 words, half of it (a lane then holds half the prev row segment:
- * fewer VGPRs and more waves per SIMD, for a second load of the segment's
- * overlap).  For fields up to 128 px a plan may also
+ * follows from the field size.  For fields up to 128 px a plan may also
  * stage the field in LDS (read by broadcast into a VGPR operand) instead of
  * reading it with scalar loads (SGPR operand), which trades scalar-cache
  * misses for LDS reads.  Every plan computes the same
@@ -17,9 +14,7 @@
  * buffers) and keeps the fastest; results are cached per device and stage
  * geometry for the life of the process.
  *
- *   VIDSTAB_AMD_SEARCH_K=4|6|8   force a plan (tests, sweeps); with it
- *   VIDSTAB_AMD_SEARCH_LDS=0|1   the field's source and
- *   VIDSTAB_AMD_SEARCH_NARROW=0|1  the column chunk
+ *   VIDSTAB_AMD_SEARCH_K=4|6|8   force a plan (tests, sweeps)
  *   VIDSTAB_AMD_AUTOTUNE=0       no timing: the cost model's plan
  */
 #include <stdlib.h>
@@ -38,7 +33,7 @@ constexpr int kPlans[3] = {4, 6, 8};
 
 typedef std::tuple<int, int, int, int, int, int, int> TuneKey;   /* device, S, step, maxShift, B, nf, bpp */
 std::mutex g_tuneMu;
-std::map<TuneKey, int> g_tuned;   /* K + 16 * lds + 32 * narrow */
+std::map<TuneKey, int> g_tuned;   /* K + 16 * lds */
 constexpr int kMaxLdsPatch = 128 * 128 * 3;   /* bytes of a field staged in LDS */
 
 int env_int(const char* name, int def) {
@@ -53,20 +48,13 @@ int env_int(const char* name, int def) {
     if (e_ != hipSuccess) return vsg_fail(#call, e_);     \
   } while (0)
 
-/* the stage's grid under plan (K, lds, narrow); returns the workgroups per job */
-int plan_of(const VSGStage& S, int bpp, int K, int lds, int narrow, VSGGrid* g) {
-  const int tiles = vsg_plan_grid(g, S.S, S.step, S.maxShift, K, bpp);
-  g->lds = lds;
-  if (narrow && vsg_narrow_cw(*g)) g->CW = vsg_narrow_cw(*g);
-  return tiles;
-}
-
-/* times plan (K, lds, narrow) on stage k (ms, best of 2 after a warm-up) */
-int time_plan(VSGContext* c, int k, int K, int lds, int narrow, const VSGJob* dJobs, int njobs, float* ms) {
+/* times plan (K, lds) on stage k (ms, best of 2 after a warm-up) */
+int time_plan(VSGContext* c, int k, int K, int lds, const VSGJob* dJobs, int njobs, float* ms) {
   VSGStage& S = c->st[k];
   VSGSet& s = c->set[0];
   VSGStage trial = S;
-  trial.tiles = plan_of(S, c->bpp, K, lds, narrow, &trial.g);
+  trial.tiles = vsg_plan_grid(&trial.g, S.S, S.step, S.maxShift, K, c->bpp);
+  trial.g.lds = lds;
   VSGSearchArgs a = vsg_ctx_search_args(c, k, dJobs, njobs, s.d_best, s.d_done, k == 0 ? s.d_resC : s.d_resF);
   a.g = trial.g;
   a.tiles = trial.tiles;
@@ -113,19 +101,14 @@ int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift, int bpp) {
 int vsg_ctx_tune(VSGContext* c) {
   const int forced = env_int("VIDSTAB_AMD_SEARCH_K", 0);
   const int forcedLds = env_int("VIDSTAB_AMD_SEARCH_LDS", -1);
-  const int forcedNarrow = env_int("VIDSTAB_AMD_SEARCH_NARROW", -1);
   const bool timing = env_int("VIDSTAB_AMD_AUTOTUNE", 1) != 0;
   for (int k = 0; k < 2; ++k) {
     VSGStage& S = c->st[k];
-    int K = 0, lds = 0, narrow = 0;
+    int K = 0, lds = 0;
     const bool ldsOk = S.S <= 128 && (size_t)S.S * S.S * c->bpp <= (size_t)kMaxLdsPatch;
-    VSGGrid probe;
-    plan_of(S, c->bpp, 4, 0, 0, &probe);
-    const bool narrowOk = vsg_narrow_cw(probe) != 0;
     if (forced == 4 || forced == 6 || forced == 8) {
       K = forced;
       lds = forcedLds == 1 && ldsOk;
-      narrow = forcedNarrow == 1 && narrowOk;
     } else if (timing) {
       const TuneKey key(c->device, S.S, S.step, S.maxShift, c->B, S.nf, c->bpp);
       {
@@ -133,8 +116,7 @@ int vsg_ctx_tune(VSGContext* c) {
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) {
           K = it->second & 15;
-          lds = (it->second >> 4) & 1;
-          narrow = (it->second >> 5) & 1;
+          lds = it->second >> 4;
         }
       }
       if (K == 0) {
@@ -156,27 +138,25 @@ int vsg_ctx_tune(VSGContext* c) {
         for (int l = 0; l < 2; ++l) {
           if (l == 1 && (!ldsOk || forcedLds == 0)) break;
           if (l == 0 && forcedLds == 1 && ldsOk) continue;
-          for (int q = 0; q < 3; ++q)
-            for (int w = 0; w < 2; ++w) {
-              if (w == 1 && (!narrowOk || forcedNarrow == 0)) break;
-              if (w == 0 && forcedNarrow == 1 && narrowOk) continue;
-              float ms = 0;
-              if (time_plan(c, k, kPlans[q], l, w, dJobs, njobs, &ms) != 0) return -1;
-              float& slot = S.tuneMs[3 * l + q];
-              if (slot == 0 || ms < slot) slot = ms;
-              if (K == 0 || ms < bestMs) {
-                bestMs = ms;
-                K = kPlans[q];
-                lds = l;
-                narrow = w;
-              }
+          for (int q = 0; q < 3; ++q) {
+            float ms = 0;
+            if (time_plan(c, k, kPlans[q], l, dJobs, njobs, &ms) != 0) return -1;
+            S.tuneMs[3 * l + q] = ms;
+            if (K == 0 || ms < bestMs) {
+              bestMs = ms;
+              K = kPlans[q];
+              lds = l;
             }
+          }
         }
         std::lock_guard<std::mutex> lk(g_tuneMu);
-        g_tuned[key] = K + 16 * lds + 32 * narrow;
+        g_tuned[key] = K + 16 * lds;
       }
     }
-    if (K != 0) S.tiles = plan_of(S, c->bpp, K, lds, narrow, &S.g);
+    if (K != 0) {
+      S.tiles = vsg_plan_grid(&S.g, S.S, S.step, S.maxShift, K, c->bpp);
+      S.g.lds = lds;
+    }
   }
   return 0;
 }
