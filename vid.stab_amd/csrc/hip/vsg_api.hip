@@ -177,17 +177,29 @@ static hipError_t hmap(T** dev, T* host) {
   return hipHostGetDevicePointer(reinterpret_cast<void**>(dev), host, 0);
 }
 
-/* small batches (fewer than kUploadAsideMin frames: a single frame per
-   vsMotionDetection call) skip the copies across PCIe: their host frames go
-   up by a kernel, their descriptors and fine jobs are read and their
-   selection and search results written by the kernels in pinned memory.
-   VIDSTAB_AMD_ZERO_COPY=0 keeps the copies. */
+/* batches of up to zero_copy_max_batch() frames skip the small copies
+   across PCIe: their descriptors and fine jobs are read and their selection
+   and search results written by the kernels in pinned host memory, which
+   saves the copy launches and the gaps between them at every batch
+   boundary.  Host frames of small batches (fewer than kUploadAsideMin: a
+   single frame per vsMotionDetection call) also go up by a kernel on the
+   compute queue; those of larger batches by DMA on a stream beside it.
+   VIDSTAB_AMD_ZERO_COPY=0 keeps every copy; VIDSTAB_AMD_ZERO_COPY_MAX_BATCH
+   sets the largest batch that skips them (default: all). */
 static bool zero_copy_enabled() {
   static const bool on = [] {
     const char* e = getenv("VIDSTAB_AMD_ZERO_COPY");
     return !(e && atoi(e) == 0);
   }();
   return on;
+}
+
+static int zero_copy_max_batch() {
+  static const int n = [] {
+    const char* e = getenv("VIDSTAB_AMD_ZERO_COPY_MAX_BATCH");
+    return e && *e ? atoi(e) : 1 << 30;
+  }();
+  return n;
 }
 
 int vsg_ctx_init(VSGContext* c, int device, int width, int height, int bpp, int blurSize, const VSGStageDesc* coarse,
@@ -527,7 +539,10 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   /* host frames of a large batch go up on a stream of their own, beside the
      compute stream's earlier batches (the set's raw planes are free: its
      previous batch completed above); a few frames go in order */
-  s.zc = zero_copy_enabled() && n < kUploadAsideMin && !c->learned && !s.borrowed;
+  s.zc = zero_copy_enabled() && n <= zero_copy_max_batch() && !c->learned && !s.borrowed;
+  /* the frames of a zero-copy batch of fewer than kUploadAsideMin frames go
+     up by the copy kernel */
+  const bool kernelUpload = s.zc && n < kUploadAsideMin;
   hipStream_t up = stream;
   if (onDevice != VSG_FRAMES_DEVICE && n >= kUploadAsideMin) {
     if (!c->upStream) VSG_CHECK(hipStreamCreateWithFlags(&c->upStream, hipStreamNonBlocking));
@@ -542,7 +557,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
       const size_t plane = (size_t)c->pitch * c->H;
       if (!s.h_stage) return vsg_fail("vsg_submit: frames not staged", hipSuccess);
       uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * plane;
-      if (!s.zc) VSG_CHECK(hipMemcpyAsync(dst, s.h_stage + plane * i, plane, hipMemcpyHostToDevice, up));
+      if (!kernelUpload) VSG_CHECK(hipMemcpyAsync(dst, s.h_stage + plane * i, plane, hipMemcpyHostToDevice, up));
       s.h_src[i].p = dst;
       s.h_src[i].pitch = c->pitch;
     } else {
@@ -560,7 +575,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
         for (int y = 0; y < c->H; ++y) memcpy(stg + (size_t)y * c->pitch, luma[i] + (size_t)y * linesize[i], c->Wb);
       }
       uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * plane;
-      if (!s.zc) VSG_CHECK(hipMemcpyAsync(dst, stg, plane, hipMemcpyHostToDevice, up));
+      if (!kernelUpload) VSG_CHECK(hipMemcpyAsync(dst, stg, plane, hipMemcpyHostToDevice, up));
       s.h_src[i].p = dst;
       s.h_src[i].pitch = c->pitch;
     }
@@ -572,7 +587,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
     VSG_CHECK(hipEventRecord(s.uploaded, up));
     VSG_CHECK(hipStreamWaitEvent(stream, s.uploaded, 0));
   }
-  if (s.zc && onDevice != VSG_FRAMES_DEVICE) {
+  if (kernelUpload && onDevice != VSG_FRAMES_DEVICE) {
     /* the staged planes are consecutive, as are the batch's raw slots */
     const size_t plane = (size_t)c->pitch * c->H;
     VSG_CHECK(vsg_launch_upload(s.m_stage, c->d_raw + (size_t)slot0 * plane, plane * n, stream));
