@@ -13,13 +13,14 @@
  *   minmax[3B]    8-bit (min, max) per field (packed: per colour channel)
  *   per set       selection lists, coarse/fine job lists, best[] keys, results
  *
- * Each set has its own stream, so one batch's search can fill the tail of the
- * previous batch's kernels.  Cross-set hazards are ordered with events: a
- * batch's coarse search waits for the previous batch's planesReady (which in
- * turn waited for the one before, so the tripod plane is covered), and a set
- * is overwritten only after the batch that followed its previous use has
- * finished its latest search (it read the set's last plane as its previous
- * frame).
+ * The sets share one compute stream by default, which orders every batch's
+ * stages by submission.  With a stream per set (VIDSTAB_AMD_STREAMS=3, or
+ * streams handed in by the multi-clip form) cross-set hazards are ordered
+ * with events: a batch's coarse search waits for the previous batch's
+ * planesReady (which in turn waited for the one before, so the tripod plane
+ * is covered), and a set is overwritten only after the batch that followed
+ * its previous use has finished its latest search (it read the set's last
+ * plane as its previous frame).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -224,6 +225,8 @@ int vsg_ctx_init(VSGContext* c, int device, int width, int height, int bpp, int 
     else
       VSG_CHECK(hipStreamCreateWithFlags(&c->set[k].stream, hipStreamNonBlocking));
   }
+  c->oneStream = true;
+  for (int k = 1; k < VSG_NSETS; ++k) c->oneStream = c->oneStream && c->set[k].stream == c->set[0].stream;
   c->W = width;
   c->H = height;
   c->bpp = bpp;
@@ -535,7 +538,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
   /* the set's planes are overwritten below; the batch that followed the set's
      previous batch reads its last plane as the previous frame, so its latest
      search must be over first */
-  if (s.follower >= 0) VSG_CHECK(hipStreamWaitEvent(stream, c->set[s.follower].searchDone, 0));
+  if (s.follower >= 0 && !c->oneStream) VSG_CHECK(hipStreamWaitEvent(stream, c->set[s.follower].searchDone, 0));
   /* host frames of a large batch go up on a stream of their own, beside the
      compute stream's earlier batches (the set's raw planes are free: its
      previous batch completed above); a few frames go in order */
@@ -615,12 +618,14 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
                              hipMemcpyDeviceToDevice, stream));
   }
   /* planes of earlier batches (previous frames, the tripod plane) come from
-     other sets' streams: planesReady chains them, so one wait covers all */
+     other sets' streams: planesReady chains them, so one wait covers all.
+     On one shared stream the stream order already covers them, and the
+     cross-set events are neither recorded nor waited for. */
   if (c->lastSet >= 0 && c->lastSet != set) {
-    VSG_CHECK(hipStreamWaitEvent(stream, c->set[c->lastSet].planesReady, 0));
+    if (!c->oneStream) VSG_CHECK(hipStreamWaitEvent(stream, c->set[c->lastSet].planesReady, 0));
     c->set[c->lastSet].follower = set;
   }
-  VSG_CHECK(hipEventRecord(s.planesReady, stream));
+  if (!c->oneStream) VSG_CHECK(hipEventRecord(s.planesReady, stream));
   c->lastSet = set;
   s.follower = -1;
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID0], stream));
@@ -632,7 +637,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
     VSG_CHECK(vsg_launch_search_any(ga, stream));
   }
   if (s.prof) VSG_CHECK(hipEventRecord(s.ev[VSG_EV_GRID1], stream));
-  VSG_CHECK(hipEventRecord(s.searchDone, stream));
+  if (!c->oneStream) VSG_CHECK(hipEventRecord(s.searchDone, stream));
   if (!s.zc && vsg_ctx_enqueue_coarse_readback(c, set, n) != 0) return -1;
   VSG_CHECK(hipEventRecord(s.done, stream));
   s.n = n;
@@ -696,7 +701,7 @@ extern "C" int vsg_submit_fine(VSGContext* c, int set, int njobs, const VSGJob* 
     VSG_CHECK(vsg_launch_search_any(fa, s.stream));
   }
   if (s.profF) VSG_CHECK(hipEventRecord(s.evf[VSG_EVF_GRID1], s.stream));
-  VSG_CHECK(hipEventRecord(s.searchDone, s.stream));
+  if (!c->oneStream) VSG_CHECK(hipEventRecord(s.searchDone, s.stream));
   if (!s.zc)
     VSG_CHECK(hipMemcpyAsync(s.h_resF, s.d_resF, sizeof(VSGResult) * njobs, hipMemcpyDeviceToHost, s.stream));
   VSG_CHECK(hipEventRecord(s.doneF, s.stream));

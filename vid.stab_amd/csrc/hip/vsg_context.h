@@ -86,6 +86,7 @@ struct VSGContext {
   bool ownStreams = true;
   hipStream_t upStream = nullptr;   /* host-frame uploads of large batches, beside the compute stream */
   int lastSet = -1;       /* set of the latest batch */
+  bool oneStream = true;  /* every set runs on the same stream: stream order alone orders the batches */
   int W = 0, H = 0, pitch = 0, rowsAlloc = 0;
   int bpp = 1;            /* bytes per pixel: 1 planar luma, 3 or 4 packed */
   int Wb = 0;             /* bytes per row of a plane: W * bpp */
