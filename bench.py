@@ -235,6 +235,7 @@ def measure_plain_4k_step4(steps=2, warmup=1):
     t0 = time.perf_counter()
     for _ in range(steps):
         det.detect_device(ptrs, [s.width] * n)
+    api.release_wait()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     L.vsg_profile_enable(0)
@@ -404,6 +405,7 @@ def main():
     t0 = time.perf_counter()
     for s in range(args.steps):
         det.detect_device(ptrs, ls)
+    api.release_wait()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     L.vsg_profile_enable(0)
@@ -468,6 +470,7 @@ def main():
         det.detect_batch(frames, keep=False)
         t0 = time.perf_counter()
         det.detect_batch(frames, keep=False)
+        api.release_wait()
         host_fps = nfr / (time.perf_counter() - t0)
         det.close()
         log(f"host frames (PCIe-inclusive): {host_fps:.1f} frames/s")
@@ -556,6 +559,7 @@ def run_multi(args):
     t0 = time.perf_counter()
     for _ in range(args.steps):
         api.detect_multi(clips, keep=False)
+    api.release_wait()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     multi_fps = frames * args.steps / (t1 - t0)
@@ -571,6 +575,7 @@ def run_multi(args):
         torch.cuda.synchronize()
         tc = time.perf_counter()
         det.detect_device(cl["ptrs"], [cl["linesize"]] * len(cl["ptrs"]))
+        api.release_wait()
         torch.cuda.synchronize()
         seq_s += time.perf_counter() - tc
         det.close()
@@ -623,6 +628,7 @@ def measure_small_clips(steps, warmup, nclips=64, width=640, height=360, nframes
     t0 = time.perf_counter()
     for _ in range(steps):
         api.detect_multi(clips, keep=False)
+    api.release_wait()
     torch.cuda.synchronize()
     multi_fps = frames * steps / (time.perf_counter() - t0)
     dets = [api.Detector(cl["width"], cl["height"], shakiness=cl["shakiness"], accuracy=cl["accuracy"],
@@ -633,6 +639,7 @@ def measure_small_clips(steps, warmup, nclips=64, width=640, height=360, nframes
     t0 = time.perf_counter()
     for det, cl in zip(dets, clips):
         det.detect_device(cl["ptrs"], [cl["linesize"]] * nframes)
+    api.release_wait()
     torch.cuda.synchronize()
     seq_fps = frames / (time.perf_counter() - t0)
     for det in dets:
@@ -676,6 +683,7 @@ def measure_forms_4k(nframes, steps, warmup, weights, clip_index=0):
         t0 = time.perf_counter()
         for _ in range(steps):
             det.detect_device(ptrs, ls)
+        api.release_wait()
         torch.cuda.synchronize()
         return nfr * steps / (time.perf_counter() - t0)
 

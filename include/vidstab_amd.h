@@ -70,8 +70,15 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
                                const int* nframes, const uint8_t* const* const* lumaDev, const int* linesize,
                                LocalMotions* const* motions);
 
-/* vs_vector_del on motions[0..n-1]: the elements go back through vs_free */
+/* vs_vector_del on motions[0..n-1]: the vectors are empty when it returns
+   (and may be reused); the elements of arrays of 16 or more go back through
+   vs_free on a library thread while the caller carries on (one vs_free per
+   LocalMotion, about 300 per 1080p frame).  At most four such releases are
+   pending at a time; vs_free must not change while any is. */
 VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n);
+/* returns once every release vsLocalMotionsArrayDel handed to the library
+   thread is done (every element returned to vs_free) */
+VS_API void vsLocalMotionsReleaseWait(void);
 
 #ifdef __cplusplus
 }

@@ -781,18 +781,6 @@ VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions, co
   return detect_frames(md, motions, lumaDev, linesize, 1, NULL, n);
 }
 
-/* bulk release on a library-wide pool: the vectors are independent */
-static VSPool* g_delPool = NULL;
-static pthread_once_t g_delOnce = PTHREAD_ONCE_INIT;
-static void del_pool_init(void) {
-  const long cores = sysconf(_SC_NPROCESSORS_ONLN);
-  g_delPool = vs_pool_create(VS_MIN(8, VS_MAX(1, (int)(cores * 0.8))));
-}
-static void del_fn(void* ctx, int i) {
-  LocalMotions* m = (LocalMotions*)ctx;
-  for (int k = 0; k < 16; k++) vs_vector_del(&m[16 * i + k]);
-}
-
 VS_API int vsMotionDetectSetProposal(VSMotionDetect* md, const char* weightsFile) {
   MDState* s = state_find(md);
   if (!s || !weightsFile) return VS_ERROR;
@@ -849,18 +837,6 @@ VS_API int vsMotionDetectProposalStats(const VSMotionDetect* md, long long* fiel
     }
   }
   return VS_OK;
-}
-
-VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n) {
-  if (!motions || n <= 0) return;
-  const int full = n / 16;
-  if (full >= 2) {
-    pthread_once(&g_delOnce, del_pool_init);
-    vs_pool_run(g_delPool, full, del_fn, motions);
-  } else {
-    for (int i = 0; i < 16 * full; i++) vs_vector_del(&motions[i]);
-  }
-  for (int i = 16 * full; i < n; i++) vs_vector_del(&motions[i]);
 }
 
 VS_API int vsMotionDetectGetBlurred(const VSMotionDetect* md, uint8_t* dst, int dstStride) {

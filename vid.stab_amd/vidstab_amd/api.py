@@ -164,6 +164,8 @@ def lib():
                                           c_void_p]
         L.vsMotionDetectMulti.restype = c_int
         L.vsLocalMotionsArrayDel.restype = None
+        L.vsLocalMotionsReleaseWait.argtypes = []
+        L.vsLocalMotionsReleaseWait.restype = None
         L.vsSimpleMotionsToTransform.argtypes = [VSFrameInfo, ctypes.c_char_p, ctypes.POINTER(VSVector)]
         L.vsSimpleMotionsToTransform.restype = VSTransform
         L.vsMotionsToTransformBatch.argtypes = [ctypes.POINTER(VSFrameInfo), ctypes.POINTER(VSVector), c_int, c_int,
@@ -489,6 +491,12 @@ def localmotions_to_transforms(width, height, frames, fov=0.0, estimate=1, simpl
     if trans.ts:
         L.vs_free_fn(ctypes.cast(trans.ts, c_void_p))
     return out, td.lensK
+
+
+def release_wait():
+    """vsLocalMotionsReleaseWait: returns once every LocalMotions array
+    released by earlier calls has gone back through vs_free"""
+    lib().vsLocalMotionsReleaseWait()
 
 
 def estimate_lens(width, height, frames, focal=0.0):
