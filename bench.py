@@ -326,8 +326,8 @@ def measure_second_pass_default(spec, frames, clip):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2,
                     help="workload: 2 = 1080p, 3 = 4K, 4 = 64 clips of uneven size (batched multi-clip "
                          "form), 5 = 4K motion blur, plain vs learned-proposal form")
