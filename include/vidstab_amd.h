@@ -26,8 +26,10 @@ extern "C" {
 VS_API int vsMotionDetectionBatch(VSMotionDetect* md, LocalMotions* motions, VSFrame* frames, int n);
 
 /* as vsMotionDetectionBatch, but the luma planes are already in device memory
-   of the detector's GPU (lumaDev[i], linesize[i] bytes per row).  No show
-   overlays are drawn into device frames. */
+   of the detector's GPU (lumaDev[i], linesize[i] bytes per row).  With
+   show > 0 the overlays are drawn into those device frames, as
+   vsMotionDetection draws them into a host frame (the frames must then be
+   writable). */
 VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions,
                                    const uint8_t* const* lumaDev, const int* linesize, int n);
 
@@ -61,7 +63,9 @@ VS_API int vsMotionDetectProposalStageStats(const VSMotionDetect* md, int stage,
    info fis[c], planar; virtualTripod as for vsMotionDetection) has nframes[c] luma planes
    lumaDev[c][0..nframes[c]-1] with linesize[c] bytes per row; motions[c][t]
    receives exactly what the t-th of nframes[c] vsMotionDetection calls on a
-   fresh detector for that clip would return.  Up to VIDSTAB_AMD_MULTI_BATCH
+   fresh detector for that clip would return; with confs[c].show > 0 the
+   overlays are drawn into the clip's device frames, as those calls would
+   draw them.  Up to VIDSTAB_AMD_MULTI_BATCH
    (default 16) frames of every clip go through each device batch.  The
    device state of a call is kept for the next call over clips of the same
    shapes (frame sizes, settings), so repeated calls pay the set-up once;

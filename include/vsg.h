@@ -128,6 +128,12 @@ VS_API int vsg_wait_fine(VSGContext* ctx, int set, const VSGResult** results);
 /* Copies a blurred plane (global slot; < 0: the stored reference) to the host. */
 VS_API int vsg_read_blurred(VSGContext* ctx, int slot, uint8_t* dst, int dstStride);
 
+/* Copies rows x rowBytes bytes between host and device memory of `device`,
+   either way (synchronous; the show overlays of device frames go through
+   the host with it). */
+VS_API int vsg_copy_plane(int device, uint8_t* dst, int dstPitch, const uint8_t* src, int srcPitch, int rowBytes,
+                          int rows);
+
 VS_API const char* vsg_last_error(void);
 
 /* Optional per-kernel timing: HIP events on the context's stream around each

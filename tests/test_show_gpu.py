@@ -1,7 +1,8 @@
 """GPU detect with show = 1 / 2 through the vid.stab C API against the
 reference's show fixtures (tests/golden/show_golden.json): the caller's frame
 after each call (planar luma, packed RGBA / BGR24) and every LocalMotion, bit
-for bit, per frame and through the batch entry point."""
+for bit, per frame, through the batch entry point, with the frames in device
+memory (vsMotionDetectionDevice) and through the multi-clip form."""
 import json
 import os
 import sys
@@ -72,3 +73,57 @@ def test_show_batch_matches_reference_fixture(show_golden, name):
         assert "%016x" % hash_plane(frames[t], wb) == fr["drawn"], f"{name} overlay frame {t}"
         assert got[t] == golden_motions(fr), f"{name} frame {t}"
     det.close()
+
+
+def _to_device(frames):
+    """the frames as one device block, each frame's rows at its own stride"""
+    import numpy as np
+    import torch
+    host = np.stack(frames)
+    dev = torch.from_numpy(host).to("cuda:0")
+    torch.cuda.synchronize()
+    step = host.shape[1] * host.shape[2]
+    return dev, [dev.data_ptr() + i * step for i in range(len(frames))], host.shape[2]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_show_device_frames_match_reference_fixture(show_golden, name):
+    """the overlays drawn into device-resident frames, two device batches"""
+    import torch
+    case = show_golden[name]
+    wb = case["width"] * case["bpp"]
+    _say(f"[gpu] show device frames {name} show={case['show']} bpp {case['bpp']}")
+    frames = _frames(case)
+    dev, ptrs, stride = _to_device(frames)
+    det = _detector(case)
+    k = max(1, len(frames) // 2)
+    got = det.detect_device(ptrs[:k], [stride] * k, keep=True) + \
+        det.detect_device(ptrs[k:], [stride] * (len(frames) - k), keep=True)
+    det.close()
+    torch.cuda.synchronize()
+    drawn = dev.cpu().numpy()
+    for t, fr in enumerate(case["frames"]):
+        assert "%016x" % hash_plane(drawn[t], wb) == fr["drawn"], f"{name} overlay frame {t}"
+        assert got[t] == golden_motions(fr), f"{name} frame {t}"
+
+
+def test_show_multi_clip_matches_reference_fixture(show_golden):
+    """the planar show cases as clips of one vsMotionDetectMulti call"""
+    import torch
+    names = ["s_c1_show1", "s_c1_show2", "s_c1_step3_show2", "s_odd_show2", "s_1080p_show2"]
+    _say(f"[gpu] show multi-clip {names}")
+    clips, devs = [], []
+    for name in names:
+        case = show_golden[name]
+        dev, ptrs, stride = _to_device(_frames(case))
+        devs.append(dev)
+        clips.append(dict(width=case["width"], height=case["height"], ptrs=ptrs, linesize=stride,
+                          show=case["show"], **case_params(case)))
+    got = api.detect_multi(clips)
+    torch.cuda.synchronize()
+    for name, dev, motions in zip(names, devs, got):
+        case = show_golden[name]
+        drawn = dev.cpu().numpy()
+        for t, fr in enumerate(case["frames"]):
+            assert "%016x" % hash_plane(drawn[t], case["width"]) == fr["drawn"], f"{name} overlay frame {t}"
+            assert motions[t] == golden_motions(fr), f"{name} frame {t}"

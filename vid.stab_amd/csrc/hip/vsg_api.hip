@@ -725,6 +725,15 @@ extern "C" int vsg_wait_fine(VSGContext* c, int set, const VSGResult** results) 
   return 0;
 }
 
+extern "C" int vsg_copy_plane(int device, uint8_t* dst, int dstPitch, const uint8_t* src, int srcPitch, int rowBytes,
+                              int rows) {
+  if (!dst || !src || rowBytes <= 0 || rows <= 0 || dstPitch < rowBytes || srcPitch < rowBytes)
+    return vsg_fail("vsg_copy_plane", hipSuccess);
+  VSG_CHECK(hipSetDevice(device));
+  VSG_CHECK(hipMemcpy2D(dst, dstPitch, src, srcPitch, rowBytes, rows, hipMemcpyDefault));
+  return 0;
+}
+
 extern "C" int vsg_read_blurred(VSGContext* c, int slot, uint8_t* dst, int dstStride) {
   if (!c || slot >= VSG_NSETS * c->B || !dst || dstStride < c->Wb) return vsg_fail("vsg_read_blurred", hipSuccess);
   VSG_CHECK(hipSetDevice(c->device));

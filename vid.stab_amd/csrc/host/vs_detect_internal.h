@@ -31,11 +31,12 @@ int vsd_frame_coarse(const VSMotionDetect* md, const VSGBatchView* v, int i, int
 
 /* Frame i: fine results of its njobs jobs -> the cleaned fine motions; with
    md->conf.show > 0 and a frame to draw into (plane 0, linesize bytes a
-   row; null: none), the overlays; then out = coarse ++ fine (the elements of
-   coarse move into out).  Reads md only. */
+   row, in host memory or, for drawDevice >= 0, on that HIP device; null:
+   none), the overlays; then out = coarse ++ fine (the elements of coarse
+   move into out).  Reads md only. */
 void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
                     const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, uint8_t* draw,
-                    int drawLinesize, LocalMotions* out);
+                    int drawLinesize, int drawDevice, LocalMotions* out);
 
 /* the show overlays of one frame (src/motiondetect.c:271-286, 816-952) into
    plane 0 of the caller's frame; may rewrite the field position of motions

@@ -45,6 +45,7 @@ typedef struct Batch {
   int fnum0;                   /* frame number of the batch's first frame */
   uint8_t* draw[MD_MAX_BATCH]; /* show > 0: plane 0 of the caller's frames (null: no overlays) */
   int drawLs[MD_MAX_BATCH];
+  int drawDevice;              /* the HIP device holding the frames drawn into, -1: host frames */
   int prevSlot[MD_MAX_BATCH];
   uint8_t detect[MD_MAX_BATCH];
   LocalMotions* out;
@@ -69,6 +70,7 @@ typedef struct MDState {
   const VSMotionDetect* key;
   struct MDState* next;
   VSGContext* gpu;
+  int device;      /* HIP device of gpu */
   VSPool* pool;    /* per-frame host steps of a batch */
   VSPool* stagePool;   /* copies of a few host frames into pinned staging (at most 4 threads) */
   int maxBatch;
@@ -315,6 +317,7 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
   const int device = env_int("VIDSTAB_AMD_DEVICE", 0);
   VSGStageDesc dc = vsd_stage_desc(&md->fieldscoarse), df = vsd_stage_desc(&md->fieldsfine);
   const int bpp = md->fi.pFormat > PF_PACKED ? md->fi.bytesPerPixel : 1;
+  s->device = device;
   if (vsg_create_fmt(&s->gpu, device, md->fi.width, md->fi.height, bpp, vsd_blur_size(md), &dc, &df,
                      s->maxBatch) != 0) {
     vs_log_error(md->conf.modName, "GPU initialisation failed: %s\n", vsg_last_error());
@@ -445,9 +448,30 @@ int vsd_frame_coarse(const VSMotionDetect* md, const VSGBatchView* v, int i, int
   return nj;
 }
 
+/* the overlays of a frame in device memory: its plane 0 comes down to the
+   host, is drawn by the host-frame code and goes back (vsg_copy_plane,
+   synchronous; the frame's own kernels finished before its fine results came
+   back, and later batches read only their own frames).  The LocalMotions are
+   rewritten as for a host frame even if a copy fails. */
+static void draw_device_frame(const VSMotionDetect* md, int device, uint8_t* plane, int linesize,
+                              LocalMotions* coarse, LocalMotions* fine) {
+  const int rowBytes = md->fi.width * (md->fi.pFormat > PF_PACKED ? md->fi.bytesPerPixel : 1);
+  const int rows = md->fi.height;
+  uint8_t* h = (uint8_t*)calloc((size_t)rowBytes * (size_t)rows, 1);
+  if (!h) {
+    vs_log_error(md->conf.modName, "show: out of host memory for a device frame\n");
+    return;
+  }
+  const int down = vsg_copy_plane(device, h, rowBytes, plane, linesize, rowBytes, rows) == 0;
+  vsd_draw_overlays(md, h, rowBytes, coarse, fine);
+  if (!down || vsg_copy_plane(device, plane, linesize, h, rowBytes, rowBytes, rows) != 0)
+    vs_log_error(md->conf.modName, "show: device frame copy failed: %s\n", vsg_last_error());
+  free(h);
+}
+
 void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, LocalMotions* coarse,
                     const VSGJob* jobs, const int* jobSel, const VSGResult* res, int njobs, uint8_t* draw,
-                    int drawLinesize, LocalMotions* out) {
+                    int drawLinesize, int drawDevice, LocalMotions* out) {
   const VSMotionDetectFields* ff = &md->fieldsfine;
   LocalMotions fine;
   vs_vector_init(&fine, 0);
@@ -466,7 +490,12 @@ void vsd_frame_fine(const VSMotionDetect* md, const VSGBatchView* v, int i, Loca
       if (lm_match_better(&meanMatch, &lm)) vs_vector_append_dup(&fine, &lm, sizeof(LocalMotion));
     }
   }
-  if (draw) vsd_draw_overlays(md, draw, drawLinesize, coarse, &fine);
+  if (draw && md->conf.show > 0) {
+    if (drawDevice >= 0)
+      draw_device_frame(md, drawDevice, draw, drawLinesize, coarse, &fine);
+    else
+      vsd_draw_overlays(md, draw, drawLinesize, coarse, &fine);
+  }
   *out = vs_vector_concat(coarse, &fine);
   vs_vector_fini(coarse);
   vs_vector_fini(&fine);
@@ -544,6 +573,7 @@ static int batch_submit(VSMotionDetect* md, MDState* s, Batch* b, const uint8_t*
     b->prevSlot[i] = tripod >= 1 ? -1 : (i == 0 ? s->lastSlot : b->slot0 + i - 1);
     b->draw[i] = draw ? draw[i] : NULL;
     b->drawLs[i] = linesize[i];
+    b->drawDevice = onDevice ? s->device : -1;
     if (tripod >= 1 && fnum == tripod - 1) refSlot = b->slot0 + i;
   }
   int mode = onDevice ? VSG_FRAMES_DEVICE : VSG_FRAMES_STAGED;
@@ -595,7 +625,7 @@ static void frame_fine_fn(void* ctx, int i) {
   }
   const int j0 = b->jobStart[i];
   vsd_frame_fine(w->md, w->v, i, &b->coarse[i], b->jobs + j0, b->jobSel + j0, w->res + j0,
-                 b->jobStart[i + 1] - j0, b->draw[i], b->drawLs[i], &b->out[i]);
+                 b->jobStart[i + 1] - j0, b->draw[i], b->drawLs[i], b->drawDevice, &b->out[i]);
 }
 
 /* coarse results -> coarse LocalMotions, transform, fine jobs; submits them */
@@ -777,8 +807,9 @@ VS_API int vsMotionDetectionBatch(VSMotionDetect* md, LocalMotions* motions, VSF
 VS_API int vsMotionDetectionDevice(VSMotionDetect* md, LocalMotions* motions, const uint8_t* const* lumaDev,
                                    const int* linesize, int n) {
   if (n == 0) return VS_OK;
-  /* device-resident frames get no overlays: show applies to host frames */
-  return detect_frames(md, motions, lumaDev, linesize, 1, NULL, n);
+  /* show > 0: the overlays are drawn into the device frames, as into a host
+     frame by vsMotionDetection */
+  return detect_frames(md, motions, lumaDev, linesize, 1, md->conf.show > 0 ? (uint8_t* const*)lumaDev : NULL, n);
 }
 
 VS_API int vsMotionDetectSetProposal(VSMotionDetect* md, const char* weightsFile) {

@@ -60,6 +60,7 @@ typedef struct {
 
 typedef struct {
   int nclips, B;
+  int device;            /* the HIP device of the clips' frames */
   MClip* clips;
   VSGMulti* gpu;
   VSPool* pool;
@@ -114,8 +115,11 @@ static void fine_fn(void* ctx, int k) {
     return;
   }
   const int j0 = cb->jobStart[i];
+  /* show > 0: the overlays go into the clip's device frame, as into a host
+     frame by vsMotionDetection */
+  uint8_t* draw = cl->md.conf.show > 0 ? (uint8_t*)cb->luma[i] : NULL;
   vsd_frame_fine(&cl->md, &cb->view, i, &cb->coarse[i], cb->jobs + j0, cb->jobSel + j0, cb->res + j0,
-                 cb->jobStart[i + 1] - j0, NULL, 0, out);
+                 cb->jobStart[i + 1] - j0, draw, w->M->linesize[c], w->M->device, out);
 }
 
 /* next frames of every clip into set `set` */
@@ -422,6 +426,7 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
     maxFrames = VS_MAX(maxFrames, nframes[c]);
   }
   MultiShape shape = {env_int("VIDSTAB_AMD_DEVICE", 0), M.B, nclips, w, h, blur, dc, df};
+  M.device = shape.device;
   if (ok) kept_take(&shape, &M.gpu, &M.pool);
   if (ok && !M.gpu && vsg_multi_create(&M.gpu, shape.device, nclips, w, h, blur, dc, df, M.B) != 0) {
     vs_log_error("vidstab_amd", "GPU multi-clip initialisation failed: %s\n", vsg_last_error());

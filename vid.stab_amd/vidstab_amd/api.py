@@ -328,7 +328,8 @@ def detect_multi(clips, keep=True, mod=b"vidstab_amd"):
 
     clips: sequence of dicts with width, height, shakiness, accuracy,
     stepsize, mincontrast, ptrs (device addresses of the luma planes, one per
-    frame), linesize and optionally tripod (virtualTripod, default 0).
+    frame), linesize and optionally tripod (virtualTripod, default 0) and
+    show (overlays drawn into the device frames, default 0).
     Returns per clip the per-frame LocalMotions
     (tuples, as motions_of) or, with keep=False, the per-frame counts."""
     L = lib()
@@ -347,6 +348,7 @@ def detect_multi(clips, keep=True, mod=b"vidstab_amd"):
         conf.shakiness, conf.accuracy, conf.stepSize = cl["shakiness"], cl["accuracy"], cl["stepsize"]
         conf.contrastThreshold = float(cl["mincontrast"])
         conf.virtualTripod = int(cl.get("tripod", 0))
+        conf.show = int(cl.get("show", 0))
         confs[c] = conf
         nf = len(cl["ptrs"])
         nframes[c] = nf
