@@ -92,6 +92,7 @@ struct VSGMulti {
   VSGMultiSeg* d_segs[2][2][VSG_NSETS] = {};   /* [kind][stage][set] */
   int nseg[2][2] = {};                          /* [kind][stage] */
   VSGQueueTotals totals[2][2] = {};             /* [kind][stage]: units per XCD queue */
+  std::vector<int> direct[2];   /* per stage: clips searched by their own tuned kernel, not a queue */
   unsigned* d_queue = nullptr;                  /* [kind][stage][set][xcd][2] counters */
   Arena ar[VSG_NSETS][AR_COUNT];
   std::vector<int> jobsFilled[VSG_NSETS];   /* per clip: fine entries that may hold jobs */
@@ -234,6 +235,23 @@ static int env_pair(const char* name, int stage, int def) {
    5,053; K 6 / 6 6,013, 4 / 6 5,651, 4 / 8 5,255 */
 static int queue_wg(int stage) { return env_pair("VIDSTAB_AMD_QUEUE_WG", stage, stage == 0 ? 256 : 128); }
 static int queue_k(int stage) { return env_pair("VIDSTAB_AMD_QUEUE_K", stage, 6); }
+/* a clip whose stage launch holds at least this many lanes (B frames' jobs
+   x workgroups per job x workgroup size, under its own tuned plan) keeps
+   the chip busy on its own: it is searched by its own k_sad_search launch
+   (the autotuned plan: the field staged in LDS where that is faster, the
+   units of a frame on one XCD) inside the multi-clip call, and only the
+   smaller clips share the queues.  Measured on config 4's 64 clips (frames/s
+   of the whole call, two runs each): every clip queued 5,973-5,978; direct
+   from 262,144 lanes (4 waves per SIMD) 6,325-6,452, from 131,072
+   6,469-6,510, from 65,536 (one wave per SIMD) 6,505-6,642, from 32,768
+   6,425-6,513 with 64 small clips down from 81,000-94,000 to 62,000.
+   VIDSTAB_AMD_QUEUE_DIRECT_LANES overrides (0: every clip queued). */
+static long long queue_direct_lanes(void) {
+  const char* v = getenv("VIDSTAB_AMD_QUEUE_DIRECT_LANES");
+  if (!v || !*v) return 1LL << 16;
+  const long long n = atoll(v);
+  return n > 0 ? n : (1LL << 62);
+}
 /* VIDSTAB_AMD_QUEUE_SPLIT=0 keeps workgroup-wide units for every job */
 static bool queue_split_on(void) {
   const char* v = getenv("VIDSTAB_AMD_QUEUE_SPLIT");
@@ -272,8 +290,17 @@ static int build_segments(VSGMulti* m, int stage, int set) {
     tot.queues = queues;
     tot.chunk = 1;
   }
+  /* kind 0: wide queue, 1: split queue, 2: its own launch */
   std::vector<int> kindOf(m->ctx.size());
+  m->direct[stage].clear();
   for (int c : m->order[stage]) {
+    const VSGStage& st = m->ctx[c]->st[stage];
+    const long long lanes = (long long)m->B * st.nf * st.tiles * st.g.wg;
+    if (lanes >= queue_direct_lanes()) {
+      kindOf[c] = 2;
+      m->direct[stage].push_back(c);
+      continue;
+    }
     VSGSearchArgs a = vsg_queue_args(vsg_ctx_search_args(m->ctx[c], stage, nullptr, 0, nullptr, nullptr, nullptr),
                                      wg, queue_k(stage));
     kindOf[c] = queue_split_on() && a.g.U * a.g.V <= queue_split_lanes(wg) ? 1 : 0;
@@ -288,6 +315,7 @@ static int build_segments(VSGMulti* m, int stage, int set) {
     int units[2] = {0, 0}, k[2] = {0, 0};
     for (int c : m->order[stage]) {
       const int kind = kindOf[c];
+      if (kind == 2) continue;
       VSGContext* x_ = m->ctx[c];
       VSGSet& s = x_->set[set];
       const int njobs = m->B * x_->st[stage].nf;
@@ -316,10 +344,18 @@ static int build_segments(VSGMulti* m, int stage, int set) {
   return 0;
 }
 
-/* the stage's wide-unit queue, then its split-unit queue (run as two
-   branches of the graph, side by side, they measured the same: config 4
-   5,894 against 5,843 frames/s, 64 small clips 95,251 against 94,705) */
+/* the stage's launches of the clips that fill the chip alone, then its
+   wide-unit queue, then its split-unit queue (the two queues run as two
+   branches of the graph, side by side, measured the same: config 4 5,894
+   against 5,843 frames/s, 64 small clips 95,251 against 94,705) */
 static int launch_queue(VSGMulti* m, int stage, int set) {
+  for (int c : m->direct[stage]) {
+    VSGContext* x = m->ctx[c];
+    VSGSet& s = x->set[set];
+    const VSGSearchArgs a = vsg_ctx_search_args(x, stage, stage == 0 ? s.d_jobsC : s.d_jobsF, m->B * x->st[stage].nf,
+                                                s.d_best, s.d_done, stage == 0 ? s.d_resC : s.d_resF);
+    MCHECK(vsg_launch_search(a, m->stream));
+  }
   for (int kind = 0; kind < 2; ++kind) {
     if (m->nseg[kind][stage] == 0) continue;
     unsigned* q = m->d_queue + 2 * VSG_XCDS * ((kind * 2 + stage) * VSG_NSETS + set);
