@@ -13,8 +13,9 @@
  *
  * At most kMaxPending releases are queued or running at a time: a caller
  * that outruns the release thread waits in vsLocalMotionsArrayDel.
- * vsLocalMotionsReleaseWait returns once every queued release is done, and
- * the queue is drained before the library is unloaded.  Arrays of fewer than
+ * vsLocalMotionsReleaseWait returns once every queued release is done; when
+ * the library is unloaded (or the process exits) the queue is drained and
+ * the release thread ends and is joined.  Arrays of fewer than
  * kAsyncMin vectors, or any when the thread cannot be started, are released
  * in the calling thread.  The hooks (vs_free) must stay the same while
  * releases are pending.
@@ -40,13 +41,16 @@ static pthread_cond_t g_done = PTHREAD_COND_INITIALIZER;     /* a release finish
 static Release* g_head = NULL;
 static Release* g_tail = NULL;
 static int g_pending = 0;   /* releases queued or running */
-static int g_thread = 0;    /* 0: not started, 1: running, -1: could not start */
+static int g_thread = 0;    /* 0: not started, 1: running, -1: could not start or ended */
+static int g_quit = 0;      /* the thread ends once the queue is empty */
+static pthread_t g_tid;
 
 static void* release_main(void* arg) {
   (void)arg;
   pthread_mutex_lock(&g_mu);
   for (;;) {
-    while (!g_head) pthread_cond_wait(&g_queued, &g_mu);
+    while (!g_head && !g_quit) pthread_cond_wait(&g_queued, &g_mu);
+    if (!g_head) break;
     Release* r = g_head;
     g_head = r->next;
     if (!g_head) g_tail = NULL;
@@ -57,23 +61,14 @@ static void* release_main(void* arg) {
     g_pending--;
     pthread_cond_broadcast(&g_done);
   }
+  pthread_mutex_unlock(&g_mu);
   return NULL;
 }
 
-/* with g_mu held: 1 when the release thread runs */
+/* with g_mu held: 1 when the release thread runs and takes new releases */
 static int thread_running(void) {
-  if (g_thread == 0) {
-    pthread_t t;
-    pthread_attr_t at;
-    g_thread = -1;
-    if (pthread_attr_init(&at) == 0) {
-      if (pthread_attr_setdetachstate(&at, PTHREAD_CREATE_DETACHED) == 0 &&
-          pthread_create(&t, &at, release_main, NULL) == 0)
-        g_thread = 1;
-      pthread_attr_destroy(&at);
-    }
-  }
-  return g_thread == 1;
+  if (g_thread == 0 && !g_quit) g_thread = pthread_create(&g_tid, NULL, release_main, NULL) == 0 ? 1 : -1;
+  return g_thread == 1 && !g_quit;
 }
 
 VS_API void vsLocalMotionsArrayDel(LocalMotions* motions, int n) {
@@ -113,5 +108,17 @@ VS_API void vsLocalMotionsReleaseWait(void) {
   pthread_mutex_unlock(&g_mu);
 }
 
-/* the queue is drained before the library goes away (process exit or unload) */
-__attribute__((destructor)) static void release_drain(void) { vsLocalMotionsReleaseWait(); }
+/* before the library goes away (process exit or unload): the queue is
+   drained, then the thread ends and is joined; later releases run in the
+   calling thread */
+__attribute__((destructor)) static void release_drain(void) {
+  pthread_mutex_lock(&g_mu);
+  const int running = g_thread == 1;
+  g_quit = 1;
+  pthread_cond_broadcast(&g_queued);
+  pthread_mutex_unlock(&g_mu);
+  if (running) pthread_join(g_tid, NULL);
+  pthread_mutex_lock(&g_mu);
+  g_thread = -1;
+  pthread_mutex_unlock(&g_mu);
+}
