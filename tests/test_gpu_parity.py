@@ -109,11 +109,13 @@ def test_flat_frames_give_empty_lists():
 
 
 @pytest.mark.parametrize("lds", [0, 1])
-@pytest.mark.parametrize("K", [4, 6, 8])
+@pytest.mark.parametrize("K", [4, 6, 7, 8])
 @pytest.mark.parametrize("name", ["c1_step5", "c2_1080p", "c3_4k"])
 def test_every_search_plan_is_bit_identical(golden, monkeypatch, name, K, lds):
-    """the autotuner may pick any plan (K rows per lane, field by scalar loads
-    or staged in LDS for fields up to 128 px): each must reproduce the fixture"""
+    """the autotuner may pick any plan (K rows per lane -- 7 is the
+    three-column tile where it applies, the four-column K = 4 elsewhere --,
+    field by scalar loads or staged in LDS for fields up to 128 px): each
+    must reproduce the fixture"""
     monkeypatch.setenv("VIDSTAB_AMD_SEARCH_K", str(K))
     monkeypatch.setenv("VIDSTAB_AMD_SEARCH_LDS", str(lds))
     case = golden[name]

@@ -72,7 +72,7 @@ def test_packed_batch_matches_reference_fixture(packed_golden, name):
 
 
 @pytest.mark.parametrize("lds", [0, 1])
-@pytest.mark.parametrize("K", [4, 6, 8])
+@pytest.mark.parametrize("K", [4, 6, 7, 8])
 @pytest.mark.parametrize("name", ["p_rgba_step8acc6", "p_rgb24_1080p"])
 def test_every_packed_search_plan_is_bit_identical(packed_golden, monkeypatch, name, K, lds):
     monkeypatch.setenv("VIDSTAB_AMD_SEARCH_K", str(K))

@@ -739,12 +739,21 @@ hipError_t launch_fallback(const VSGSearchArgs& b, const int* list, const unsign
   return hipGetLastError();
 }
 
+/* a grid on the three-column tile (J = 3, K = 7: a plan of the per-job
+   search only) back on four columns, as the split and queue kernels take it */
+void four_columns(VSGGrid& g) {
+  if (g.J != 3) return;
+  g.J = 4;
+  g.U = g.m * (((g.G + g.m - 1) / g.m + 3) / 4);
+}
+
 /* the split kernel with WPT waves per tile */
 template <int WPT>
 hipError_t launch_split(const VSGSearchArgs& a, const int* list, const unsigned* count, hipStream_t st) {
   if (a.njobs <= 0) return hipSuccess;
   if ((!list) != (!count) || a.g.S > 256 || a.g.S * a.g.bpp > 4 * 256) return hipErrorInvalidValue;
   VSGSearchArgs b = a;
+  four_columns(b.g);
   b.g.K = 4;
   b.g.V = (b.g.G + 3) / 4;
   b.tiles = (b.g.U * b.g.V + 63) / 64;
@@ -784,7 +793,7 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K, int bpp) {
   const int stepBytes = step * bpp;
   const int lcm4 = 4 * stepBytes / gcd_int(4, stepBytes);
   const int D = lcm4 / 4;
-  g->K = (K == 6 || K == 8) ? K : 4;
+  g->K = (K == 6 || K == 7 || K == 8) ? K : 4;
   g->lds = 0;
   /* the widest column chunk (<= 28 words) that divides a field row: one prev
      row segment load then feeds J*K*CW v_sad_u8 */
@@ -797,7 +806,8 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K, int bpp) {
       break;
     }
   if (D == 1 || D == 3) {
-    g->J = 4;
+    /* K = 7 is the three-column tile, for grids one word apart (D = 1) */
+    g->J = g->K == 7 && D == 1 ? 3 : 4;
     g->D = D;
     g->m = lcm4 / stepBytes;
   } else {
@@ -805,6 +815,7 @@ int vsg_plan_grid(VSGGrid* g, int S, int step, int maxShift, int K, int bpp) {
     g->D = 1;
     g->m = 1;
   }
+  if (g->K == 7 && g->J != 3) g->K = 4;
   const int perClass = (G + g->m - 1) / g->m;
   g->U = g->m * ((perClass + g->J - 1) / g->J);
   g->V = (G + g->K - 1) / g->K;
@@ -842,7 +853,9 @@ hipError_t vsg_launch_search_any(const VSGSearchArgs& a, hipStream_t st) {
     return v == 4 || v == 8 ? v : 0;
   }();
   if (a.njobs > 0 && (long long)a.njobs * a.tiles < 2LL * kSplitCus) {
-    const long long units = (long long)a.njobs * ((a.g.U * ((a.g.G + 3) / 4) + 63) / 64);
+    VSGGrid g4 = a.g;
+    four_columns(g4);
+    const long long units = (long long)a.njobs * ((g4.U * ((g4.G + 3) / 4) + 63) / 64);
     const int wpt = forced ? forced : (units <= 2LL * kSplitCus ? 8 : 4);
     return wpt == 4 ? launch_split<4>(a, nullptr, nullptr, st) : launch_split<8>(a, nullptr, nullptr, st);
   }
@@ -854,6 +867,7 @@ hipError_t vsg_launch_search(const VSGSearchArgs& a, hipStream_t st) {
   if (a.chunk <= 0 || a.g.S > 256 || a.g.wg < 64 || a.g.wg > 256 || a.g.S * a.g.bpp > 4 * a.g.wg)
     return hipErrorInvalidValue;
   const VSGGrid& g = a.g;
+  if (g.J == 3 && g.D == 1 && g.K == 7) return launch_grid_cw<3, 7, 1>(a, st);
   if (g.J == 4 && g.D == 1) return launch_grid_k<4, 1>(a, st);
   if (g.J == 4 && g.D == 3) return launch_grid_k<4, 3>(a, st);
   return launch_grid_k<1, 1>(a, st);
@@ -870,6 +884,7 @@ int vsg_queue_variant(const VSGGrid& g) {
 
 VSGSearchArgs vsg_queue_args(const VSGSearchArgs& a, int wg, int K) {
   VSGSearchArgs b = a;
+  four_columns(b.g);
   b.g.K = K;
   b.g.V = (b.g.G + K - 1) / K;
   b.g.wg = wg;

@@ -2,8 +2,12 @@
  * vsg_tune.hip -- the search-plan autotuner.
  *
  * A plan fixes how a job's (2n+1)^2 candidate grid is spread over lanes:
- * K candidate rows per lane (4, 6 or 8; J columns follow from the step)
- * and with it the workgroup size and workgroups per job; the column chunk CW
+ * K candidate rows per lane (4, 6 or 8; J columns follow from the step: 4
+ * where the candidates of a lane sit whole words apart), or, for grids whose
+ * neighbouring columns are one word apart, a tile of 3 columns by 7 rows
+ * (K = 7), which wastes fewer lane slots on the fine stage's 33 x 33 and
+ * 49 x 49 grids than 4 x 6; with it the workgroup size and workgroups per
+ * job; the column chunk CW
  * follows from the field size.  For fields up to 128 px a plan may also
  * stage the field in LDS (read by broadcast into a VGPR operand) instead of
  * reading it with scalar loads (SGPR operand), which trades scalar-cache
@@ -14,7 +18,8 @@
  * buffers) and keeps the fastest; results are cached per device and stage
  * geometry for the life of the process.
  *
- *   VIDSTAB_AMD_SEARCH_K=4|6|8   force a plan (tests, sweeps)
+ *   VIDSTAB_AMD_SEARCH_K=4|6|7|8 force a plan (tests, sweeps; 7 only where
+ *                                the three-column tile applies, else 4)
  *   VIDSTAB_AMD_AUTOTUNE=0       no timing: the cost model's plan
  */
 #include <stdlib.h>
@@ -85,7 +90,7 @@ int time_plan(VSGContext* c, int k, int K, int lds, const VSGJob* dJobs, int njo
 int vsg_plan_model(VSGGrid* g, int S, int step, int maxShift, int bpp) {
   int bestTiles = 0;
   double bestCost = 0;
-  for (int K : kPlans) {
+  for (int K : kPlans) {   /* the four-column plans */
     VSGGrid t;
     const int tiles = vsg_plan_grid(&t, S, step, maxShift, K, bpp);
     const double cost = vsg_plan_cost(t, tiles);
@@ -106,7 +111,7 @@ int vsg_ctx_tune(VSGContext* c) {
     VSGStage& S = c->st[k];
     int K = 0, lds = 0;
     const bool ldsOk = S.S <= 128 && (size_t)S.S * S.S * c->bpp <= (size_t)kMaxLdsPatch;
-    if (forced == 4 || forced == 6 || forced == 8) {
+    if (forced == 4 || forced == 6 || forced == 7 || forced == 8) {
       K = forced;
       lds = forcedLds == 1 && ldsOk;
     } else if (timing) {
@@ -138,13 +143,19 @@ int vsg_ctx_tune(VSGContext* c) {
         for (int l = 0; l < 2; ++l) {
           if (l == 1 && (!ldsOk || forcedLds == 0)) break;
           if (l == 0 && forcedLds == 1 && ldsOk) continue;
-          for (int q = 0; q < 3; ++q) {
+          /* K = 4, 6, 8, then the three-column tile where it applies (its
+             time is not among tuneMs) */
+          VSGGrid probe;
+          vsg_plan_grid(&probe, S.S, S.step, S.maxShift, 7, c->bpp);
+          const int nplans = probe.J == 3 ? 4 : 3;
+          for (int q = 0; q < nplans; ++q) {
+            const int Kq = q < 3 ? kPlans[q] : 7;
             float ms = 0;
-            if (time_plan(c, k, kPlans[q], l, dJobs, njobs, &ms) != 0) return -1;
-            S.tuneMs[3 * l + q] = ms;
+            if (time_plan(c, k, Kq, l, dJobs, njobs, &ms) != 0) return -1;
+            if (q < 3) S.tuneMs[3 * l + q] = ms;
             if (K == 0 || ms < bestMs) {
               bestMs = ms;
-              K = kPlans[q];
+              K = Kq;
               lds = l;
             }
           }
