@@ -102,7 +102,8 @@ VS_API void vsg_destroy(VSGContext* ctx);
 enum {
   VSG_FRAMES_HOST = 0,     /* luma[i]: host memory, staged by vsg_submit */
   VSG_FRAMES_DEVICE = 1,   /* luma[i]: device memory of the context's GPU, read in place */
-  VSG_FRAMES_STAGED = 2    /* already copied by the caller into vsg_stage_frames' buffers */
+  VSG_FRAMES_STAGED = 2,   /* already copied by the caller into vsg_stage_frames' buffers */
+  VSG_FRAMES_UPLOADED = 3  /* staged, and sent up to the device by vsg_stage_upload */
 };
 
 /* Enqueues a batch of n <= maxBatch frames into `set`: upload (host luma),
@@ -119,6 +120,13 @@ VS_API int vsg_submit(VSGContext* ctx, int set, int n, const uint8_t* const* lum
    over; the caller copies each frame's luma rows (width bytes each) into
    them, on as many threads as it likes, and submits with VSG_FRAMES_STAGED. */
 VS_API int vsg_stage_frames(VSGContext* ctx, int set, int n, uint8_t** stage, int* pitch);
+/* Sends bytes [offset, offset + bytes) of the set's staging area (frame i at
+   offset i x pitch x height) up to the device frames the next vsg_submit of
+   the set reads, by a copy kernel queued on the compute stream ahead of it:
+   a caller that stages a frame in row chunks uploads each chunk while it
+   stages the next, then submits with VSG_FRAMES_UPLOADED.  offset and bytes
+   are multiples of the pitch. */
+VS_API int vsg_stage_upload(VSGContext* ctx, int set, size_t offset, size_t bytes);
 VS_API int vsg_wait_coarse(VSGContext* ctx, int set, VSGBatchView* view);
 
 /* Enqueues / waits for the fine searches of a set (jobs built by the host). */

@@ -517,6 +517,17 @@ extern "C" int vsg_stage_frames(VSGContext* c, int set, int n, uint8_t** stage, 
   return 0;
 }
 
+extern "C" int vsg_stage_upload(VSGContext* c, int set, size_t offset, size_t bytes) {
+  if (!c || set < 0 || set >= VSG_NSETS) return vsg_fail("vsg_stage_upload", hipSuccess);
+  VSGSet& s = c->set[set];
+  const size_t plane = (size_t)c->pitch * c->H;
+  if (!s.m_stage || offset % (size_t)c->pitch || bytes % (size_t)c->pitch || offset + bytes > plane * c->B)
+    return vsg_fail("vsg_stage_upload: range", hipSuccess);
+  VSG_CHECK(hipSetDevice(c->device));
+  VSG_CHECK(vsg_launch_upload(s.m_stage + offset, c->d_raw + (size_t)set * c->B * plane + offset, bytes, s.stream));
+  return 0;
+}
+
 extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* luma, const int* linesize,
                           int onDevice, const int* prevSlot, const uint8_t* detect, int refSlot) {
   if (!c || set < 0 || set >= VSG_NSETS || n <= 0 || n > c->B || !luma || !linesize || !prevSlot || !detect)
@@ -547,7 +558,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
      up by the copy kernel */
   const bool kernelUpload = s.zc && n < kUploadAsideMin;
   hipStream_t up = stream;
-  if (onDevice != VSG_FRAMES_DEVICE && n >= kUploadAsideMin) {
+  if (onDevice != VSG_FRAMES_DEVICE && onDevice != VSG_FRAMES_UPLOADED && n >= kUploadAsideMin) {
     if (!c->upStream) VSG_CHECK(hipStreamCreateWithFlags(&c->upStream, hipStreamNonBlocking));
     up = c->upStream;
   }
@@ -555,12 +566,14 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
     if (onDevice == VSG_FRAMES_DEVICE) {
       s.h_src[i].p = luma[i];
       s.h_src[i].pitch = linesize[i];
-    } else if (onDevice == VSG_FRAMES_STAGED) {
-      /* the caller copied the frame into the set's staging area */
+    } else if (onDevice == VSG_FRAMES_STAGED || onDevice == VSG_FRAMES_UPLOADED) {
+      /* the caller copied the frame into the set's staging area (and, for
+         VSG_FRAMES_UPLOADED, queued its upload with vsg_stage_upload) */
       const size_t plane = (size_t)c->pitch * c->H;
       if (!s.h_stage) return vsg_fail("vsg_submit: frames not staged", hipSuccess);
       uint8_t* dst = c->d_raw + (size_t)(slot0 + i) * plane;
-      if (!kernelUpload) VSG_CHECK(hipMemcpyAsync(dst, s.h_stage + plane * i, plane, hipMemcpyHostToDevice, up));
+      if (!kernelUpload && onDevice == VSG_FRAMES_STAGED)
+        VSG_CHECK(hipMemcpyAsync(dst, s.h_stage + plane * i, plane, hipMemcpyHostToDevice, up));
       s.h_src[i].p = dst;
       s.h_src[i].pitch = c->pitch;
     } else {
@@ -590,7 +603,7 @@ extern "C" int vsg_submit(VSGContext* c, int set, int n, const uint8_t* const* l
     VSG_CHECK(hipEventRecord(s.uploaded, up));
     VSG_CHECK(hipStreamWaitEvent(stream, s.uploaded, 0));
   }
-  if (kernelUpload && onDevice != VSG_FRAMES_DEVICE) {
+  if (kernelUpload && (onDevice == VSG_FRAMES_STAGED || onDevice == VSG_FRAMES_HOST)) {
     /* the staged planes are consecutive, as are the batch's raw slots */
     const size_t plane = (size_t)c->pitch * c->H;
     VSG_CHECK(vsg_launch_upload(s.m_stage, c->d_raw + (size_t)slot0 * plane, plane * n, stream));
