@@ -95,13 +95,25 @@ __device__ __forceinline__ void vsg_sad_acc(const VSGTile& t, int lid, int p0, i
       for (int k = 0; k < K; ++k) {
         const int y = p - k * s;
         if (y >= 0 && y < S) {
-          vsg_cword_ptr crow = cbase + y * wordsPerRow;
-          const uint32_t* lrow = lbase + y * wordsPerRow;
+          if constexpr (LDSP) {
+            /* the chunk's field words (same address in every lane: broadcast
+               32-bit reads), all issued before the SADs that use them */
+            const uint32_t* lrow = lbase + y * wordsPerRow;
+            uint32_t cw[CW];
 #pragma unroll
-          for (int w = 0; w < CW; ++w) {
-            const uint32_t cw = LDSP ? lrow[w] : crow[w];
+            for (int w = 0; w < CW; ++w) cw[w] = lrow[w];
 #pragma unroll
-            for (int j = 0; j < J; ++j) L.acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw, L.acc[k][j]);
+            for (int w = 0; w < CW; ++w)
+#pragma unroll
+              for (int j = 0; j < J; ++j) L.acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw[w], L.acc[k][j]);
+          } else {
+            vsg_cword_ptr crow = cbase + y * wordsPerRow;
+#pragma unroll
+            for (int w = 0; w < CW; ++w) {
+              const uint32_t cw = crow[w];
+#pragma unroll
+              for (int j = 0; j < J; ++j) L.acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw, L.acc[k][j]);
+            }
           }
         }
       }
