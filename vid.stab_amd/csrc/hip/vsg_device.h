@@ -107,13 +107,15 @@ __device__ __forceinline__ void vsg_sad_acc(const VSGTile& t, int lid, int p0, i
 #pragma unroll
               for (int j = 0; j < J; ++j) L.acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw[w], L.acc[k][j]);
           } else {
+            /* the same from wave-uniform scalar loads (SGPR operands) */
             vsg_cword_ptr crow = cbase + y * wordsPerRow;
+            uint32_t cw[CW];
 #pragma unroll
-            for (int w = 0; w < CW; ++w) {
-              const uint32_t cw = crow[w];
+            for (int w = 0; w < CW; ++w) cw[w] = crow[w];
 #pragma unroll
-              for (int j = 0; j < J; ++j) L.acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw, L.acc[k][j]);
-            }
+            for (int w = 0; w < CW; ++w)
+#pragma unroll
+              for (int j = 0; j < J; ++j) L.acc[k][j] = __builtin_amdgcn_sad_u8(P[w + D * j], cw[w], L.acc[k][j]);
           }
         }
       }
