@@ -72,7 +72,9 @@ typedef struct _vsmotiondetect {
 VS_API VSMotionDetectConfig vsMotionDetectGetDefaultConfig(const char* modName);
 
 /* Validates the format, clamps the config, lays out the coarse and fine field
-   grids and creates the GPU context.  VS_OK or VS_ERROR.
+   grids and creates the GPU context.  VS_OK or VS_ERROR; VS_ERROR also for
+   frames too small for the grids (a coarse field moved as far as its search
+   reaches, or a fine field, would leave the frame).
    src/motiondetect.h:123-128, src/motiondetect.c:83-165 */
 VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* conf,
                               const VSFrameInfo* fi);

@@ -12,7 +12,7 @@ import pytest
 import pyoracle
 from golden_util import case_clip, case_params, golden_motions
 from vidstab_amd import api
-from vidstab_amd.synth import Clip, hash_plane
+from vidstab_amd.synth import Clip, hash_plane, pack
 
 pytestmark = pytest.mark.gpu
 
@@ -97,6 +97,31 @@ def test_matches_oracle_on_fresh_seeds(config, clip_id, nframes, step):
         assert gpu.detect(luma) == ref.detect(luma), f"frame {t}"
         assert np.array_equal(gpu.blurred(), ref.blurred()), f"blur frame {t}"
     gpu.close()
+
+
+@pytest.mark.parametrize("width,height,step,pformat", [(72, 72, 6, api.PF_GRAY8), (75, 75, 7, api.PF_GRAY8),
+                                                       (66, 66, 1, api.PF_GRAY8), (74, 90, 6, api.PF_YUV420P),
+                                                       (75, 75, 6, api.PF_RGB24), (66, 66, 1, api.PF_RGB24)])
+def test_smallest_accepted_frames_match_oracle(width, height, step, pformat):
+    """frames at the edge of what vsMotionDetectInit accepts: a 3 x 3 grid of
+    fields with little or no spacing, search windows that touch the frame's
+    edges (tests/test_small_frames_cpu.py covers the refusal)"""
+    clip = Clip(1, 61, width=width, height=height, nframes=6)
+    bpp = 3 if pformat == api.PF_RGB24 else 1
+    _say(f"[gpu] smallest frames {width}x{height} step {step} bpp {bpp}")
+    kw = dict(shakiness=5, accuracy=6, stepsize=step, mincontrast=0.1)
+    gpu = api.Detector(width, height, pformat=pformat, **kw)
+    ref = pyoracle.Detector(width, height, bpp=bpp, **kw)
+    found = 0
+    for t in range(6):
+        frame = clip.render(t) if bpp == 1 else pack(clip.render(t), bpp)
+        want = ref.detect(frame)
+        assert gpu.detect(frame) == want, f"frame {t}"
+        assert np.array_equal(gpu.blurred(), ref.blurred()), f"working plane frame {t}"
+        found += len(want)
+    gpu.close()
+    ref.close()
+    assert found > 0
 
 
 def test_flat_frames_give_empty_lists():

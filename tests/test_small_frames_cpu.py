@@ -1,0 +1,137 @@
+"""Frames too small for the detection grid, refused by vsMotionDetectInit.
+
+The reference centres its fields size/2 + maxShift + stepsize from the border
+(src/motiondetect.c:314-358); on frames narrower or shorter than twice that
+the grid's spacing turns negative and its searches read outside the frame.
+The library refuses a frame exactly when some coarse field, shifted as far as
+the search can take it, or some fine field (whose contrast is taken at its
+own position) leaves the frame.
+
+Pinned here without the library's own formulas: how far a search reaches, by
+walking the reference's search loops for every visited shift (the planar
+spiral, src/motiondetect.c:466-518, and its refinement levels, :545-567; the
+packed grid from -maxShift and its one level, :613-643) with every possible
+best shift carried into the next level; and which frames are refused, against
+the oracle's field grid over a sweep of small sizes, steps and formats.
+"""
+import ctypes
+
+import pytest
+
+import pyoracle
+from vidstab_amd import api
+
+
+def _spiral(max_shift, step):
+    """the shifts (one axis) the planar spiral visits, walked as the reference walks it"""
+    i = j = 0
+    limit, n, d = 1, 0, 0
+    xs = set()
+    while -max_shift <= j <= max_shift and -max_shift <= i <= max_shift:
+        xs.add(i)
+        xs.add(j)
+        n += 1
+        if d == 0:
+            i += step
+            if n == limit:
+                d, n = 1, 0
+        elif d == 1:
+            j += step
+            if n == limit:
+                d, n, limit = 2, 0, limit + 1
+        elif d == 2:
+            i -= step
+            if n == limit:
+                d, n = 3, 0
+        else:
+            j -= step
+            if n == limit:
+                d, n, limit = 0, 0, limit + 1
+    return xs
+
+
+def _reach(max_shift, step, packed):
+    """(left, right) reach of every shift a coarse search can read"""
+    if packed:
+        best = set(range(-max_shift, max_shift + 1, step))
+        seen = set(best)
+        if step > 1:
+            r = step - 1
+            seen |= {b + k for b in best for k in range(-r, r + 1)}
+    else:
+        best = _spiral(max_shift, step)
+        seen = set(best)
+        st = step
+        while st > 1:
+            ns = st // 2
+            r = st - ns
+            new = {b + k for b in best for k in range(-r, r + 1, ns)}
+            seen |= new
+            best |= new
+            st //= 2
+    return -min(seen), max(seen)
+
+
+def _leaves(fields, n, width, height, left, right):
+    for k in range(n):
+        f = fields[k]
+        lo, hi = f.size // 2 + left, f.size - f.size // 2 - 1 + right
+        if f.x - lo < 0 or f.x + hi > width - 1 or f.y - lo < 0 or f.y + hi > height - 1:
+            return True
+    return False
+
+
+def test_reach_of_the_walked_search_loops():
+    """the spiral covers the whole (2n+1)^2 grid, n = maxShift // step, and the
+    refinement levels add up to step - 1 on the left; the right can fall short
+    of it where a level's grid does not end on its radius"""
+    for step in range(1, 13):
+        for max_shift in range(16, 80, 3):
+            n = max_shift // step
+            assert _spiral(max_shift, step) == {k * step for k in range(-n, n + 1)}, (max_shift, step)
+            left, right = _reach(max_shift, step, packed=False)
+            assert left == n * step + step - 1, (max_shift, step)
+            assert right <= left
+            pl, pr = _reach(max_shift, step, packed=True)
+            assert pl == max_shift + step - 1
+    assert _reach(16, 7, packed=False) == (20, 18)
+
+
+def _init(width, height, step, pformat):
+    L = api.lib()
+    fi = api.VSFrameInfo()
+    assert L.vsFrameInfoInit(ctypes.byref(fi), width, height, pformat) == 1
+    conf = L.vsMotionDetectGetDefaultConfig(b"small")
+    conf.stepSize = step
+    conf.accuracy = 6   # leaves steps above 6 as they are
+    md = api.VSMotionDetect()
+    rc = L.vsMotionDetectInit(ctypes.byref(md), ctypes.byref(conf), ctypes.byref(fi))
+    if rc == 0:
+        L.vsMotionDetectionCleanup(ctypes.byref(md))
+    return rc
+
+
+SIZES = [(40, 40), (60, 200), (200, 60), (64, 64), (66, 66), (70, 70), (72, 72), (74, 90), (75, 75),
+         (76, 76), (80, 64), (90, 90), (100, 100), (113, 113), (128, 96), (160, 120)]
+
+
+@pytest.mark.parametrize("pformat,bpp", [(api.PF_GRAY8, 1), (api.PF_RGB24, 3)])
+@pytest.mark.parametrize("step", [1, 2, 4, 6, 7, 9])
+def test_refused_exactly_where_the_reference_reads_outside(capfd, pformat, bpp, step):
+    refused = accepted = 0
+    for width, height in SIZES:
+        ref = pyoracle.Detector(width, height, accuracy=6, stepsize=step, bpp=bpp)
+        st = ref.state
+        left, right = _reach(st.coarse.maxShift, st.coarse.stepSize, packed=bpp > 1)
+        outside = (_leaves(st.coarse.fields, st.coarse.fieldNum, width, height, left, right)
+                   or _leaves(st.fine.fields, st.fine.fieldNum, width, height, 0, 0))
+        ref.close()
+        rc = _init(width, height, step, pformat)
+        err = capfd.readouterr().err
+        assert ("too small" in err) == outside, f"{width}x{height} step {step}: {err.strip()}"
+        if outside:
+            assert rc == -1
+            refused += 1
+        else:
+            accepted += 1
+    assert refused > 0 and accepted > 0

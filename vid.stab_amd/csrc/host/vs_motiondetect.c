@@ -216,6 +216,46 @@ static int init_fields(VSMotionDetect* md, VSMotionDetectFields* fs, int size, i
   return 1;
 }
 
+/* How far a coarse search reads from its field's position: neg px to the
+   left / up, pos px to the right / down.  Planar frames: the spiral's n =
+   maxShift / step rings, then refinement levels of step/2, step/4, ...
+   around the best shift (src/motiondetect.c:466-518, 545-567); packed
+   frames: the grid from -maxShift in steps of step, then one level of
+   step - 1 (src/motiondetect.c:613-643). */
+static void search_reach(int maxShift, int step, int packed, int* neg, int* pos) {
+  if (packed) {
+    const int r = step > 1 ? step - 1 : 0;
+    *neg = maxShift + r;
+    *pos = -maxShift + (2 * maxShift / step) * step + r;
+    return;
+  }
+  *neg = *pos = (maxShift / step) * step;
+  for (int st = step; st > 1; st /= 2) {
+    const int ns = st / 2, r = st - ns;
+    *neg += r;
+    *pos += -r + (2 * r / ns) * ns;
+  }
+}
+
+/* 1 when every field of `fs`, shifted up to neg px left / up and pos px
+   right / down, lies inside the frame; otherwise logs the first field that
+   does not */
+static int fields_inside(const VSMotionDetect* md, const VSMotionDetectFields* fs, int neg, int pos,
+                         const char* stage) {
+  for (int i = 0; i < fs->fieldNum; i++) {
+    const Field* f = &fs->fields[i];
+    const int lo = f->size / 2 + neg, hi = f->size - f->size / 2 - 1 + pos;
+    if (f->x - lo < 0 || f->x + hi > md->fi.width - 1 || f->y - lo < 0 || f->y + hi > md->fi.height - 1) {
+      vs_log_error(md->conf.modName,
+                   "frame %ix%i is too small: the %s field at (%i, %i), %i px searched %i px left and up and "
+                   "%i px right and down, leaves it\n",
+                   md->fi.width, md->fi.height, stage, f->x, f->y, f->size, neg, pos);
+      return 0;
+    }
+  }
+  return 1;
+}
+
 int vsd_blur_size(const VSMotionDetect* md) {
   /* packed frames are searched unblurred (src/motiondetect.c:197-201) */
   if (md->fi.pFormat > PF_PACKED) return 1;
@@ -284,6 +324,15 @@ int vsd_configure(VSMotionDetect* md, const VSMotionDetectConfig* conf, const VS
     return VS_ERROR;
   if (!init_fields(md, &md->fieldsfine, fieldSizeFine, fieldSizeFine, 2, 1, fieldSizeFine,
                    md->conf.contrastThreshold / 2))
+    return VS_ERROR;
+  /* A fine field's contrast is taken at its own position, and its searches
+     are dropped per frame when they would leave the frame (vsd_frame_coarse).
+     On frames so small that the grid's spacing turns negative
+     (src/motiondetect.c:343-346) coarse search windows leave the frame and
+     the reference reads outside the frame's rows; such frames are refused. */
+  int neg, pos;
+  search_reach(maxShift, md->conf.stepSize, md->fi.pFormat > PF_PACKED, &neg, &pos);
+  if (!fields_inside(md, &md->fieldscoarse, neg, pos, "coarse") || !fields_inside(md, &md->fieldsfine, 0, 0, "fine"))
     return VS_ERROR;
   return VS_OK;
 }
