@@ -843,10 +843,12 @@ double vsg_plan_cost(const VSGGrid& g, int tiles) {
 hipError_t vsg_launch_search_any(const VSGSearchArgs& a, hipStream_t st) {
   /* a batch too small to give every CU two grid workgroups is latency-bound:
      the split kernel spreads each job's rows over four or eight waves.
-     Eight halve each wave's rows, but (88 VGPRs, 30 KB of LDS) only two
-     such workgroups fit a CU: they are used while every unit finds a
-     slot at once (a single frame's coarse jobs), four otherwise (its fine
-     jobs).  VIDSTAB_AMD_SPLIT_WAVES=4 or 8 forces one. */
+     Eight halve each wave's rows, but (175 VGPRs: 2 waves a SIMD) only one
+     such workgroup fits a CU: they are used while every unit finds a CU at
+     once (a single frame's coarse jobs), four otherwise (its fine jobs).
+     Sixteen (1,024 threads, at most 128 VGPRs) spill 712 bytes a lane and
+     made a per-frame call slower (0.315 against 0.298 ms).
+     VIDSTAB_AMD_SPLIT_WAVES=4 or 8 forces one. */
   static const int forced = [] {
     const char* e = getenv("VIDSTAB_AMD_SPLIT_WAVES");
     const int v = e ? atoi(e) : 0;
