@@ -66,7 +66,7 @@ VS_API int vsMotionDetectProposalStageStats(const VSMotionDetect* md, int stage,
    fresh detector for that clip would return; with confs[c].show > 0 the
    overlays are drawn into the clip's device frames, as those calls would
    draw them.  Up to VIDSTAB_AMD_MULTI_BATCH
-   (default 32 when the longest clip has 96 frames or more, else 16) frames
+   (default 64 when the longest clip has 96 frames or more, else 16) frames
    of every clip go through each device batch.  Clips too small for the
    field grid are refused as vsMotionDetectInit refuses them.  The
    device state of a call is kept for the next call over clips of the same

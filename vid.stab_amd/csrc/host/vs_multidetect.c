@@ -24,7 +24,7 @@
 #include "vs_pool.h"
 #include "vsg.h"
 
-#define MULTI_MAX_B 32
+#define MULTI_MAX_B 64
 #define MULTI_INFLIGHT 3
 
 typedef struct {
@@ -385,16 +385,19 @@ VS_API int vsMotionDetectMulti(const VSMotionDetectConfig* confs, const VSFrameI
   Multi M;
   memset(&M, 0, sizeof(M));
   M.nclips = nclips;
-  /* frames of every clip per device batch (at most MULTI_MAX_B = 32): 32
-     when the longest clip fills at least three such batches (the three slot
-     sets in flight), else 16.  Measured on config 4 (64 clips, 30-120
-     frames; frames/s of the whole call, against one clip after another in
-     the same run): 16: 6,599-6,615 (0.92-0.94x), 32: 6,978-7,065
-     (0.99-1.01x); on 64 clips of 30 frames: 16: 88,600-91,600,
-     32: 79,700-83,400. */
+  /* frames of every clip per device batch (at most MULTI_MAX_B = 64): 64
+     when the longest clip has 96 frames or more, else 16.  Long clips gain
+     from fewer, larger batches (fewer front launches and host steps per
+     frame); short clips from more batches in flight.  Measured on config 4
+     (64 clips, 30-120 frames; frames/s of the whole call, against one clip
+     after another in the same run): 16: 6,599-6,615 (0.92-0.94x), 32:
+     7,349-7,369 (1.00x), 48: 7,602-7,607 (1.04x), 64: 7,633-7,668
+     (1.05x); on 64 clips of 30 frames: 16: 88,600-91,600, 32:
+     79,700-83,400.  Device memory grows with B: three slot sets of B planes
+     per clip. */
   int longest = 0;
   for (int c = 0; c < nclips; c++) longest = VS_MAX(longest, nframes[c]);
-  M.B = VS_MIN(MULTI_MAX_B, VS_MAX(1, env_int("VIDSTAB_AMD_MULTI_BATCH", longest >= 3 * 32 ? 32 : 16)));
+  M.B = VS_MIN(MULTI_MAX_B, VS_MAX(1, env_int("VIDSTAB_AMD_MULTI_BATCH", longest >= 96 ? 64 : 16)));
   M.motions = motions;
   M.luma = lumaDev;
   M.nframes = nframes;
