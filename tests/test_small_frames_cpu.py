@@ -135,3 +135,12 @@ def test_refused_exactly_where_the_reference_reads_outside(capfd, pformat, bpp, 
         else:
             accepted += 1
     assert refused > 0 and accepted > 0
+
+
+def test_multi_clip_form_refuses_a_clip_too_small(capfd):
+    """vsMotionDetectMulti checks every clip's grid as vsMotionDetectInit does,
+    before any device work (the frame pointers are never read)"""
+    clip = dict(shakiness=5, accuracy=15, stepsize=6, mincontrast=0.25, linesize=640, ptrs=[0, 0])
+    with pytest.raises(RuntimeError):
+        api.detect_multi([dict(clip, width=640, height=360), dict(clip, width=64, height=64, linesize=64)])
+    assert "frame 64x64 is too small" in capfd.readouterr().err
