@@ -26,7 +26,7 @@ for p in (os.path.join(REPO, "vid.stab_amd"), os.path.join(REPO, "oracle")):
 METRIC = "detect-pass frames/s at 1080p & 4K (plain vs learned form, same shift accuracy)"
 SAD_PEAK_T = 256 * 64 * 2.4e9 * 4 / 1e12      # T abs-diff/s
 HBM_PEAK = 8000.0                              # GB/s
-COARSE_TRAFFIC_1080P = 807.1e6 + 0.05e6        # bytes read + written per 64-frame coarse launch
+COARSE_TRAFFIC_1080P = 1709.2e6 + 0.13e6       # bytes read + written per 100-frame coarse launch
 
 
 class VSGProfile(ctypes.Structure):

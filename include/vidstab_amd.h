@@ -5,7 +5,9 @@
  *
  * Environment (read by vsMotionDetectInit):
  *   VIDSTAB_AMD_DEVICE     HIP device ordinal of new detectors (default 0)
- *   VIDSTAB_AMD_MAX_BATCH  frames per device batch (default 64, at most 64)
+ *   VIDSTAB_AMD_MAX_BATCH  most frames per device batch (default 128, at most 128;
+ *                          a call's frames go through as few batches as that allows,
+ *                          all of one size but the last)
  *   VIDSTAB_AMD_SEARCH_LDS 1: field patch in LDS, 0: scalar loads (default: autotuned)
  *   VIDSTAB_AMD_HOST_THREADS  threads for the per-frame host steps of a batch
  *                          (default: numThreads, else 0.8 x cores; at most 16)

@@ -36,7 +36,7 @@
 #include "vs_pool.h"
 #include "vsg.h"
 
-#define MD_MAX_BATCH 64
+#define MD_MAX_BATCH 128
 
 /* ---- per-detector state ------------------------------------------------- */
 
@@ -361,7 +361,7 @@ VS_API int vsMotionDetectInit(VSMotionDetect* md, const VSMotionDetectConfig* co
     return VS_ERROR;
   }
   s->key = md;
-  s->maxBatch = VS_MIN(MD_MAX_BATCH, VS_MAX(1, env_int("VIDSTAB_AMD_MAX_BATCH", 64)));
+  s->maxBatch = VS_MIN(MD_MAX_BATCH, VS_MAX(1, env_int("VIDSTAB_AMD_MAX_BATCH", MD_MAX_BATCH)));
   s->lastSlot = -1;
   const int device = env_int("VIDSTAB_AMD_DEVICE", 0);
   VSGStageDesc dc = vsd_stage_desc(&md->fieldscoarse), df = vsd_stage_desc(&md->fieldsfine);
@@ -786,7 +786,10 @@ static int batch_submit_t(VSMotionDetect* md, MDState* s, Batch* b, const uint8_
   return r;
 }
 
-/* Runs n frames through batches of maxBatch, three in flight: batch k+2's
+/* Runs n frames through as few batches of at most maxBatch as there can be,
+   all of one size but the last (300 frames: 3 x 100, not 2 x 128 + 44 --
+   1080p 31,250 frames/s against 29,790-29,880 with 4 x 64 + 44; 4 x 75:
+   29,440-29,740), three in flight: batch k+2's
    front stages and coarse search are queued, then batch k+1's coarse results
    become its fine jobs, and only then does the host wait for batch k's fine
    search and clean it up -- so the device always holds the next batch's
@@ -801,8 +804,8 @@ static int detect_frames(VSMotionDetect* md, LocalMotions* motions, const uint8_
   }
   if (n == 0) return VS_OK;
   const double tcall = s->timed ? now_us() : 0;
-  const int B = s->maxBatch;
-  const int nb = (n + B - 1) / B;
+  const int nb = (n + s->maxBatch - 1) / s->maxBatch;
+  const int B = (n + nb - 1) / nb;   /* <= maxBatch, and (nb - 1) * B < n */
   Batch* bs = s->bs;
   for (int k = 0; k < MD_INFLIGHT; k++) bs[k].active = 0;
   int ok = 1, done = 0;
